@@ -1,0 +1,178 @@
+/*
+ * ps_sift.h -- C ABI of the MI355X SIFT extract + match engine (libps_sift.so).
+ *
+ * Drop-in boundary for pure-sift 0.2.0's hot path. Every entry point below
+ * names the reference interface it replaces (file:line in pure-sift 0.2.0).
+ * Plain pointers and sizes only; all functions return ps_status (0 = OK,
+ * negative = error; ps_status_string / ps_last_error explain). The library
+ * never allocates memory that crosses the boundary: callers own every buffer.
+ *
+ * Thread safety: every function may be called concurrently from any thread.
+ * Host-pointer entry points borrow an engine (device stream + workspace) from
+ * an internal pool for the duration of the call. Engine entry points run
+ * asynchronously on that engine's stream and take device pointers.
+ *
+ * There is no CPU fallback: without a usable gfx950 device every compute call
+ * returns PS_E_NO_DEVICE.
+ */
+#ifndef PS_SIFT_H
+#define PS_SIFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t ps_status;
+#define PS_OK 0
+#define PS_E_INVALID_ARG (-1)   /* null pointer / bad size */
+#define PS_E_CONFIG (-2)        /* SiftConfig / MatchOptions validation failure */
+#define PS_E_DEVICE (-3)        /* HIP runtime error */
+#define PS_E_CAPACITY (-4)      /* output buffer too small; *out_count holds the needed size */
+#define PS_E_NO_DEVICE (-5)     /* no usable GPU */
+#define PS_E_OUT_OF_MEMORY (-6) /* device allocation failed */
+#define PS_E_INJECTED (-7)      /* failure forced by ps_debug_fail_next (tests only) */
+
+#define PS_DESCRIPTOR_LEN 128      /* pipeline.rs:998 SIFT_DESCRIPTOR_LEN */
+#define PS_MAX_FEATURES 4096       /* pipeline.rs:1054 MAX_FEATURES_PER_IMAGE */
+
+#define PS_DESCRIPTOR_SIFT 0       /* pipeline.rs:1002-1008 DescriptorKind::Sift */
+#define PS_DESCRIPTOR_ROOTSIFT 1   /* DescriptorKind::RootSift */
+
+/* pipeline.rs:22-34 SiftConfig (same fields, same order) */
+typedef struct {
+    uint32_t octaves;
+    uint32_t scales;
+    float contrast_threshold;
+    uint8_t upsample_first_octave;
+    uint8_t descriptor_kind;
+    uint8_t _pad[2];
+} ps_sift_config;
+
+/* pipeline.rs:1720-1727 MatchOptions */
+typedef struct {
+    float ratio;
+    uint8_t mutual;
+    uint8_t _pad[3];
+} ps_match_options;
+
+/* pipeline.rs:251-267 Keypoint */
+typedef struct {
+    float x, y, scale, response;
+    uint32_t octave, layer;
+} ps_keypoint;
+
+/* pipeline.rs:1058-1072 SiftFeature without the descriptor Vec; descriptors travel
+ * as a separate row-major [n][128] float array (feature i <-> row i). */
+typedef struct {
+    float x, y, scale, response, orientation;
+} ps_feature;
+
+/* pipeline.rs:1075-1091 FeatureMatch */
+typedef struct {
+    uint64_t query_index, target_index;
+    float qx, qy;   /* query_point */
+    float tx, ty;   /* target_point */
+    float distance, ratio;
+    float dx, dy;   /* displacement */
+} ps_feature_match;
+
+/* ------------------------------------------------------------------ setup */
+
+/* Selects and initialises a device (idempotent, thread-safe). device < 0 -> current/0. */
+ps_status ps_init(int device);
+const char *ps_status_string(ps_status s);
+/* Message of the last error raised on the calling thread ("" if none). */
+const char *ps_last_error(void);
+/* pipeline.rs:80-103 SiftConfig::validate; msg (optional) gets the reference's message. */
+ps_status ps_validate_sift_config(const ps_sift_config *cfg, char *msg, size_t msg_cap);
+/* pipeline.rs:156-164 Matcher::new validation. */
+ps_status ps_validate_match_options(const ps_match_options *opt, char *msg, size_t msg_cap);
+
+/* --------------------------------------------- host-pointer entry points */
+
+/* pipeline.rs:136-145 Sift::detect_and_compute -> extract_sift_features_with_kind
+ * (pipeline.rs:1633-1658). img: h x w f32 luma in [0,1], row stride in elements.
+ * Writes at most `cap` features (<= 4096 are ever produced; cap >= 4096 always suffices). */
+ps_status ps_detect_and_compute(const float *img, size_t h, size_t w, size_t row_stride,
+                                const ps_sift_config *cfg, ps_feature *out_feat,
+                                float *out_desc, size_t cap, size_t *out_count);
+
+/* Same as ps_detect_and_compute on u8 luma; the (px * (1/255)) conversion of
+ * luma_to_array2 (pipeline.rs:957-966) runs fused on the device, identical values. */
+ps_status ps_detect_and_compute_u8(const uint8_t *img, size_t h, size_t w, size_t row_stride,
+                                   const ps_sift_config *cfg, ps_feature *out_feat,
+                                   float *out_desc, size_t cap, size_t *out_count);
+
+/* pipeline.rs:125-133 Sift::detect_keypoints (uncapped, scan order). If more than `cap`
+ * keypoints exist, returns PS_E_CAPACITY and sets *out_count to the needed count. */
+ps_status ps_detect_keypoints(const float *img, size_t h, size_t w, size_t row_stride,
+                              const ps_sift_config *cfg, ps_keypoint *out, size_t cap,
+                              size_t *out_count);
+
+/* pipeline.rs:167-169 Matcher::match_ -> match_descriptors_with_options (pipeline.rs:1744-1841).
+ * Descriptors are [n][128] f32. out needs capacity >= nq. */
+ps_status ps_match(const ps_feature *q, const float *q_desc, size_t nq,
+                   const ps_feature *t, const float *t_desc, size_t nt,
+                   const ps_match_options *opt, ps_feature_match *out, size_t cap, size_t *out_count);
+
+/* Debug/parity: the full Gaussian + DoG pyramid of build_scale_space_pyramid_with_options
+ * (pipeline.rs:550-638). Layout: for each octave o, (scales+3) Gaussian planes then
+ * (scales+2) DoG planes of dims[2o] x dims[2o+1] floats, octaves back to back.
+ * Pass out=NULL to query *out_len (floats) and dims (cap 2*8 ints) / *n_octaves. */
+ps_status ps_build_pyramid(const float *img, size_t h, size_t w, size_t row_stride,
+                           const ps_sift_config *cfg, float *out, size_t out_cap, size_t *out_len,
+                           int32_t *dims, int32_t *n_octaves);
+
+/* Nearest / second-nearest squared distances and nearest index per query
+ * (pipeline.rs:1764-1786), bit-exact with the reference's sequential f32 sum. */
+ps_status ps_nn2(const float *q_desc, size_t nq, const float *t_desc, size_t nt,
+                 uint32_t *best_idx, float *best_d2, float *second_d2);
+
+/* ---------------------------------------------------- engine entry points */
+/* An engine owns one HIP stream and a workspace. Engine calls enqueue work on
+ * that stream and return immediately; device pointers only. */
+typedef struct ps_engine ps_engine;
+
+ps_status ps_engine_create(int device, ps_engine **out);
+ps_status ps_engine_destroy(ps_engine *e);
+/* hipStream_t of the engine, as an opaque pointer. */
+void *ps_engine_stream(ps_engine *e);
+ps_status ps_engine_sync(ps_engine *e);
+
+/* Device-resident extract: d_img u8 h x w (row stride in bytes). Writes up to cap
+ * features + descriptors and *d_count (device u32). */
+ps_status ps_engine_extract_u8(ps_engine *e, const uint8_t *d_img, size_t h, size_t w,
+                               size_t row_stride, const ps_sift_config *cfg,
+                               ps_feature *d_feat, float *d_desc, size_t cap, uint32_t *d_count);
+
+/* Device-resident match. Counts are device u32s (as written by ps_engine_extract_u8);
+ * nq_max / nt_max bound them (array capacities). Writes d_out[0..*d_out_count). */
+ps_status ps_engine_match(ps_engine *e, const ps_feature *d_qf, const float *d_qd,
+                          const uint32_t *d_nq, size_t nq_max, const ps_feature *d_tf,
+                          const float *d_td, const uint32_t *d_nt, size_t nt_max,
+                          const ps_match_options *opt, ps_feature_match *d_out, uint32_t *d_out_count);
+
+/* Device-side error word of the engine (0 = none; set when an internal capacity
+ * bound was exceeded during an async call). Synchronises the engine. */
+ps_status ps_engine_check(ps_engine *e);
+
+/* Stage timing with HIP events recorded on the engine's stream (profiling hook; off by
+ * default). Between begin and end, every engine extract/match call records its stage
+ * boundaries; end synchronises and returns per-stage totals in milliseconds:
+ * [0] pyramid (blur + DoG), [1] extrema, [2] order + cap, [3] orientation + descriptors,
+ * [4] match; *calls gets the number of extract calls recorded. */
+#define PS_NUM_STAGES 5
+ps_status ps_engine_profile_begin(ps_engine *e, uint32_t max_calls);
+ps_status ps_engine_profile_end(ps_engine *e, float stage_ms[PS_NUM_STAGES], uint32_t *calls);
+
+/* Tests only: the next compute call on this thread fails with PS_E_INJECTED before
+ * touching the device. */
+void ps_debug_fail_next(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,137 @@
+/*
+ * ps_oracle.h -- CPU oracle for the SIFT extract + match hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a plain-C restatement of pure-sift 0.2.0's
+ * CPU pipeline (src/pipeline.rs, src/catalog.rs) used as the parity checker
+ * for the HIP implementation and, timed, as the CPU baseline. Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it. The
+ * product library (libps_sift.so) never links or calls it.
+ *
+ * Numerics follow the reference's f32 code exactly: no FMA contraction
+ * (built with -ffp-contract=off), left-to-right evaluation as written in the
+ * Rust source, glibc libm for exp/atan2/pow/sin/cos (the platform libm Rust
+ * binds on Linux), binary-exponentiation powi (compiler-rt __powisf2 order),
+ * stable sorts, sequential f32 sums.
+ *
+ * Parity status of each function (how it is pinned) is listed in the header
+ * comment of ps_oracle.c.
+ */
+#ifndef PS_ORACLE_H
+#define PS_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSR_DESC_LEN 128
+#define PSR_MAX_FEATURES 4096
+#define PSR_MAX_OCTAVES 8
+
+typedef struct { float x, y, scale, response; uint32_t octave, layer; } psr_keypoint;
+typedef struct { float x, y, scale, response, orientation; } psr_feature;
+typedef struct {
+    uint64_t query_index, target_index;
+    float qx, qy, tx, ty, distance, ratio, dx, dy;
+} psr_match;
+
+typedef struct {
+    int n_octaves;          /* octaves actually built (stops when h<3 || w<3) */
+    int scales;
+    int h[PSR_MAX_OCTAVES], w[PSR_MAX_OCTAVES];
+    float scale_factor[PSR_MAX_OCTAVES];
+    float *gauss[PSR_MAX_OCTAVES]; /* (scales+3) planes of h*w, level-major */
+    float *dog[PSR_MAX_OCTAVES];   /* (scales+2) planes of h*w */
+} psr_pyramid;
+
+/* threading (mirrors the reference's Rayon sites); n<=0 -> all cores */
+void psr_set_threads(int n);
+int psr_get_threads(void);
+
+/* pipeline.rs:275-296 */
+int psr_gaussian_kernel_1d(float sigma, float *out, int cap);
+/* pipeline.rs:314-425 */
+void psr_gaussian_blur(const float *in, int h, int w, float sigma, float *out);
+/* pipeline.rs:429-449 (out is (h/2)x(w/2)) */
+void psr_downsample_2x(const float *in, int h, int w, float *out);
+/* pipeline.rs:455-483 (out is 2h x 2w) */
+void psr_upsample_2x(const float *in, int h, int w, float *out);
+/* pipeline.rs:957-966 */
+void psr_luma_to_f32(const uint8_t *in, size_t n, float *out);
+/* pipeline.rs:550-638 */
+int psr_build_pyramid(const float *img, int h, int w, int octaves, int scales,
+                      int upsample, psr_pyramid *out);
+void psr_free_pyramid(psr_pyramid *p);
+/* pipeline.rs:535-547 / 810-916; *out is malloc'd, caller frees with psr_free */
+size_t psr_extract_extrema(const psr_pyramid *p, float threshold, psr_keypoint **out);
+/* pipeline.rs:969-978 */
+size_t psr_detect_keypoints(const float *img, int h, int w, int octaves, int scales,
+                            float threshold, int upsample, psr_keypoint **out);
+/* pipeline.rs:1233-1347, on caller-provided gradient planes; returns count (<=4) */
+int psr_keypoint_orientations(const float *mag, const float *ang, int w, int h,
+                              const psr_keypoint *octave_local_kp, float out[4]);
+/* pipeline.rs:1168-1203 */
+void psr_gradient_planes(const float *gauss, int h, int w, float *mag, float *ang);
+/* pipeline.rs:1352-1480 */
+void psr_descriptor_128(const float *mag, const float *ang, int w, int h,
+                        const psr_keypoint *octave_local_kp, float orientation,
+                        float out[PSR_DESC_LEN]);
+/* pipeline.rs:1012-1019 */
+void psr_rootsift(const float *in, float *out);
+/* pipeline.rs:1482-1577: features for an ordered keypoint list (no 4096 truncation) */
+size_t psr_descriptors_from_pyramid(const psr_pyramid *p, const psr_keypoint *kps, size_t n,
+                                    int kind, psr_feature **feats, float **descs);
+/* pipeline.rs:1633-1658 (extract_sift_features_with_kind); cap + truncate to 4096 */
+size_t psr_extract_features(const float *img, int h, int w, int octaves, int scales,
+                            float threshold, int upsample, int kind,
+                            psr_feature **feats, float **descs);
+/* pipeline.rs:1744-1841; out has capacity nq; returns count */
+size_t psr_match_descriptors(const psr_feature *q, const float *qd, size_t nq,
+                             const psr_feature *t, const float *td, size_t nt,
+                             float ratio, int mutual, psr_match *out);
+/* per-query nearest / second-nearest squared distance (pipeline.rs:1764-1786) */
+void psr_nn2(const float *qd, size_t nq, const float *td, size_t nt,
+             uint32_t *best_idx, float *best_d2, float *second_d2);
+/* catalog.rs:37-45 (two little-endian u64 words per descriptor) */
+void psr_quantize_u128(const float *desc, size_t n, uint64_t *out);
+/* catalog.rs:707-728 */
+size_t psr_count_coarse_hamming_votes(const uint64_t *q, size_t nq, const uint64_t *e, size_t ne);
+/* top-k Hamming shortlist, ordered by (distance asc, index asc) */
+void psr_hamming_topk(const uint64_t *catalog, size_t n, const uint64_t *queries, size_t nq,
+                      int k, uint32_t *out_idx, uint8_t *out_dist);
+
+/* RANSAC (pipeline.rs:1860-2755) */
+uint64_t psr_splitmix64_next(uint64_t *state);
+typedef struct { float a, b, tx, ty, scale, rotation_deg; } psr_similarity;
+/* returns inlier count; inlier_idx (cap n) gets indices into matches; *has_model set */
+size_t psr_filter_matches_ransac(const psr_match *m, size_t n, float thresh, uint64_t seed,
+                                 uint32_t *inlier_idx, psr_similarity *model, int *has_model);
+/* sampler: 0 = reference 1/ratio CDF, 1 = uniform, 2 = caller weights (guided) */
+typedef struct {
+    int sampler;
+    const float *weights;   /* sampler 2: per-match non-negative weights */
+    double confidence;      /* adaptive-termination confidence (reference: 0.999) */
+    int min_iters, max_iters;
+    int local_opt;          /* 1 = LO-RANSAC as the reference */
+    int sim_seed_model;     /* 1 = seed with the similarity RANSAC model as the reference */
+    int adaptive;           /* 0 = run exactly max_iters hypotheses */
+} psr_ransac_params;
+size_t psr_homography_ransac(const psr_match *m, size_t n, float thresh, uint64_t sim_seed,
+                             uint64_t hom_seed, const psr_ransac_params *prm,
+                             uint32_t *inlier_idx, float h_out[9], int *has_model,
+                             uint32_t *hyps_used);
+/* reference entry point estimate_homography_ransac_with_seed semantics */
+size_t psr_homography_ransac_ref(const psr_match *m, size_t n, float thresh, uint64_t seed,
+                                 int seed_is_default, uint32_t *inlier_idx, float h_out[9],
+                                 int *has_model);
+int psr_fit_homography_dlt(const psr_match *m, size_t n, double h_out[9]);
+size_t psr_adaptive_ransac_iters(size_t inl, size_t total, int sample_size, double conf,
+                                 size_t min_iters, size_t max_cap);
+
+void psr_free(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,402 @@
+"""ctypes binding of the CPU oracle (libps_oracle.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker (or the timed CPU
+baseline), never as the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libps_oracle.so")
+
+DESC_LEN = 128
+MAX_FEATURES = 4096
+
+
+class Keypoint(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("scale", C.c_float), ("response", C.c_float),
+                ("octave", C.c_uint32), ("layer", C.c_uint32)]
+
+
+class Feature(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("scale", C.c_float), ("response", C.c_float),
+                ("orientation", C.c_float)]
+
+
+class Match(C.Structure):
+    _fields_ = [("query_index", C.c_uint64), ("target_index", C.c_uint64),
+                ("qx", C.c_float), ("qy", C.c_float), ("tx", C.c_float), ("ty", C.c_float),
+                ("distance", C.c_float), ("ratio", C.c_float), ("dx", C.c_float), ("dy", C.c_float)]
+
+
+class Pyramid(C.Structure):
+    _fields_ = [("n_octaves", C.c_int), ("scales", C.c_int), ("h", C.c_int * 8), ("w", C.c_int * 8),
+                ("scale_factor", C.c_float * 8), ("gauss", C.POINTER(C.c_float) * 8),
+                ("dog", C.POINTER(C.c_float) * 8)]
+
+
+class Similarity(C.Structure):
+    _fields_ = [("a", C.c_float), ("b", C.c_float), ("tx", C.c_float), ("ty", C.c_float),
+                ("scale", C.c_float), ("rotation_deg", C.c_float)]
+
+
+class RansacParams(C.Structure):
+    _fields_ = [("sampler", C.c_int), ("weights", C.POINTER(C.c_float)), ("confidence", C.c_double),
+                ("min_iters", C.c_int), ("max_iters", C.c_int), ("local_opt", C.c_int),
+                ("sim_seed_model", C.c_int), ("adaptive", C.c_int)]
+
+
+KP_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("scale", "<f4"), ("response", "<f4"),
+                     ("octave", "<u4"), ("layer", "<u4")])
+FEAT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("scale", "<f4"), ("response", "<f4"),
+                       ("orientation", "<f4")])
+MATCH_DTYPE = np.dtype([("query_index", "<u8"), ("target_index", "<u8"), ("qx", "<f4"), ("qy", "<f4"),
+                        ("tx", "<f4"), ("ty", "<f4"), ("distance", "<f4"), ("ratio", "<f4"),
+                        ("dx", "<f4"), ("dy", "<f4")])
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        fp = C.POINTER(C.c_float)
+        L.psr_gaussian_kernel_1d.argtypes = [C.c_float, fp, C.c_int]
+        L.psr_gaussian_kernel_1d.restype = C.c_int
+        L.psr_gaussian_blur.argtypes = [fp, C.c_int, C.c_int, C.c_float, fp]
+        L.psr_downsample_2x.argtypes = [fp, C.c_int, C.c_int, fp]
+        L.psr_upsample_2x.argtypes = [fp, C.c_int, C.c_int, fp]
+        L.psr_luma_to_f32.argtypes = [C.c_void_p, C.c_size_t, fp]
+        L.psr_build_pyramid.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Pyramid)]
+        L.psr_free_pyramid.argtypes = [C.POINTER(Pyramid)]
+        L.psr_extract_extrema.argtypes = [C.POINTER(Pyramid), C.c_float, C.POINTER(C.POINTER(Keypoint))]
+        L.psr_extract_extrema.restype = C.c_size_t
+        L.psr_detect_keypoints.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int,
+                                           C.POINTER(C.POINTER(Keypoint))]
+        L.psr_detect_keypoints.restype = C.c_size_t
+        L.psr_gradient_planes.argtypes = [fp, C.c_int, C.c_int, fp, fp]
+        L.psr_keypoint_orientations.argtypes = [fp, fp, C.c_int, C.c_int, C.POINTER(Keypoint), fp]
+        L.psr_keypoint_orientations.restype = C.c_int
+        L.psr_descriptor_128.argtypes = [fp, fp, C.c_int, C.c_int, C.POINTER(Keypoint), C.c_float, fp]
+        L.psr_rootsift.argtypes = [fp, fp]
+        L.psr_descriptors_from_pyramid.argtypes = [C.POINTER(Pyramid), C.POINTER(Keypoint), C.c_size_t, C.c_int,
+                                                   C.POINTER(C.POINTER(Feature)), C.POINTER(fp)]
+        L.psr_descriptors_from_pyramid.restype = C.c_size_t
+        L.psr_extract_features.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int,
+                                           C.POINTER(C.POINTER(Feature)), C.POINTER(fp)]
+        L.psr_extract_features.restype = C.c_size_t
+        L.psr_match_descriptors.argtypes = [C.POINTER(Feature), fp, C.c_size_t, C.POINTER(Feature), fp, C.c_size_t,
+                                            C.c_float, C.c_int, C.POINTER(Match)]
+        L.psr_match_descriptors.restype = C.c_size_t
+        L.psr_nn2.argtypes = [fp, C.c_size_t, fp, C.c_size_t, C.POINTER(C.c_uint32), fp, fp]
+        L.psr_quantize_u128.argtypes = [fp, C.c_size_t, C.POINTER(C.c_uint64)]
+        L.psr_count_coarse_hamming_votes.argtypes = [C.POINTER(C.c_uint64), C.c_size_t, C.POINTER(C.c_uint64),
+                                                     C.c_size_t]
+        L.psr_count_coarse_hamming_votes.restype = C.c_size_t
+        L.psr_hamming_topk.argtypes = [C.POINTER(C.c_uint64), C.c_size_t, C.POINTER(C.c_uint64), C.c_size_t,
+                                       C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint8)]
+        L.psr_splitmix64_next.argtypes = [C.POINTER(C.c_uint64)]
+        L.psr_splitmix64_next.restype = C.c_uint64
+        L.psr_filter_matches_ransac.argtypes = [C.POINTER(Match), C.c_size_t, C.c_float, C.c_uint64,
+                                                C.POINTER(C.c_uint32), C.POINTER(Similarity), C.POINTER(C.c_int)]
+        L.psr_filter_matches_ransac.restype = C.c_size_t
+        L.psr_homography_ransac.argtypes = [C.POINTER(Match), C.c_size_t, C.c_float, C.c_uint64, C.c_uint64,
+                                            C.POINTER(RansacParams), C.POINTER(C.c_uint32), fp, C.POINTER(C.c_int),
+                                            C.POINTER(C.c_uint32)]
+        L.psr_homography_ransac.restype = C.c_size_t
+        L.psr_homography_ransac_ref.argtypes = [C.POINTER(Match), C.c_size_t, C.c_float, C.c_uint64, C.c_int,
+                                                C.POINTER(C.c_uint32), fp, C.POINTER(C.c_int)]
+        L.psr_homography_ransac_ref.restype = C.c_size_t
+        L.psr_fit_homography_dlt.argtypes = [C.POINTER(Match), C.c_size_t, C.POINTER(C.c_double)]
+        L.psr_fit_homography_dlt.restype = C.c_int
+        L.psr_adaptive_ransac_iters.argtypes = [C.c_size_t, C.c_size_t, C.c_int, C.c_double, C.c_size_t, C.c_size_t]
+        L.psr_adaptive_ransac_iters.restype = C.c_size_t
+        L.psr_set_threads.argtypes = [C.c_int]
+        L.psr_get_threads.restype = C.c_int
+        L.psr_free.argtypes = [C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def set_threads(n: int) -> None:
+    lib().psr_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return lib().psr_get_threads()
+
+
+def gaussian_kernel_1d(sigma: float) -> np.ndarray:
+    out = np.zeros(1024, np.float32)
+    n = lib().psr_gaussian_kernel_1d(C.c_float(sigma), _fp(out), 1024)
+    assert n > 0
+    return out[:n].copy()
+
+
+def gaussian_blur(img: np.ndarray, sigma: float) -> np.ndarray:
+    img = _f32(img)
+    out = np.zeros_like(img)
+    lib().psr_gaussian_blur(_fp(img), img.shape[0], img.shape[1], C.c_float(sigma), _fp(out))
+    return out
+
+
+def upsample_2x(img: np.ndarray) -> np.ndarray:
+    img = _f32(img)
+    out = np.zeros((img.shape[0] * 2, img.shape[1] * 2), np.float32)
+    lib().psr_upsample_2x(_fp(img), img.shape[0], img.shape[1], _fp(out))
+    return out
+
+
+def downsample_2x(img: np.ndarray) -> np.ndarray:
+    img = _f32(img)
+    out = np.zeros((img.shape[0] // 2, img.shape[1] // 2), np.float32)
+    if out.size:
+        lib().psr_downsample_2x(_fp(img), img.shape[0], img.shape[1], _fp(out))
+    return out
+
+
+def luma_to_f32(img_u8: np.ndarray) -> np.ndarray:
+    img_u8 = np.ascontiguousarray(img_u8, dtype=np.uint8)
+    out = np.zeros(img_u8.shape, np.float32)
+    lib().psr_luma_to_f32(img_u8.ctypes.data, img_u8.size, _fp(out))
+    return out
+
+
+def build_pyramid(img: np.ndarray, octaves: int, scales: int, upsample: bool):
+    """Returns a list of octaves: dict(gauss=[S+3,h,w], dog=[S+2,h,w], scale_factor)."""
+    img = _f32(img)
+    p = Pyramid()
+    lib().psr_build_pyramid(_fp(img), img.shape[0], img.shape[1], octaves, scales, int(upsample), C.byref(p))
+    res = []
+    for o in range(p.n_octaves):
+        h, w = p.h[o], p.w[o]
+        g = np.ctypeslib.as_array(p.gauss[o], shape=((scales + 3) * h * w,)).reshape(scales + 3, h, w).copy()
+        d = np.ctypeslib.as_array(p.dog[o], shape=((scales + 2) * h * w,)).reshape(scales + 2, h, w).copy()
+        res.append(dict(gauss=g, dog=d, scale_factor=p.scale_factor[o]))
+    lib().psr_free_pyramid(C.byref(p))
+    return res
+
+
+def _take_kps(ptr, n):
+    if n == 0:
+        lib().psr_free(ptr)
+        return np.zeros(0, KP_DTYPE)
+    arr = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_uint8)), shape=(n * KP_DTYPE.itemsize,))
+    out = arr.view(KP_DTYPE).copy()
+    lib().psr_free(ptr)
+    return out
+
+
+def detect_keypoints(img, octaves=3, scales=3, threshold=0.015, upsample=True) -> np.ndarray:
+    img = _f32(img)
+    ptr = C.POINTER(Keypoint)()
+    n = lib().psr_detect_keypoints(_fp(img), img.shape[0], img.shape[1], octaves, scales, C.c_float(threshold),
+                                   int(upsample), C.byref(ptr))
+    return _take_kps(ptr, n)
+
+
+def gradient_planes(g: np.ndarray):
+    g = _f32(g)
+    mag = np.zeros_like(g)
+    ang = np.zeros_like(g)
+    lib().psr_gradient_planes(_fp(g), g.shape[0], g.shape[1], _fp(mag), _fp(ang))
+    return mag, ang
+
+
+def keypoint_orientations(mag, ang, kp) -> np.ndarray:
+    mag, ang = _f32(mag), _f32(ang)
+    k = Keypoint(*[kp[f] for f in KP_DTYPE.names])
+    out = np.zeros(4, np.float32)
+    n = lib().psr_keypoint_orientations(_fp(mag), _fp(ang), mag.shape[1], mag.shape[0], C.byref(k), _fp(out))
+    return out[:n].copy()
+
+
+def descriptor_128(mag, ang, kp, orientation: float) -> np.ndarray:
+    mag, ang = _f32(mag), _f32(ang)
+    k = Keypoint(*[kp[f] for f in KP_DTYPE.names])
+    out = np.zeros(128, np.float32)
+    lib().psr_descriptor_128(_fp(mag), _fp(ang), mag.shape[1], mag.shape[0], C.byref(k), C.c_float(orientation),
+                             _fp(out))
+    return out
+
+
+def _take_feats(fptr, dptr, n):
+    if n == 0:
+        lib().psr_free(fptr)
+        lib().psr_free(dptr)
+        return np.zeros(0, FEAT_DTYPE), np.zeros((0, 128), np.float32)
+    f = np.ctypeslib.as_array(C.cast(fptr, C.POINTER(C.c_uint8)), shape=(n * FEAT_DTYPE.itemsize,))
+    feats = f.view(FEAT_DTYPE).copy()
+    descs = np.ctypeslib.as_array(dptr, shape=(n * 128,)).reshape(n, 128).copy()
+    lib().psr_free(fptr)
+    lib().psr_free(dptr)
+    return feats, descs
+
+
+def extract_features(img, octaves=3, scales=3, threshold=0.015, upsample=True, kind=0):
+    """extract_sift_features_with_kind: returns (features[FEAT_DTYPE], descriptors[n,128])."""
+    img = _f32(img)
+    fptr = C.POINTER(Feature)()
+    dptr = C.POINTER(C.c_float)()
+    n = lib().psr_extract_features(_fp(img), img.shape[0], img.shape[1], octaves, scales, C.c_float(threshold),
+                                   int(upsample), int(kind), C.byref(fptr), C.byref(dptr))
+    return _take_feats(fptr, dptr, n)
+
+
+def descriptors_for_keypoints(img, kps, octaves, scales, upsample, kind=0):
+    """compute_sift_descriptors_from_pyramid_with_kind on the pyramid of img (no truncation)."""
+    img = _f32(img)
+    p = Pyramid()
+    lib().psr_build_pyramid(_fp(img), img.shape[0], img.shape[1], octaves, scales, int(upsample), C.byref(p))
+    kps = np.ascontiguousarray(kps, KP_DTYPE)
+    fptr = C.POINTER(Feature)()
+    dptr = C.POINTER(C.c_float)()
+    n = lib().psr_descriptors_from_pyramid(C.byref(p), kps.ctypes.data_as(C.POINTER(Keypoint)), len(kps), int(kind),
+                                           C.byref(fptr), C.byref(dptr))
+    lib().psr_free_pyramid(C.byref(p))
+    return _take_feats(fptr, dptr, n)
+
+
+def match(qf, qd, tf, td, ratio=0.75, mutual=False) -> np.ndarray:
+    qf = np.ascontiguousarray(qf, FEAT_DTYPE)
+    tf = np.ascontiguousarray(tf, FEAT_DTYPE)
+    qd, td = _f32(qd), _f32(td)
+    out = np.zeros(max(len(qf), 1), MATCH_DTYPE)
+    n = lib().psr_match_descriptors(qf.ctypes.data_as(C.POINTER(Feature)), _fp(qd), len(qf),
+                                    tf.ctypes.data_as(C.POINTER(Feature)), _fp(td), len(tf),
+                                    C.c_float(ratio), int(mutual), out.ctypes.data_as(C.POINTER(Match)))
+    return out[:n].copy()
+
+
+def nn2(qd, td):
+    qd, td = _f32(qd), _f32(td)
+    nq = len(qd)
+    bi = np.zeros(nq, np.uint32)
+    b = np.zeros(nq, np.float32)
+    s = np.zeros(nq, np.float32)
+    lib().psr_nn2(_fp(qd), nq, _fp(td), len(td), bi.ctypes.data_as(C.POINTER(C.c_uint32)), _fp(b), _fp(s))
+    return bi, b, s
+
+
+def quantize_u128(desc) -> np.ndarray:
+    desc = _f32(desc).reshape(-1, 128)
+    out = np.zeros((len(desc), 2), np.uint64)
+    lib().psr_quantize_u128(_fp(desc), len(desc), out.ctypes.data_as(C.POINTER(C.c_uint64)))
+    return out
+
+
+def count_coarse_hamming_votes(q, e) -> int:
+    q = np.ascontiguousarray(q, np.uint64)
+    e = np.ascontiguousarray(e, np.uint64)
+    return lib().psr_count_coarse_hamming_votes(q.ctypes.data_as(C.POINTER(C.c_uint64)), len(q),
+                                                e.ctypes.data_as(C.POINTER(C.c_uint64)), len(e))
+
+
+def hamming_topk(catalog, queries, k=100):
+    catalog = np.ascontiguousarray(catalog, np.uint64)
+    queries = np.ascontiguousarray(queries, np.uint64)
+    nq = len(queries)
+    idx = np.zeros((nq, k), np.uint32)
+    dist = np.zeros((nq, k), np.uint8)
+    lib().psr_hamming_topk(catalog.ctypes.data_as(C.POINTER(C.c_uint64)), len(catalog),
+                           queries.ctypes.data_as(C.POINTER(C.c_uint64)), nq, k,
+                           idx.ctypes.data_as(C.POINTER(C.c_uint32)), dist.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return idx, dist
+
+
+def splitmix64(state: int, n: int):
+    st = C.c_uint64(state)
+    return [lib().psr_splitmix64_next(C.byref(st)) for _ in range(n)]
+
+
+def make_matches(qpts, tpts, ratio=0.5, distance=0.0) -> np.ndarray:
+    qpts = np.asarray(qpts, np.float32).reshape(-1, 2)
+    tpts = np.asarray(tpts, np.float32).reshape(-1, 2)
+    n = len(qpts)
+    m = np.zeros(n, MATCH_DTYPE)
+    m["query_index"] = np.arange(n)
+    m["target_index"] = np.arange(n)
+    m["qx"], m["qy"] = qpts[:, 0], qpts[:, 1]
+    m["tx"], m["ty"] = tpts[:, 0], tpts[:, 1]
+    m["distance"] = distance
+    m["ratio"] = ratio
+    m["dx"] = tpts[:, 0] - qpts[:, 0]
+    m["dy"] = tpts[:, 1] - qpts[:, 1]
+    return m
+
+
+def filter_matches_ransac(m, thresh=3.0, seed=0xD1B54A32D192ED03):
+    m = np.ascontiguousarray(m, MATCH_DTYPE)
+    inl = np.zeros(max(len(m), 1), np.uint32)
+    model = Similarity()
+    has = C.c_int(0)
+    n = lib().psr_filter_matches_ransac(m.ctypes.data_as(C.POINTER(Match)), len(m), C.c_float(thresh),
+                                        C.c_uint64(seed), inl.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                        C.byref(model), C.byref(has))
+    mod = dict(a=model.a, b=model.b, tx=model.tx, ty=model.ty, scale=model.scale,
+               rotation_deg=model.rotation_deg) if has.value else None
+    return inl[:n].copy(), mod
+
+
+def homography_ransac_ref(m, thresh=3.0, seed=None):
+    m = np.ascontiguousarray(m, MATCH_DTYPE)
+    inl = np.zeros(max(len(m), 1), np.uint32)
+    h = np.zeros(9, np.float32)
+    has = C.c_int(0)
+    n = lib().psr_homography_ransac_ref(m.ctypes.data_as(C.POINTER(Match)), len(m), C.c_float(thresh),
+                                        C.c_uint64(0 if seed is None else seed), int(seed is None),
+                                        inl.ctypes.data_as(C.POINTER(C.c_uint32)), _fp(h), C.byref(has))
+    return inl[:n].copy(), (h.reshape(3, 3).copy() if has.value else None)
+
+
+def homography_ransac(m, thresh=3.0, sim_seed=0, hom_seed=0, sampler=1, weights=None, confidence=0.99,
+                      min_iters=1, max_iters=10000, local_opt=False, sim_seed_model=False, adaptive=True):
+    """Generalised RANSAC: sampler 0 = reference 1/ratio CDF, 1 = uniform, 2 = weights (guided)."""
+    m = np.ascontiguousarray(m, MATCH_DTYPE)
+    w = None
+    prm = RansacParams(sampler, None, confidence, min_iters, max_iters, int(local_opt), int(sim_seed_model),
+                       int(adaptive))
+    if sampler == 2:
+        w = _f32(weights)
+        prm.weights = _fp(w)
+    inl = np.zeros(max(len(m), 1), np.uint32)
+    h = np.zeros(9, np.float32)
+    has = C.c_int(0)
+    hyps = C.c_uint32(0)
+    n = lib().psr_homography_ransac(m.ctypes.data_as(C.POINTER(Match)), len(m), C.c_float(thresh),
+                                    C.c_uint64(sim_seed), C.c_uint64(hom_seed), C.byref(prm),
+                                    inl.ctypes.data_as(C.POINTER(C.c_uint32)), _fp(h), C.byref(has), C.byref(hyps))
+    return inl[:n].copy(), (h.reshape(3, 3).copy() if has.value else None), hyps.value
+
+
+def fit_homography_dlt(m):
+    m = np.ascontiguousarray(m, MATCH_DTYPE)
+    h = np.zeros(9, np.float64)
+    ok = lib().psr_fit_homography_dlt(m.ctypes.data_as(C.POINTER(Match)), len(m),
+                                      h.ctypes.data_as(C.POINTER(C.c_double)))
+    return h.reshape(3, 3) if ok else None
+
+
+def adaptive_ransac_iters(inl, total, sample_size, conf, min_iters, max_cap) -> int:
+    return lib().psr_adaptive_ransac_iters(inl, total, sample_size, conf, min_iters, max_cap)

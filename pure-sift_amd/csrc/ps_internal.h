@@ -1,0 +1,112 @@
+// Internal declarations shared by the HIP translation units of libps_sift.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/ps_sift.h"
+
+namespace ps {
+
+constexpr int kMaxOct = 8;
+constexpr int kMaxScales = 10;
+constexpr int kMaxLevels = kMaxScales + 3;    // Gaussians per octave
+constexpr int kMaxTaps = 33;                  // 2*16+1
+constexpr uint32_t kCandCap = 1u << 20;       // extremum candidates per image before ordering
+constexpr uint32_t kErrCandOverflow = 1u;
+constexpr uint32_t kErrRadius = 2u;
+
+// Keypoint plus its scan-order key: (octave << 56) | (slice << 48) | (y << 24) | x.
+struct Cand {
+    float x, y, scale, response;
+    uint32_t octave, layer;
+    uint64_t key;
+};
+static_assert(sizeof(Cand) == 32, "Cand layout");
+
+struct Taps {
+    float w[kMaxTaps];
+};
+
+// Geometry of one image's pyramid inside the engine's pyramid buffer. Octave o holds
+// (S+3) Gaussian planes followed by (S+2) DoG planes of h[o] x w[o] floats.
+struct PyrPlan {
+    int n_oct = 0, S = 0, upsample = 0, in_h = 0, in_w = 0;
+    int h[kMaxOct] = {}, w[kMaxOct] = {};
+    size_t off[kMaxOct] = {};                 // float offset of octave o
+    float scale_factor[kMaxOct] = {};
+    int init_r = 0;
+    Taps init_taps{};
+    int lvl_r[kMaxLevels] = {};               // radius of the blur producing Gaussian s (s >= 1)
+    Taps lvl_taps[kMaxLevels]{};
+    float eff_scale[kMaxOct][kMaxLevels] = {};// BASE_SIGMA * mult * k^s (pipeline.rs:839)
+    float k = 0.f;                            // 2^(1/S) (glibc powf, pipeline.rs:561/827)
+    double log2k = 0.0;
+    size_t total_floats = 0;
+    size_t gauss(int o, int s) const { return off[o] + (size_t)s * h[o] * w[o]; }
+    size_t dog(int o, int s) const { return off[o] + (size_t)(S + 3 + s) * h[o] * w[o]; }
+};
+
+// Read-only view of the pyramid for the descriptor kernels (passed by value).
+struct PyrView {
+    const float *base;
+    size_t off[kMaxOct];
+    int h[kMaxOct], w[kMaxOct];
+    float scale_factor[kMaxOct];
+    int n_oct, S;
+};
+
+enum SrcMode { SRC_F32 = 0, SRC_U8 = 1, SRC_F32_UP = 2, SRC_U8_UP = 3, SRC_DOWN2 = 4 };
+
+struct BlurArgs {
+    const void *src;
+    size_t src_stride;   // elements of the source type
+    int src_h, src_w;    // source dims (pre-upsample / pre-downsample)
+    int h, w;            // output dims
+    float *dst_g;        // Gaussian out (nullable)
+    float *dst_d;        // DoG out = G - src (nullable)
+    float *dst_src;      // the source value itself (nullable; materialises G0 for SRC_DOWN2)
+};
+
+// host helpers (engine.hip)
+bool make_plan(int h, int w, const ps_sift_config &cfg, PyrPlan &p);
+
+// launches (pyramid.hip)
+hipError_t launch_blur(int radius, int src_mode, const BlurArgs &a, const Taps &t, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st);
+hipError_t launch_extrema(const float *dog_octave, int h, int w, int S, float thr, float mult,
+                          const float *eff_scale, double log2k, int octave, Cand *out, uint32_t *count,
+                          uint32_t cap, uint32_t *err, hipStream_t st);
+// Orders candidates: scan order if n <= limit, else (|response| desc, scan order); writes the
+// first min(n, limit) into out and that count into out_count.
+hipError_t launch_rank(const Cand *cand, const uint32_t *count, uint32_t cap_in, uint32_t limit,
+                       Cand *out, uint32_t *out_count, hipStream_t st);
+
+// describe.hip
+hipError_t launch_orientations(const PyrView &pv, const Cand *kps, const uint32_t *nkp, uint32_t max_kp,
+                               float *ori, uint32_t *nori, hipStream_t st);
+hipError_t launch_feature_scan(const uint32_t *nkp, const uint32_t *nori, uint32_t max_kp, uint32_t cap,
+                               uint32_t *feat_kp, uint32_t *feat_j, uint32_t *nfeat, hipStream_t st);
+hipError_t launch_descriptors(const PyrView &pv, const Cand *kps, const float *ori, const uint32_t *feat_kp,
+                              const uint32_t *feat_j, const uint32_t *nfeat, uint32_t cap, int kind,
+                              ps_feature *out_f, float *out_d, hipStream_t st);
+hipError_t launch_cand_to_keypoints(const Cand *c, const uint32_t *n, uint32_t cap, ps_keypoint *out,
+                                    hipStream_t st);
+
+// match.hip
+struct NNPart {
+    float best, second;
+    uint32_t idx, pad;
+};
+int nn_splits(size_t nq_max, size_t nt_max);
+hipError_t launch_nn_partial(const float *qd, const uint32_t *nq, size_t nq_max, const float *td,
+                             const uint32_t *nt, size_t nt_max, int nsplit, NNPart *part, hipStream_t st);
+hipError_t launch_nn_merge(const NNPart *part, int nsplit, const uint32_t *nq, size_t nq_max, uint32_t *best_idx,
+                           float *best, float *second, hipStream_t st);
+hipError_t launch_match_finish(const ps_feature *qf, const ps_feature *tf, const uint32_t *nq, size_t nq_max,
+                               const uint32_t *nt, const uint32_t *best_idx, const float *best,
+                               const float *second, const uint32_t *rev_idx, float ratio, int mutual,
+                               ps_feature_match *out, uint32_t *out_count, hipStream_t st);
+
+}  // namespace ps

@@ -1,0 +1,383 @@
+// Scale-space pyramid (fused upsample + separable Gaussian + DoG), extremum
+// detection with Hessian edge rejection and 3-D Taylor refinement, and the
+// keypoint ordering / 4096 cap -- gfx950 kernels.
+//
+// Bit-exactness with pure-sift 0.2.0 (src/pipeline.rs): every output pixel sums
+// its taps in the reference's order with separate multiply and add roundings
+// (no FMA: -ffp-contract=off plus the pragma below), Gaussian weights come from
+// the host (glibc expf, exactly pipeline.rs:275-296), and the extremum tests /
+// refinement repeat the reference's f32 expressions operation by operation.
+#include "ps_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace ps {
+
+// ------------------------------------------------------------------ blur + DoG
+// One workgroup produces a TW x TH tile of Gaussian level s (and DoG s) from level
+// s-1: the (TH+2R) x (TW+2R) clamped source window is staged in LDS, the horizontal
+// pass writes (TH+2R) x TW partial rows to LDS, the vertical pass reads them back.
+// Clamping the staged window reproduces the reference's per-tap index clamp
+// (pipeline.rs:339, :401) because every staged element is the clamped source pixel.
+constexpr int kTW = 64;
+constexpr int kTH = 32;
+constexpr int kBlurThreads = 256;
+
+template <int SRC>
+__device__ __forceinline__ float load_src(const BlurArgs &a, int y, int x) {
+    const float inv255 = 1.0f / 255.0f;   // luma_to_array2, pipeline.rs:961
+    if constexpr (SRC == SRC_F32) {
+        return static_cast<const float *>(a.src)[(size_t)y * a.src_stride + x];
+    } else if constexpr (SRC == SRC_U8) {
+        return (float)static_cast<const uint8_t *>(a.src)[(size_t)y * a.src_stride + x] * inv255;
+    } else if constexpr (SRC == SRC_DOWN2) {   // downsample_2x, pipeline.rs:443-446
+        return static_cast<const float *>(a.src)[(size_t)(2 * y) * a.src_stride + 2 * x];
+    } else {                                   // upsample_2x_bilinear, pipeline.rs:469-479
+        int y0 = y >> 1, x0 = x >> 1;
+        int y1 = min(y0 + (y & 1), a.src_h - 1), x1 = min(x0 + (x & 1), a.src_w - 1);
+        float p00, p01, p10, p11;
+        if constexpr (SRC == SRC_F32_UP) {
+            const float *s = static_cast<const float *>(a.src);
+            p00 = s[(size_t)y0 * a.src_stride + x0]; p01 = s[(size_t)y0 * a.src_stride + x1];
+            p10 = s[(size_t)y1 * a.src_stride + x0]; p11 = s[(size_t)y1 * a.src_stride + x1];
+        } else {
+            const uint8_t *s = static_cast<const uint8_t *>(a.src);
+            p00 = (float)s[(size_t)y0 * a.src_stride + x0] * inv255;
+            p01 = (float)s[(size_t)y0 * a.src_stride + x1] * inv255;
+            p10 = (float)s[(size_t)y1 * a.src_stride + x0] * inv255;
+            p11 = (float)s[(size_t)y1 * a.src_stride + x1] * inv255;
+        }
+        return 0.25f * (p00 + p01 + p10 + p11);
+    }
+}
+
+template <int R, int SRC>
+__global__ __launch_bounds__(kBlurThreads) void blur_kernel(BlurArgs a, Taps taps) {
+    constexpr int IW = kTW + 2 * R;
+    constexpr int IWP = (IW + 3) & ~3;
+    constexpr int IH = kTH + 2 * R;
+    constexpr int NT = 2 * R + 1;
+    __shared__ __attribute__((aligned(16))) float s_in[IH * IWP];
+    __shared__ __attribute__((aligned(16))) float s_tmp[IH * kTW];
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * kTW, y0 = blockIdx.y * kTH;
+    const int h = a.h, w = a.w;
+
+    for (int i = tid; i < IH * IW; i += kBlurThreads) {
+        int r = i / IW, c = i - r * IW;
+        int gy = min(max(y0 - R + r, 0), h - 1);
+        int gx = min(max(x0 - R + c, 0), w - 1);
+        s_in[r * IWP + c] = load_src<SRC>(a, gy, gx);
+    }
+    __syncthreads();
+
+    // horizontal: 4 consecutive outputs per thread, taps in k order (pipeline.rs:351-354)
+    constexpr int HC = kTW / 4;
+    for (int i = tid; i < IH * HC; i += kBlurThreads) {
+        int r = i / HC, c = (i - r * HC) * 4;
+        const float *src = s_in + r * IWP + c;
+        float v[NT + 3];
+#pragma unroll
+        for (int j = 0; j < NT + 3; j++) v[j] = src[j];
+        float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+#pragma unroll
+        for (int k = 0; k < NT; k++) {
+            float wk = taps.w[k];
+            a0 += v[k] * wk;
+            a1 += v[k + 1] * wk;
+            a2 += v[k + 2] * wk;
+            a3 += v[k + 3] * wk;
+        }
+        float4 o = make_float4(a0, a1, a2, a3);
+        *reinterpret_cast<float4 *>(s_tmp + r * kTW + c) = o;
+    }
+    __syncthreads();
+
+    // vertical: 8 consecutive rows of one column per thread (pipeline.rs:400-406)
+    constexpr int VR = 8;
+    for (int i = tid; i < (kTH / VR) * kTW; i += kBlurThreads) {
+        int c = i % kTW, r0 = (i / kTW) * VR;
+        float v[VR + NT - 1];
+#pragma unroll
+        for (int j = 0; j < VR + NT - 1; j++) v[j] = s_tmp[(r0 + j) * kTW + c];
+        float acc[VR];
+#pragma unroll
+        for (int j = 0; j < VR; j++) acc[j] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < NT; k++) {
+            float wk = taps.w[k];
+#pragma unroll
+            for (int j = 0; j < VR; j++) acc[j] += v[j + k] * wk;
+        }
+        int x = x0 + c;
+        if (x < w) {
+#pragma unroll
+            for (int j = 0; j < VR; j++) {
+                int y = y0 + r0 + j;
+                if (y < h) {
+                    size_t o = (size_t)y * w + x;
+                    float center = s_in[(r0 + j + R) * IWP + c + R];
+                    if (a.dst_g) a.dst_g[o] = acc[j];
+                    if (a.dst_d) a.dst_d[o] = acc[j] - center;   // pipeline.rs:611
+                    if (a.dst_src) a.dst_src[o] = center;
+                }
+            }
+        }
+    }
+}
+
+template <int R>
+static hipError_t blur_dispatch_src(int src_mode, const BlurArgs &a, const Taps &t, hipStream_t st) {
+    dim3 grid((a.w + kTW - 1) / kTW, (a.h + kTH - 1) / kTH);
+    switch (src_mode) {
+    case SRC_F32: blur_kernel<R, SRC_F32><<<grid, kBlurThreads, 0, st>>>(a, t); break;
+    case SRC_DOWN2: blur_kernel<R, SRC_DOWN2><<<grid, kBlurThreads, 0, st>>>(a, t); break;
+    case SRC_U8:
+    case SRC_F32_UP:
+    case SRC_U8_UP:
+        if constexpr (R == 4 || R == 5) {
+            if (src_mode == SRC_U8) blur_kernel<R, SRC_U8><<<grid, kBlurThreads, 0, st>>>(a, t);
+            else if (src_mode == SRC_F32_UP) blur_kernel<R, SRC_F32_UP><<<grid, kBlurThreads, 0, st>>>(a, t);
+            else blur_kernel<R, SRC_U8_UP><<<grid, kBlurThreads, 0, st>>>(a, t);
+            break;
+        } else {
+            return hipErrorInvalidValue;
+        }
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_blur(int radius, int src_mode, const BlurArgs &a, const Taps &t, hipStream_t st) {
+    if (a.h <= 0 || a.w <= 0) return hipSuccess;
+    switch (radius) {
+    case 1: return blur_dispatch_src<1>(src_mode, a, t, st);
+    case 2: return blur_dispatch_src<2>(src_mode, a, t, st);
+    case 3: return blur_dispatch_src<3>(src_mode, a, t, st);
+    case 4: return blur_dispatch_src<4>(src_mode, a, t, st);
+    case 5: return blur_dispatch_src<5>(src_mode, a, t, st);
+    case 6: return blur_dispatch_src<6>(src_mode, a, t, st);
+    case 7: return blur_dispatch_src<7>(src_mode, a, t, st);
+    case 8: return blur_dispatch_src<8>(src_mode, a, t, st);
+    case 9: return blur_dispatch_src<9>(src_mode, a, t, st);
+    case 10: return blur_dispatch_src<10>(src_mode, a, t, st);
+    case 11: return blur_dispatch_src<11>(src_mode, a, t, st);
+    case 12: return blur_dispatch_src<12>(src_mode, a, t, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, uint32_t v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    fill_u32_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(p, v, n);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ extrema
+struct EffScale {
+    float v[kMaxLevels];
+};
+
+// pipeline.rs:664-687
+__device__ __forceinline__ bool hessian_edge_ok(const float *c, int w, int x, int y, float center) {
+    const float *rp = c + (size_t)(y - 1) * w, *rc = c + (size_t)y * w, *rn = c + (size_t)(y + 1) * w;
+    float dxx = rc[x + 1] - 2.0f * center + rc[x - 1];
+    float dyy = rn[x] - 2.0f * center + rp[x];
+    float dxy = 0.25f * ((rn[x + 1] - rn[x - 1]) - (rp[x + 1] - rp[x - 1]));
+    float trace = dxx + dyy;
+    float det = dxx * dyy - dxy * dxy;
+    const float THRESHOLD_RATIO = ((10.0f + 1.0f) * (10.0f + 1.0f)) / 10.0f;
+    return det > 1e-9f && (trace * trace) < THRESHOLD_RATIO * det;
+}
+
+// pipeline.rs:697-770 (f32, operation by operation)
+__device__ __forceinline__ void refine3d(const float *ps, const float *cs, const float *ns, int w, int x, int y,
+                                         float &odx, float &ody, float &ods, float &oi) {
+    const float *c_prev = cs + (size_t)(y - 1) * w, *c_curr = cs + (size_t)y * w, *c_next = cs + (size_t)(y + 1) * w;
+    const float *p_prev = ps + (size_t)(y - 1) * w, *p_curr = ps + (size_t)y * w, *p_next = ps + (size_t)(y + 1) * w;
+    const float *n_prev = ns + (size_t)(y - 1) * w, *n_curr = ns + (size_t)y * w, *n_next = ns + (size_t)(y + 1) * w;
+    float center = c_curr[x];
+    float gx = 0.5f * (c_curr[x + 1] - c_curr[x - 1]);
+    float gy = 0.5f * (c_next[x] - c_prev[x]);
+    float gs = 0.5f * (n_curr[x] - p_curr[x]);
+    float dxx = c_curr[x + 1] - 2.0f * center + c_curr[x - 1];
+    float dyy = c_next[x] - 2.0f * center + c_prev[x];
+    float dss = n_curr[x] - 2.0f * center + p_curr[x];
+    float dxy = 0.25f * ((c_next[x + 1] - c_next[x - 1]) - (c_prev[x + 1] - c_prev[x - 1]));
+    float dxs = 0.25f * ((n_curr[x + 1] - n_curr[x - 1]) - (p_curr[x + 1] - p_curr[x - 1]));
+    float dys = 0.25f * ((n_next[x] - n_prev[x]) - (p_next[x] - p_prev[x]));
+    float c00 = dyy * dss - dys * dys;
+    float c01 = dys * dxs - dxy * dss;
+    float c02 = dxy * dys - dyy * dxs;
+    float c11 = dxx * dss - dxs * dxs;
+    float c12 = dxy * dxs - dxx * dys;
+    float c22 = dxx * dyy - dxy * dxy;
+    float det3 = dxx * c00 + dxy * c01 + dxs * c02;
+    if (fabsf(det3) > 1e-9f) {
+        float inv_det = -1.0f / det3;
+        float ox = (c00 * gx + c01 * gy + c02 * gs) * inv_det;
+        float oy = (c01 * gx + c11 * gy + c12 * gs) * inv_det;
+        float os = (c02 * gx + c12 * gy + c22 * gs) * inv_det;
+        if (fabsf(ox) <= 0.5f && fabsf(oy) <= 0.5f && fabsf(os) <= 0.5f) {
+            odx = ox; ody = oy; ods = os;
+            oi = center + 0.5f * (gx * ox + gy * oy + gs * os);
+            return;
+        }
+    }
+    if (fabsf(c22) > 1e-9f) {
+        float inv_det2 = -1.0f / c22;
+        float ox = (dyy * gx - dxy * gy) * inv_det2;
+        float oy = (-dxy * gx + dxx * gy) * inv_det2;
+        ox = ox < -0.5f ? -0.5f : (ox > 0.5f ? 0.5f : ox);
+        oy = oy < -0.5f ? -0.5f : (oy > 0.5f ? 0.5f : oy);
+        float os = 0.0f;
+        if (fabsf(dss) > 1e-6f) {
+            os = -gs / dss;
+            os = os < -0.5f ? -0.5f : (os > 0.5f ? 0.5f : os);
+        }
+        odx = ox; ody = oy; ods = os;
+        oi = center + 0.5f * (gx * ox + gy * oy + gs * os);
+        return;
+    }
+    odx = 0.0f; ody = 0.0f; ods = 0.0f; oi = center;
+}
+
+// One thread per interior pixel of one searchable slice (pipeline.rs:841-897). Accepted
+// keypoints are appended with their scan-order key; launch_rank restores the order.
+__global__ __launch_bounds__(256) void extrema_kernel(const float *dog, int h, int w, float thr, float mult,
+                                                      EffScale eff, double log2k, int S, int octave, Cand *out,
+                                                      uint32_t *count, uint32_t cap, uint32_t *err) {
+    const int x = 1 + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int s = blockIdx.z + 1;
+    if (x >= w - 1 || y >= h - 1) return;
+    const size_t P = (size_t)h * w;
+    const float *ps = dog + (size_t)(s - 1) * P, *cs = dog + (size_t)s * P, *ns = dog + (size_t)(s + 1) * P;
+    const float val = cs[(size_t)y * w + x];
+    if (fabsf(val) <= thr) return;
+    bool ext = true;
+    if (val > 0.0f) {
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++) {
+                size_t o = (size_t)(y + dy) * w + (x + dx);
+                if (dx != 0 || dy != 0) ext = ext && (val > cs[o]);
+                ext = ext && (val > ps[o]) && (val > ns[o]);
+            }
+    } else {
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++) {
+                size_t o = (size_t)(y + dy) * w + (x + dx);
+                if (dx != 0 || dy != 0) ext = ext && (val < cs[o]);
+                ext = ext && (val < ps[o]) && (val < ns[o]);
+            }
+    }
+    if (!ext || !hessian_edge_ok(cs, w, x, y, val)) return;
+    float dx, dy, ds, interp;
+    refine3d(ps, cs, ns, w, x, y, dx, dy, ds, interp);
+    if (!(fabsf(interp) > thr)) return;
+    // k.powf(ds): evaluated in f64 and rounded once (glibc powf agrees except in ~0.06% of
+    // arguments, where the two differ by one ulp)
+    float kpow = (float)exp2((double)ds * log2k);
+    float refined_scale = eff.v[s] * kpow;
+    float lf = roundf((float)s + ds);   // pipeline.rs:882-885
+    lf = lf < 1.0f ? 1.0f : (lf > (float)S ? (float)S : lf);
+    Cand c;
+    c.x = ((float)x + dx) * mult;
+    c.y = ((float)y + dy) * mult;
+    c.scale = fmaxf(refined_scale, 0.25f * 1.6f);
+    c.response = interp;
+    c.octave = (uint32_t)octave;
+    c.layer = (uint32_t)lf;
+    c.key = ((uint64_t)octave << 56) | ((uint64_t)s << 48) | ((uint64_t)y << 24) | (uint64_t)x;
+    uint32_t i = atomicAdd(count, 1u);
+    if (i < cap) out[i] = c;
+    else atomicOr(err, kErrCandOverflow);
+}
+
+hipError_t launch_extrema(const float *dog_octave, int h, int w, int S, float thr, float mult, const float *eff_scale,
+                          double log2k, int octave, Cand *out, uint32_t *count, uint32_t cap, uint32_t *err,
+                          hipStream_t st) {
+    if (h < 3 || w < 3 || S < 1) return hipSuccess;
+    EffScale e;
+    for (int i = 0; i < kMaxLevels; i++) e.v[i] = eff_scale[i];
+    dim3 grid((w - 2 + 63) / 64, (h - 2 + 3) / 4, S);
+    extrema_kernel<<<grid, 256, 0, st>>>(dog_octave, h, w, thr, mult, e, log2k, S, octave, out, count, cap, err);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ ordering + cap
+// rank[i] = #{j : j precedes i}, where "precedes" is scan order when n <= limit (no
+// sort, pipeline.rs:1643) and otherwise the stable |response|-descending order of
+// pipeline.rs:1644-1649 (ties keep scan order). A fixed 64x64 grid of workgroups walks
+// the (i-tile, j-tile) pairs; partial counts are summed with integer atomics, so the
+// result is exact and independent of scheduling.
+constexpr int kRankTile = 256;
+constexpr int kRankGrid = 64;
+
+__global__ __launch_bounds__(kRankTile) void rank_count_kernel(const Cand *c, const uint32_t *count, uint32_t cap_in,
+                                                               uint32_t limit, uint32_t *rank) {
+    __shared__ float s_r[kRankTile];
+    __shared__ uint64_t s_k[kRankTile];
+    const uint32_t n = min(*count, cap_in);
+    const bool by_resp = n > limit;
+    const uint32_t ntile = (n + kRankTile - 1) / kRankTile;
+    for (uint32_t ti = blockIdx.x; ti < ntile; ti += kRankGrid) {
+        const uint32_t i = ti * kRankTile + threadIdx.x;
+        float ri = 0.0f;
+        uint64_t ki = 0;
+        if (i < n) { ri = fabsf(c[i].response); ki = c[i].key; }
+        uint32_t acc = 0;
+        for (uint32_t tj = blockIdx.y; tj < ntile; tj += kRankGrid) {
+            const uint32_t j = tj * kRankTile + threadIdx.x;
+            __syncthreads();
+            if (j < n) { s_r[threadIdx.x] = fabsf(c[j].response); s_k[threadIdx.x] = c[j].key; }
+            __syncthreads();
+            const uint32_t m = min((uint32_t)kRankTile, n - tj * kRankTile);
+            if (by_resp) {
+                for (uint32_t t = 0; t < m; t++) {
+                    float rj = s_r[t];
+                    acc += (rj > ri || (rj == ri && s_k[t] < ki)) ? 1u : 0u;
+                }
+            } else {
+                for (uint32_t t = 0; t < m; t++) acc += s_k[t] < ki ? 1u : 0u;
+            }
+        }
+        if (i < n && acc) atomicAdd(&rank[i], acc);
+    }
+}
+
+__global__ void rank_scatter_kernel(const Cand *c, const uint32_t *count, uint32_t cap_in, uint32_t limit,
+                                    const uint32_t *rank, Cand *out, uint32_t *out_count) {
+    const uint32_t n = min(*count, cap_in);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *out_count = min(n, limit);
+    if (i < n && rank[i] < limit) out[rank[i]] = c[i];
+}
+
+__global__ void fill_upto_kernel(uint32_t *p, const uint32_t *count, uint32_t cap) {
+    const uint32_t n = min(*count, cap);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+hipError_t launch_rank(const Cand *cand, const uint32_t *count, uint32_t cap_in, uint32_t limit, Cand *out,
+                       uint32_t *out_count, hipStream_t st) {
+    // the rank array lives right after the candidates' capacity in the caller's scratch
+    uint32_t *rank = reinterpret_cast<uint32_t *>(const_cast<Cand *>(cand) + cap_in);
+    unsigned blocks = (cap_in + 255) / 256;
+    fill_upto_kernel<<<blocks, 256, 0, st>>>(rank, count, cap_in);
+    rank_count_kernel<<<dim3(kRankGrid, kRankGrid), kRankTile, 0, st>>>(cand, count, cap_in, limit, rank);
+    rank_scatter_kernel<<<blocks, 256, 0, st>>>(cand, count, cap_in, limit, rank, out, out_count);
+    return hipGetLastError();
+}
+
+}  // namespace ps

@@ -1,0 +1,399 @@
+"""Python host mirror of pure-sift 0.2.0's hot-path API over libps_sift.so.
+
+Same names, argument meaning and error behaviour as the Rust API
+(src/pipeline.rs:22-170):
+
+    cfg = SiftConfig().octaves(4).scales(3).contrast_threshold(0.015)
+    sift = Sift.new(cfg)                      # raises ValueError(msg) like Result::Err(String)
+    feats = sift.detect_and_compute(img)      # img: (h, w) float32 in [0, 1] or uint8 luma
+    matcher = Matcher.new(MatchOptions(ratio=0.8, mutual=False))
+    matches = matcher.match_(feats_q, feats_t)
+
+Features are returned as a `Features` record of numpy arrays (x, y, scale,
+response, orientation, descriptors[n, 128]); `Features.to_list()` gives
+per-feature `SiftFeature` objects like the reference's Vec<SiftFeature>.
+
+Every compute call runs the HIP kernels; there is no CPU fallback. Without a
+gfx950 device the calls raise DeviceError (PS_E_NO_DEVICE).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+from typing import List, Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libps_sift.so")
+REPO = os.path.dirname(HERE)
+HEADER_PATH = os.path.join(REPO, "include", "ps_sift.h")
+
+# pipeline.rs:10-19, :998, :1047-1054
+BASE_SIGMA = 1.6
+ASSUMED_INPUT_SIGMA = 0.5
+DEFAULT_UPSAMPLE_FIRST_OCTAVE = True
+DEFAULT_CONTRAST_THRESHOLD = 0.015
+SIFT_DESCRIPTOR_LEN = 128
+DEFAULT_MATCH_RATIO = 0.75
+DEFAULT_RANSAC_REPROJ_THRESHOLD = 3.0
+MAX_FEATURES_PER_IMAGE = 4096
+
+PS_OK = 0
+PS_E_INVALID_ARG = -1
+PS_E_CONFIG = -2
+PS_E_DEVICE = -3
+PS_E_CAPACITY = -4
+PS_E_NO_DEVICE = -5
+PS_E_OUT_OF_MEMORY = -6
+PS_E_INJECTED = -7
+
+
+class DeviceError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"[{status}] {msg}")
+        self.status = status
+
+
+class DescriptorKind:
+    Sift = 0
+    RootSift = 1
+
+
+class _SiftConfigC(C.Structure):
+    _fields_ = [("octaves", C.c_uint32), ("scales", C.c_uint32), ("contrast_threshold", C.c_float),
+                ("upsample_first_octave", C.c_uint8), ("descriptor_kind", C.c_uint8), ("_pad", C.c_uint8 * 2)]
+
+
+class _MatchOptionsC(C.Structure):
+    _fields_ = [("ratio", C.c_float), ("mutual", C.c_uint8), ("_pad", C.c_uint8 * 3)]
+
+
+FEAT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("scale", "<f4"), ("response", "<f4"),
+                       ("orientation", "<f4")])
+KP_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("scale", "<f4"), ("response", "<f4"),
+                     ("octave", "<u4"), ("layer", "<u4")])
+MATCH_DTYPE = np.dtype([("query_index", "<u8"), ("target_index", "<u8"), ("qx", "<f4"), ("qy", "<f4"),
+                        ("tx", "<f4"), ("ty", "<f4"), ("distance", "<f4"), ("ratio", "<f4"),
+                        ("dx", "<f4"), ("dy", "<f4")])
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Loads the in-tree libps_sift.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() (make -C pure-sift_amd)")
+        L = C.CDLL(LIB_PATH)
+        vp, sz, i32, u32p = C.c_void_p, C.c_size_t, C.c_int32, C.POINTER(C.c_uint32)
+        L.ps_init.argtypes = [C.c_int]
+        L.ps_status_string.argtypes = [i32]
+        L.ps_status_string.restype = C.c_char_p
+        L.ps_last_error.restype = C.c_char_p
+        L.ps_validate_sift_config.argtypes = [C.POINTER(_SiftConfigC), C.c_char_p, sz]
+        L.ps_validate_match_options.argtypes = [C.POINTER(_MatchOptionsC), C.c_char_p, sz]
+        L.ps_detect_and_compute.argtypes = [vp, sz, sz, sz, C.POINTER(_SiftConfigC), vp, vp, sz, C.POINTER(sz)]
+        L.ps_detect_and_compute_u8.argtypes = [vp, sz, sz, sz, C.POINTER(_SiftConfigC), vp, vp, sz, C.POINTER(sz)]
+        L.ps_detect_keypoints.argtypes = [vp, sz, sz, sz, C.POINTER(_SiftConfigC), vp, sz, C.POINTER(sz)]
+        L.ps_match.argtypes = [vp, vp, sz, vp, vp, sz, C.POINTER(_MatchOptionsC), vp, sz, C.POINTER(sz)]
+        L.ps_build_pyramid.argtypes = [vp, sz, sz, sz, C.POINTER(_SiftConfigC), vp, sz, C.POINTER(sz),
+                                       C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.ps_nn2.argtypes = [vp, sz, vp, sz, vp, vp, vp]
+        L.ps_engine_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.ps_engine_destroy.argtypes = [vp]
+        L.ps_engine_stream.argtypes = [vp]
+        L.ps_engine_stream.restype = vp
+        L.ps_engine_sync.argtypes = [vp]
+        L.ps_engine_check.argtypes = [vp]
+        L.ps_engine_extract_u8.argtypes = [vp, vp, sz, sz, sz, C.POINTER(_SiftConfigC), vp, vp, sz, u32p]
+        L.ps_engine_match.argtypes = [vp, vp, vp, u32p, sz, vp, vp, u32p, sz, C.POINTER(_MatchOptionsC), vp, u32p]
+        L.ps_debug_fail_next.argtypes = []
+        L.ps_engine_profile_begin.argtypes = [vp, C.c_uint32]
+        L.ps_engine_profile_end.argtypes = [vp, C.POINTER(C.c_float), u32p]
+        for name in ("ps_init", "ps_validate_sift_config", "ps_validate_match_options", "ps_detect_and_compute",
+                     "ps_detect_and_compute_u8", "ps_detect_keypoints", "ps_match", "ps_build_pyramid", "ps_nn2",
+                     "ps_engine_create", "ps_engine_destroy", "ps_engine_sync", "ps_engine_check",
+                     "ps_engine_extract_u8", "ps_engine_match", "ps_engine_profile_begin",
+                     "ps_engine_profile_end"):
+            getattr(L, name).restype = i32
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != PS_OK:
+        msg = lib().ps_last_error().decode() or lib().ps_status_string(status).decode()
+        if status == PS_E_CONFIG:
+            raise ValueError(msg)
+        raise DeviceError(status, msg)
+
+
+@dataclasses.dataclass
+class SiftConfig:
+    """pipeline.rs:22-104 (builder methods return a modified copy, like the Rust by-value builder)."""
+    octaves_: int = 3
+    scales_: int = 3
+    contrast_threshold_: float = DEFAULT_CONTRAST_THRESHOLD
+    upsample_first_octave_: bool = DEFAULT_UPSAMPLE_FIRST_OCTAVE
+    descriptor_kind_: int = DescriptorKind.Sift
+
+    def octaves(self, v: int) -> "SiftConfig":
+        return dataclasses.replace(self, octaves_=int(v))
+
+    def scales(self, v: int) -> "SiftConfig":
+        return dataclasses.replace(self, scales_=int(v))
+
+    def contrast_threshold(self, v: float) -> "SiftConfig":
+        return dataclasses.replace(self, contrast_threshold_=float(v))
+
+    def upsample_first_octave(self, v: bool) -> "SiftConfig":
+        return dataclasses.replace(self, upsample_first_octave_=bool(v))
+
+    def descriptor_kind(self, v: int) -> "SiftConfig":
+        return dataclasses.replace(self, descriptor_kind_=int(v))
+
+    def _c(self) -> _SiftConfigC:
+        if not (0 <= self.octaves_ < 2 ** 32 and 0 <= self.scales_ < 2 ** 32):
+            raise ValueError(f"Invalid `octaves`={self.octaves_}: must be between 1 and 8")
+        return _SiftConfigC(self.octaves_, self.scales_, self.contrast_threshold_,
+                            1 if self.upsample_first_octave_ else 0, self.descriptor_kind_)
+
+    def validate(self) -> None:
+        """Raises ValueError with the reference's message (pipeline.rs:80-103)."""
+        c = self._c()
+        buf = C.create_string_buffer(256)
+        st = lib().ps_validate_sift_config(C.byref(c), buf, 256)
+        if st == PS_E_CONFIG:
+            raise ValueError(buf.value.decode())
+        check(st)
+
+
+@dataclasses.dataclass
+class MatchOptions:
+    """pipeline.rs:1720-1736"""
+    ratio: float = DEFAULT_MATCH_RATIO
+    mutual: bool = False
+
+    def _c(self) -> _MatchOptionsC:
+        return _MatchOptionsC(self.ratio, 1 if self.mutual else 0)
+
+
+@dataclasses.dataclass
+class SiftFeature:
+    """pipeline.rs:1058-1072"""
+    x: float
+    y: float
+    scale: float
+    response: float
+    orientation: float
+    descriptor: np.ndarray
+
+
+@dataclasses.dataclass
+class Features:
+    """A Vec<SiftFeature> as struct-of-arrays: header records + [n, 128] descriptors."""
+    header: np.ndarray        # FEAT_DTYPE
+    descriptors: np.ndarray   # float32 [n, 128]
+
+    def __len__(self) -> int:
+        return len(self.header)
+
+    def to_list(self) -> List[SiftFeature]:
+        return [SiftFeature(float(h["x"]), float(h["y"]), float(h["scale"]), float(h["response"]),
+                            float(h["orientation"]), d.copy()) for h, d in zip(self.header, self.descriptors)]
+
+    @staticmethod
+    def from_arrays(header, descriptors) -> "Features":
+        return Features(np.ascontiguousarray(header, FEAT_DTYPE),
+                        np.ascontiguousarray(descriptors, np.float32).reshape(-1, 128))
+
+
+def _img_arg(img: np.ndarray):
+    if img.ndim != 2:
+        raise ValueError("image must be 2-D (h, w)")
+    if img.dtype == np.uint8:
+        a = img if img.strides[1] == 1 and img.strides[0] >= img.shape[1] else np.ascontiguousarray(img)
+        return a, a.strides[0], True
+    a = img.astype(np.float32, copy=False)
+    if a.strides[1] != 4 or a.strides[0] % 4:
+        a = np.ascontiguousarray(a)
+    return a, a.strides[0] // 4, False
+
+
+class Sift:
+    """pipeline.rs:106-146"""
+
+    def __init__(self, config: SiftConfig):
+        config.validate()
+        self._config = config
+        self._c = config._c()
+
+    @staticmethod
+    def new(config: SiftConfig) -> "Sift":
+        return Sift(config)
+
+    def config(self) -> SiftConfig:
+        return self._config
+
+    def detect_keypoints(self, img: np.ndarray) -> np.ndarray:
+        """Uncapped keypoints in scan order (KP_DTYPE records). img: float32 [0,1] luma."""
+        a, stride, is_u8 = _img_arg(img)
+        if is_u8:
+            a = a.astype(np.float32) * np.float32(1.0 / 255.0)
+            stride = a.strides[0] // 4
+        n = C.c_size_t(0)
+        cap = 1 << 16
+        while True:
+            out = np.zeros(cap, KP_DTYPE)
+            st = lib().ps_detect_keypoints(a.ctypes.data, a.shape[0], a.shape[1], stride, C.byref(self._c),
+                                           out.ctypes.data, cap, C.byref(n))
+            if st == PS_E_CAPACITY and n.value > cap:
+                cap = n.value
+                continue
+            check(st)
+            return out[:n.value].copy()
+
+    def detect_and_compute(self, img: np.ndarray) -> Features:
+        """Sift::detect_and_compute: <= 4096 features with 128-D descriptors."""
+        a, stride, is_u8 = _img_arg(img)
+        hdr = np.zeros(MAX_FEATURES_PER_IMAGE, FEAT_DTYPE)
+        desc = np.zeros((MAX_FEATURES_PER_IMAGE, 128), np.float32)
+        n = C.c_size_t(0)
+        fn = lib().ps_detect_and_compute_u8 if is_u8 else lib().ps_detect_and_compute
+        check(fn(a.ctypes.data, a.shape[0], a.shape[1], stride, C.byref(self._c), hdr.ctypes.data, desc.ctypes.data,
+                 MAX_FEATURES_PER_IMAGE, C.byref(n)))
+        return Features(hdr[:n.value].copy(), desc[:n.value].copy())
+
+    def build_pyramid(self, img: np.ndarray):
+        """Debug/parity: [(gauss[S+3,h,w], dog[S+2,h,w]), ...] of the device pyramid."""
+        a, stride, is_u8 = _img_arg(img)
+        if is_u8:
+            a = a.astype(np.float32) * np.float32(1.0 / 255.0)
+            stride = a.strides[0] // 4
+        n = C.c_size_t(0)
+        dims = (C.c_int32 * 16)()
+        noct = C.c_int32(0)
+        check(lib().ps_build_pyramid(a.ctypes.data, a.shape[0], a.shape[1], stride, C.byref(self._c), None, 0,
+                                     C.byref(n), dims, C.byref(noct)))
+        buf = np.zeros(max(n.value, 1), np.float32)
+        check(lib().ps_build_pyramid(a.ctypes.data, a.shape[0], a.shape[1], stride, C.byref(self._c),
+                                     buf.ctypes.data, buf.size, C.byref(n), dims, C.byref(noct)))
+        S = self._config.scales_
+        res, off = [], 0
+        for o in range(noct.value):
+            h, w = dims[2 * o], dims[2 * o + 1]
+            g = buf[off:off + (S + 3) * h * w].reshape(S + 3, h, w)
+            off += (S + 3) * h * w
+            d = buf[off:off + (S + 2) * h * w].reshape(S + 2, h, w)
+            off += (S + 2) * h * w
+            res.append((g, d))
+        return res
+
+
+class Matcher:
+    """pipeline.rs:148-170"""
+
+    def __init__(self, options: MatchOptions):
+        c = options._c()
+        buf = C.create_string_buffer(160)
+        st = lib().ps_validate_match_options(C.byref(c), buf, 160)
+        if st == PS_E_CONFIG:
+            raise ValueError(buf.value.decode())
+        check(st)
+        self._options = options
+        self._c = c
+
+    @staticmethod
+    def new(options: MatchOptions) -> "Matcher":
+        return Matcher(options)
+
+    def match_(self, query: Features, target: Features) -> np.ndarray:
+        """Matcher::match_: FeatureMatch records (MATCH_DTYPE) in query order."""
+        qh = np.ascontiguousarray(query.header, FEAT_DTYPE)
+        th = np.ascontiguousarray(target.header, FEAT_DTYPE)
+        qd = np.ascontiguousarray(query.descriptors, np.float32)
+        td = np.ascontiguousarray(target.descriptors, np.float32)
+        out = np.zeros(max(len(qh), 1), MATCH_DTYPE)
+        n = C.c_size_t(0)
+        check(lib().ps_match(qh.ctypes.data, qd.ctypes.data, len(qh), th.ctypes.data, td.ctypes.data, len(th),
+                             C.byref(self._c), out.ctypes.data, out.size, C.byref(n)))
+        return out[:n.value].copy()
+
+
+def nn2(q_desc: np.ndarray, t_desc: np.ndarray):
+    """(best_idx, best_d2, second_d2) per query, bit-exact with the reference's f32 sums."""
+    q = np.ascontiguousarray(q_desc, np.float32).reshape(-1, 128)
+    t = np.ascontiguousarray(t_desc, np.float32).reshape(-1, 128)
+    bi = np.zeros(len(q), np.uint32)
+    b = np.zeros(len(q), np.float32)
+    s = np.zeros(len(q), np.float32)
+    check(lib().ps_nn2(q.ctypes.data, len(q), t.ctypes.data, len(t), bi.ctypes.data, b.ctypes.data, s.ctypes.data))
+    return bi, b, s
+
+
+def extract_sift_features(img, octaves, scales, threshold, upsample_first_octave=True,
+                          kind=DescriptorKind.Sift) -> Features:
+    """Free-function funnel extract_sift_features_with_kind (pipeline.rs:1633-1658)."""
+    cfg = SiftConfig(octaves, scales, threshold, upsample_first_octave, kind)
+    return Sift(cfg).detect_and_compute(img)
+
+
+def match_descriptors_with_options(query: Features, target: Features, options: MatchOptions) -> np.ndarray:
+    """Free-function funnel (pipeline.rs:1744-1841)."""
+    return Matcher(options).match_(query, target)
+
+
+class Engine:
+    """Device-resident pipeline (one HIP stream + workspace); takes device pointers (ints)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().ps_engine_create(device, C.byref(h)))
+        self.h = h
+
+    def stream(self) -> int:
+        return lib().ps_engine_stream(self.h) or 0
+
+    def extract_u8(self, d_img: int, h: int, w: int, stride: int, cfg: SiftConfig, d_feat: int, d_desc: int,
+                   cap: int, d_count: int) -> None:
+        c = cfg._c()
+        check(lib().ps_engine_extract_u8(self.h, d_img, h, w, stride, C.byref(c), d_feat, d_desc, cap,
+                                         C.cast(d_count, C.POINTER(C.c_uint32))))
+
+    def match(self, d_qf, d_qd, d_nq, nq_max, d_tf, d_td, d_nt, nt_max, opt: MatchOptions, d_out, d_out_count):
+        c = opt._c()
+        u = lambda p: C.cast(p, C.POINTER(C.c_uint32))
+        check(lib().ps_engine_match(self.h, d_qf, d_qd, u(d_nq), nq_max, d_tf, d_td, u(d_nt), nt_max, C.byref(c),
+                                    d_out, u(d_out_count)))
+
+    def sync(self) -> None:
+        check(lib().ps_engine_sync(self.h))
+
+    STAGES = ("pyramid", "extrema", "order", "describe", "match")
+
+    def profile_begin(self, max_calls: int) -> None:
+        check(lib().ps_engine_profile_begin(self.h, max_calls))
+
+    def profile_end(self):
+        """({stage: total ms}, number of extract calls) since profile_begin (HIP events)."""
+        ms = (C.c_float * 5)()
+        calls = C.c_uint32(0)
+        check(lib().ps_engine_profile_end(self.h, ms, C.byref(calls)))
+        return dict(zip(self.STAGES, [float(v) for v in ms])), calls.value
+
+    def check(self) -> None:
+        check(lib().ps_engine_check(self.h))
+
+    def close(self) -> None:
+        if self.h:
+            lib().ps_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

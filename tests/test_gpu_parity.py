@@ -1,0 +1,222 @@
+"""GPU parity: the HIP pipeline (through the C ABI) against the CPU oracle on the
+same seeded inputs.
+
+Bars:
+  * Gaussian/DoG pyramid, keypoint x/y/response/octave/layer, match indices,
+    distances and ratios: bit-exact.
+  * keypoint scale: within 1 ulp (the device evaluates k^ds in f64 and rounds once;
+    glibc powf differs from that in ~0.06% of arguments), i.e. << 1e-4 octave.
+  * descriptors: L2 <= 1e-3 per feature on unit vectors, orientation within 1e-3 rad,
+    after aligning features by keypoint (orientation-count flips are counted and
+    bounded, since device expf/atan2f differ from glibc in the last ulp).
+"""
+import numpy as np
+import pytest
+
+import psr
+import synth
+
+pytestmark = pytest.mark.gpu
+
+SMALL_SHAPES = [(64, 64), (37, 53), (7, 5), (3, 3), (96, 130)]
+
+
+def _rand_img(h, w, seed):
+    rng = np.random.default_rng(seed)
+    img = rng.random((h, w)).astype(np.float32)
+    # add smooth structure so the pyramid has extrema
+    yy, xx = np.mgrid[0:h, 0:w]
+    img += np.sin(xx / 3.0).astype(np.float32) * np.cos(yy / 5.0).astype(np.float32)
+    return np.clip(img * 0.5, 0, 1).astype(np.float32)
+
+
+def _blob_f32(w, h, n, seed, band=False):
+    a = synth.blob_image(w, h, n, seed, band)
+    return a, psr.luma_to_f32(a)
+
+
+@pytest.mark.parametrize("shape", SMALL_SHAPES)
+@pytest.mark.parametrize("upsample", [True, False])
+def test_pyramid_bit_exact_small(gpu, shape, upsample):
+    img = _rand_img(*shape, seed=shape[0] * 7 + shape[1])
+    cfg = gpu.SiftConfig().octaves(4).scales(3).upsample_first_octave(upsample)
+    dev = gpu.Sift(cfg).build_pyramid(img)
+    ref = psr.build_pyramid(img, 4, 3, upsample)
+    assert len(dev) == len(ref)
+    for (g, d), r in zip(dev, ref):
+        assert np.array_equal(g, r["gauss"]), "Gaussian planes differ"
+        assert np.array_equal(d, r["dog"]), "DoG planes differ"
+
+
+@pytest.mark.parametrize("scales", [3, 4, 6, 10])
+def test_pyramid_bit_exact_scales(gpu, scales):
+    u8, img = _blob_f32(160, 120, 30, seed=scales)
+    cfg = gpu.SiftConfig().octaves(3).scales(scales)
+    dev = gpu.Sift(cfg).build_pyramid(img)
+    ref = psr.build_pyramid(img, 3, scales, True)
+    assert len(dev) == len(ref)
+    for (g, d), r in zip(dev, ref):
+        assert np.array_equal(g, r["gauss"])
+        assert np.array_equal(d, r["dog"])
+
+
+def _cmp_keypoints(dev, ref):
+    assert len(dev) == len(ref), (len(dev), len(ref))
+    for f in ("x", "y", "response", "octave", "layer"):
+        assert np.array_equal(dev[f], ref[f]), f
+    ulp = np.abs(dev["scale"].view(np.int32).astype(np.int64) - ref["scale"].view(np.int32).astype(np.int64))
+    assert ulp.max(initial=0) <= 1
+    return int((ulp > 0).sum())
+
+
+@pytest.mark.parametrize("upsample", [True, False])
+def test_keypoints_exact_640x480(gpu, upsample):
+    u8, img = _blob_f32(640, 480, 200, seed=0)
+    cfg = gpu.SiftConfig().octaves(4).scales(3).upsample_first_octave(upsample)
+    dev = gpu.Sift(cfg).detect_keypoints(img)
+    ref = psr.detect_keypoints(img, 4, 3, 0.015, upsample)
+    assert len(ref) > 50
+    _cmp_keypoints(dev, ref)
+
+
+def _align_features(dh, dd, rh, rd):
+    """Pairs device and oracle features by (x, y, response) in order; returns index pairs
+    and the number of keypoints whose orientation count differs."""
+    from collections import defaultdict
+    rk = defaultdict(list)
+    for i, h in enumerate(rh):
+        rk[(h["x"].tobytes(), h["y"].tobytes(), h["response"].tobytes())].append(i)
+    dk = defaultdict(list)
+    for i, h in enumerate(dh):
+        dk[(h["x"].tobytes(), h["y"].tobytes(), h["response"].tobytes())].append(i)
+    pairs, flips = [], 0
+    for k, ri in rk.items():
+        di = dk.get(k, [])
+        if len(di) != len(ri):
+            flips += 1
+            continue
+        pairs.extend(zip(di, ri))
+    return pairs, flips
+
+
+def _check_features(dev, rh, rd, max_flip_frac=0.01):
+    dh, dd = dev.header, dev.descriptors
+    pairs, flips = _align_features(dh, dd, rh, rd)
+    nkp = len({(h["x"].tobytes(), h["y"].tobytes()) for h in rh})
+    assert flips <= max(1, max_flip_frac * nkp), f"{flips} orientation-count flips of {nkp}"
+    assert len(pairs) >= 0.98 * len(rh)
+    di = np.array([p[0] for p in pairs])
+    ri = np.array([p[1] for p in pairs])
+    dori = np.abs(dh["orientation"][di] - rh["orientation"][ri])
+    dori = np.minimum(dori, 2 * np.pi - dori)
+    ok_ori = dori <= 1e-3
+    assert ok_ori.mean() >= 0.99, f"orientation agreement {ok_ori.mean()}"
+    l2 = np.linalg.norm(dd[di] - rd[ri], axis=1)
+    assert np.percentile(l2[ok_ori], 99) <= 1e-3 and (l2[ok_ori] <= 1e-3).mean() >= 0.99, l2.max()
+    norms = np.linalg.norm(dd, axis=1)
+    assert np.all(np.abs(norms - 1) < 1e-4)
+    return flips, float(l2[ok_ori].max())
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_detect_and_compute_640x480(gpu, kind):
+    u8, img = _blob_f32(640, 480, 200, seed=0)
+    cfg = gpu.SiftConfig().octaves(4).scales(3).descriptor_kind(kind)
+    dev = gpu.Sift(cfg).detect_and_compute(u8)
+    rh, rd = psr.extract_features(img, 4, 3, 0.015, True, kind)
+    _check_features(dev, rh, rd)
+    dev_f = gpu.Sift(cfg).detect_and_compute(img)
+    assert np.array_equal(dev_f.header, dev.header) and np.array_equal(dev_f.descriptors, dev.descriptors)
+
+
+def test_detect_and_compute_capped_1080p(gpu):
+    """1080p band-noise image: > 4096 keypoints, so the stable |response| cap is exercised."""
+    u8, _, _ = synth.blob_pair(1920, 1080, 2000, seed=1, band_noise=True)
+    img = psr.luma_to_f32(u8)
+    cfg = gpu.SiftConfig().octaves(4).scales(3)
+    kps = gpu.Sift(cfg).detect_keypoints(img)
+    assert len(kps) > 4096
+    dev = gpu.Sift(cfg).detect_and_compute(u8)
+    rh, rd = psr.extract_features(img, 4, 3, 0.015, True)
+    assert len(dev) == 4096 == len(rh)
+    # the keypoint order after the cap is exact, so headers agree except for orientation flips
+    flips, _ = _check_features(dev, rh, rd)
+    assert flips <= 8
+
+
+def test_determinism(gpu):
+    u8, img = _blob_f32(640, 480, 200, seed=3)
+    cfg = gpu.SiftConfig().octaves(4)
+    a = gpu.Sift(cfg).detect_and_compute(u8)
+    b = gpu.Sift(cfg).detect_and_compute(u8)
+    assert np.array_equal(a.header, b.header)
+    assert np.array_equal(a.descriptors, b.descriptors)
+
+
+@pytest.mark.parametrize("nq,nt", [(1, 2), (5, 3), (300, 257), (1000, 1500), (64, 1)])
+def test_nn2_exact(gpu, nq, nt):
+    rng = np.random.default_rng(nq * 31 + nt)
+    q = np.maximum(rng.standard_normal((nq, 128)), 0).astype(np.float32)
+    t = np.maximum(rng.standard_normal((nt, 128)), 0).astype(np.float32)
+    q /= np.maximum(np.linalg.norm(q, axis=1, keepdims=True), 1e-12)
+    t /= np.maximum(np.linalg.norm(t, axis=1, keepdims=True), 1e-12)
+    if nt >= 3:
+        t[nt // 2] = t[0]          # exact duplicate target -> tie on best
+        q[0] = t[0]                # zero distance
+    bi, b, s = gpu.nn2(q, t)
+    rbi, rb, rs = psr.nn2(q, t)
+    assert np.array_equal(bi, rbi)
+    assert np.array_equal(b, rb)
+    assert np.array_equal(s, rs)
+
+
+@pytest.mark.parametrize("mutual", [False, True])
+def test_match_exact_on_oracle_features(gpu, mutual):
+    a, b, _ = synth.blob_pair(640, 480, 200, seed=5)
+    qh, qd = psr.extract_features(psr.luma_to_f32(a), 4, 3, 0.015, True)
+    th, td = psr.extract_features(psr.luma_to_f32(b), 4, 3, 0.015, True)
+    ref = psr.match(qh, qd, th, td, 0.8, mutual)
+    m = gpu.Matcher(gpu.MatchOptions(0.8, mutual)).match_(gpu.Features(qh, qd), gpu.Features(th, td))
+    assert len(ref) > 20
+    assert np.array_equal(m, ref)
+
+
+def test_mutual_known_answer(gpu):
+    """pipeline_tests.rs:1221-1280"""
+    def desc(v0, v1):
+        d = np.zeros(128, np.float32)
+        n = np.float32(np.sqrt(np.float32(v0 * v0 + v1 * v1)))
+        d[0], d[1] = np.float32(v0) / n, np.float32(v1) / n
+        return d
+    def feats(xs, ds):
+        h = np.zeros(len(xs), gpu.FEAT_DTYPE)
+        h["x"] = xs
+        h["y"] = 10.0
+        h["scale"] = 1.6
+        h["response"] = 0.05
+        return gpu.Features(h, np.stack(ds))
+    q = feats([10.0, 20.0], [desc(1.0, 0.0), desc(0.95, 0.31)])
+    t = feats([12.0, 80.0], [desc(1.0, 0.0), desc(0.0, 1.0)])
+    one = gpu.Matcher(gpu.MatchOptions(0.75, False)).match_(q, t)
+    mut = gpu.Matcher(gpu.MatchOptions(0.75, True)).match_(q, t)
+    assert len(one) == 2
+    assert len(mut) == 1 and mut[0]["query_index"] == 0 and mut[0]["target_index"] == 0
+
+
+def test_end_to_end_match_indices(gpu):
+    """GPU extract + GPU match vs oracle extract + oracle match on the 640x480 pair."""
+    a, b, _ = synth.blob_pair(640, 480, 200, seed=0)
+    cfg = gpu.SiftConfig().octaves(4).scales(3)
+    s = gpu.Sift(cfg)
+    fq, ft = s.detect_and_compute(a), s.detect_and_compute(b)
+    m = gpu.Matcher(gpu.MatchOptions(0.8)).match_(fq, ft)
+    qh, qd = psr.extract_features(psr.luma_to_f32(a), 4, 3, 0.015, True)
+    th, td = psr.extract_features(psr.luma_to_f32(b), 4, 3, 0.015, True)
+    ref = psr.match(qh, qd, th, td, 0.8)
+    # exact agreement whenever the feature lists agree element-for-element
+    if len(fq) == len(qh) and len(ft) == len(th):
+        same = (m["query_index"].size == ref["query_index"].size)
+        agree = np.mean([x in set(zip(ref["query_index"], ref["target_index"]))
+                         for x in zip(m["query_index"], m["target_index"])]) if len(m) else 1.0
+        assert same and agree >= 0.995, agree
+    assert len(m) > 50
