@@ -1,0 +1,205 @@
+// pure_sift.hpp -- header-only C++ host mirror of pure-sift 0.2.0's hot-path API
+// (src/pipeline.rs:22-170) over the C ABI in include/ps_sift.h.
+//
+//   auto sift = pure_sift::Sift::create(pure_sift::SiftConfig().octaves(4));   // throws ConfigError
+//   std::vector<pure_sift::SiftFeature> f = sift.detect_and_compute(img, h, w, w);
+//   auto m = pure_sift::Matcher::create({0.8f, false}).match_(fq, ft);
+//
+// Validation happens at construction (Result<_, String> in the reference -> ConfigError);
+// compute calls never fall back to the CPU: a device failure throws DeviceError (the
+// reference's infallible signatures become a panic in the Rust wrapper).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../include/ps_sift.h"
+
+namespace pure_sift {
+
+constexpr float BASE_SIGMA = 1.6f;                 // pipeline.rs:10
+constexpr float ASSUMED_INPUT_SIGMA = 0.5f;        // pipeline.rs:13
+constexpr bool DEFAULT_UPSAMPLE_FIRST_OCTAVE = true;
+constexpr float DEFAULT_CONTRAST_THRESHOLD = 0.015f;
+constexpr std::size_t SIFT_DESCRIPTOR_LEN = 128;   // pipeline.rs:998
+constexpr float DEFAULT_MATCH_RATIO = 0.75f;       // pipeline.rs:1047
+constexpr std::size_t MAX_FEATURES_PER_IMAGE = 4096;
+
+struct ConfigError : std::invalid_argument {
+    using std::invalid_argument::invalid_argument;
+};
+struct DeviceError : std::runtime_error {
+    ps_status status;
+    DeviceError(ps_status s, const std::string &m) : std::runtime_error(m), status(s) {}
+};
+
+inline void check(ps_status s) {
+    if (s == PS_OK) return;
+    std::string msg = ps_last_error();
+    if (msg.empty()) msg = ps_status_string(s);
+    if (s == PS_E_CONFIG) throw ConfigError(msg);
+    throw DeviceError(s, msg);
+}
+
+enum class DescriptorKind : uint8_t { Sift = PS_DESCRIPTOR_SIFT, RootSift = PS_DESCRIPTOR_ROOTSIFT };
+
+// pipeline.rs:22-104
+struct SiftConfig {
+    std::size_t octaves_ = 3, scales_ = 3;
+    float contrast_threshold_ = DEFAULT_CONTRAST_THRESHOLD;
+    bool upsample_first_octave_ = DEFAULT_UPSAMPLE_FIRST_OCTAVE;
+    DescriptorKind descriptor_kind_ = DescriptorKind::Sift;
+
+    SiftConfig octaves(std::size_t v) const { auto c = *this; c.octaves_ = v; return c; }
+    SiftConfig scales(std::size_t v) const { auto c = *this; c.scales_ = v; return c; }
+    SiftConfig contrast_threshold(float v) const { auto c = *this; c.contrast_threshold_ = v; return c; }
+    SiftConfig upsample_first_octave(bool v) const { auto c = *this; c.upsample_first_octave_ = v; return c; }
+    SiftConfig descriptor_kind(DescriptorKind v) const { auto c = *this; c.descriptor_kind_ = v; return c; }
+
+    ps_sift_config c_config() const {
+        ps_sift_config c{};
+        c.octaves = octaves_ > 0xffffffffu ? 0xffffffffu : (uint32_t)octaves_;
+        c.scales = scales_ > 0xffffffffu ? 0xffffffffu : (uint32_t)scales_;
+        c.contrast_threshold = contrast_threshold_;
+        c.upsample_first_octave = upsample_first_octave_ ? 1 : 0;
+        c.descriptor_kind = (uint8_t)descriptor_kind_;
+        return c;
+    }
+    // Returns the reference's error message, or "" when valid.
+    std::string validate() const {
+        ps_sift_config c = c_config();
+        char msg[256];
+        return ps_validate_sift_config(&c, msg, sizeof msg) == PS_OK ? std::string() : std::string(msg);
+    }
+};
+
+struct Keypoint {            // pipeline.rs:251-267
+    float x, y, scale, response;
+    std::size_t octave, layer;
+};
+
+struct SiftFeature {         // pipeline.rs:1058-1072
+    float x, y, scale, response, orientation;
+    std::vector<float> descriptor;
+};
+
+struct FeatureMatch {        // pipeline.rs:1075-1091
+    std::size_t query_index, target_index;
+    std::pair<float, float> query_point, target_point;
+    float distance, ratio;
+    std::pair<float, float> displacement;
+};
+
+struct MatchOptions {        // pipeline.rs:1720-1736
+    float ratio = DEFAULT_MATCH_RATIO;
+    bool mutual = false;
+};
+
+// pipeline.rs:106-146
+class Sift {
+public:
+    static Sift create(const SiftConfig &config) {
+        std::string err = config.validate();
+        if (!err.empty()) throw ConfigError(err);
+        return Sift(config);
+    }
+    SiftConfig config() const { return config_; }
+
+    // img: h x w f32 luma in [0, 1], row stride in elements
+    std::vector<Keypoint> detect_keypoints(const float *img, std::size_t h, std::size_t w, std::size_t stride) const {
+        ps_sift_config c = config_.c_config();
+        std::size_t n = 0;
+        std::vector<ps_keypoint> buf(1 << 14);
+        ps_status s = ps_detect_keypoints(img, h, w, stride, &c, buf.data(), buf.size(), &n);
+        if (s == PS_E_CAPACITY && n > buf.size()) {
+            buf.resize(n);
+            s = ps_detect_keypoints(img, h, w, stride, &c, buf.data(), buf.size(), &n);
+        }
+        check(s);
+        std::vector<Keypoint> out(n);
+        for (std::size_t i = 0; i < n; i++)
+            out[i] = {buf[i].x, buf[i].y, buf[i].scale, buf[i].response, buf[i].octave, buf[i].layer};
+        return out;
+    }
+
+    std::vector<SiftFeature> detect_and_compute(const float *img, std::size_t h, std::size_t w,
+                                                std::size_t stride) const {
+        return run(img, h, w, stride, false);
+    }
+    std::vector<SiftFeature> detect_and_compute_u8(const uint8_t *img, std::size_t h, std::size_t w,
+                                                   std::size_t stride) const {
+        return run(img, h, w, stride, true);
+    }
+
+private:
+    explicit Sift(const SiftConfig &c) : config_(c) {}
+    std::vector<SiftFeature> run(const void *img, std::size_t h, std::size_t w, std::size_t stride, bool u8) const {
+        ps_sift_config c = config_.c_config();
+        std::vector<ps_feature> hdr(MAX_FEATURES_PER_IMAGE);
+        std::vector<float> desc(MAX_FEATURES_PER_IMAGE * SIFT_DESCRIPTOR_LEN);
+        std::size_t n = 0;
+        check(u8 ? ps_detect_and_compute_u8(static_cast<const uint8_t *>(img), h, w, stride, &c, hdr.data(),
+                                            desc.data(), hdr.size(), &n)
+                 : ps_detect_and_compute(static_cast<const float *>(img), h, w, stride, &c, hdr.data(), desc.data(),
+                                         hdr.size(), &n));
+        std::vector<SiftFeature> out(n);
+        for (std::size_t i = 0; i < n; i++) {
+            out[i] = {hdr[i].x, hdr[i].y, hdr[i].scale, hdr[i].response, hdr[i].orientation,
+                      std::vector<float>(desc.begin() + i * SIFT_DESCRIPTOR_LEN,
+                                         desc.begin() + (i + 1) * SIFT_DESCRIPTOR_LEN)};
+        }
+        return out;
+    }
+    SiftConfig config_;
+};
+
+// pipeline.rs:148-170
+class Matcher {
+public:
+    static Matcher create(const MatchOptions &o) {
+        ps_match_options c{o.ratio, (uint8_t)(o.mutual ? 1 : 0), {0, 0, 0}};
+        char msg[160];
+        if (ps_validate_match_options(&c, msg, sizeof msg) != PS_OK) throw ConfigError(msg);
+        return Matcher(o);
+    }
+
+    std::vector<FeatureMatch> match_(const std::vector<SiftFeature> &query,
+                                     const std::vector<SiftFeature> &target) const {
+        std::vector<ps_feature> qh, th;
+        std::vector<float> qd, td;
+        pack(query, qh, qd);
+        pack(target, th, td);
+        ps_match_options c{opt_.ratio, (uint8_t)(opt_.mutual ? 1 : 0), {0, 0, 0}};
+        std::vector<ps_feature_match> buf(query.size() ? query.size() : 1);
+        std::size_t n = 0;
+        check(ps_match(qh.data(), qd.data(), qh.size(), th.data(), td.data(), th.size(), &c, buf.data(), buf.size(),
+                       &n));
+        std::vector<FeatureMatch> out(n);
+        for (std::size_t i = 0; i < n; i++) {
+            const ps_feature_match &m = buf[i];
+            out[i] = {(std::size_t)m.query_index, (std::size_t)m.target_index, {m.qx, m.qy}, {m.tx, m.ty},
+                      m.distance, m.ratio, {m.dx, m.dy}};
+        }
+        return out;
+    }
+
+private:
+    explicit Matcher(const MatchOptions &o) : opt_(o) {}
+    static void pack(const std::vector<SiftFeature> &f, std::vector<ps_feature> &h, std::vector<float> &d) {
+        h.resize(f.size());
+        d.resize(f.size() * SIFT_DESCRIPTOR_LEN);
+        for (std::size_t i = 0; i < f.size(); i++) {
+            if (f[i].descriptor.size() != SIFT_DESCRIPTOR_LEN)
+                throw ConfigError("descriptor length must be 128");
+            h[i] = {f[i].x, f[i].y, f[i].scale, f[i].response, f[i].orientation};
+            std::copy(f[i].descriptor.begin(), f[i].descriptor.end(), d.begin() + i * SIFT_DESCRIPTOR_LEN);
+        }
+    }
+    MatchOptions opt_;
+};
+
+}  // namespace pure_sift

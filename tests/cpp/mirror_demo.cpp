@@ -1,0 +1,50 @@
+// Exercises the C++ host mirror (pure-sift_amd/pure_sift.hpp) end to end: validation
+// errors, detect_keypoints, detect_and_compute (f32 and u8), match_ (plain and mutual).
+// Reads a raw u8 image (h, w from argv) and prints a one-line summary the GPU test checks.
+#include <cstdio>
+#include <fstream>
+#include <vector>
+
+#include "pure_sift.hpp"
+
+int main(int argc, char **argv) {
+    if (argc < 5) {
+        std::fprintf(stderr, "usage: %s query.u8 target.u8 h w\n", argv[0]);
+        return 2;
+    }
+    std::size_t h = std::stoul(argv[3]), w = std::stoul(argv[4]);
+    auto load = [&](const char *p) {
+        std::vector<uint8_t> v(h * w);
+        std::ifstream f(p, std::ios::binary);
+        f.read(reinterpret_cast<char *>(v.data()), (std::streamsize)v.size());
+        return v;
+    };
+    std::vector<uint8_t> q8 = load(argv[1]), t8 = load(argv[2]);
+    int bad = 0;
+    try {
+        pure_sift::Sift::create(pure_sift::SiftConfig().scales(2));
+        bad++;
+    } catch (const pure_sift::ConfigError &e) {
+        if (std::string(e.what()).find("Invalid `scales`=2") == std::string::npos) bad++;
+    }
+    try {
+        pure_sift::Matcher::create({1.5f, false});
+        bad++;
+    } catch (const pure_sift::ConfigError &) {
+    }
+    auto sift = pure_sift::Sift::create(pure_sift::SiftConfig().octaves(4).scales(3));
+    std::vector<float> qf(h * w);
+    for (std::size_t i = 0; i < h * w; i++) qf[i] = (float)q8[i] * (1.0f / 255.0f);
+    auto kps = sift.detect_keypoints(qf.data(), h, w, w);
+    auto fq = sift.detect_and_compute_u8(q8.data(), h, w, w);
+    auto fq32 = sift.detect_and_compute(qf.data(), h, w, w);
+    auto ft = sift.detect_and_compute_u8(t8.data(), h, w, w);
+    if (fq.size() != fq32.size()) bad++;
+    for (std::size_t i = 0; i < fq.size() && i < fq32.size(); i++)
+        if (fq[i].descriptor != fq32[i].descriptor || fq[i].orientation != fq32[i].orientation) bad++;
+    auto m = pure_sift::Matcher::create({0.8f, false}).match_(fq, ft);
+    auto mm = pure_sift::Matcher::create({0.8f, true}).match_(fq, ft);
+    std::printf("keypoints=%zu features=%zu,%zu matches=%zu mutual=%zu bad=%d\n", kps.size(), fq.size(), ft.size(),
+                m.size(), mm.size(), bad);
+    return bad ? 1 : 0;
+}

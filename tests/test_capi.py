@@ -1,0 +1,97 @@
+"""C ABI checks that need no GPU: the library loads, exports every entry point the
+header declares, validates configs with the reference's messages, and fails loudly
+(no CPU fallback) when no device is usable."""
+import ctypes as C
+import re
+
+import numpy as np
+import pytest
+
+import pure_sift as ps
+from conftest import has_gpu
+
+
+def declared_symbols():
+    txt = open(ps.HEADER_PATH).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(ps_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_symbols_exported():
+    names = declared_symbols()
+    assert len(names) >= 20
+    L = ps.lib()
+    for n in names:
+        assert hasattr(L, n), f"{n} declared in include/ps_sift.h but not exported"
+
+
+def test_struct_layouts_match_header():
+    assert C.sizeof(ps._SiftConfigC) == 16
+    assert C.sizeof(ps._MatchOptionsC) == 8
+    assert ps.FEAT_DTYPE.itemsize == 20
+    assert ps.KP_DTYPE.itemsize == 24
+    assert ps.MATCH_DTYPE.itemsize == 48
+
+
+@pytest.mark.parametrize("cfg,msg", [
+    (ps.SiftConfig().octaves(0), "Invalid `octaves`=0: must be between 1 and 8"),
+    (ps.SiftConfig().octaves(9), "Invalid `octaves`=9: must be between 1 and 8"),
+    (ps.SiftConfig().scales(2), "Invalid `scales`=2: must be between 3 and 10 for 3D extrema extraction"),
+    (ps.SiftConfig().scales(11), "Invalid `scales`=11: must be between 3 and 10 for 3D extrema extraction"),
+    (ps.SiftConfig().contrast_threshold(0.0), "Invalid `contrast_threshold`=0: must be finite and in (0.0, 1.0]"),
+    (ps.SiftConfig().contrast_threshold(float("nan")), "must be finite and in (0.0, 1.0]"),
+    (ps.SiftConfig().contrast_threshold(1.5), "must be finite and in (0.0, 1.0]"),
+])
+def test_sift_config_validation(cfg, msg):
+    """pipeline.rs:80-103, pipeline_tests.rs:1334-1338"""
+    with pytest.raises(ValueError) as ei:
+        ps.Sift.new(cfg)
+    assert msg in str(ei.value)
+
+
+def test_valid_configs_accepted():
+    for o in (1, 8):
+        for s in (3, 10):
+            ps.SiftConfig().octaves(o).scales(s).validate()
+    ps.SiftConfig().contrast_threshold(1.0).validate()
+
+
+@pytest.mark.parametrize("ratio", [0.0, -0.1, 1.5, float("inf"), float("nan")])
+def test_matcher_validation(ratio):
+    """pipeline.rs:156-164"""
+    with pytest.raises(ValueError, match="Invalid `ratio`"):
+        ps.Matcher.new(ps.MatchOptions(ratio=ratio))
+    ps.Matcher.new(ps.MatchOptions(ratio=1.0))
+
+
+def test_injected_failure_is_reported():
+    """The host-settable fault switch forces the error path without touching a device."""
+    ps.lib().ps_debug_fail_next()
+    with pytest.raises(ps.DeviceError) as ei:
+        ps.Sift(ps.SiftConfig()).detect_and_compute(np.zeros((16, 16), np.float32))
+    assert ei.value.status == ps.PS_E_INJECTED
+
+
+@pytest.mark.skipif(has_gpu(), reason="checks the no-device path")
+def test_no_device_fails_loudly():
+    with pytest.raises(ps.DeviceError) as ei:
+        ps.Sift(ps.SiftConfig()).detect_and_compute(np.zeros((16, 16), np.float32))
+    assert ei.value.status == ps.PS_E_NO_DEVICE
+    with pytest.raises(ps.DeviceError):
+        ps.nn2(np.zeros((2, 128), np.float32), np.zeros((3, 128), np.float32))
+
+
+def test_empty_match_needs_no_device():
+    """pipeline.rs:1749-1751: empty query or < 2 targets -> empty result."""
+    f = ps.Features(np.zeros(0, ps.FEAT_DTYPE), np.zeros((0, 128), np.float32))
+    g = ps.Features(np.zeros(1, ps.FEAT_DTYPE), np.zeros((1, 128), np.float32))
+    m = ps.Matcher(ps.MatchOptions(0.8))
+    assert len(m.match_(f, g)) == 0
+    assert len(m.match_(g, g)) == 0
+
+
+def test_status_strings():
+    L = ps.lib()
+    for s in (ps.PS_OK, ps.PS_E_INVALID_ARG, ps.PS_E_CONFIG, ps.PS_E_DEVICE, ps.PS_E_CAPACITY, ps.PS_E_NO_DEVICE,
+              ps.PS_E_OUT_OF_MEMORY, ps.PS_E_INJECTED):
+        assert L.ps_status_string(s).decode() not in ("", "unknown status")
