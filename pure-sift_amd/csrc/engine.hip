@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -138,7 +139,8 @@ struct ps_engine {
     DevBuf pyr, cand, sorted, ori, nori, feat_kp, feat_j, cnt;
     DevBuf img, feat, desc, kpout;                 // host-API staging
     DevBuf qf, qd, tf, td, mcnt, mout;             // host-API match staging
-    DevBuf part, part2, bidx, bdist, sdist, ridx;  // matcher scratch
+    DevBuf part, part2, bidx, bdist, sdist, ridx, ridx_d;  // matcher scratch
+    bool match_valu = false;                       // PS_MATCH_KERNEL=valu: direct f32 kernel
     ps::PyrPlan plan;
     uint32_t host_counts[4] = {};
     // stage profiling: per recorded call, events [extract: 0..4][match: 5..6]
@@ -199,6 +201,8 @@ ps_status engine_new(int device, ps_engine **out) {
     e->device = device;
     hipError_t err = hipSetDevice(device);
     if (err == hipSuccess) err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    const char *mk = std::getenv("PS_MATCH_KERNEL");
+    e->match_valu = mk && std::strcmp(mk, "valu") == 0;
     if (err != hipSuccess) {
         delete e;
         return set_err(PS_E_DEVICE, std::string("engine stream: ") + hipGetErrorString(err));
@@ -230,7 +234,7 @@ void engine_free(ps_engine *e) {
     (void)hipSetDevice(e->device);
     DevBuf *bufs[] = {&e->pyr, &e->cand, &e->sorted, &e->ori, &e->nori, &e->feat_kp, &e->feat_j, &e->cnt,
                       &e->img, &e->feat, &e->desc, &e->kpout, &e->qf, &e->qd, &e->tf, &e->td,
-                      &e->mcnt, &e->mout, &e->part, &e->part2, &e->bidx, &e->bdist, &e->sdist, &e->ridx};
+                      &e->mcnt, &e->mout, &e->part, &e->part2, &e->bidx, &e->bdist, &e->sdist, &e->ridx, &e->ridx_d};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -421,27 +425,38 @@ ps_status enqueue_extract(ps_engine *e, const void *src, bool src_u8, size_t h, 
     return PS_OK;
 }
 
+// best/second/index per query into e->bidx / bdist / sdist (or the given outputs)
+ps_status enqueue_nn(ps_engine *e, const float *qd, const uint32_t *nq, size_t nqm, const float *td,
+                     const uint32_t *nt, size_t ntm, DevBuf &scratch, uint32_t *bidx, float *bd, float *sd) {
+    if (e->match_valu) {
+        int ns = ps::nn_splits(nqm, ntm);
+        PS_TRY(ensure(scratch, (size_t)ns * nqm * sizeof(ps::NNPart)));
+        PS_HIP(ps::launch_nn_partial(qd, nq, nqm, td, nt, ntm, ns, as<ps::NNPart>(scratch), e->stream));
+        PS_HIP(ps::launch_nn_merge(as<ps::NNPart>(scratch), ns, nq, nqm, bidx, bd, sd, e->stream));
+        return PS_OK;
+    }
+    PS_TRY(ensure(scratch, ps::nn_mfma_scratch_bytes(nqm, ntm)));
+    PS_HIP(ps::launch_nn_mfma(qd, nq, nqm, td, nt, ntm, scratch.p, bidx, bd, sd, e->stream));
+    return PS_OK;
+}
+
 ps_status enqueue_match(ps_engine *e, const ps_feature *qf, const float *qd, const uint32_t *nq, size_t nq_max,
                         const ps_feature *tf, const float *td, const uint32_t *nt, size_t nt_max,
                         const ps_match_options &opt, ps_feature_match *out, uint32_t *out_count) {
     size_t nqm = std::max<size_t>(nq_max, 1), ntm = std::max<size_t>(nt_max, 1);
-    int ns = ps::nn_splits(nqm, ntm);
     prof_mark(e, 5);
-    PS_TRY(ensure(e->part, (size_t)ns * nqm * sizeof(ps::NNPart)));
     PS_TRY(ensure(e->bidx, nqm * sizeof(uint32_t)));
     PS_TRY(ensure(e->bdist, nqm * sizeof(float)));
     PS_TRY(ensure(e->sdist, nqm * sizeof(float)));
-    PS_HIP(ps::launch_nn_partial(qd, nq, nqm, td, nt, ntm, ns, as<ps::NNPart>(e->part), e->stream));
-    PS_HIP(ps::launch_nn_merge(as<ps::NNPart>(e->part), ns, nq, nqm, as<uint32_t>(e->bidx), as<float>(e->bdist),
-                               as<float>(e->sdist), e->stream));
+    PS_TRY(enqueue_nn(e, qd, nq, nqm, td, nt, ntm, e->part, as<uint32_t>(e->bidx), as<float>(e->bdist),
+                      as<float>(e->sdist)));
     uint32_t *rev = nullptr;
     if (opt.mutual) {
-        int ns2 = ps::nn_splits(ntm, nqm);
-        PS_TRY(ensure(e->part2, (size_t)ns2 * ntm * sizeof(ps::NNPart)));
+        // reverse pass (pipeline.rs:1815-1840): nearest query of every target, first index on ties
         PS_TRY(ensure(e->ridx, ntm * sizeof(uint32_t)));
-        PS_HIP(ps::launch_nn_partial(td, nt, ntm, qd, nq, nqm, ns2, as<ps::NNPart>(e->part2), e->stream));
-        PS_HIP(ps::launch_nn_merge(as<ps::NNPart>(e->part2), ns2, nt, ntm, as<uint32_t>(e->ridx), nullptr, nullptr,
-                                   e->stream));
+        PS_TRY(ensure(e->ridx_d, 2 * ntm * sizeof(float)));
+        PS_TRY(enqueue_nn(e, td, nt, ntm, qd, nq, nqm, e->part2, as<uint32_t>(e->ridx), as<float>(e->ridx_d),
+                          as<float>(e->ridx_d) + ntm));
         rev = as<uint32_t>(e->ridx);
     }
     PS_HIP(ps::launch_match_finish(qf, tf, nq, nqm, nt, as<uint32_t>(e->bidx), as<float>(e->bdist),
@@ -626,15 +641,11 @@ ps_status ps_nn2(const float *q_desc, size_t nq, const float *t_desc, size_t nt,
     PS_HIP(hipMemcpyAsync(dc, e->host_counts, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, e->stream));
     PS_HIP(hipMemcpyAsync(e->qd.p, q_desc, nq * 128 * sizeof(float), hipMemcpyHostToDevice, e->stream));
     if (nt) PS_HIP(hipMemcpyAsync(e->td.p, t_desc, nt * 128 * sizeof(float), hipMemcpyHostToDevice, e->stream));
-    int ns = ps::nn_splits(nq, std::max<size_t>(nt, 1));
-    PS_TRY(ensure(e->part, (size_t)ns * nq * sizeof(ps::NNPart)));
     PS_TRY(ensure(e->bidx, nq * sizeof(uint32_t)));
     PS_TRY(ensure(e->bdist, nq * sizeof(float)));
     PS_TRY(ensure(e->sdist, nq * sizeof(float)));
-    PS_HIP(ps::launch_nn_partial(as<float>(e->qd), dc, nq, as<float>(e->td), dc + 1, std::max<size_t>(nt, 1), ns,
-                                 as<ps::NNPart>(e->part), e->stream));
-    PS_HIP(ps::launch_nn_merge(as<ps::NNPart>(e->part), ns, dc, nq, as<uint32_t>(e->bidx), as<float>(e->bdist),
-                               as<float>(e->sdist), e->stream));
+    PS_TRY(enqueue_nn(e, as<float>(e->qd), dc, nq, as<float>(e->td), dc + 1, std::max<size_t>(nt, 1), e->part,
+                      as<uint32_t>(e->bidx), as<float>(e->bdist), as<float>(e->sdist)));
     PS_HIP(hipMemcpyAsync(best_idx, e->bidx.p, nq * sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream));
     PS_HIP(hipMemcpyAsync(best_d2, e->bdist.p, nq * sizeof(float), hipMemcpyDeviceToHost, e->stream));
     PS_HIP(hipMemcpyAsync(second_d2, e->sdist.p, nq * sizeof(float), hipMemcpyDeviceToHost, e->stream));

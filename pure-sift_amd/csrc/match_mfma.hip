@@ -1,0 +1,297 @@
+// Exact Lowe-ratio matching through an FP32 MFMA distance GEMM.
+//
+// Pass 1 (nn_mfma_kernel): d'(q,t) = |t|^2 - 2 q.t on v_mfma_f32_32x32x2f32 (the per-query
+// constant |q|^2 does not change the ranking); every lane keeps the 4 smallest (d', index)
+// of one query over its targets in registers.
+// Pass 2 (nn_rescore_kernel): the per-split lists are merged into the global top-4, each
+// candidate is rescored with the reference's sequential f32 sum (pipeline.rs:1700-1717),
+// and best/second/index follow the reference's rule. Exactness guard: with E a bound on
+// |d'+|q|^2 - exact| for any pair, every non-candidate t has exact(t) >= d'_4 + |q|^2 - E;
+// if that exceeds the exact second-best the result equals the full scan's. Otherwise the
+// query is flagged and pass 3 (nn_exact_flagged_kernel) scans all targets exactly.
+#include "ps_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace ps {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+constexpr int kMQ = 128;   // queries per workgroup (4 waves x 32 query columns)
+constexpr int kMT = 64;    // targets per LDS tile (2 MFMA row blocks)
+constexpr int kMP = 132;   // LDS row stride in floats
+constexpr uint32_t kNoIdx = 0xffffffffu;
+
+// |row|^2 of each descriptor (any summation order: only feeds the approximate distance and
+// the error bound) and the maximum over rows as float bits.
+__global__ void row_norms_kernel(const float *d, const uint32_t *n_p, uint32_t n_max, float *out,
+                                 uint32_t *max_bits) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = min(*n_p, n_max);
+    float s = 0.0f;
+    if (r < n) {
+        const float4 *p = reinterpret_cast<const float4 *>(d + (size_t)r * 128);
+        for (int i = 0; i < 32; i++) {
+            float4 v = p[i];
+            s += v.x * v.x + v.y * v.y + v.z * v.z + v.w * v.w;
+        }
+        out[r] = s;
+        if (isfinite(s)) atomicMax(max_bits, __float_as_uint(s));
+        else atomicMax(max_bits, 0x7f800000u);
+    }
+}
+
+__device__ __forceinline__ void top4_insert(float d, uint32_t t, float (&k)[4], uint32_t (&ix)[4]) {
+    if (!(d < k[3])) return;
+    if (d < k[2]) {
+        k[3] = k[2]; ix[3] = ix[2];
+        if (d < k[1]) {
+            k[2] = k[1]; ix[2] = ix[1];
+            if (d < k[0]) { k[1] = k[0]; ix[1] = ix[0]; k[0] = d; ix[0] = t; }
+            else { k[1] = d; ix[1] = t; }
+        } else { k[2] = d; ix[2] = t; }
+    } else { k[3] = d; ix[3] = t; }
+}
+
+// (d, idx) lexicographic insertion for merging lists whose indices are not monotone
+__device__ __forceinline__ void top4_insert_lex(float d, uint32_t t, float (&k)[4], uint32_t (&ix)[4]) {
+    auto less = [](float a, uint32_t ia, float b, uint32_t ib) { return a < b || (a == b && ia < ib); };
+    if (!less(d, t, k[3], ix[3])) return;
+    int p = 3;
+    while (p > 0 && less(d, t, k[p - 1], ix[p - 1])) {
+        k[p] = k[p - 1];
+        ix[p] = ix[p - 1];
+        p--;
+    }
+    k[p] = d;
+    ix[p] = t;
+}
+
+struct Cand4 {
+    float d[4];
+    uint32_t i[4];
+};
+
+__global__ __launch_bounds__(256) void nn_mfma_kernel(const float *qd, const uint32_t *nq_p, uint32_t nq_max,
+                                                      const float *td, const float *tn, const uint32_t *nt_p,
+                                                      uint32_t nt_max, int nsplit, Cand4 *part) {
+    __shared__ __attribute__((aligned(16))) float s_t[kMT * kMP];
+    __shared__ float s_tn[kMT];
+    const uint32_t nq = min(*nq_p, nq_max), nt = min(*nt_p, nt_max);
+    const uint32_t qb = blockIdx.x * kMQ;
+    if (qb >= nq) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col = lane & 31, half = lane >> 5;
+    const uint32_t q = qb + wave * 32 + col;
+
+    // B operand: this lane's query, dims half*64 .. half*64+63 (k order is free for a dot product)
+    float b[64];
+    if (q < nq) {
+        const float4 *p = reinterpret_cast<const float4 *>(qd + (size_t)q * 128 + half * 64);
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            float4 v = p[m];
+            b[4 * m] = v.x; b[4 * m + 1] = v.y; b[4 * m + 2] = v.z; b[4 * m + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < 64; s++) b[s] = 0.0f;
+    }
+    float kd[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+    uint32_t ki[4] = {kNoIdx, kNoIdx, kNoIdx, kNoIdx};
+
+    const uint32_t ntiles = (nt + kMT - 1) / kMT;
+    const uint32_t per = (ntiles + nsplit - 1) / nsplit;
+    const uint32_t tb = blockIdx.y * per, te = min(tb + per, ntiles);
+    for (uint32_t tile = tb; tile < te; tile++) {
+        const uint32_t t0 = tile * kMT;
+        __syncthreads();
+        for (int i = tid; i < kMT * 32; i += 256) {
+            int r = i >> 5, c = (i & 31) * 4;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (t0 + r < nt) v = *reinterpret_cast<const float4 *>(td + (size_t)(t0 + r) * 128 + c);
+            *reinterpret_cast<float4 *>(s_t + r * kMP + c) = v;
+        }
+        if (tid < kMT) s_tn[tid] = (t0 + tid < nt) ? tn[t0 + tid] : 0.0f;
+        __syncthreads();
+        floatx16 acc0 = {}, acc1 = {};
+        const float *r0 = s_t + col * kMP + half * 64;
+        const float *r1 = s_t + (32 + col) * kMP + half * 64;
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            float4 a0 = *reinterpret_cast<const float4 *>(r0 + 4 * m);
+            float4 a1 = *reinterpret_cast<const float4 *>(r1 + 4 * m);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b[4 * m], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, b[4 * m], acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b[4 * m + 1], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, b[4 * m + 1], acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, b[4 * m + 2], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, b[4 * m + 2], acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, b[4 * m + 3], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, b[4 * m + 3], acc1, 0, 0, 0);
+        }
+        // D layout: lane holds column (query) lane%32, rows 8*(r/4) + 4*(lane/32) + r%4
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            int i = 8 * (r >> 2) + 4 * half + (r & 3);
+            uint32_t t = t0 + i;
+            if (t < nt) top4_insert(s_tn[i] - 2.0f * acc0[r], t, kd, ki);
+        }
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            int i = 32 + 8 * (r >> 2) + 4 * half + (r & 3);
+            uint32_t t = t0 + i;
+            if (t < nt) top4_insert(s_tn[i] - 2.0f * acc1[r], t, kd, ki);
+        }
+    }
+    // merge the two lane halves that hold the same query
+    float od[4];
+    uint32_t oi[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        od[j] = __shfl_xor(kd[j], 32);
+        oi[j] = __shfl_xor(ki[j], 32);
+    }
+    if (half == 0 && q < nq) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) top4_insert_lex(od[j], oi[j], kd, ki);
+        Cand4 c;
+#pragma unroll
+        for (int j = 0; j < 4; j++) { c.d[j] = kd[j]; c.i[j] = ki[j]; }
+        part[(size_t)blockIdx.y * nq_max + q] = c;
+    }
+}
+
+// sequential f32 distance of pipeline.rs:1700-1717 (without the exactness-neutral early exit)
+__device__ __forceinline__ float exact_dist(const float *a, const float *b) {
+    float sum = 0.0f;
+    const float4 *pa = reinterpret_cast<const float4 *>(a), *pb = reinterpret_cast<const float4 *>(b);
+    for (int i = 0; i < 32; i++) {
+        float4 x = pa[i], y = pb[i];
+        float e;
+        e = x.x - y.x; sum += e * e;
+        e = x.y - y.y; sum += e * e;
+        e = x.z - y.z; sum += e * e;
+        e = x.w - y.w; sum += e * e;
+    }
+    return sum;
+}
+
+__global__ void nn_rescore_kernel(const Cand4 *part, int nsplit, const float *qd, const float *qn,
+                                  const uint32_t *nq_p, uint32_t nq_max, const float *td, const uint32_t *nt_p,
+                                  const uint32_t *tmax_bits, uint32_t *best_idx, float *best, float *second,
+                                  uint32_t *flag) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= min(*nq_p, nq_max)) return;
+    const uint32_t nt = *nt_p;
+    float kd[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+    uint32_t ki[4] = {kNoIdx, kNoIdx, kNoIdx, kNoIdx};
+    for (int s = 0; s < nsplit; s++) {
+        Cand4 c = part[(size_t)s * nq_max + q];
+        for (int j = 0; j < 4; j++)
+            if (c.i[j] != kNoIdx) top4_insert_lex(c.d[j], c.i[j], kd, ki);
+    }
+    float b = INFINITY, sc = INFINITY;
+    uint32_t bi = 0;
+    int ncand = 0;
+    for (int j = 0; j < 4; j++) {
+        if (ki[j] == kNoIdx) continue;
+        ncand++;
+        float d = exact_dist(qd + (size_t)q * 128, td + (size_t)ki[j] * 128);
+        if (d < b || (d == b && ki[j] < bi)) { sc = b; b = d; bi = ki[j]; }
+        else if (d < sc) sc = d;
+    }
+    // exactness guard (see header comment); all targets are candidates when nt <= 4
+    bool ok = ncand >= (int)min(nt, 4u);
+    if (ok && nt > 4) {
+        const float qq = qn[q], tt = __uint_as_float(*tmax_bits);
+        const float E = 1.0e-4f * (qq + tt + 2.0f * sqrtf(qq * tt)) + 1e-30f;
+        ok = (kd[3] + qq) - E > sc;
+    }
+    best_idx[q] = bi;
+    best[q] = b;
+    second[q] = sc;
+    flag[q] = ok ? 0u : 1u;
+}
+
+// Exact full scan for the flagged queries: one workgroup per query.
+__global__ __launch_bounds__(256) void nn_exact_flagged_kernel(const float *qd, const uint32_t *nq_p, uint32_t nq_max,
+                                                               const float *td, const uint32_t *nt_p, uint32_t nt_max,
+                                                               const uint32_t *flag, uint32_t *best_idx, float *best,
+                                                               float *second) {
+    __shared__ float sb[256], ss[256];
+    __shared__ uint32_t si[256];
+    const uint32_t q = blockIdx.x;
+    if (q >= min(*nq_p, nq_max) || flag[q] == 0) return;
+    const uint32_t nt = min(*nt_p, nt_max);
+    float b = INFINITY, s = INFINITY;
+    uint32_t bi = 0;
+    for (uint32_t t = threadIdx.x; t < nt; t += 256) {
+        float d = exact_dist(qd + (size_t)q * 128, td + (size_t)t * 128);
+        if (d < b) { s = b; b = d; bi = t; }
+        else if (d < s) s = d;
+    }
+    sb[threadIdx.x] = b; ss[threadIdx.x] = s; si[threadIdx.x] = bi;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float B = sb[0], S = ss[0];
+        uint32_t BI = si[0];
+        for (int k = 1; k < 256; k++) {
+            float b2 = sb[k], s2 = ss[k];
+            uint32_t i2 = si[k];
+            float other;
+            if (b2 < B || (b2 == B && i2 < BI)) { other = B; B = b2; BI = i2; }
+            else other = b2;
+            S = fminf(fminf(S, s2), other);
+        }
+        best_idx[q] = BI;
+        best[q] = B;
+        second[q] = S;
+    }
+}
+
+int nn_mfma_splits(size_t nq_max, size_t nt_max) {
+    size_t qb = (nq_max + kMQ - 1) / kMQ, tt = (nt_max + kMT - 1) / kMT;
+    if (qb == 0 || tt == 0) return 1;
+    size_t ns = (768 + qb - 1) / qb;
+    if (ns > tt) ns = tt;
+    if (ns > 1024) ns = 1024;
+    return (int)(ns < 1 ? 1 : ns);
+}
+
+size_t nn_mfma_scratch_bytes(size_t nq_max, size_t nt_max) {
+    size_t ns = (size_t)nn_mfma_splits(nq_max, nt_max);
+    return ns * nq_max * sizeof(Cand4) + (nq_max + nt_max) * sizeof(float) + nq_max * sizeof(uint32_t) + 256;
+}
+
+// Scratch layout: [part: ns*nq_max Cand4][qn: nq_max][tn: nt_max][flag: nq_max][tmax: 1]
+hipError_t launch_nn_mfma(const float *qd, const uint32_t *nq, size_t nq_max, const float *td, const uint32_t *nt,
+                          size_t nt_max, void *scratch, uint32_t *best_idx, float *best, float *second,
+                          hipStream_t st) {
+    if (nq_max == 0) return hipSuccess;
+    const int ns = nn_mfma_splits(nq_max, nt_max);
+    char *p = static_cast<char *>(scratch);
+    Cand4 *part = reinterpret_cast<Cand4 *>(p);
+    p += (size_t)ns * nq_max * sizeof(Cand4);
+    float *qn = reinterpret_cast<float *>(p);
+    p += nq_max * sizeof(float);
+    float *tn = reinterpret_cast<float *>(p);
+    p += nt_max * sizeof(float);
+    uint32_t *flag = reinterpret_cast<uint32_t *>(p);
+    p += nq_max * sizeof(uint32_t);
+    uint32_t *tmax = reinterpret_cast<uint32_t *>(p);
+    hipError_t e = launch_fill_u32(tmax, 0u, 2, st);
+    if (e != hipSuccess) return e;
+    uint32_t *qmax = tmax + 1;
+    row_norms_kernel<<<(unsigned)((nq_max + 255) / 256), 256, 0, st>>>(qd, nq, (uint32_t)nq_max, qn, qmax);
+    row_norms_kernel<<<(unsigned)((nt_max + 255) / 256), 256, 0, st>>>(td, nt, (uint32_t)nt_max, tn, tmax);
+    dim3 grid((unsigned)((nq_max + kMQ - 1) / kMQ), (unsigned)ns);
+    nn_mfma_kernel<<<grid, 256, 0, st>>>(qd, nq, (uint32_t)nq_max, td, tn, nt, (uint32_t)nt_max, ns, part);
+    nn_rescore_kernel<<<(unsigned)((nq_max + 255) / 256), 256, 0, st>>>(part, ns, qd, qn, nq, (uint32_t)nq_max, td, nt,
+                                                                        tmax, best_idx, best, second, flag);
+    nn_exact_flagged_kernel<<<(unsigned)nq_max, 256, 0, st>>>(qd, nq, (uint32_t)nq_max, td, nt, (uint32_t)nt_max, flag,
+                                                              best_idx, best, second);
+    return hipGetLastError();
+}
+
+}  // namespace ps

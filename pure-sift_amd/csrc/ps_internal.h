@@ -109,4 +109,11 @@ hipError_t launch_match_finish(const ps_feature *qf, const ps_feature *tf, const
                                const float *second, const uint32_t *rev_idx, float ratio, int mutual,
                                ps_feature_match *out, uint32_t *out_count, hipStream_t st);
 
+// match_mfma.hip: exact NN/2NN through an FP32 MFMA distance GEMM + exact rescoring
+int nn_mfma_splits(size_t nq_max, size_t nt_max);
+size_t nn_mfma_scratch_bytes(size_t nq_max, size_t nt_max);
+hipError_t launch_nn_mfma(const float *qd, const uint32_t *nq, size_t nq_max, const float *td, const uint32_t *nt,
+                          size_t nt_max, void *scratch, uint32_t *best_idx, float *best, float *second,
+                          hipStream_t st);
+
 }  // namespace ps
