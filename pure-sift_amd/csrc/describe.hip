@@ -4,8 +4,8 @@
 // Gradients are evaluated on the fly from the octave's Gaussian level instead of
 // materialising magnitude/angle planes (pipeline.rs:1168-1203): same expressions,
 // and the windows never touch the zeroed border of the reference's planes. Device
-// expf/atan2f/sinf/cosf and the parallel histogram accumulation make these results
-// tolerance-level (not bit-level) equal to the reference.
+// expf/sinf/cosf, a polynomial atan2 and the parallel histogram accumulation make these
+// results tolerance-level (not bit-level) equal to the reference.
 #include "ps_internal.h"
 
 #pragma clang fp contract(off)
@@ -14,9 +14,42 @@ namespace ps {
 
 constexpr float kTwoPi = 2.0f * 3.14159265358979323846f;
 
+// f32::rem_euclid. fmodf(x, m) returns x exactly when |x| < |m|, which covers every call
+// below (angles in (-2pi, 2pi]); the library fmodf only runs for the remaining inputs.
 __device__ __forceinline__ float rem_euclid(float x, float m) {
-    float r = fmodf(x, m);
+    float r = fabsf(x) < fabsf(m) ? x : fmodf(x, m);
     return r < 0.0f ? r + fabsf(m) : r;
+}
+
+// atan2 for the gradient angle: octant reduction, a = min/max via the hardware reciprocal,
+// atan(a) = a * P(a^2) with P a degree-8 polynomial fitted here (max error 2.9e-7 rad over all
+// inputs, about 3 ulp -- the same order as glibc's atan2f, which the device cannot reproduce
+// bit for bit anyway). Callers guarantee (x, y) != (0, 0).
+__device__ __forceinline__ float grad_atan2(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float a = mn * __builtin_amdgcn_rcpf(mx);
+    const float s = a * a;
+    float p = 0.00284997f;
+    p = p * s + -0.01606893f;
+    p = p * s + 0.042692f;
+    p = p * s + -0.07504334f;
+    p = p * s + 0.10640952f;
+    p = p * s + -0.14203648f;
+    p = p * s + 0.1999262f;
+    p = p * s + -0.33333072f;
+    p = p * s + 1.0f;
+    p = p * a;
+    if (ay > ax) p = 1.57079632679489662f - p;
+    if (x < 0.0f) p = 3.14159265358979324f - p;
+    return y < 0.0f ? -p : p;
+}
+
+// (p / w, p % w) for 0 <= p < 2^20 and 1 <= w < 2^10 via a float reciprocal: p/w has
+// fractional part in [0, 1 - 1/w], so floor((p + 0.5) * (1/w)) is exact.
+__device__ __forceinline__ void divmod_small(int p, int w, float inv_w, int &q, int &r) {
+    q = (int)(((float)p + 0.5f) * inv_w);
+    r = p - q * w;
 }
 
 struct LocalKp {
@@ -46,14 +79,21 @@ __device__ __forceinline__ void grad(const LocalKp &l, int x, int y, float &m, f
     float dx = row[x + 1] - row[x - 1];
     float dy = row[x + l.W] - row[x - l.W];
     m = sqrtf(dx * dx + dy * dy);
-    a = rem_euclid(atan2f(dy, dx), kTwoPi);
+    a = rem_euclid(grad_atan2(dy, dx), kTwoPi);
 }
 
-// pipeline.rs:1233-1347. One 64-lane workgroup per keypoint.
-__global__ __launch_bounds__(64) void orient_kernel(PyrView pv, const Cand *kps, const uint32_t *nkp, uint32_t max_kp,
-                                                    float *ori, uint32_t *nori) {
+// pipeline.rs:1233-1347. One 256-thread workgroup per keypoint. Histogram updates go to one
+// of 32 private LDS copies (wave, lane % 8), so lanes that share a copy are 8 pixels apart and
+// rarely hit the same bin; the copies are summed in a fixed order (deterministic results).
+// Each thread gathers the gradient samples of 4 pixels before any histogram update, so
+// their loads overlap.
+constexpr int kDescThreads = 256;
+constexpr int kCopies = 32;   // 4 waves x 8 lane groups
+constexpr int kBatch = 4;
+__global__ __launch_bounds__(kDescThreads) void orient_kernel(PyrView pv, const Cand *kps, const uint32_t *nkp,
+                                                              uint32_t max_kp, float *ori, uint32_t *nori) {
+    __shared__ float s_whist[kCopies][37];   // +1 pad: copies of a bin sit in different LDS banks
     __shared__ float s_hist[36];
-    __shared__ float s_sm[36];
     const uint32_t k = blockIdx.x;
     const int lane = threadIdx.x;
     if (k >= min(*nkp, max_kp)) return;
@@ -68,71 +108,102 @@ __global__ __launch_bounds__(64) void orient_kernel(PyrView pv, const Cand *kps,
     const int y_min = max(cy - radius, 1), y_max = max(min(cy + radius, l.H - 2), 1);
     const int x_min = max(cx - radius, 1), x_max = max(min(cx + radius, l.W - 2), 1);
     const float r_sq_max = (float)(radius * radius);
-    if (lane < 36) s_hist[lane] = 0.0f;
+    for (int i = lane; i < kCopies * 37; i += kDescThreads) (&s_whist[0][0])[i] = 0.0f;
     __syncthreads();
+    float *wh = s_whist[(lane >> 6) * 8 + (lane & 7)];
     const int ww = x_max - x_min + 1, hh = y_max - y_min + 1;
     const int total = (ww > 0 && hh > 0) ? ww * hh : 0;
-    for (int p = lane; p < total; p += 64) {
-        int yy = y_min + p / ww, xx = x_min + p % ww;
-        float dx = (float)(xx - cx), dy = (float)(yy - cy);
-        float r_sq = dx * dx + dy * dy;
-        if (r_sq > r_sq_max) continue;
-        float m, a;
-        grad(l, xx, yy, m, a);
-        if (m <= 1e-7f) continue;
-        float weight = expf(-r_sq / two_sigma_sq);
-        int bin = ((int)floorf(a * bin_scale)) % 36;
-        atomicAdd(&s_hist[bin], m * weight);
+    const float inv_ww = 1.0f / (float)max(ww, 1);
+    for (int p0 = lane; p0 < total; p0 += kBatch * kDescThreads) {
+        float gxa[kBatch], gxb[kBatch], gya[kBatch], gyb[kBatch], rsq[kBatch];
+        bool ok[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            int p = p0 + u * kDescThreads;
+            int qy, qx;
+            divmod_small(p, ww, inv_ww, qy, qx);
+            int yy = y_min + qy, xx = x_min + qx;
+            float dx = (float)(xx - cx), dy = (float)(yy - cy);
+            rsq[u] = dx * dx + dy * dy;
+            ok[u] = p < total && !(rsq[u] > r_sq_max);
+            const float *row = l.g + (size_t)(ok[u] ? yy : y_min) * l.W + (ok[u] ? xx : x_min);
+            gxa[u] = row[1]; gxb[u] = row[-1]; gyb[u] = row[l.W]; gya[u] = row[-l.W];
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            if (!ok[u]) continue;
+            float dx = gxa[u] - gxb[u], dy = gyb[u] - gya[u];    // pipeline.rs:1186-1189
+            float m = sqrtf(dx * dx + dy * dy);
+            if (m <= 1e-7f) continue;
+            float a = rem_euclid(grad_atan2(dy, dx), kTwoPi);
+            float weight = expf(-rsq[u] / two_sigma_sq);
+            int bin = ((int)floorf(a * bin_scale)) % 36;
+            atomicAdd(&wh[bin], m * weight);
+        }
     }
     __syncthreads();
     if (lane < 36) {
-        float prev = s_hist[(lane + 35) % 36], curr = s_hist[lane], next = s_hist[(lane + 1) % 36];
-        s_sm[lane] = (prev + curr + next) * (float)(1.0 / 3.0);
+        float acc = 0.0f;
+        for (int c = 0; c < kCopies; c++) acc += s_whist[c][lane];
+        s_hist[lane] = acc;
     }
     __syncthreads();
-    if (lane != 0) return;
-    int best_bin = 0;
-    float best_val = s_sm[0];
-    for (int b = 1; b < 36; b++)
-        if (s_sm[b] > best_val) { best_val = s_sm[b]; best_bin = b; }
-    float *o = ori + (size_t)k * 4;
-    if (best_val <= 1e-7f) { o[0] = 0.0f; nori[k] = 1; return; }
-    const float split = 0.80f * best_val;
-    int pb[4];
-    float pv4[4];
-    int np = 0;
-    // collect peaks in bin order and keep the 4 best by (value desc, bin asc) (pipeline.rs:1306-1329)
-    for (int b = 0; b < 36; b++) {
-        float v = s_sm[b];
-        if (v < split) continue;
-        float lft = s_sm[(b + 35) % 36], rgt = s_sm[(b + 1) % 36];
-        if (!(v > lft && v >= rgt)) continue;
-        int pos = np < 4 ? np : 4;
-        while (pos > 0 && pv4[pos - 1] < v) pos--;
-        if (pos < 4) {
-            for (int q = min(np, 3); q > pos; q--) { pb[q] = pb[q - 1]; pv4[q] = pv4[q - 1]; }
-            pb[pos] = b;
-            pv4[pos] = v;
-            if (np < 4) np++;
-        }
+    if (lane >= 64) return;
+    // wave 0: lane b < 36 owns smoothed bin b (pipeline.rs:1280-1287)
+    const bool own = lane < 36;
+    float sm = -INFINITY;
+    if (own) {
+        float prev = s_hist[(lane + 35) % 36], curr = s_hist[lane], next = s_hist[(lane + 1) % 36];
+        sm = (prev + curr + next) * (float)(1.0 / 3.0);
     }
-    if (np == 0) { pb[0] = best_bin; pv4[0] = best_val; np = 1; }
-    for (int i = 0; i < np; i++) {
-        int bin = pb[i];
-        float val = pv4[i];
-        float lft = s_sm[(bin + 35) % 36], rgt = s_sm[(bin + 1) % 36];
-        float denom = lft - 2.0f * val + rgt;
+    // global maximum, first bin on ties (pipeline.rs:1289-1297)
+    float best_val = sm;
+    int best_bin = own ? lane : 64;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        float ov = __shfl_xor(best_val, o);
+        int ob = __shfl_xor(best_bin, o);
+        if (ov > best_val || (ov == best_val && ob < best_bin)) { best_val = ov; best_bin = ob; }
+    }
+    float *o = ori + (size_t)k * 4;
+    if (best_val <= 1e-7f) {
+        if (lane == 0) { o[0] = 0.0f; nori[k] = 1; }
+        return;
+    }
+    // peaks >= 80% of the maximum, strict on the left, >= on the right (pipeline.rs:1303-1317)
+    const float lft = __shfl(sm, (lane + 35) % 36), rgt = __shfl(sm, (lane + 1) % 36);
+    const bool peak = own && !(sm < 0.80f * best_val) && sm > lft && sm >= rgt;
+    const uint64_t pm = __ballot(peak);
+    const int npeaks = __popcll(pm);
+    // order by (value desc, bin asc), keep 4 (pipeline.rs:1323-1328)
+    int rank = 0;
+    for (int j = 0; j < 36; j++) {
+        float vj = __shfl(sm, j);
+        if (((pm >> j) & 1ull) && (vj > sm || (vj == sm && j < lane))) rank++;
+    }
+    if (npeaks == 0) {
+        if (lane == best_bin) {
+            float denom = lft - 2.0f * sm + rgt;
+            float off = fabsf(denom) > 1e-6f ? 0.5f * (lft - rgt) / denom : 0.0f;
+            off = off < -0.5f ? -0.5f : (off > 0.5f ? 0.5f : off);
+            o[0] = rem_euclid(((float)lane + 0.5f + off) * (kTwoPi / 36.0f), kTwoPi);
+            nori[k] = 1;
+        }
+        return;
+    }
+    if (peak && rank < 4) {   // parabolic refinement (pipeline.rs:1333-1345)
+        float denom = lft - 2.0f * sm + rgt;
         float off = fabsf(denom) > 1e-6f ? 0.5f * (lft - rgt) / denom : 0.0f;
         off = off < -0.5f ? -0.5f : (off > 0.5f ? 0.5f : off);
-        o[i] = rem_euclid(((float)bin + 0.5f + off) * (kTwoPi / 36.0f), kTwoPi);
+        o[rank] = rem_euclid(((float)lane + 0.5f + off) * (kTwoPi / 36.0f), kTwoPi);
     }
-    nori[k] = (uint32_t)np;
+    if (lane == 0) nori[k] = (uint32_t)min(npeaks, 4);
 }
 
 hipError_t launch_orientations(const PyrView &pv, const Cand *kps, const uint32_t *nkp, uint32_t max_kp, float *ori,
                                uint32_t *nori, hipStream_t st) {
     if (max_kp == 0) return hipSuccess;
-    orient_kernel<<<max_kp, 64, 0, st>>>(pv, kps, nkp, max_kp, ori, nori);
+    orient_kernel<<<max_kp, kDescThreads, 0, st>>>(pv, kps, nkp, max_kp, ori, nori);
     return hipGetLastError();
 }
 
@@ -173,13 +244,15 @@ hipError_t launch_feature_scan(const uint32_t *nkp, const uint32_t *nori, uint32
     return hipGetLastError();
 }
 
-// pipeline.rs:1352-1480 (+ RootSIFT, :1012-1019). One 64-lane workgroup per feature.
-__global__ __launch_bounds__(64) void descriptor_kernel(PyrView pv, const Cand *kps, const float *ori,
-                                                        const uint32_t *feat_kp, const uint32_t *feat_j,
-                                                        const uint32_t *nfeat, uint32_t cap, int kind,
-                                                        ps_feature *out_f, float *out_d) {
+// pipeline.rs:1352-1480 (+ RootSIFT, :1012-1019). One 256-thread workgroup per feature, with
+// 32 private LDS histograms summed in a fixed order and batched gradient loads (as in
+// orient_kernel).
+__global__ __launch_bounds__(kDescThreads) void descriptor_kernel(PyrView pv, const Cand *kps, const float *ori,
+                                                                  const uint32_t *feat_kp, const uint32_t *feat_j,
+                                                                  const uint32_t *nfeat, uint32_t cap, int kind,
+                                                                  ps_feature *out_f, float *out_d) {
+    __shared__ float s_wdesc[kCopies][129];   // +1 pad: copies of a bin sit in different LDS banks
     __shared__ float s_desc[128];
-    __shared__ float s_scale[2];
     const uint32_t f = blockIdx.x;
     const int lane = threadIdx.x;
     if (f >= min(*nfeat, cap)) return;
@@ -187,8 +260,7 @@ __global__ __launch_bounds__(64) void descriptor_kernel(PyrView pv, const Cand *
     const Cand kp = kps[k];
     const float theta = ori[(size_t)k * 4 + feat_j[f]];
     const LocalKp l = local_kp(pv, kp);
-    s_desc[lane] = 0.0f;
-    s_desc[lane + 64] = 0.0f;
+    for (int i = lane; i < kCopies * 129; i += kDescThreads) (&s_wdesc[0][0])[i] = 0.0f;
     __syncthreads();
     if (l.W >= 3 && l.H >= 3) {
         const float cos_t = cosf(theta), sin_t = sinf(theta);
@@ -202,90 +274,107 @@ __global__ __launch_bounds__(64) void descriptor_kernel(PyrView pv, const Cand *
         const float bin_per_rad = 8.0f / kTwoPi;
         const int ww = x_max - x_min + 1, hh = y_max - y_min + 1;
         const int total = (ww > 0 && hh > 0) ? ww * hh : 0;
-        for (int p = lane; p < total; p += 64) {
-            int yy = y_min + p / ww, xx = x_min + p % ww;
-            float dy = (float)yy - l.y, dx = (float)xx - l.x;
-            float r_rot = (dx * cos_t + dy * sin_t) * inv_hw;
-            float c_rot = (-dx * sin_t + dy * cos_t) * inv_hw;
-            float r_bin = r_rot + 4.0f * 0.5f - 0.5f;
-            float c_bin = c_rot + 4.0f * 0.5f - 0.5f;
-            if (r_bin <= -1.0f || r_bin >= 4.0f || c_bin <= -1.0f || c_bin >= 4.0f) continue;
-            float m, a;
-            grad(l, xx, yy, m, a);
-            if (m <= 1e-7f) continue;
-            float wm = m * expf((r_rot * r_rot + c_rot * c_rot) * exp_scale);
-            float o_bin = rem_euclid(a - theta, kTwoPi) * bin_per_rad;
-            int r0 = (int)floorf(r_bin), c0 = (int)floorf(c_bin), o0 = (int)floorf(o_bin);
-            float dr = r_bin - (float)r0, dc = c_bin - (float)c0, dof = o_bin - (float)o0;
+        const float inv_ww = 1.0f / (float)max(ww, 1);
+        float *wd = s_wdesc[(lane >> 6) * 8 + (lane & 7)];
+        for (int p0 = lane; p0 < total; p0 += kBatch * kDescThreads) {
+            float gxa[kBatch], gxb[kBatch], gya[kBatch], gyb[kBatch], rr[kBatch], cc[kBatch];
+            bool ok[kBatch];
 #pragma unroll
-            for (int ir = 0; ir <= 1; ir++) {
-                int rb = r0 + ir;
-                if (rb < 0 || rb >= 4) continue;
-                float wr = ir == 0 ? 1.0f - dr : dr;
+            for (int u = 0; u < kBatch; u++) {
+                int p = p0 + u * kDescThreads;
+                int qy, qx;
+                divmod_small(p, ww, inv_ww, qy, qx);
+                int yy = y_min + qy, xx = x_min + qx;
+                float dy = (float)yy - l.y, dx = (float)xx - l.x;
+                float r_rot = (dx * cos_t + dy * sin_t) * inv_hw;
+                float c_rot = (-dx * sin_t + dy * cos_t) * inv_hw;
+                float r_bin = r_rot + 4.0f * 0.5f - 0.5f;
+                float c_bin = c_rot + 4.0f * 0.5f - 0.5f;
+                rr[u] = r_rot;
+                cc[u] = c_rot;
+                ok[u] = p < total && !(r_bin <= -1.0f || r_bin >= 4.0f || c_bin <= -1.0f || c_bin >= 4.0f);
+                const float *row = l.g + (size_t)(ok[u] ? yy : y_min) * l.W + (ok[u] ? xx : x_min);
+                gxa[u] = row[1]; gxb[u] = row[-1]; gyb[u] = row[l.W]; gya[u] = row[-l.W];
+            }
 #pragma unroll
-                for (int ic = 0; ic <= 1; ic++) {
-                    int cb = c0 + ic;
-                    if (cb < 0 || cb >= 4) continue;
-                    float wc = ic == 0 ? 1.0f - dc : dc;
+            for (int u = 0; u < kBatch; u++) {
+                if (!ok[u]) continue;
+                float gdx = gxa[u] - gxb[u], gdy = gyb[u] - gya[u];   // pipeline.rs:1186-1189
+                float m = sqrtf(gdx * gdx + gdy * gdy);
+                if (m <= 1e-7f) continue;
+                float a = rem_euclid(grad_atan2(gdy, gdx), kTwoPi);
+                float r_rot = rr[u], c_rot = cc[u];
+                float r_bin = r_rot + 4.0f * 0.5f - 0.5f;
+                float c_bin = c_rot + 4.0f * 0.5f - 0.5f;
+                float wm = m * expf((r_rot * r_rot + c_rot * c_rot) * exp_scale);
+                float o_bin = rem_euclid(a - theta, kTwoPi) * bin_per_rad;
+                int r0 = (int)floorf(r_bin), c0 = (int)floorf(c_bin), o0 = (int)floorf(o_bin);
+                float dr = r_bin - (float)r0, dc = c_bin - (float)c0, dof = o_bin - (float)o0;
 #pragma unroll
-                    for (int io = 0; io <= 1; io++) {
-                        int ob = (o0 + io) & 7;
-                        float wo = io == 0 ? 1.0f - dof : dof;
-                        atomicAdd(&s_desc[(rb * 4 + cb) * 8 + ob], wm * wr * wc * wo);
+                for (int ir = 0; ir <= 1; ir++) {
+                    int rb = r0 + ir;
+                    if (rb < 0 || rb >= 4) continue;
+                    float wr = ir == 0 ? 1.0f - dr : dr;
+#pragma unroll
+                    for (int ic = 0; ic <= 1; ic++) {
+                        int cb = c0 + ic;
+                        if (cb < 0 || cb >= 4) continue;
+                        float wc = ic == 0 ? 1.0f - dc : dc;
+#pragma unroll
+                        for (int io = 0; io <= 1; io++) {
+                            int ob = (o0 + io) & 7;
+                            float wo = io == 0 ? 1.0f - dof : dof;
+                            atomicAdd(&wd[(rb * 4 + cb) * 8 + ob], wm * wr * wc * wo);
+                        }
                     }
                 }
             }
         }
     }
     __syncthreads();
-    // normalisation sums run sequentially in element order, as the reference's iterator sums
-    if (lane == 0) {
-        float n1 = 0.0f;
-        for (int i = 0; i < 128; i++) n1 += s_desc[i] * s_desc[i];
+    if (lane < 128) {
+        float acc = 0.0f;
+        for (int c = 0; c < kCopies; c++) acc += s_wdesc[c][lane];
+        s_desc[lane] = acc;
+    }
+    __syncthreads();
+    // Normalisation (pipeline.rs:1461-1477) by wave 0: lane l holds elements l and l + 64; the
+    // sums use a fixed butterfly order (deterministic; within tolerance of the reference's
+    // sequential sums, like the rest of the descriptor).
+    if (lane < 64) {
+        float v0 = s_desc[lane], v1 = s_desc[lane + 64];
+        float n1 = v0 * v0 + v1 * v1;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) n1 += __shfl_xor(n1, o);
         n1 = sqrtf(n1);
-        s_scale[0] = n1;
-        s_scale[1] = 1.0f / n1;
-    }
-    __syncthreads();
-    const bool norm_ok = s_scale[0] > 1e-7f;
-    if (norm_ok) {
-        float inv1 = s_scale[1];
-        s_desc[lane] = fminf(s_desc[lane] * inv1, 0.2f);
-        s_desc[lane + 64] = fminf(s_desc[lane + 64] * inv1, 0.2f);
-    }
-    __syncthreads();
-    if (norm_ok && lane == 0) {
-        float n2 = 0.0f;
-        for (int i = 0; i < 128; i++) n2 += s_desc[i] * s_desc[i];
-        n2 = sqrtf(n2);
-        s_scale[0] = n2;
-        s_scale[1] = 1.0f / n2;
-    }
-    __syncthreads();
-    if (norm_ok && s_scale[0] > 1e-7f) {
-        float inv2 = s_scale[1];
-        s_desc[lane] *= inv2;
-        s_desc[lane + 64] *= inv2;
-    }
-    __syncthreads();
-    if (kind == PS_DESCRIPTOR_ROOTSIFT) {
-        if (lane == 0) {
-            float l1 = 0.0f;
-            for (int i = 0; i < 128; i++) l1 += fmaxf(s_desc[i], 0.0f);
-            s_scale[0] = l1;
-            s_scale[1] = 1.0f / l1;
+        if (n1 > 1e-7f) {
+            const float inv1 = 1.0f / n1;
+            v0 = fminf(v0 * inv1, 0.2f);
+            v1 = fminf(v1 * inv1, 0.2f);
+            float n2 = v0 * v0 + v1 * v1;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) n2 += __shfl_xor(n2, o);
+            n2 = sqrtf(n2);
+            if (n2 > 1e-7f) {
+                const float inv2 = 1.0f / n2;
+                v0 *= inv2;
+                v1 *= inv2;
+            }
         }
-        __syncthreads();
-        if (s_scale[0] > 1e-7f) {
-            float inv = s_scale[1];
-            s_desc[lane] = sqrtf(fmaxf(s_desc[lane], 0.0f) * inv);
-            s_desc[lane + 64] = sqrtf(fmaxf(s_desc[lane + 64], 0.0f) * inv);
+        if (kind == PS_DESCRIPTOR_ROOTSIFT) {   // pipeline.rs:1012-1019
+            float l1 = fmaxf(v0, 0.0f) + fmaxf(v1, 0.0f);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) l1 += __shfl_xor(l1, o);
+            if (l1 > 1e-7f) {
+                const float inv = 1.0f / l1;
+                v0 = sqrtf(fmaxf(v0, 0.0f) * inv);
+                v1 = sqrtf(fmaxf(v1, 0.0f) * inv);
+            }
         }
-        __syncthreads();
+        float *od = out_d + (size_t)f * 128;
+        od[lane] = v0;
+        od[lane + 64] = v1;
     }
-    float *od = out_d + (size_t)f * 128;
-    od[lane] = s_desc[lane];
-    od[lane + 64] = s_desc[lane + 64];
     if (lane == 0) {
         ps_feature o;
         o.x = kp.x; o.y = kp.y; o.scale = kp.scale; o.response = kp.response; o.orientation = theta;
@@ -297,7 +386,7 @@ hipError_t launch_descriptors(const PyrView &pv, const Cand *kps, const float *o
                               const uint32_t *feat_j, const uint32_t *nfeat, uint32_t cap, int kind, ps_feature *out_f,
                               float *out_d, hipStream_t st) {
     if (cap == 0) return hipSuccess;
-    descriptor_kernel<<<cap, 64, 0, st>>>(pv, kps, ori, feat_kp, feat_j, nfeat, cap, kind, out_f, out_d);
+    descriptor_kernel<<<cap, kDescThreads, 0, st>>>(pv, kps, ori, feat_kp, feat_j, nfeat, cap, kind, out_f, out_d);
     return hipGetLastError();
 }
 

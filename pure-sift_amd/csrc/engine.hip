@@ -136,10 +136,10 @@ struct DevBuf {
 struct ps_engine {
     int device = 0;
     hipStream_t stream = nullptr;
-    DevBuf pyr, cand, sorted, ori, nori, feat_kp, feat_j, cnt;
+    DevBuf pyr, cand, rank, sorted, ori, nori, feat_kp, feat_j, cnt;
     DevBuf img, feat, desc, kpout;                 // host-API staging
     DevBuf qf, qd, tf, td, mcnt, mout;             // host-API match staging
-    DevBuf part, part2, bidx, bdist, sdist, ridx, ridx_d;  // matcher scratch
+    DevBuf part, part2, bidx, bdist, sdist, ridx, ridx_d, fin;  // matcher scratch
     bool match_valu = false;                       // PS_MATCH_KERNEL=valu: direct f32 kernel
     ps::PyrPlan plan;
     uint32_t host_counts[4] = {};
@@ -232,9 +232,9 @@ void engine_free(ps_engine *e) {
     if (!e) return;
     prof_release(e);
     (void)hipSetDevice(e->device);
-    DevBuf *bufs[] = {&e->pyr, &e->cand, &e->sorted, &e->ori, &e->nori, &e->feat_kp, &e->feat_j, &e->cnt,
+    DevBuf *bufs[] = {&e->pyr, &e->cand, &e->rank, &e->sorted, &e->ori, &e->nori, &e->feat_kp, &e->feat_j, &e->cnt,
                       &e->img, &e->feat, &e->desc, &e->kpout, &e->qf, &e->qd, &e->tf, &e->td,
-                      &e->mcnt, &e->mout, &e->part, &e->part2, &e->bidx, &e->bdist, &e->sdist, &e->ridx, &e->ridx_d};
+                      &e->mcnt, &e->mout, &e->part, &e->part2, &e->bidx, &e->bdist, &e->sdist, &e->ridx, &e->ridx_d, &e->fin};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -385,7 +385,8 @@ ps_status enqueue_keypoints(ps_engine *e, const void *src, bool src_u8, size_t h
     prof_mark(e, 0);
     PS_TRY(enqueue_pyramid(e, src, src_u8, h, w, stride, cfg));
     prof_mark(e, 1);
-    PS_TRY(ensure(e->cand, (size_t)ps::kCandCap * (sizeof(ps::Cand) + sizeof(uint32_t))));
+    PS_TRY(ensure(e->cand, (size_t)ps::kCandCap * sizeof(ps::Cand)));
+    PS_TRY(ensure(e->rank, ps::rank_scratch_bytes(ps::kCandCap)));
     PS_TRY(ensure(e->sorted, (size_t)limit * sizeof(ps::Cand)));
     ps::PyrPlan &p = e->plan;
     float *P = as<float>(e->pyr);
@@ -396,7 +397,7 @@ ps_status enqueue_keypoints(ps_engine *e, const void *src, bool src_u8, size_t h
     }
     prof_mark(e, 2);
     PS_HIP(ps::launch_rank(as<ps::Cand>(e->cand), cnt + 0, ps::kCandCap, limit, as<ps::Cand>(e->sorted), cnt + 2,
-                           e->stream));
+                           e->rank.p, e->stream));
     prof_mark(e, 3);
     return PS_OK;
 }
@@ -459,8 +460,10 @@ ps_status enqueue_match(ps_engine *e, const ps_feature *qf, const float *qd, con
                           as<float>(e->ridx_d) + ntm));
         rev = as<uint32_t>(e->ridx);
     }
+    if (nqm > 262144) return set_err(PS_E_INVALID_ARG, "at most 262144 query features per match call");
+    PS_TRY(ensure(e->fin, ps::match_finish_scratch_bytes(nqm)));
     PS_HIP(ps::launch_match_finish(qf, tf, nq, nqm, nt, as<uint32_t>(e->bidx), as<float>(e->bdist),
-                                   as<float>(e->sdist), rev, opt.ratio, opt.mutual ? 1 : 0, out, out_count,
+                                   as<float>(e->sdist), rev, opt.ratio, opt.mutual ? 1 : 0, out, out_count, e->fin.p,
                                    e->stream));
     prof_mark(e, 6);
     if (e->prof_on) e->prof_mat++;

@@ -59,12 +59,18 @@ __global__ __launch_bounds__(256) void nn_partial_kernel(const float *qd, const 
 
     for (uint32_t tile = tb; tile < te; tile++) {
         const uint32_t t0 = tile * kTT;
+        float4 v[kTT * 32 / 256];
+#pragma unroll
+        for (int j = 0; j < kTT * 32 / 256; j++) {
+            int i = tid + j * 256, r = i >> 5, c = (i & 31) * 4;
+            v[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (t0 + r < nt) v[j] = *reinterpret_cast<const float4 *>(td + (size_t)(t0 + r) * 128 + c);
+        }
         __syncthreads();
-        for (int i = tid; i < kTT * 32; i += 256) {
-            int r = i >> 5, c = (i & 31) * 4;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (t0 + r < nt) v = *reinterpret_cast<const float4 *>(td + (size_t)(t0 + r) * 128 + c);
-            *reinterpret_cast<float4 *>(st + r * kDP + c) = v;
+#pragma unroll
+        for (int j = 0; j < kTT * 32 / 256; j++) {
+            int i = tid + j * 256, r = i >> 5, c = (i & 31) * 4;
+            *reinterpret_cast<float4 *>(st + r * kDP + c) = v[j];
         }
         __syncthreads();
         float acc[4][4];
@@ -164,60 +170,95 @@ hipError_t launch_nn_merge(const NNPart *part, int nsplit, const uint32_t *nq, s
 }
 
 // Ratio test (pipeline.rs:1788-1805), optional mutual check (pipeline.rs:1815-1840) and
-// ordered compaction of the accepted matches in query order.
+// ordered compaction of the accepted matches in query order: match_flag_kernel decides every
+// query in parallel and counts per 256-query block; match_compact_kernel (one workgroup) scans
+// the block counts and writes the records.
 constexpr int kFinThreads = 1024;
-__global__ __launch_bounds__(kFinThreads) void match_finish_kernel(const ps_feature *qf, const ps_feature *tf,
-                                                                   const uint32_t *nq_p, uint32_t nq_max,
-                                                                   const uint32_t *nt_p, const uint32_t *best_idx,
-                                                                   const float *best, const float *second,
-                                                                   const uint32_t *rev_idx, float ratio_thr,
-                                                                   int mutual, ps_feature_match *out, uint32_t *out_count) {
-    __shared__ uint32_t s[kFinThreads];
+__global__ __launch_bounds__(256) void match_flag_kernel(const uint32_t *nq_p, uint32_t nq_max, const uint32_t *nt_p,
+                                                         const uint32_t *best_idx, const float *best,
+                                                         const float *second, const uint32_t *rev_idx,
+                                                         float ratio_thr, int mutual, uint8_t *flag,
+                                                         uint32_t *block_count) {
+    __shared__ uint32_t s_cnt;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    const uint32_t q = blockIdx.x * 256 + threadIdx.x;
     const uint32_t nq = min(*nq_p, nq_max);
-    const bool nt_ok = *nt_p >= 2;
-    const uint32_t per = (nq_max + kFinThreads - 1) / kFinThreads;
-    const uint32_t b = threadIdx.x * per, e = min(b + per, nq);
-    uint32_t local = 0;
-    for (uint32_t q = b; q < e && nt_ok; q++) {
+    bool acc = false;
+    if (q < nq && *nt_p >= 2) {
         float bd = best[q], sd = second[q];
         if (sd > 1e-12f && isfinite(bd)) {
             float r = sqrtf(bd) / sqrtf(sd);
-            if (r < ratio_thr && (!mutual || rev_idx[best_idx[q]] == q)) local++;
+            acc = r < ratio_thr && (!mutual || rev_idx[best_idx[q]] == q);
         }
     }
-    s[threadIdx.x] = local;
+    if (q < nq_max) flag[q] = acc ? 1 : 0;
+    if (acc) atomicAdd(&s_cnt, 1u);
     __syncthreads();
-    for (int d = 1; d < kFinThreads; d <<= 1) {
-        uint32_t v = threadIdx.x >= (unsigned)d ? s[threadIdx.x - d] : 0u;
-        __syncthreads();
-        s[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t off = s[threadIdx.x] - local;
-    for (uint32_t q = b; q < e && nt_ok; q++) {
-        float bd = best[q], sd = second[q];
-        if (!(sd > 1e-12f && isfinite(bd))) continue;
-        float d1 = sqrtf(bd), d2 = sqrtf(sd);
-        float r = d1 / d2;
-        uint32_t t = best_idx[q];
-        if (!(r < ratio_thr && (!mutual || rev_idx[t] == q))) continue;
-        ps_feature a = qf[q], c = tf[t];
-        ps_feature_match m;
-        m.query_index = q; m.target_index = t;
-        m.qx = a.x; m.qy = a.y; m.tx = c.x; m.ty = c.y;
-        m.distance = d1; m.ratio = r;
-        m.dx = c.x - a.x; m.dy = c.y - a.y;
-        out[off++] = m;
-    }
-    if (threadIdx.x == kFinThreads - 1) *out_count = nt_ok ? s[kFinThreads - 1] : 0u;
+    if (threadIdx.x == 0) block_count[blockIdx.x] = s_cnt;
 }
+
+__global__ __launch_bounds__(kFinThreads) void match_compact_kernel(const ps_feature *qf, const ps_feature *tf,
+                                                                    const uint32_t *nq_p, uint32_t nq_max,
+                                                                    const uint32_t *best_idx, const float *best,
+                                                                    const float *second, const uint8_t *flag,
+                                                                    const uint32_t *block_count, uint32_t nblocks,
+                                                                    ps_feature_match *out, uint32_t *out_count) {
+    __shared__ uint32_t s_off[1024];
+    __shared__ uint32_t s_total;
+    const uint32_t nq = min(*nq_p, nq_max);
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (uint32_t b = 0; b < nblocks; b++) {
+            s_off[b] = acc;
+            acc += block_count[b];
+        }
+        s_total = acc;
+    }
+    __syncthreads();
+    // each group of 256 threads owns one query block at a time: prefix over its flags by ballot
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;   // 16 waves
+    for (uint32_t b = wave >> 2; b < nblocks; b += kFinThreads / 256) {
+        const uint32_t sub = wave & 3;                                  // 64-query quarter of the block
+        // offset of this quarter inside the block
+        uint32_t before = 0;
+        for (uint32_t k = 0; k < sub; k++) {
+            uint32_t qq = b * 256 + k * 64 + lane;
+            before += __popcll(__ballot(qq < nq && flag[qq]));
+        }
+        const uint32_t q = b * 256 + sub * 64 + lane;
+        const bool f = q < nq && flag[q];
+        const uint64_t m = __ballot(f);
+        if (f) {
+            uint32_t pos = s_off[b] + before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            float d1 = sqrtf(best[q]), d2 = sqrtf(second[q]);
+            uint32_t t = best_idx[q];
+            ps_feature a = qf[q], c = tf[t];
+            ps_feature_match r;
+            r.query_index = q; r.target_index = t;
+            r.qx = a.x; r.qy = a.y; r.tx = c.x; r.ty = c.y;
+            r.distance = d1; r.ratio = d1 / d2;
+            r.dx = c.x - a.x; r.dy = c.y - a.y;
+            out[pos] = r;
+        }
+    }
+    if (threadIdx.x == 0) *out_count = s_total;
+}
+
+size_t match_finish_scratch_bytes(size_t nq_max) { return nq_max + ((nq_max + 255) / 256) * 4 + 64; }
 
 hipError_t launch_match_finish(const ps_feature *qf, const ps_feature *tf, const uint32_t *nq, size_t nq_max,
                                const uint32_t *nt, const uint32_t *best_idx, const float *best, const float *second,
-                               const uint32_t *rev_idx, float ratio, int mutual, ps_feature_match *out, uint32_t *out_count,
-                               hipStream_t st) {
-    match_finish_kernel<<<1, kFinThreads, 0, st>>>(qf, tf, nq, (uint32_t)nq_max, nt, best_idx, best, second, rev_idx,
-                                                   ratio, mutual, out, out_count);
+                               const uint32_t *rev_idx, float ratio, int mutual, ps_feature_match *out,
+                               uint32_t *out_count, void *scratch, hipStream_t st) {
+    const uint32_t nblocks = (uint32_t)((nq_max + 255) / 256);
+    if (nblocks > 1024) return hipErrorInvalidValue;
+    uint32_t *block_count = static_cast<uint32_t *>(scratch);
+    uint8_t *flag = reinterpret_cast<uint8_t *>(block_count + nblocks);
+    match_flag_kernel<<<nblocks, 256, 0, st>>>(nq, (uint32_t)nq_max, nt, best_idx, best, second, rev_idx, ratio, mutual,
+                                               flag, block_count);
+    match_compact_kernel<<<1, kFinThreads, 0, st>>>(qf, tf, nq, (uint32_t)nq_max, best_idx, best, second, flag,
+                                                    block_count, nblocks, out, out_count);
     return hipGetLastError();
 }
 

@@ -8,7 +8,7 @@
 // and best/second/index follow the reference's rule. Exactness guard: with E a bound on
 // |d'+|q|^2 - exact| for any pair, every non-candidate t has exact(t) >= d'_4 + |q|^2 - E;
 // if that exceeds the exact second-best the result equals the full scan's. Otherwise the
-// query is flagged and pass 3 (nn_exact_flagged_kernel) scans all targets exactly.
+// query is flagged and pass 3 (nn_fallback_*) scans all targets exactly.
 #include "ps_internal.h"
 
 #pragma clang fp contract(off)
@@ -23,21 +23,25 @@ constexpr int kMP = 132;   // LDS row stride in floats
 constexpr uint32_t kNoIdx = 0xffffffffu;
 
 // |row|^2 of each descriptor (any summation order: only feeds the approximate distance and
-// the error bound) and the maximum over rows as float bits.
+// the error bound) and the maximum over rows as float bits. 8 lanes per row.
 __global__ void row_norms_kernel(const float *d, const uint32_t *n_p, uint32_t n_max, float *out,
                                  uint32_t *max_bits) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t r = gid >> 3, part = gid & 7;
     const uint32_t n = min(*n_p, n_max);
     float s = 0.0f;
     if (r < n) {
-        const float4 *p = reinterpret_cast<const float4 *>(d + (size_t)r * 128);
-        for (int i = 0; i < 32; i++) {
-            float4 v = p[i];
-            s += v.x * v.x + v.y * v.y + v.z * v.z + v.w * v.w;
-        }
+        const float4 *p = reinterpret_cast<const float4 *>(d + (size_t)r * 128 + part * 16);
+        float4 a = p[0], b = p[1], c = p[2], e = p[3];
+        s = a.x * a.x + a.y * a.y + a.z * a.z + a.w * a.w + b.x * b.x + b.y * b.y + b.z * b.z + b.w * b.w +
+            c.x * c.x + c.y * c.y + c.z * c.z + c.w * c.w + e.x * e.x + e.y * e.y + e.z * e.z + e.w * e.w;
+    }
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    s += __shfl_xor(s, 4);
+    if (r < n && part == 0) {
         out[r] = s;
-        if (isfinite(s)) atomicMax(max_bits, __float_as_uint(s));
-        else atomicMax(max_bits, 0x7f800000u);
+        atomicMax(max_bits, isfinite(s) ? __float_as_uint(s) : 0x7f800000u);
     }
 }
 
@@ -54,17 +58,21 @@ __device__ __forceinline__ void top4_insert(float d, uint32_t t, float (&k)[4], 
 }
 
 // (d, idx) lexicographic insertion for merging lists whose indices are not monotone
+// (fully unrolled so the lists stay in registers)
+__device__ __forceinline__ bool lex_less(float a, uint32_t ia, float b, uint32_t ib) {
+    return a < b || (a == b && ia < ib);
+}
 __device__ __forceinline__ void top4_insert_lex(float d, uint32_t t, float (&k)[4], uint32_t (&ix)[4]) {
-    auto less = [](float a, uint32_t ia, float b, uint32_t ib) { return a < b || (a == b && ia < ib); };
-    if (!less(d, t, k[3], ix[3])) return;
-    int p = 3;
-    while (p > 0 && less(d, t, k[p - 1], ix[p - 1])) {
-        k[p] = k[p - 1];
-        ix[p] = ix[p - 1];
-        p--;
+    if (!lex_less(d, t, k[3], ix[3])) return;
+    k[3] = d;
+    ix[3] = t;
+#pragma unroll
+    for (int p = 3; p > 0; p--) {
+        if (lex_less(k[p], ix[p], k[p - 1], ix[p - 1])) {
+            float td = k[p]; k[p] = k[p - 1]; k[p - 1] = td;
+            uint32_t ti = ix[p]; ix[p] = ix[p - 1]; ix[p - 1] = ti;
+        }
     }
-    k[p] = d;
-    ix[p] = t;
 }
 
 struct Cand4 {
@@ -105,14 +113,21 @@ __global__ __launch_bounds__(256) void nn_mfma_kernel(const float *qd, const uin
     const uint32_t tb = blockIdx.y * per, te = min(tb + per, ntiles);
     for (uint32_t tile = tb; tile < te; tile++) {
         const uint32_t t0 = tile * kMT;
-        __syncthreads();
-        for (int i = tid; i < kMT * 32; i += 256) {
-            int r = i >> 5, c = (i & 31) * 4;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (t0 + r < nt) v = *reinterpret_cast<const float4 *>(td + (size_t)(t0 + r) * 128 + c);
-            *reinterpret_cast<float4 *>(s_t + r * kMP + c) = v;
+        float4 v[kMT * 32 / 256];
+#pragma unroll
+        for (int j = 0; j < kMT * 32 / 256; j++) {
+            int i = tid + j * 256, r = i >> 5, c = (i & 31) * 4;
+            v[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (t0 + r < nt) v[j] = *reinterpret_cast<const float4 *>(td + (size_t)(t0 + r) * 128 + c);
         }
-        if (tid < kMT) s_tn[tid] = (t0 + tid < nt) ? tn[t0 + tid] : 0.0f;
+        float tnv = (tid < kMT && t0 + tid < nt) ? tn[t0 + tid] : 0.0f;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kMT * 32 / 256; j++) {
+            int i = tid + j * 256, r = i >> 5, c = (i & 31) * 4;
+            *reinterpret_cast<float4 *>(s_t + r * kMP + c) = v[j];
+        }
+        if (tid < kMT) s_tn[tid] = tnv;
         __syncthreads();
         floatx16 acc0 = {}, acc1 = {};
         const float *r0 = s_t + col * kMP + half * 64;
@@ -177,29 +192,61 @@ __device__ __forceinline__ float exact_dist(const float *a, const float *b) {
     return sum;
 }
 
-__global__ void nn_rescore_kernel(const Cand4 *part, int nsplit, const float *qd, const float *qn,
-                                  const uint32_t *nq_p, uint32_t nq_max, const float *td, const uint32_t *nt_p,
-                                  const uint32_t *tmax_bits, uint32_t *best_idx, float *best, float *second,
-                                  uint32_t *flag) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= min(*nq_p, nq_max)) return;
+// One thread per (query, candidate): the 4 exact distances of a query are computed by 4
+// adjacent lanes (each a 128-step sequential sum) and combined with shuffles.
+__global__ __launch_bounds__(256) void nn_rescore_kernel(const Cand4 *part, int nsplit, const float *qd, const float *qn,
+                                                         const uint32_t *nq_p, uint32_t nq_max, const float *td,
+                                                         const uint32_t *nt_p, const uint32_t *tmax_bits,
+                                                         uint32_t *best_idx, float *best, float *second,
+                                                         uint32_t *flag_list, uint32_t *flag_count) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t q = gid >> 2;
+    const int j = gid & 3;
+    const bool active = q < min(*nq_p, nq_max);
     const uint32_t nt = *nt_p;
     float kd[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
     uint32_t ki[4] = {kNoIdx, kNoIdx, kNoIdx, kNoIdx};
-    for (int s = 0; s < nsplit; s++) {
-        Cand4 c = part[(size_t)s * nq_max + q];
-        for (int j = 0; j < 4; j++)
-            if (c.i[j] != kNoIdx) top4_insert_lex(c.d[j], c.i[j], kd, ki);
+    if (active) {
+        // the per-split lists are loaded 8 at a time so their loads overlap
+        for (int s0 = 0; s0 < nsplit; s0 += 8) {
+            Cand4 c[8];
+#pragma unroll
+            for (int v = 0; v < 8; v++)
+                if (s0 + v < nsplit) c[v] = part[(size_t)(s0 + v) * nq_max + q];
+#pragma unroll
+            for (int v = 0; v < 8; v++) {
+                if (s0 + v >= nsplit) break;
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (c[v].i[u] != kNoIdx) top4_insert_lex(c[v].d[u], c[v].i[u], kd, ki);
+            }
+        }
     }
+    uint32_t my = kNoIdx;
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+        if (u == j) my = ki[u];
+    float d = INFINITY;
+    if (active && my != kNoIdx) d = exact_dist(qd + (size_t)q * 128, td + (size_t)my * 128);
+    // gather the 4 (d, idx) of this query into every one of its lanes
+    float dv[4];
+    uint32_t iv[4];
+    const int base = (threadIdx.x & 63) & ~3;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        dv[u] = __shfl(d, base + u);
+        iv[u] = __shfl(my, base + u);
+    }
+    if (!active || j != 0) return;
     float b = INFINITY, sc = INFINITY;
     uint32_t bi = 0;
     int ncand = 0;
-    for (int j = 0; j < 4; j++) {
-        if (ki[j] == kNoIdx) continue;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        if (iv[u] == kNoIdx) continue;
         ncand++;
-        float d = exact_dist(qd + (size_t)q * 128, td + (size_t)ki[j] * 128);
-        if (d < b || (d == b && ki[j] < bi)) { sc = b; b = d; bi = ki[j]; }
-        else if (d < sc) sc = d;
+        if (dv[u] < b || (dv[u] == b && iv[u] < bi)) { sc = b; b = dv[u]; bi = iv[u]; }
+        else if (dv[u] < sc) sc = dv[u];
     }
     // exactness guard (see header comment); all targets are candidates when nt <= 4
     bool ok = ncand >= (int)min(nt, 4u);
@@ -211,43 +258,74 @@ __global__ void nn_rescore_kernel(const Cand4 *part, int nsplit, const float *qd
     best_idx[q] = bi;
     best[q] = b;
     second[q] = sc;
-    flag[q] = ok ? 0u : 1u;
+    if (!ok) flag_list[atomicAdd(flag_count, 1u)] = q;
 }
 
-// Exact full scan for the flagged queries: one workgroup per query.
-__global__ __launch_bounds__(256) void nn_exact_flagged_kernel(const float *qd, const uint32_t *nq_p, uint32_t nq_max,
-                                                               const float *td, const uint32_t *nt_p, uint32_t nt_max,
-                                                               const uint32_t *flag, uint32_t *best_idx, float *best,
-                                                               float *second) {
+// Exact scan for flagged queries over a fixed 64 x 16 grid: workgroup (a, b) takes flagged
+// queries a, a+64, ... and targets b*256 + tid (+ 4096 strides), one target per thread, then
+// reduces its 256 partial states; nn_fallback_merge_kernel combines the 16 splits in order.
+constexpr int kFbQ = 64, kFbT = 16;
+__global__ __launch_bounds__(256) void nn_fallback_partial_kernel(const float *qd, const uint32_t *nq_p,
+                                                                  uint32_t nq_max, const float *td,
+                                                                  const uint32_t *nt_p, uint32_t nt_max,
+                                                                  const uint32_t *flag_list,
+                                                                  const uint32_t *flag_count, NNPart *fb) {
     __shared__ float sb[256], ss[256];
     __shared__ uint32_t si[256];
-    const uint32_t q = blockIdx.x;
-    if (q >= min(*nq_p, nq_max) || flag[q] == 0) return;
+    const uint32_t nflag = min(*flag_count, min(*nq_p, nq_max));
     const uint32_t nt = min(*nt_p, nt_max);
-    float b = INFINITY, s = INFINITY;
-    uint32_t bi = 0;
-    for (uint32_t t = threadIdx.x; t < nt; t += 256) {
-        float d = exact_dist(qd + (size_t)q * 128, td + (size_t)t * 128);
-        if (d < b) { s = b; b = d; bi = t; }
-        else if (d < s) s = d;
-    }
-    sb[threadIdx.x] = b; ss[threadIdx.x] = s; si[threadIdx.x] = bi;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float B = sb[0], S = ss[0];
-        uint32_t BI = si[0];
-        for (int k = 1; k < 256; k++) {
-            float b2 = sb[k], s2 = ss[k];
-            uint32_t i2 = si[k];
-            float other;
-            if (b2 < B || (b2 == B && i2 < BI)) { other = B; B = b2; BI = i2; }
-            else other = b2;
-            S = fminf(fminf(S, s2), other);
+    for (uint32_t f = blockIdx.x; f < nflag; f += kFbQ) {
+        const uint32_t q = flag_list[f];
+        float b = INFINITY, s = INFINITY;
+        uint32_t bi = 0;
+        for (uint32_t t = blockIdx.y * 256 + threadIdx.x; t < nt; t += kFbT * 256) {
+            float d = exact_dist(qd + (size_t)q * 128, td + (size_t)t * 128);
+            if (d < b) { s = b; b = d; bi = t; }
+            else if (d < s) s = d;
         }
-        best_idx[q] = BI;
-        best[q] = B;
-        second[q] = S;
+        __syncthreads();
+        sb[threadIdx.x] = b; ss[threadIdx.x] = s; si[threadIdx.x] = bi;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {   // tree reduction with the order-independent rule
+            if (threadIdx.x < (unsigned)w) {
+                float B = sb[threadIdx.x], S = ss[threadIdx.x];
+                uint32_t BI = si[threadIdx.x];
+                float b2 = sb[threadIdx.x + w], s2 = ss[threadIdx.x + w];
+                uint32_t i2 = si[threadIdx.x + w];
+                float other;
+                if (b2 < B || (b2 == B && i2 < BI)) { other = B; B = b2; BI = i2; }
+                else other = b2;
+                sb[threadIdx.x] = B; ss[threadIdx.x] = fminf(fminf(S, s2), other); si[threadIdx.x] = BI;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            NNPart p;
+            p.best = sb[0]; p.second = ss[0]; p.idx = si[0]; p.pad = 0;
+            fb[(size_t)f * kFbT + blockIdx.y] = p;
+        }
     }
+}
+
+__global__ void nn_fallback_merge_kernel(const NNPart *fb, const uint32_t *flag_list, const uint32_t *flag_count,
+                                         const uint32_t *nq_p, uint32_t nq_max, uint32_t *best_idx, float *best,
+                                         float *second) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= min(*flag_count, min(*nq_p, nq_max))) return;
+    NNPart p = fb[(size_t)f * kFbT];
+    float B = p.best, S = p.second;
+    uint32_t BI = p.idx;
+    for (int k = 1; k < kFbT; k++) {
+        NNPart o = fb[(size_t)f * kFbT + k];
+        float other;
+        if (o.best < B || (o.best == B && o.idx < BI)) { other = B; B = o.best; BI = o.idx; }
+        else other = o.best;
+        S = fminf(fminf(S, o.second), other);
+    }
+    const uint32_t q = flag_list[f];
+    best_idx[q] = BI;
+    best[q] = B;
+    second[q] = S;
 }
 
 int nn_mfma_splits(size_t nq_max, size_t nt_max) {
@@ -261,10 +339,12 @@ int nn_mfma_splits(size_t nq_max, size_t nt_max) {
 
 size_t nn_mfma_scratch_bytes(size_t nq_max, size_t nt_max) {
     size_t ns = (size_t)nn_mfma_splits(nq_max, nt_max);
-    return ns * nq_max * sizeof(Cand4) + (nq_max + nt_max) * sizeof(float) + nq_max * sizeof(uint32_t) + 256;
+    return ns * nq_max * sizeof(Cand4) + (nq_max + nt_max) * sizeof(float) + nq_max * sizeof(uint32_t) +
+           nq_max * kFbT * sizeof(NNPart) + 256;
 }
 
-// Scratch layout: [part: ns*nq_max Cand4][qn: nq_max][tn: nt_max][flag: nq_max][tmax: 1]
+// Scratch layout: [part: ns*nq_max Cand4][fb: nq_max*16 NNPart][qn: nq_max][tn: nt_max]
+// [flag_list: nq_max][counters: tmax, qmax, flag_count]
 hipError_t launch_nn_mfma(const float *qd, const uint32_t *nq, size_t nq_max, const float *td, const uint32_t *nt,
                           size_t nt_max, void *scratch, uint32_t *best_idx, float *best, float *second,
                           hipStream_t st) {
@@ -273,24 +353,31 @@ hipError_t launch_nn_mfma(const float *qd, const uint32_t *nq, size_t nq_max, co
     char *p = static_cast<char *>(scratch);
     Cand4 *part = reinterpret_cast<Cand4 *>(p);
     p += (size_t)ns * nq_max * sizeof(Cand4);
+    NNPart *fb = reinterpret_cast<NNPart *>(p);
+    p += nq_max * kFbT * sizeof(NNPart);
     float *qn = reinterpret_cast<float *>(p);
     p += nq_max * sizeof(float);
     float *tn = reinterpret_cast<float *>(p);
     p += nt_max * sizeof(float);
-    uint32_t *flag = reinterpret_cast<uint32_t *>(p);
+    uint32_t *flag_list = reinterpret_cast<uint32_t *>(p);
     p += nq_max * sizeof(uint32_t);
     uint32_t *tmax = reinterpret_cast<uint32_t *>(p);
-    hipError_t e = launch_fill_u32(tmax, 0u, 2, st);
-    if (e != hipSuccess) return e;
     uint32_t *qmax = tmax + 1;
-    row_norms_kernel<<<(unsigned)((nq_max + 255) / 256), 256, 0, st>>>(qd, nq, (uint32_t)nq_max, qn, qmax);
-    row_norms_kernel<<<(unsigned)((nt_max + 255) / 256), 256, 0, st>>>(td, nt, (uint32_t)nt_max, tn, tmax);
+    uint32_t *flag_count = tmax + 2;
+    hipError_t e = launch_fill_u32(tmax, 0u, 3, st);
+    if (e != hipSuccess) return e;
+    row_norms_kernel<<<(unsigned)((8 * nq_max + 255) / 256), 256, 0, st>>>(qd, nq, (uint32_t)nq_max, qn, qmax);
+    row_norms_kernel<<<(unsigned)((8 * nt_max + 255) / 256), 256, 0, st>>>(td, nt, (uint32_t)nt_max, tn, tmax);
     dim3 grid((unsigned)((nq_max + kMQ - 1) / kMQ), (unsigned)ns);
     nn_mfma_kernel<<<grid, 256, 0, st>>>(qd, nq, (uint32_t)nq_max, td, tn, nt, (uint32_t)nt_max, ns, part);
-    nn_rescore_kernel<<<(unsigned)((nq_max + 255) / 256), 256, 0, st>>>(part, ns, qd, qn, nq, (uint32_t)nq_max, td, nt,
-                                                                        tmax, best_idx, best, second, flag);
-    nn_exact_flagged_kernel<<<(unsigned)nq_max, 256, 0, st>>>(qd, nq, (uint32_t)nq_max, td, nt, (uint32_t)nt_max, flag,
-                                                              best_idx, best, second);
+    nn_rescore_kernel<<<(unsigned)((4 * nq_max + 255) / 256), 256, 0, st>>>(part, ns, qd, qn, nq, (uint32_t)nq_max, td,
+                                                                            nt, tmax, best_idx, best, second,
+                                                                            flag_list, flag_count);
+    nn_fallback_partial_kernel<<<dim3(kFbQ, kFbT), 256, 0, st>>>(qd, nq, (uint32_t)nq_max, td, nt, (uint32_t)nt_max,
+                                                                 flag_list, flag_count, fb);
+    nn_fallback_merge_kernel<<<(unsigned)((nq_max + 255) / 256), 256, 0, st>>>(fb, flag_list, flag_count, nq,
+                                                                               (uint32_t)nq_max, best_idx, best,
+                                                                               second);
     return hipGetLastError();
 }
 

@@ -80,8 +80,9 @@ hipError_t launch_extrema(const float *dog_octave, int h, int w, int S, float th
                           uint32_t cap, uint32_t *err, hipStream_t st);
 // Orders candidates: scan order if n <= limit, else (|response| desc, scan order); writes the
 // first min(n, limit) into out and that count into out_count.
+size_t rank_scratch_bytes(uint32_t cap_in);
 hipError_t launch_rank(const Cand *cand, const uint32_t *count, uint32_t cap_in, uint32_t limit,
-                       Cand *out, uint32_t *out_count, hipStream_t st);
+                       Cand *out, uint32_t *out_count, void *scratch, hipStream_t st);
 
 // describe.hip
 hipError_t launch_orientations(const PyrView &pv, const Cand *kps, const uint32_t *nkp, uint32_t max_kp,
@@ -104,10 +105,12 @@ hipError_t launch_nn_partial(const float *qd, const uint32_t *nq, size_t nq_max,
                              const uint32_t *nt, size_t nt_max, int nsplit, NNPart *part, hipStream_t st);
 hipError_t launch_nn_merge(const NNPart *part, int nsplit, const uint32_t *nq, size_t nq_max, uint32_t *best_idx,
                            float *best, float *second, hipStream_t st);
+// nq_max <= 262144 (1024 blocks of 256 queries)
+size_t match_finish_scratch_bytes(size_t nq_max);
 hipError_t launch_match_finish(const ps_feature *qf, const ps_feature *tf, const uint32_t *nq, size_t nq_max,
                                const uint32_t *nt, const uint32_t *best_idx, const float *best,
                                const float *second, const uint32_t *rev_idx, float ratio, int mutual,
-                               ps_feature_match *out, uint32_t *out_count, hipStream_t st);
+                               ps_feature_match *out, uint32_t *out_count, void *scratch, hipStream_t st);
 
 // match_mfma.hip: exact NN/2NN through an FP32 MFMA distance GEMM + exact rescoring
 int nn_mfma_splits(size_t nq_max, size_t nt_max);

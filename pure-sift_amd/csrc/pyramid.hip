@@ -63,11 +63,25 @@ __global__ __launch_bounds__(kBlurThreads) void blur_kernel(BlurArgs a, Taps tap
     const int x0 = blockIdx.x * kTW, y0 = blockIdx.y * kTH;
     const int h = a.h, w = a.w;
 
-    for (int i = tid; i < IH * IW; i += kBlurThreads) {
-        int r = i / IW, c = i - r * IW;
-        int gy = min(max(y0 - R + r, 0), h - 1);
-        int gx = min(max(x0 - R + c, 0), w - 1);
-        s_in[r * IWP + c] = load_src<SRC>(a, gy, gx);
+    // stage the clamped window: all loads of a thread are issued before its LDS stores so
+    // they overlap instead of paying one memory latency each
+    constexpr int NL = (IH * IW + kBlurThreads - 1) / kBlurThreads;
+    {
+        float v[NL];
+#pragma unroll
+        for (int j = 0; j < NL; j++) {
+            int i = tid + j * kBlurThreads;
+            int r = i / IW, c = i - r * IW;
+            int gy = min(max(y0 - R + r, 0), h - 1);
+            int gx = min(max(x0 - R + c, 0), w - 1);
+            v[j] = (i < IH * IW) ? load_src<SRC>(a, gy, gx) : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < NL; j++) {
+            int i = tid + j * kBlurThreads;
+            int r = i / IW, c = i - r * IW;
+            if (i < IH * IW) s_in[r * IWP + c] = v[j];
+        }
     }
     __syncthreads();
 
@@ -247,60 +261,129 @@ __device__ __forceinline__ void refine3d(const float *ps, const float *cs, const
     odx = 0.0f; ody = 0.0f; ods = 0.0f; oi = center;
 }
 
-// One thread per interior pixel of one searchable slice (pipeline.rs:841-897). Accepted
-// keypoints are appended with their scan-order key; launch_rank restores the order.
+// 64 x 8 interior pixels per 256-thread workgroup: all PL = S+2 DoG planes of the tile plus a
+// one-pixel halo are staged in LDS (each plane read from HBM once, every load of a thread in
+// flight together). Phase 1: each thread tests its 2 pixels x S slices against the contrast
+// threshold and the 8 in-plane neighbours (the cheap tests that reject almost everything) and
+// queues survivors in LDS. Phase 2: the queue is processed by all lanes in parallel -- the 18
+// cross-scale neighbours, the Hessian edge test and the Taylor refinement -- so no wave idles
+// while a few lanes run the expensive path.
+constexpr int kExRows = 8;
+constexpr int kExW = 66;
+constexpr int kExH = kExRows + 2;
+template <int PL>
 __global__ __launch_bounds__(256) void extrema_kernel(const float *dog, int h, int w, float thr, float mult,
-                                                      EffScale eff, double log2k, int S, int octave, Cand *out,
+                                                      EffScale eff, double log2k, int octave, Cand *out,
                                                       uint32_t *count, uint32_t cap, uint32_t *err) {
-    const int x = 1 + blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int s = blockIdx.z + 1;
-    if (x >= w - 1 || y >= h - 1) return;
+    constexpr int S = PL - 2;
+    constexpr int per_plane = kExH * kExW;
+    constexpr int NL = (per_plane + 255) / 256;
+    __shared__ float s_dog[PL * per_plane];
+    __shared__ uint32_t s_q[64 * kExRows * S];
+    __shared__ uint32_t s_qn;
+    const int x0 = 1 + blockIdx.x * 64, y0 = 1 + blockIdx.y * kExRows;
     const size_t P = (size_t)h * w;
-    const float *ps = dog + (size_t)(s - 1) * P, *cs = dog + (size_t)s * P, *ns = dog + (size_t)(s + 1) * P;
-    const float val = cs[(size_t)y * w + x];
-    if (fabsf(val) <= thr) return;
-    bool ext = true;
-    if (val > 0.0f) {
+    if (threadIdx.x == 0) s_qn = 0;
+    {
+        float v[PL][NL];
 #pragma unroll
-        for (int dy = -1; dy <= 1; dy++)
+        for (int j = 0; j < NL; j++) {
+            int i = threadIdx.x + j * 256;
+            int r = i / kExW, c = i - r * kExW;
+            size_t o = (size_t)min(y0 - 1 + r, h - 1) * w + min(x0 - 1 + c, w - 1);
 #pragma unroll
-            for (int dx = -1; dx <= 1; dx++) {
-                size_t o = (size_t)(y + dy) * w + (x + dx);
-                if (dx != 0 || dy != 0) ext = ext && (val > cs[o]);
-                ext = ext && (val > ps[o]) && (val > ns[o]);
-            }
-    } else {
+            for (int pl = 0; pl < PL; pl++) v[pl][j] = i < per_plane ? dog[(size_t)pl * P + o] : 0.0f;
+        }
 #pragma unroll
-        for (int dy = -1; dy <= 1; dy++)
+        for (int j = 0; j < NL; j++) {
+            int i = threadIdx.x + j * 256;
 #pragma unroll
-            for (int dx = -1; dx <= 1; dx++) {
-                size_t o = (size_t)(y + dy) * w + (x + dx);
-                if (dx != 0 || dy != 0) ext = ext && (val < cs[o]);
-                ext = ext && (val < ps[o]) && (val < ns[o]);
-            }
+            for (int pl = 0; pl < PL; pl++)
+                if (i < per_plane) s_dog[pl * per_plane + i] = v[pl][j];
+        }
     }
-    if (!ext || !hessian_edge_ok(cs, w, x, y, val)) return;
-    float dx, dy, ds, interp;
-    refine3d(ps, cs, ns, w, x, y, dx, dy, ds, interp);
-    if (!(fabsf(interp) > thr)) return;
-    // k.powf(ds): evaluated in f64 and rounded once (glibc powf agrees except in ~0.06% of
-    // arguments, where the two differ by one ulp)
-    float kpow = (float)exp2((double)ds * log2k);
-    float refined_scale = eff.v[s] * kpow;
-    float lf = roundf((float)s + ds);   // pipeline.rs:882-885
-    lf = lf < 1.0f ? 1.0f : (lf > (float)S ? (float)S : lf);
-    Cand c;
-    c.x = ((float)x + dx) * mult;
-    c.y = ((float)y + dy) * mult;
-    c.scale = fmaxf(refined_scale, 0.25f * 1.6f);
-    c.response = interp;
-    c.octave = (uint32_t)octave;
-    c.layer = (uint32_t)lf;
-    c.key = ((uint64_t)octave << 56) | ((uint64_t)s << 48) | ((uint64_t)y << 24) | (uint64_t)x;
-    uint32_t i = atomicAdd(count, 1u);
-    if (i < cap) out[i] = c;
-    else atomicOr(err, kErrCandOverflow);
+    __syncthreads();
+    // phase 1
+    {
+        const int tx = threadIdx.x & 63, lx = tx + 1;
+        const int ly0 = 1 + (threadIdx.x >> 6) * (kExRows / 4);
+        if (x0 + tx < w - 1) {
+            for (int j = 0; j < kExRows / 4; j++) {
+                const int ly = ly0 + j;
+                if (y0 + ly - 1 >= h - 1) break;
+#pragma unroll
+                for (int s = 1; s <= S; s++) {
+                    const float *cs = s_dog + s * per_plane;
+                    const float val = cs[ly * kExW + lx];
+                    if (fabsf(val) <= thr) continue;
+                    bool ext = true;
+                    if (val > 0.0f) {
+#pragma unroll
+                        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+                            for (int dx = -1; dx <= 1; dx++)
+                                if (dx != 0 || dy != 0) ext = ext && (val > cs[(ly + dy) * kExW + lx + dx]);
+                    } else {
+#pragma unroll
+                        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+                            for (int dx = -1; dx <= 1; dx++)
+                                if (dx != 0 || dy != 0) ext = ext && (val < cs[(ly + dy) * kExW + lx + dx]);
+                    }
+                    if (ext) s_q[atomicAdd(&s_qn, 1u)] = ((uint32_t)s << 16) | ((uint32_t)ly << 8) | (uint32_t)lx;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // phase 2 (pipeline.rs:852-893)
+    const uint32_t nq = s_qn;
+    for (uint32_t qi = threadIdx.x; qi < nq; qi += 256) {
+        const uint32_t e = s_q[qi];
+        const int s = (int)(e >> 16), ly = (int)((e >> 8) & 255u), lx = (int)(e & 255u);
+        const int x = x0 + lx - 1, y = y0 + ly - 1;
+        const float *ps = s_dog + (s - 1) * per_plane, *cs = s_dog + s * per_plane, *ns = s_dog + (s + 1) * per_plane;
+        const float val = cs[ly * kExW + lx];
+        bool ext = true;
+        if (val > 0.0f) {
+#pragma unroll
+            for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+                for (int dx = -1; dx <= 1; dx++) {
+                    int o = (ly + dy) * kExW + lx + dx;
+                    ext = ext && (val > ps[o]) && (val > ns[o]);
+                }
+        } else {
+#pragma unroll
+            for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+                for (int dx = -1; dx <= 1; dx++) {
+                    int o = (ly + dy) * kExW + lx + dx;
+                    ext = ext && (val < ps[o]) && (val < ns[o]);
+                }
+        }
+        if (!ext || !hessian_edge_ok(cs, kExW, lx, ly, val)) continue;
+        float dx, dy, ds, interp;
+        refine3d(ps, cs, ns, kExW, lx, ly, dx, dy, ds, interp);
+        if (!(fabsf(interp) > thr)) continue;
+        // k.powf(ds): evaluated in f64 and rounded once (glibc powf agrees except in ~0.06% of
+        // arguments, where the two differ by one ulp)
+        float kpow = (float)exp2((double)ds * log2k);
+        float refined_scale = eff.v[s] * kpow;
+        float lf = roundf((float)s + ds);   // pipeline.rs:882-885
+        lf = lf < 1.0f ? 1.0f : (lf > (float)S ? (float)S : lf);
+        Cand c;
+        c.x = ((float)x + dx) * mult;
+        c.y = ((float)y + dy) * mult;
+        c.scale = fmaxf(refined_scale, 0.25f * 1.6f);
+        c.response = interp;
+        c.octave = (uint32_t)octave;
+        c.layer = (uint32_t)lf;
+        c.key = ((uint64_t)octave << 56) | ((uint64_t)s << 48) | ((uint64_t)y << 24) | (uint64_t)x;
+        uint32_t i = atomicAdd(count, 1u);
+        if (i < cap) out[i] = c;
+        else atomicOr(err, kErrCandOverflow);
+    }
 }
 
 hipError_t launch_extrema(const float *dog_octave, int h, int w, int S, float thr, float mult, const float *eff_scale,
@@ -309,59 +392,32 @@ hipError_t launch_extrema(const float *dog_octave, int h, int w, int S, float th
     if (h < 3 || w < 3 || S < 1) return hipSuccess;
     EffScale e;
     for (int i = 0; i < kMaxLevels; i++) e.v[i] = eff_scale[i];
-    dim3 grid((w - 2 + 63) / 64, (h - 2 + 3) / 4, S);
-    extrema_kernel<<<grid, 256, 0, st>>>(dog_octave, h, w, thr, mult, e, log2k, S, octave, out, count, cap, err);
+    dim3 grid((w - 2 + 63) / 64, (h - 2 + kExRows - 1) / kExRows);
+#define PS_EXTREMA_CASE(SS)                                                                                   \
+    case SS:                                                                                                  \
+        extrema_kernel<SS + 2><<<grid, 256, 0, st>>>(dog_octave, h, w, thr, mult, e, log2k, octave, out, count, \
+                                                     cap, err);                                                \
+        break;
+    switch (S) {
+        PS_EXTREMA_CASE(3) PS_EXTREMA_CASE(4) PS_EXTREMA_CASE(5) PS_EXTREMA_CASE(6)
+        PS_EXTREMA_CASE(7) PS_EXTREMA_CASE(8) PS_EXTREMA_CASE(9) PS_EXTREMA_CASE(10)
+    default: return hipErrorInvalidValue;
+    }
+#undef PS_EXTREMA_CASE
     return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ ordering + cap
-// rank[i] = #{j : j precedes i}, where "precedes" is scan order when n <= limit (no
-// sort, pipeline.rs:1643) and otherwise the stable |response|-descending order of
-// pipeline.rs:1644-1649 (ties keep scan order). A fixed 64x64 grid of workgroups walks
-// the (i-tile, j-tile) pairs; partial counts are summed with integer atomics, so the
-// result is exact and independent of scheduling.
+// Output order (pipeline.rs:1643-1651): scan order when n <= limit (no sort); otherwise the
+// stable |response|-descending sort truncated to `limit`, i.e. the order
+// (|response| desc, scan key asc). For n > limit an exact radix select (four 8-bit passes
+// over the |response| bit patterns, which order like the positive floats) finds T, the
+// limit-th largest |response|; only candidates with |response| >= T (limit plus ties) are
+// ranked. rank[i] = #{j in subset : j precedes i}; a fixed 64x64 grid of workgroups walks
+// (i-tile, j-tile) pairs of the subset and adds integer partial counts, so the result is
+// exact and independent of scheduling.
 constexpr int kRankTile = 256;
 constexpr int kRankGrid = 64;
-
-__global__ __launch_bounds__(kRankTile) void rank_count_kernel(const Cand *c, const uint32_t *count, uint32_t cap_in,
-                                                               uint32_t limit, uint32_t *rank) {
-    __shared__ float s_r[kRankTile];
-    __shared__ uint64_t s_k[kRankTile];
-    const uint32_t n = min(*count, cap_in);
-    const bool by_resp = n > limit;
-    const uint32_t ntile = (n + kRankTile - 1) / kRankTile;
-    for (uint32_t ti = blockIdx.x; ti < ntile; ti += kRankGrid) {
-        const uint32_t i = ti * kRankTile + threadIdx.x;
-        float ri = 0.0f;
-        uint64_t ki = 0;
-        if (i < n) { ri = fabsf(c[i].response); ki = c[i].key; }
-        uint32_t acc = 0;
-        for (uint32_t tj = blockIdx.y; tj < ntile; tj += kRankGrid) {
-            const uint32_t j = tj * kRankTile + threadIdx.x;
-            __syncthreads();
-            if (j < n) { s_r[threadIdx.x] = fabsf(c[j].response); s_k[threadIdx.x] = c[j].key; }
-            __syncthreads();
-            const uint32_t m = min((uint32_t)kRankTile, n - tj * kRankTile);
-            if (by_resp) {
-                for (uint32_t t = 0; t < m; t++) {
-                    float rj = s_r[t];
-                    acc += (rj > ri || (rj == ri && s_k[t] < ki)) ? 1u : 0u;
-                }
-            } else {
-                for (uint32_t t = 0; t < m; t++) acc += s_k[t] < ki ? 1u : 0u;
-            }
-        }
-        if (i < n && acc) atomicAdd(&rank[i], acc);
-    }
-}
-
-__global__ void rank_scatter_kernel(const Cand *c, const uint32_t *count, uint32_t cap_in, uint32_t limit,
-                                    const uint32_t *rank, Cand *out, uint32_t *out_count) {
-    const uint32_t n = min(*count, cap_in);
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *out_count = min(n, limit);
-    if (i < n && rank[i] < limit) out[rank[i]] = c[i];
-}
 
 __global__ void fill_upto_kernel(uint32_t *p, const uint32_t *count, uint32_t cap) {
     const uint32_t n = min(*count, cap);
@@ -369,14 +425,136 @@ __global__ void fill_upto_kernel(uint32_t *p, const uint32_t *count, uint32_t ca
     if (i < n) p[i] = 0;
 }
 
+// scratch words: [0] subset size, [1] threshold bits T, [2] by-response flag
+__global__ __launch_bounds__(1024) void rank_select_kernel(const Cand *c, const uint32_t *count, uint32_t cap_in,
+                                                           uint32_t limit, uint32_t *words) {
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t s_prefix, s_need;
+    const uint32_t n = min(*count, cap_in);
+    if (n <= limit) {
+        if (threadIdx.x == 0) { words[1] = 0; words[2] = 0; }
+        return;
+    }
+    if (threadIdx.x == 0) { s_prefix = 0; s_need = limit; }
+    uint32_t mask = 0;
+    for (int pass = 0; pass < 4; pass++) {
+        const int shift = 24 - 8 * pass;
+        if (threadIdx.x < 256) hist[threadIdx.x] = 0;
+        __syncthreads();
+        const uint32_t prefix = s_prefix;
+        for (uint32_t i0 = threadIdx.x; i0 < n; i0 += 4 * 1024) {
+            uint32_t v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                uint32_t i = i0 + u * 1024;
+                v[u] = i < n ? __float_as_uint(fabsf(c[i].response)) : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (i0 + u * 1024 < n && (v[u] & mask) == prefix) atomicAdd(&hist[(v[u] >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            // lane l owns buckets 255-4l .. 252-4l (descending); suffix counts by an inclusive scan
+            const int lane = threadIdx.x;
+            uint32_t b4[4], loc = 0;
+#pragma unroll
+            for (int u = 0; u < 4; u++) { b4[u] = hist[255 - 4 * lane - u]; loc += b4[u]; }
+            uint32_t inc = loc;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                uint32_t t = __shfl_up(inc, o);
+                if (lane >= o) inc += t;
+            }
+            const uint32_t need = s_need, above = inc - loc;
+            if (above < need && inc >= need) {
+                uint32_t acc = above;
+                for (int u = 0; u < 4; u++) {
+                    if (acc + b4[u] >= need) {
+                        s_prefix = prefix | ((uint32_t)(255 - 4 * lane - u) << shift);
+                        s_need = need - acc;
+                        break;
+                    }
+                    acc += b4[u];
+                }
+            }
+        }
+        mask |= 255u << shift;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { words[1] = s_prefix; words[2] = 1; }
+}
+
+__global__ void rank_gather_kernel(const Cand *c, const uint32_t *count, uint32_t cap_in, uint32_t limit,
+                                   uint32_t *subset, uint32_t *words) {
+    const uint32_t n = min(*count, cap_in);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (n > limit && __float_as_uint(fabsf(c[i].response)) < words[1]) return;
+    subset[atomicAdd(&words[0], 1u)] = i;
+}
+
+__global__ __launch_bounds__(kRankTile) void rank_count_kernel(const Cand *c, const uint32_t *subset,
+                                                               const uint32_t *words, uint32_t *rank) {
+    __shared__ float s_r[kRankTile];
+    __shared__ uint64_t s_k[kRankTile];
+    const uint32_t m = words[0];
+    const bool by_resp = words[2] != 0;
+    const uint32_t ntile = (m + kRankTile - 1) / kRankTile;
+    for (uint32_t ti = blockIdx.x; ti < ntile; ti += kRankGrid) {
+        const uint32_t i = ti * kRankTile + threadIdx.x;
+        float ri = 0.0f;
+        uint64_t ki = 0;
+        if (i < m) { Cand ci = c[subset[i]]; ri = fabsf(ci.response); ki = ci.key; }
+        uint32_t acc = 0;
+        for (uint32_t tj = blockIdx.y; tj < ntile; tj += kRankGrid) {
+            const uint32_t j = tj * kRankTile + threadIdx.x;
+            float rj = 0.0f;
+            uint64_t kj = 0;
+            if (j < m) { Cand cj = c[subset[j]]; rj = fabsf(cj.response); kj = cj.key; }
+            __syncthreads();
+            s_r[threadIdx.x] = rj;
+            s_k[threadIdx.x] = kj;
+            __syncthreads();
+            const uint32_t mm = min((uint32_t)kRankTile, m - tj * kRankTile);
+            if (by_resp) {
+#pragma unroll 8
+                for (uint32_t t = 0; t < mm; t++) {
+                    float r = s_r[t];
+                    acc += (r > ri || (r == ri && s_k[t] < ki)) ? 1u : 0u;
+                }
+            } else {
+#pragma unroll 8
+                for (uint32_t t = 0; t < mm; t++) acc += s_k[t] < ki ? 1u : 0u;
+            }
+        }
+        if (i < m && acc) atomicAdd(&rank[i], acc);
+    }
+}
+
+__global__ void rank_scatter_kernel(const Cand *c, const uint32_t *subset, const uint32_t *words, uint32_t limit,
+                                    const uint32_t *rank, Cand *out, uint32_t *out_count) {
+    const uint32_t m = words[0];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *out_count = min(m, limit);
+    if (i < m && rank[i] < limit) out[rank[i]] = c[subset[i]];
+}
+
+size_t rank_scratch_bytes(uint32_t cap_in) { return ((size_t)2 * cap_in + 64) * sizeof(uint32_t); }
+
 hipError_t launch_rank(const Cand *cand, const uint32_t *count, uint32_t cap_in, uint32_t limit, Cand *out,
-                       uint32_t *out_count, hipStream_t st) {
-    // the rank array lives right after the candidates' capacity in the caller's scratch
-    uint32_t *rank = reinterpret_cast<uint32_t *>(const_cast<Cand *>(cand) + cap_in);
-    unsigned blocks = (cap_in + 255) / 256;
+                       uint32_t *out_count, void *scratch, hipStream_t st) {
+    uint32_t *rank = static_cast<uint32_t *>(scratch);
+    uint32_t *subset = rank + cap_in;
+    uint32_t *words = subset + cap_in;
+    const unsigned blocks = (cap_in + 255) / 256;
+    hipError_t e = launch_fill_u32(words, 0u, 64, st);
+    if (e != hipSuccess) return e;
     fill_upto_kernel<<<blocks, 256, 0, st>>>(rank, count, cap_in);
-    rank_count_kernel<<<dim3(kRankGrid, kRankGrid), kRankTile, 0, st>>>(cand, count, cap_in, limit, rank);
-    rank_scatter_kernel<<<blocks, 256, 0, st>>>(cand, count, cap_in, limit, rank, out, out_count);
+    rank_select_kernel<<<1, 1024, 0, st>>>(cand, count, cap_in, limit, words);
+    rank_gather_kernel<<<blocks, 256, 0, st>>>(cand, count, cap_in, limit, subset, words);
+    rank_count_kernel<<<dim3(kRankGrid, kRankGrid), kRankTile, 0, st>>>(cand, subset, words, rank);
+    rank_scatter_kernel<<<blocks, 256, 0, st>>>(cand, subset, words, limit, rank, out, out_count);
     return hipGetLastError();
 }
 

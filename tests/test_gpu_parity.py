@@ -170,6 +170,23 @@ def test_nn2_exact(gpu, nq, nt):
     assert np.array_equal(s, rs)
 
 
+@pytest.mark.parametrize("eps", [0.0, 1e-7, 1e-5])
+def test_nn2_exact_near_ties(gpu, eps):
+    """Clusters of near-equidistant targets: the MFMA candidate guard must route every
+    ambiguous query to the exact scan, so indices and distances stay bit-exact."""
+    rng = np.random.default_rng(int(eps * 1e9) + 7)
+    base = np.maximum(rng.standard_normal((40, 128)), 0).astype(np.float32)
+    base /= np.linalg.norm(base, axis=1, keepdims=True)
+    t = np.repeat(base, 12, axis=0) + np.float32(eps) * rng.standard_normal((480, 128)).astype(np.float32)
+    q = base + np.float32(1e-3) * rng.standard_normal((40, 128)).astype(np.float32)
+    q = np.concatenate([q, t[::7]])
+    bi, b, s = gpu.nn2(q, t)
+    rbi, rb, rs = psr.nn2(q, t)
+    assert np.array_equal(bi, rbi)
+    assert np.array_equal(b, rb)
+    assert np.array_equal(s, rs)
+
+
 @pytest.mark.parametrize("mutual", [False, True])
 def test_match_exact_on_oracle_features(gpu, mutual):
     a, b, _ = synth.blob_pair(640, 480, 200, seed=5)
