@@ -244,14 +244,21 @@ hipError_t launch_feature_scan(const uint32_t *nkp, const uint32_t *nori, uint32
     return hipGetLastError();
 }
 
-// pipeline.rs:1352-1480 (+ RootSIFT, :1012-1019). One 256-thread workgroup per feature, with
-// 32 private LDS histograms summed in a fixed order and batched gradient loads (as in
-// orient_kernel).
+// pipeline.rs:1352-1480 (+ RootSIFT, :1012-1019). One 256-thread workgroup per feature and no
+// atomics: a pixel that passes the window test has r_bin, c_bin in (-1, 4), so it belongs to
+// exactly one of the 25 cells (floor(r_bin), floor(c_bin)) in {-1..3}^2 and only feeds the 4
+// spatial bins around that cell. Ten lanes own each cell; they scan the cell's padded
+// bounding box in image space, keep the pixels whose floor(r_bin), floor(c_bin) (the
+// reference's expressions) name their cell, and accumulate 4 corners x 8 orientations into a
+// lane-private LDS region with plain read-modify-writes. A fixed-order sum over the owners of
+// the 4 neighbouring cells then forms each of the 128 bins.
+constexpr int kLanesPerCell = 10;
+constexpr int kPrivStride = 33;   // 32 accumulators + 1 pad
 __global__ __launch_bounds__(kDescThreads) void descriptor_kernel(PyrView pv, const Cand *kps, const float *ori,
                                                                   const uint32_t *feat_kp, const uint32_t *feat_j,
                                                                   const uint32_t *nfeat, uint32_t cap, int kind,
                                                                   ps_feature *out_f, float *out_d) {
-    __shared__ float s_wdesc[kCopies][129];   // +1 pad: copies of a bin sit in different LDS banks
+    __shared__ float s_priv[kDescThreads * kPrivStride];
     __shared__ float s_desc[128];
     const uint32_t f = blockIdx.x;
     const int lane = threadIdx.x;
@@ -260,9 +267,12 @@ __global__ __launch_bounds__(kDescThreads) void descriptor_kernel(PyrView pv, co
     const Cand kp = kps[k];
     const float theta = ori[(size_t)k * 4 + feat_j[f]];
     const LocalKp l = local_kp(pv, kp);
-    for (int i = lane; i < kCopies * 129; i += kDescThreads) (&s_wdesc[0][0])[i] = 0.0f;
-    __syncthreads();
-    if (l.W >= 3 && l.H >= 3) {
+    float *priv = s_priv + lane * kPrivStride;
+#pragma unroll
+    for (int i = 0; i < 32; i++) priv[i] = 0.0f;
+    const int cell = lane / kLanesPerCell, sub = lane - cell * kLanesPerCell;
+    const int r0 = cell / 5 - 1, c0 = cell % 5 - 1;   // cell in {-1..3}^2 (lanes >= 250: none)
+    if (l.W >= 3 && l.H >= 3 && cell < 25) {
         const float cos_t = cosf(theta), sin_t = sinf(theta);
         const float hist_width = fmaxf(3.0f * l.scale, 1.5f);
         const float inv_hw = 1.0f / hist_width;
@@ -272,27 +282,40 @@ __global__ __launch_bounds__(kDescThreads) void descriptor_kernel(PyrView pv, co
         const int y_min = max(cy - max_radius, 1), y_max = max(min(cy + max_radius, l.H - 2), 1);
         const int x_min = max(cx - max_radius, 1), x_max = max(min(cx + max_radius, l.W - 2), 1);
         const float bin_per_rad = 8.0f / kTwoPi;
-        const int ww = x_max - x_min + 1, hh = y_max - y_min + 1;
+        // bounding box of the cell: r_rot in [r0 - 1.5, r0 - 0.5), c_rot likewise; image offset
+        // (dx, dy) = hw (r cos - c sin, r sin + c cos); padded by 1 px against rounding
+        float bx0 = 1e30f, bx1 = -1e30f, by0 = 1e30f, by1 = -1e30f;
+#pragma unroll
+        for (int cr = 0; cr < 2; cr++)
+#pragma unroll
+            for (int cc = 0; cc < 2; cc++) {
+                float rr = ((float)r0 - 1.5f + (float)cr) * hist_width, c2 = ((float)c0 - 1.5f + (float)cc) * hist_width;
+                float ddx = rr * cos_t - c2 * sin_t, ddy = rr * sin_t + c2 * cos_t;
+                bx0 = fminf(bx0, ddx); bx1 = fmaxf(bx1, ddx);
+                by0 = fminf(by0, ddy); by1 = fmaxf(by1, ddy);
+            }
+        const int xa = max(x_min, (int)floorf(l.x + bx0) - 1), xb = min(x_max, (int)ceilf(l.x + bx1) + 1);
+        const int ya = max(y_min, (int)floorf(l.y + by0) - 1), yb = min(y_max, (int)ceilf(l.y + by1) + 1);
+        const int ww = xb - xa + 1, hh = yb - ya + 1;
         const int total = (ww > 0 && hh > 0) ? ww * hh : 0;
         const float inv_ww = 1.0f / (float)max(ww, 1);
-        float *wd = s_wdesc[(lane >> 6) * 8 + (lane & 7)];
-        for (int p0 = lane; p0 < total; p0 += kBatch * kDescThreads) {
-            float gxa[kBatch], gxb[kBatch], gya[kBatch], gyb[kBatch], rr[kBatch], cc[kBatch];
+        for (int p0 = sub; p0 < total; p0 += kBatch * kLanesPerCell) {
+            float gxa[kBatch], gxb[kBatch], gya[kBatch], gyb[kBatch], rb_[kBatch], cb_[kBatch], rr_[kBatch], cc_[kBatch];
             bool ok[kBatch];
 #pragma unroll
             for (int u = 0; u < kBatch; u++) {
-                int p = p0 + u * kDescThreads;
+                int p = p0 + u * kLanesPerCell;
                 int qy, qx;
                 divmod_small(p, ww, inv_ww, qy, qx);
-                int yy = y_min + qy, xx = x_min + qx;
+                int yy = ya + qy, xx = xa + qx;
                 float dy = (float)yy - l.y, dx = (float)xx - l.x;
                 float r_rot = (dx * cos_t + dy * sin_t) * inv_hw;
                 float c_rot = (-dx * sin_t + dy * cos_t) * inv_hw;
                 float r_bin = r_rot + 4.0f * 0.5f - 0.5f;
                 float c_bin = c_rot + 4.0f * 0.5f - 0.5f;
-                rr[u] = r_rot;
-                cc[u] = c_rot;
-                ok[u] = p < total && !(r_bin <= -1.0f || r_bin >= 4.0f || c_bin <= -1.0f || c_bin >= 4.0f);
+                rr_[u] = r_rot; cc_[u] = c_rot; rb_[u] = r_bin; cb_[u] = c_bin;
+                ok[u] = p < total && !(r_bin <= -1.0f || r_bin >= 4.0f || c_bin <= -1.0f || c_bin >= 4.0f) &&
+                        (int)floorf(r_bin) == r0 && (int)floorf(c_bin) == c0;
                 const float *row = l.g + (size_t)(ok[u] ? yy : y_min) * l.W + (ok[u] ? xx : x_min);
                 gxa[u] = row[1]; gxb[u] = row[-1]; gyb[u] = row[l.W]; gya[u] = row[-l.W];
             }
@@ -303,38 +326,42 @@ __global__ __launch_bounds__(kDescThreads) void descriptor_kernel(PyrView pv, co
                 float m = sqrtf(gdx * gdx + gdy * gdy);
                 if (m <= 1e-7f) continue;
                 float a = rem_euclid(grad_atan2(gdy, gdx), kTwoPi);
-                float r_rot = rr[u], c_rot = cc[u];
-                float r_bin = r_rot + 4.0f * 0.5f - 0.5f;
-                float c_bin = c_rot + 4.0f * 0.5f - 0.5f;
+                float r_rot = rr_[u], c_rot = cc_[u], r_bin = rb_[u], c_bin = cb_[u];
                 float wm = m * expf((r_rot * r_rot + c_rot * c_rot) * exp_scale);
                 float o_bin = rem_euclid(a - theta, kTwoPi) * bin_per_rad;
-                int r0 = (int)floorf(r_bin), c0 = (int)floorf(c_bin), o0 = (int)floorf(o_bin);
+                int o0 = (int)floorf(o_bin);
                 float dr = r_bin - (float)r0, dc = c_bin - (float)c0, dof = o_bin - (float)o0;
+                const int ob0 = o0 & 7, ob1 = (o0 + 1) & 7;
 #pragma unroll
                 for (int ir = 0; ir <= 1; ir++) {
-                    int rb = r0 + ir;
-                    if (rb < 0 || rb >= 4) continue;
                     float wr = ir == 0 ? 1.0f - dr : dr;
 #pragma unroll
                     for (int ic = 0; ic <= 1; ic++) {
-                        int cb = c0 + ic;
-                        if (cb < 0 || cb >= 4) continue;
                         float wc = ic == 0 ? 1.0f - dc : dc;
-#pragma unroll
-                        for (int io = 0; io <= 1; io++) {
-                            int ob = (o0 + io) & 7;
-                            float wo = io == 0 ? 1.0f - dof : dof;
-                            atomicAdd(&wd[(rb * 4 + cb) * 8 + ob], wm * wr * wc * wo);
-                        }
+                        float w0 = wm * wr * wc;
+                        float *acc = priv + (ir * 2 + ic) * 8;
+                        acc[ob0] += w0 * (1.0f - dof);
+                        acc[ob1] += w0 * dof;
                     }
                 }
             }
         }
     }
     __syncthreads();
+    // bin (rb, cb, ob): corner (rb - r0, cb - c0) of the cells r0 in {rb-1, rb}, c0 in {cb-1, cb}
     if (lane < 128) {
+        const int rb = lane >> 5, cb = (lane >> 3) & 3, ob = lane & 7;
         float acc = 0.0f;
-        for (int c = 0; c < kCopies; c++) acc += s_wdesc[c][lane];
+#pragma unroll
+        for (int ir = 0; ir <= 1; ir++)
+#pragma unroll
+            for (int ic = 0; ic <= 1; ic++) {
+                const int cr = rb - ir, ccl = cb - ic;   // owning cell, in {-1..3}
+                const int owner = (cr + 1) * 5 + (ccl + 1);
+                const int corner = ir * 2 + ic;
+                for (int sl = 0; sl < kLanesPerCell; sl++)
+                    acc += s_priv[(owner * kLanesPerCell + sl) * kPrivStride + corner * 8 + ob];
+            }
         s_desc[lane] = acc;
     }
     __syncthreads();

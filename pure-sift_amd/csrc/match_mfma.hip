@@ -1,8 +1,8 @@
 // Exact Lowe-ratio matching through an FP32 MFMA distance GEMM.
 //
 // Pass 1 (nn_mfma_kernel): d'(q,t) = |t|^2 - 2 q.t on v_mfma_f32_32x32x2f32 (the per-query
-// constant |q|^2 does not change the ranking); every lane keeps the 4 smallest (d', index)
-// of one query over its targets in registers.
+// constant |q|^2 does not change the ranking; |t|^2 comes from the LDS tile); every lane keeps
+// the 4 smallest (d', index) of one query over its targets in registers.
 // Pass 2 (nn_rescore_kernel): the per-split lists are merged into the global top-4, each
 // candidate is rescored with the reference's sequential f32 sum (pipeline.rs:1700-1717),
 // and best/second/index follow the reference's rule. Exactness guard: with E a bound on
@@ -21,29 +21,6 @@ constexpr int kMQ = 128;   // queries per workgroup (4 waves x 32 query columns)
 constexpr int kMT = 64;    // targets per LDS tile (2 MFMA row blocks)
 constexpr int kMP = 132;   // LDS row stride in floats
 constexpr uint32_t kNoIdx = 0xffffffffu;
-
-// |row|^2 of each descriptor (any summation order: only feeds the approximate distance and
-// the error bound) and the maximum over rows as float bits. 8 lanes per row.
-__global__ void row_norms_kernel(const float *d, const uint32_t *n_p, uint32_t n_max, float *out,
-                                 uint32_t *max_bits) {
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t r = gid >> 3, part = gid & 7;
-    const uint32_t n = min(*n_p, n_max);
-    float s = 0.0f;
-    if (r < n) {
-        const float4 *p = reinterpret_cast<const float4 *>(d + (size_t)r * 128 + part * 16);
-        float4 a = p[0], b = p[1], c = p[2], e = p[3];
-        s = a.x * a.x + a.y * a.y + a.z * a.z + a.w * a.w + b.x * b.x + b.y * b.y + b.z * b.z + b.w * b.w +
-            c.x * c.x + c.y * c.y + c.z * c.z + c.w * c.w + e.x * e.x + e.y * e.y + e.z * e.z + e.w * e.w;
-    }
-    s += __shfl_xor(s, 1);
-    s += __shfl_xor(s, 2);
-    s += __shfl_xor(s, 4);
-    if (r < n && part == 0) {
-        out[r] = s;
-        atomicMax(max_bits, isfinite(s) ? __float_as_uint(s) : 0x7f800000u);
-    }
-}
 
 __device__ __forceinline__ void top4_insert(float d, uint32_t t, float (&k)[4], uint32_t (&ix)[4]) {
     if (!(d < k[3])) return;
@@ -81,8 +58,8 @@ struct Cand4 {
 };
 
 __global__ __launch_bounds__(256) void nn_mfma_kernel(const float *qd, const uint32_t *nq_p, uint32_t nq_max,
-                                                      const float *td, const float *tn, const uint32_t *nt_p,
-                                                      uint32_t nt_max, int nsplit, Cand4 *part) {
+                                                      const float *td, const uint32_t *nt_p, uint32_t nt_max,
+                                                      int nsplit, Cand4 *part, float *qn, uint32_t *tmax_bits) {
     __shared__ __attribute__((aligned(16))) float s_t[kMT * kMP];
     __shared__ float s_tn[kMT];
     const uint32_t nq = min(*nq_p, nq_max), nt = min(*nt_p, nt_max);
@@ -105,6 +82,14 @@ __global__ __launch_bounds__(256) void nn_mfma_kernel(const float *qd, const uin
 #pragma unroll
         for (int s = 0; s < 64; s++) b[s] = 0.0f;
     }
+    // |q|^2 (any order: it only feeds the error bound) for the rescoring pass
+    if (blockIdx.y == 0) {
+        float qq = 0.0f;
+#pragma unroll
+        for (int s = 0; s < 64; s++) qq += b[s] * b[s];
+        qq += __shfl_xor(qq, 32);
+        if (half == 0 && q < nq) qn[q] = qq;
+    }
     float kd[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
     uint32_t ki[4] = {kNoIdx, kNoIdx, kNoIdx, kNoIdx};
 
@@ -120,14 +105,30 @@ __global__ __launch_bounds__(256) void nn_mfma_kernel(const float *qd, const uin
             v[j] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (t0 + r < nt) v[j] = *reinterpret_cast<const float4 *>(td + (size_t)(t0 + r) * 128 + c);
         }
-        float tnv = (tid < kMT && t0 + tid < nt) ? tn[t0 + tid] : 0.0f;
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < kMT * 32 / 256; j++) {
             int i = tid + j * 256, r = i >> 5, c = (i & 31) * 4;
             *reinterpret_cast<float4 *>(s_t + r * kMP + c) = v[j];
         }
-        if (tid < kMT) s_tn[tid] = tnv;
+        __syncthreads();
+        {   // |t|^2 of the tile rows: 4 threads per row, combined by shuffles
+            const int r = tid >> 2, part = tid & 3;
+            const float *row = s_t + r * kMP + part * 32;
+            float tt = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 32; c += 4) {
+                float4 x = *reinterpret_cast<const float4 *>(row + c);
+                tt += x.x * x.x + x.y * x.y + x.z * x.z + x.w * x.w;
+            }
+            tt += __shfl_xor(tt, 1);
+            tt += __shfl_xor(tt, 2);
+            if (part == 0) {
+                s_tn[r] = tt;
+                if (blockIdx.x == 0 && t0 + r < nt)
+                    atomicMax(tmax_bits, isfinite(tt) ? __float_as_uint(tt) : 0x7f800000u);
+            }
+        }
         __syncthreads();
         floatx16 acc0 = {}, acc1 = {};
         const float *r0 = s_t + col * kMP + half * 64;
@@ -339,12 +340,12 @@ int nn_mfma_splits(size_t nq_max, size_t nt_max) {
 
 size_t nn_mfma_scratch_bytes(size_t nq_max, size_t nt_max) {
     size_t ns = (size_t)nn_mfma_splits(nq_max, nt_max);
-    return ns * nq_max * sizeof(Cand4) + (nq_max + nt_max) * sizeof(float) + nq_max * sizeof(uint32_t) +
-           nq_max * kFbT * sizeof(NNPart) + 256;
+    return ns * nq_max * sizeof(Cand4) + nq_max * kFbT * sizeof(NNPart) + nq_max * (sizeof(float) + sizeof(uint32_t)) +
+           256;
 }
 
-// Scratch layout: [part: ns*nq_max Cand4][fb: nq_max*16 NNPart][qn: nq_max][tn: nt_max]
-// [flag_list: nq_max][counters: tmax, qmax, flag_count]
+// Scratch layout: [part: ns*nq_max Cand4][fb: nq_max*16 NNPart][qn: nq_max][flag_list: nq_max]
+// [counters: tmax, flag_count]
 hipError_t launch_nn_mfma(const float *qd, const uint32_t *nq, size_t nq_max, const float *td, const uint32_t *nt,
                           size_t nt_max, void *scratch, uint32_t *best_idx, float *best, float *second,
                           hipStream_t st) {
@@ -357,19 +358,14 @@ hipError_t launch_nn_mfma(const float *qd, const uint32_t *nq, size_t nq_max, co
     p += nq_max * kFbT * sizeof(NNPart);
     float *qn = reinterpret_cast<float *>(p);
     p += nq_max * sizeof(float);
-    float *tn = reinterpret_cast<float *>(p);
-    p += nt_max * sizeof(float);
     uint32_t *flag_list = reinterpret_cast<uint32_t *>(p);
     p += nq_max * sizeof(uint32_t);
     uint32_t *tmax = reinterpret_cast<uint32_t *>(p);
-    uint32_t *qmax = tmax + 1;
-    uint32_t *flag_count = tmax + 2;
-    hipError_t e = launch_fill_u32(tmax, 0u, 3, st);
+    uint32_t *flag_count = tmax + 1;
+    hipError_t e = launch_fill_u32(tmax, 0u, 2, st);
     if (e != hipSuccess) return e;
-    row_norms_kernel<<<(unsigned)((8 * nq_max + 255) / 256), 256, 0, st>>>(qd, nq, (uint32_t)nq_max, qn, qmax);
-    row_norms_kernel<<<(unsigned)((8 * nt_max + 255) / 256), 256, 0, st>>>(td, nt, (uint32_t)nt_max, tn, tmax);
     dim3 grid((unsigned)((nq_max + kMQ - 1) / kMQ), (unsigned)ns);
-    nn_mfma_kernel<<<grid, 256, 0, st>>>(qd, nq, (uint32_t)nq_max, td, tn, nt, (uint32_t)nt_max, ns, part);
+    nn_mfma_kernel<<<grid, 256, 0, st>>>(qd, nq, (uint32_t)nq_max, td, nt, (uint32_t)nt_max, ns, part, qn, tmax);
     nn_rescore_kernel<<<(unsigned)((4 * nq_max + 255) / 256), 256, 0, st>>>(part, ns, qd, qn, nq, (uint32_t)nq_max, td,
                                                                             nt, tmax, best_idx, best, second,
                                                                             flag_list, flag_count);

@@ -419,18 +419,13 @@ hipError_t launch_extrema(const float *dog_octave, int h, int w, int S, float th
 constexpr int kRankTile = 256;
 constexpr int kRankGrid = 64;
 
-__global__ void fill_upto_kernel(uint32_t *p, const uint32_t *count, uint32_t cap) {
-    const uint32_t n = min(*count, cap);
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
-
 // scratch words: [0] subset size, [1] threshold bits T, [2] by-response flag
 __global__ __launch_bounds__(1024) void rank_select_kernel(const Cand *c, const uint32_t *count, uint32_t cap_in,
                                                            uint32_t limit, uint32_t *words) {
     __shared__ uint32_t hist[256];
     __shared__ uint32_t s_prefix, s_need;
     const uint32_t n = min(*count, cap_in);
+    if (threadIdx.x == 0) words[0] = 0;   // subset counter for rank_gather_kernel
     if (n <= limit) {
         if (threadIdx.x == 0) { words[1] = 0; words[2] = 0; }
         return;
@@ -486,12 +481,14 @@ __global__ __launch_bounds__(1024) void rank_select_kernel(const Cand *c, const 
 }
 
 __global__ void rank_gather_kernel(const Cand *c, const uint32_t *count, uint32_t cap_in, uint32_t limit,
-                                   uint32_t *subset, uint32_t *words) {
+                                   uint32_t *subset, uint32_t *rank, uint32_t *words) {
     const uint32_t n = min(*count, cap_in);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (n > limit && __float_as_uint(fabsf(c[i].response)) < words[1]) return;
-    subset[atomicAdd(&words[0], 1u)] = i;
+    const uint32_t pos = atomicAdd(&words[0], 1u);
+    subset[pos] = i;
+    rank[pos] = 0;
 }
 
 __global__ __launch_bounds__(kRankTile) void rank_count_kernel(const Cand *c, const uint32_t *subset,
@@ -548,11 +545,8 @@ hipError_t launch_rank(const Cand *cand, const uint32_t *count, uint32_t cap_in,
     uint32_t *subset = rank + cap_in;
     uint32_t *words = subset + cap_in;
     const unsigned blocks = (cap_in + 255) / 256;
-    hipError_t e = launch_fill_u32(words, 0u, 64, st);
-    if (e != hipSuccess) return e;
-    fill_upto_kernel<<<blocks, 256, 0, st>>>(rank, count, cap_in);
     rank_select_kernel<<<1, 1024, 0, st>>>(cand, count, cap_in, limit, words);
-    rank_gather_kernel<<<blocks, 256, 0, st>>>(cand, count, cap_in, limit, subset, words);
+    rank_gather_kernel<<<blocks, 256, 0, st>>>(cand, count, cap_in, limit, subset, rank, words);
     rank_count_kernel<<<dim3(kRankGrid, kRankGrid), kRankTile, 0, st>>>(cand, subset, words, rank);
     rank_scatter_kernel<<<blocks, 256, 0, st>>>(cand, subset, words, limit, rank, out, out_count);
     return hipGetLastError();
