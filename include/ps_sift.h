@@ -168,6 +168,43 @@ ps_status ps_engine_check(ps_engine *e);
 ps_status ps_engine_profile_begin(ps_engine *e, uint32_t max_calls);
 ps_status ps_engine_profile_end(ps_engine *e, float stage_ms[PS_NUM_STAGES], uint32_t *calls);
 
+/* ---- 128-bit descriptor codes and the catalog shortlist ------------------------------
+ * A code is two little-endian u64 words per descriptor: bit i (0..127) of the code is bit
+ * i % 64 of word i / 64, the layout of the reference's u128 signature (catalog.rs:436). */
+#define PS_HAMMING_MAX_K 1024
+
+/* catalog.rs:37-45 quantize_descriptor_u128: bit i = desc[i] > 0.065. out_codes holds
+ * 2 * n words. */
+ps_status ps_quantize_u128(const float *desc, size_t n, uint64_t *out_codes);
+
+/* catalog.rs:707-728 count_coarse_hamming_votes: the number of query codes whose minimum
+ * Hamming distance over the entry codes is <= 24 (0 when ne == 0). */
+ps_status ps_count_coarse_hamming_votes(const uint64_t *q_codes, size_t nq, const uint64_t *e_codes,
+                                        size_t ne, size_t *out_votes);
+
+/* Exact top-k Hamming shortlist of each query code over a catalog of n codes, ordered by
+ * (distance asc, catalog index asc); 1 <= k <= PS_HAMMING_MAX_K, n < 2^32. Slots past
+ * min(k, n) hold index UINT32_MAX and distance 255. out_idx / out_dist hold nq * k entries.
+ * The reference votes instead of ranking (catalog.rs:596-620, :707-728); this is the
+ * shortlist scan that replaces that pass in a GPU-resident catalog search. */
+ps_status ps_hamming_topk(const uint64_t *catalog, size_t n, const uint64_t *q_codes, size_t nq,
+                          uint32_t k, uint32_t *out_idx, uint8_t *out_dist);
+
+/* FP32 rerank of a shortlist: for query q, the k indices shortlist[q*k ..] into cat_desc
+ * (n x 128 f32) are reordered by (exact sequential squared L2 distance, index), the distance
+ * of ps_match (pipeline.rs:1764-1786). Slots with index >= n go last as (UINT32_MAX, +inf). */
+ps_status ps_rerank_l2(const float *q_desc, size_t nq, const float *cat_desc, size_t n,
+                       const uint32_t *shortlist, uint32_t k, uint32_t *out_idx, float *out_d2);
+
+/* Device-resident forms of the three calls above, asynchronous on the engine's stream. */
+ps_status ps_engine_quantize_u128(ps_engine *e, const float *d_desc, size_t n, uint64_t *d_codes);
+ps_status ps_engine_hamming_topk(ps_engine *e, const uint64_t *d_catalog, size_t n,
+                                 const uint64_t *d_q_codes, size_t nq, uint32_t k,
+                                 uint32_t *d_idx, uint8_t *d_dist);
+ps_status ps_engine_rerank_l2(ps_engine *e, const float *d_q_desc, size_t nq, const float *d_cat_desc,
+                              size_t n, const uint32_t *d_shortlist, uint32_t k, uint32_t *d_idx,
+                              float *d_d2);
+
 /* Tests only: the next compute call on this thread fails with PS_E_INJECTED before
  * touching the device. */
 void ps_debug_fail_next(void);

@@ -873,6 +873,42 @@ void psr_hamming_topk(const uint64_t *cat, size_t n, const uint64_t *qs, size_t 
     }
 }
 
+/* FP32 rerank of a Hamming shortlist: sequential squared L2 (the matcher's sum,
+ * pipeline.rs:1700-1717 with no cutoff), ordered by (distance asc, index asc); slots with
+ * index >= n go last as (UINT32_MAX, +inf). */
+typedef struct { float d; uint32_t idx; } psr_rr;
+static int rr_cmp(const void *a, const void *b) {
+    const psr_rr *x = (const psr_rr *)a, *y = (const psr_rr *)b;
+    if (x->d < y->d) return -1;
+    if (x->d > y->d) return 1;
+    return x->idx < y->idx ? -1 : x->idx > y->idx;
+}
+
+void psr_rerank_l2(const float *qd, size_t nq, const float *cd, size_t n, const uint32_t *sl, int k,
+                   uint32_t *out_idx, float *out_d2) {
+#pragma omp parallel
+    {
+        psr_rr *r = (psr_rr *)malloc(sizeof(psr_rr) * (size_t)(k > 0 ? k : 1));
+#pragma omp for schedule(dynamic, 4)
+        for (size_t q = 0; q < nq; q++) {
+            int m = 0;
+            for (int c = 0; c < k; c++) {
+                uint32_t idx = sl[q * (size_t)k + c];
+                if (idx >= n) continue;
+                r[m].d = dist_sq_early_exit(qd + q * PSR_DESC_LEN, cd + (size_t)idx * PSR_DESC_LEN, INFINITY);
+                r[m].idx = idx;
+                m++;
+            }
+            qsort(r, (size_t)m, sizeof(psr_rr), rr_cmp);
+            for (int c = 0; c < k; c++) {
+                out_idx[q * (size_t)k + c] = c < m ? r[c].idx : 0xffffffffu;
+                out_d2[q * (size_t)k + c] = c < m ? r[c].d : INFINITY;
+            }
+        }
+        free(r);
+    }
+}
+
 /* ---------------------------------------------------------------- RANSAC */
 
 /* pipeline.rs:1862-1868 */

@@ -100,6 +100,9 @@ size_t psr_count_coarse_hamming_votes(const uint64_t *q, size_t nq, const uint64
 /* top-k Hamming shortlist, ordered by (distance asc, index asc) */
 void psr_hamming_topk(const uint64_t *catalog, size_t n, const uint64_t *queries, size_t nq,
                       int k, uint32_t *out_idx, uint8_t *out_dist);
+/* FP32 rerank of a shortlist by (sequential squared L2, index) */
+void psr_rerank_l2(const float *qd, size_t nq, const float *cd, size_t n, const uint32_t *shortlist, int k,
+                   uint32_t *out_idx, float *out_d2);
 
 /* RANSAC (pipeline.rs:1860-2755) */
 uint64_t psr_splitmix64_next(uint64_t *state);

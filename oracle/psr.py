@@ -108,6 +108,8 @@ def lib():
         L.psr_count_coarse_hamming_votes.restype = C.c_size_t
         L.psr_hamming_topk.argtypes = [C.POINTER(C.c_uint64), C.c_size_t, C.POINTER(C.c_uint64), C.c_size_t,
                                        C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint8)]
+        L.psr_rerank_l2.argtypes = [C.POINTER(C.c_float), C.c_size_t, C.POINTER(C.c_float), C.c_size_t,
+                                    C.POINTER(C.c_uint32), C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]
         L.psr_splitmix64_next.argtypes = [C.POINTER(C.c_uint64)]
         L.psr_splitmix64_next.restype = C.c_uint64
         L.psr_filter_matches_ransac.argtypes = [C.POINTER(Match), C.c_size_t, C.c_float, C.c_uint64,
@@ -323,6 +325,19 @@ def hamming_topk(catalog, queries, k=100):
                            queries.ctypes.data_as(C.POINTER(C.c_uint64)), nq, k,
                            idx.ctypes.data_as(C.POINTER(C.c_uint32)), dist.ctypes.data_as(C.POINTER(C.c_uint8)))
     return idx, dist
+
+
+def rerank_l2(qd, cd, shortlist):
+    """FP32 rerank of a shortlist (nq x k catalog indices) by (sequential squared L2, index)."""
+    qd = _f32(qd).reshape(-1, 128)
+    cd = _f32(cd).reshape(-1, 128)
+    sl = np.ascontiguousarray(shortlist, np.uint32)
+    nq, k = sl.shape
+    idx = np.zeros((nq, k), np.uint32)
+    d2 = np.zeros((nq, k), np.float32)
+    lib().psr_rerank_l2(_fp(qd), nq, _fp(cd), len(cd), sl.ctypes.data_as(C.POINTER(C.c_uint32)), k,
+                        idx.ctypes.data_as(C.POINTER(C.c_uint32)), _fp(d2))
+    return idx, d2
 
 
 def splitmix64(state: int, n: int):

@@ -9,11 +9,11 @@
 #include <string>
 #include <vector>
 
-#include "ps_internal.h"
+#include "host_common.h"
 
 #pragma clang fp contract(off)
 
-namespace {
+namespace pshost {
 
 thread_local std::string t_last_error;
 thread_local bool t_fail_next = false;
@@ -23,19 +23,11 @@ ps_status set_err(ps_status s, const std::string &msg) {
     return s;
 }
 
-#define PS_HIP(call)                                                                                       \
-    do {                                                                                                   \
-        hipError_t _e = (call);                                                                            \
-        if (_e != hipSuccess)                                                                              \
-            return set_err(_e == hipErrorOutOfMemory ? PS_E_OUT_OF_MEMORY : PS_E_DEVICE,                   \
-                           std::string(#call) + ": " + hipGetErrorString(_e));                             \
-    } while (0)
+}  // namespace pshost
 
-#define PS_TRY(expr)                                                                                       \
-    do {                                                                                                   \
-        ps_status _s = (expr);                                                                             \
-        if (_s != PS_OK) return _s;                                                                        \
-    } while (0)
+using namespace pshost;
+
+namespace {
 
 // ------------------------------------------------------------------ host numerics
 constexpr float kBaseSigma = 1.6f;          // pipeline.rs:10
@@ -127,29 +119,7 @@ bool make_plan(int h, int w, const ps_sift_config &cfg, PyrPlan &p) {
 
 }  // namespace ps
 
-// ------------------------------------------------------------------ engine
-struct DevBuf {
-    void *p = nullptr;
-    size_t n = 0;
-};
-
-struct ps_engine {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    DevBuf pyr, cand, rank, sorted, ori, nori, feat_kp, feat_j, cnt;
-    DevBuf img, feat, desc, kpout;                 // host-API staging
-    DevBuf qf, qd, tf, td, mcnt, mout;             // host-API match staging
-    DevBuf part, part2, bidx, bdist, sdist, ridx, ridx_d, fin;  // matcher scratch
-    bool match_valu = false;                       // PS_MATCH_KERNEL=valu: direct f32 kernel
-    ps::PyrPlan plan;
-    uint32_t host_counts[4] = {};
-    // stage profiling: per recorded call, events [extract: 0..4][match: 5..6]
-    std::vector<hipEvent_t> prof_ev;
-    uint32_t prof_max = 0, prof_ext = 0, prof_mat = 0;
-    bool prof_on = false;
-};
-
-namespace {
+namespace pshost {
 
 ps_status ensure(DevBuf &b, size_t bytes) {
     if (bytes == 0) bytes = 16;
@@ -161,11 +131,6 @@ ps_status ensure(DevBuf &b, size_t bytes) {
     PS_HIP(hipMalloc(&b.p, bytes));
     b.n = bytes;
     return PS_OK;
-}
-
-template <typename T>
-T *as(DevBuf &b) {
-    return static_cast<T *>(b.p);
 }
 
 std::mutex g_mu;
@@ -237,33 +202,31 @@ void engine_free(ps_engine *e) {
                       &e->mcnt, &e->mout, &e->part, &e->part2, &e->bidx, &e->bdist, &e->sdist, &e->ridx, &e->ridx_d, &e->fin};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
+    for (DevBuf &b : e->aux)
+        if (b.p) (void)hipFree(b.p);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
 
-// RAII borrow of a pooled engine for host-pointer calls
-struct Borrow {
-    ps_engine *e = nullptr;
-    ps_status st = PS_OK;
-    Borrow() {
-        std::lock_guard<std::mutex> lk(g_mu);
-        if (!g_inited) {
-            st = init_locked(-1);
-            if (st != PS_OK) return;
-        }
-        if (!g_pool.empty()) {
-            e = g_pool.back();
-            g_pool.pop_back();
-        } else {
-            st = engine_new(g_device, &e);
-        }
+Borrow::Borrow() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_inited) {
+        st = init_locked(-1);
+        if (st != PS_OK) return;
     }
-    ~Borrow() {
-        if (!e) return;
-        std::lock_guard<std::mutex> lk(g_mu);
-        g_pool.push_back(e);
+    if (!g_pool.empty()) {
+        e = g_pool.back();
+        g_pool.pop_back();
+    } else {
+        st = engine_new(g_device, &e);
     }
-};
+}
+
+Borrow::~Borrow() {
+    if (!e) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_pool.push_back(e);
+}
 
 ps_status validate_cfg(const ps_sift_config *c, char *msg, size_t cap) {
     char buf[256];

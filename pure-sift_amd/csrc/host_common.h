@@ -1,0 +1,78 @@
+// Host-side state shared by the C ABI translation units: the engine (stream +
+// device workspace), the per-thread error message and the pooled borrow used by
+// host-pointer entry points.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "ps_internal.h"
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+};
+
+// Workspace slots of the catalog / precision-matcher / RANSAC entry points.
+enum AuxSlot {
+    AUX_HAM_CAT = 0, AUX_HAM_Q, AUX_HAM_BOUND, AUX_HAM_CNT, AUX_HAM_KEYS, AUX_HAM_IDX, AUX_HAM_DIST, AUX_HAM_FLAG,
+    AUX_RR_QD, AUX_RR_CD, AUX_RR_IDX, AUX_RR_DIST,
+    AUX_PM_0, AUX_PM_1, AUX_PM_2, AUX_PM_3, AUX_PM_4, AUX_PM_5, AUX_PM_6, AUX_PM_7,
+    AUX_RS_0, AUX_RS_1, AUX_RS_2, AUX_RS_3, AUX_RS_4, AUX_RS_5, AUX_RS_6, AUX_RS_7,
+    AUX_COUNT
+};
+
+struct ps_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevBuf pyr, cand, rank, sorted, ori, nori, feat_kp, feat_j, cnt;
+    DevBuf img, feat, desc, kpout;                 // host-API staging
+    DevBuf qf, qd, tf, td, mcnt, mout;             // host-API match staging
+    DevBuf part, part2, bidx, bdist, sdist, ridx, ridx_d, fin;  // matcher scratch
+    DevBuf aux[AUX_COUNT];
+    bool match_valu = false;                       // PS_MATCH_KERNEL=valu: direct f32 kernel
+    ps::PyrPlan plan;
+    uint32_t host_counts[4] = {};
+    // stage profiling: per recorded call, events [extract: 0..4][match: 5..6]
+    std::vector<hipEvent_t> prof_ev;
+    uint32_t prof_max = 0, prof_ext = 0, prof_mat = 0;
+    bool prof_on = false;
+};
+
+namespace pshost {
+
+ps_status set_err(ps_status s, const std::string &msg);
+ps_status ensure(DevBuf &b, size_t bytes);
+ps_status check_fail_injection();
+
+template <typename T>
+T *as(DevBuf &b) {
+    return static_cast<T *>(b.p);
+}
+
+// RAII borrow of a pooled engine for host-pointer calls
+struct Borrow {
+    ps_engine *e = nullptr;
+    ps_status st = PS_OK;
+    Borrow();
+    ~Borrow();
+    Borrow(const Borrow &) = delete;
+    Borrow &operator=(const Borrow &) = delete;
+};
+
+}  // namespace pshost
+
+#define PS_HIP(call)                                                                                       \
+    do {                                                                                                   \
+        hipError_t _e = (call);                                                                            \
+        if (_e != hipSuccess)                                                                              \
+            return pshost::set_err(_e == hipErrorOutOfMemory ? PS_E_OUT_OF_MEMORY : PS_E_DEVICE,           \
+                                   std::string(#call) + ": " + hipGetErrorString(_e));                     \
+    } while (0)
+
+#define PS_TRY(expr)                                                                                       \
+    do {                                                                                                   \
+        ps_status _s = (expr);                                                                             \
+        if (_s != PS_OK) return _s;                                                                        \
+    } while (0)

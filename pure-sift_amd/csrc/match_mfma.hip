@@ -179,20 +179,6 @@ __global__ __launch_bounds__(256) void nn_mfma_kernel(const float *qd, const uin
 }
 
 // sequential f32 distance of pipeline.rs:1700-1717 (without the exactness-neutral early exit)
-__device__ __forceinline__ float exact_dist(const float *a, const float *b) {
-    float sum = 0.0f;
-    const float4 *pa = reinterpret_cast<const float4 *>(a), *pb = reinterpret_cast<const float4 *>(b);
-    for (int i = 0; i < 32; i++) {
-        float4 x = pa[i], y = pb[i];
-        float e;
-        e = x.x - y.x; sum += e * e;
-        e = x.y - y.y; sum += e * e;
-        e = x.z - y.z; sum += e * e;
-        e = x.w - y.w; sum += e * e;
-    }
-    return sum;
-}
-
 // One thread per (query, candidate): the 4 exact distances of a query are computed by 4
 // adjacent lanes (each a 128-step sequential sum) and combined with shuffles.
 __global__ __launch_bounds__(256) void nn_rescore_kernel(const Cand4 *part, int nsplit, const float *qd, const float *qn,
@@ -228,7 +214,7 @@ __global__ __launch_bounds__(256) void nn_rescore_kernel(const Cand4 *part, int 
     for (int u = 0; u < 4; u++)
         if (u == j) my = ki[u];
     float d = INFINITY;
-    if (active && my != kNoIdx) d = exact_dist(qd + (size_t)q * 128, td + (size_t)my * 128);
+    if (active && my != kNoIdx) d = seq_dist128(qd + (size_t)q * 128, td + (size_t)my * 128);
     // gather the 4 (d, idx) of this query into every one of its lanes
     float dv[4];
     uint32_t iv[4];
@@ -280,7 +266,7 @@ __global__ __launch_bounds__(256) void nn_fallback_partial_kernel(const float *q
         float b = INFINITY, s = INFINITY;
         uint32_t bi = 0;
         for (uint32_t t = blockIdx.y * 256 + threadIdx.x; t < nt; t += kFbT * 256) {
-            float d = exact_dist(qd + (size_t)q * 128, td + (size_t)t * 128);
+            float d = seq_dist128(qd + (size_t)q * 128, td + (size_t)t * 128);
             if (d < b) { s = b; b = d; bi = t; }
             else if (d < s) s = d;
         }

@@ -119,4 +119,34 @@ hipError_t launch_nn_mfma(const float *qd, const uint32_t *nq, size_t nq_max, co
                           size_t nt_max, void *scratch, uint32_t *best_idx, float *best, float *second,
                           hipStream_t st);
 
+// Squared L2 distance of two 128-D f32 rows summed sequentially in element order
+// with separate roundings (pipeline.rs:1764-1786 without the early exit, which never
+// changes the result because partial sums are monotone).
+__device__ __forceinline__ float seq_dist128(const float *a, const float *b) {
+    float sum = 0.0f;
+    const float4 *pa = reinterpret_cast<const float4 *>(a), *pb = reinterpret_cast<const float4 *>(b);
+    for (int i = 0; i < 32; i++) {
+        float4 x = pa[i], y = pb[i];
+        float e;
+        e = x.x - y.x; sum += e * e;
+        e = x.y - y.y; sum += e * e;
+        e = x.z - y.z; sum += e * e;
+        e = x.w - y.w; sum += e * e;
+    }
+    return sum;
+}
+
+// hamming.hip: 128-bit codes (catalog.rs:37-45, :707-728) and the top-k shortlist
+constexpr uint32_t kHamMaxK = 1024;       // largest shortlist length
+constexpr uint32_t kHamCandCap = 4096;    // candidate slots per query in the fast path
+constexpr uint32_t kHamQueryChunk = 8192; // queries per pass (bounds the candidate buffer)
+hipError_t launch_quantize_u128(const float *desc, uint32_t n, uint64_t *codes, hipStream_t st);
+hipError_t launch_hamming_votes(const uint64_t *q, uint32_t nq, const uint64_t *e, uint32_t ne, uint32_t *scratch,
+                                hipStream_t st);
+size_t hamming_topk_scratch_bytes(uint32_t nq_chunk);
+hipError_t launch_hamming_topk(const uint64_t *cat, uint32_t n, const uint64_t *qs, uint32_t nq, uint32_t k,
+                               void *scratch, uint32_t *out_idx, uint8_t *out_dist, hipStream_t st);
+hipError_t launch_rerank_l2(const float *qd, uint32_t nq, const float *cd, uint32_t n, const uint32_t *shortlist,
+                            uint32_t k, uint32_t *out_idx, float *out_d2, hipStream_t st);
+
 }  // namespace ps

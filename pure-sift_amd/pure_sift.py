@@ -113,7 +113,15 @@ def lib() -> C.CDLL:
         L.ps_debug_fail_next.argtypes = []
         L.ps_engine_profile_begin.argtypes = [vp, C.c_uint32]
         L.ps_engine_profile_end.argtypes = [vp, C.POINTER(C.c_float), u32p]
-        for name in ("ps_init", "ps_validate_sift_config", "ps_validate_match_options", "ps_detect_and_compute",
+        L.ps_quantize_u128.argtypes = [vp, sz, vp]
+        L.ps_count_coarse_hamming_votes.argtypes = [vp, sz, vp, sz, C.POINTER(sz)]
+        L.ps_hamming_topk.argtypes = [vp, sz, vp, sz, C.c_uint32, vp, vp]
+        L.ps_rerank_l2.argtypes = [vp, sz, vp, sz, vp, C.c_uint32, vp, vp]
+        L.ps_engine_quantize_u128.argtypes = [vp, vp, sz, vp]
+        L.ps_engine_hamming_topk.argtypes = [vp, vp, sz, vp, sz, C.c_uint32, vp, vp]
+        L.ps_engine_rerank_l2.argtypes = [vp, vp, sz, vp, sz, vp, C.c_uint32, vp, vp]
+        for name in ("ps_quantize_u128", "ps_count_coarse_hamming_votes", "ps_hamming_topk", "ps_rerank_l2",
+                     "ps_engine_quantize_u128", "ps_engine_hamming_topk", "ps_engine_rerank_l2","ps_init", "ps_validate_sift_config", "ps_validate_match_options", "ps_detect_and_compute",
                      "ps_detect_and_compute_u8", "ps_detect_keypoints", "ps_match", "ps_build_pyramid", "ps_nn2",
                      "ps_engine_create", "ps_engine_destroy", "ps_engine_sync", "ps_engine_check",
                      "ps_engine_extract_u8", "ps_engine_match", "ps_engine_profile_begin",
@@ -334,6 +342,52 @@ def nn2(q_desc: np.ndarray, t_desc: np.ndarray):
     return bi, b, s
 
 
+HAMMING_MAX_K = 1024
+
+
+def _codes(a) -> np.ndarray:
+    a = np.ascontiguousarray(a, np.uint64)
+    return a.reshape(-1, 2)
+
+
+def quantize_descriptor_u128(desc: np.ndarray) -> np.ndarray:
+    """catalog.rs:37-45: (n, 2) u64 little-endian words, bit i = desc[i] > 0.065."""
+    d = np.ascontiguousarray(desc, np.float32).reshape(-1, 128)
+    out = np.zeros((len(d), 2), np.uint64)
+    check(lib().ps_quantize_u128(d.ctypes.data, len(d), out.ctypes.data))
+    return out
+
+
+def count_coarse_hamming_votes(query_sigs: np.ndarray, entry_sigs: np.ndarray) -> int:
+    """catalog.rs:707-728: queries whose nearest entry code is within 24 bits."""
+    q, e = _codes(query_sigs), _codes(entry_sigs)
+    n = C.c_size_t(0)
+    check(lib().ps_count_coarse_hamming_votes(q.ctypes.data, len(q), e.ctypes.data, len(e), C.byref(n)))
+    return n.value
+
+
+def hamming_topk(catalog: np.ndarray, queries: np.ndarray, k: int = 100):
+    """Exact top-k shortlist by (Hamming distance, catalog index): (idx u32 [nq,k], dist u8 [nq,k])."""
+    c, q = _codes(catalog), _codes(queries)
+    idx = np.zeros((len(q), k), np.uint32)
+    dist = np.zeros((len(q), k), np.uint8)
+    check(lib().ps_hamming_topk(c.ctypes.data, len(c), q.ctypes.data, len(q), k, idx.ctypes.data, dist.ctypes.data))
+    return idx, dist
+
+
+def rerank_l2(q_desc: np.ndarray, cat_desc: np.ndarray, shortlist: np.ndarray):
+    """FP32 rerank of a shortlist by (sequential squared L2, index): (idx [nq,k], d2 [nq,k])."""
+    q = np.ascontiguousarray(q_desc, np.float32).reshape(-1, 128)
+    c = np.ascontiguousarray(cat_desc, np.float32).reshape(-1, 128)
+    sl = np.ascontiguousarray(shortlist, np.uint32)
+    nq, k = sl.shape
+    idx = np.zeros((nq, k), np.uint32)
+    d2 = np.zeros((nq, k), np.float32)
+    check(lib().ps_rerank_l2(q.ctypes.data, nq, c.ctypes.data, len(c), sl.ctypes.data, k, idx.ctypes.data,
+                             d2.ctypes.data))
+    return idx, d2
+
+
 def extract_sift_features(img, octaves, scales, threshold, upsample_first_octave=True,
                           kind=DescriptorKind.Sift) -> Features:
     """Free-function funnel extract_sift_features_with_kind (pipeline.rs:1633-1658)."""
@@ -371,6 +425,16 @@ class Engine:
 
     def sync(self) -> None:
         check(lib().ps_engine_sync(self.h))
+
+    def quantize_u128(self, d_desc: int, n: int, d_codes: int) -> None:
+        check(lib().ps_engine_quantize_u128(self.h, d_desc, n, d_codes))
+
+    def hamming_topk(self, d_catalog: int, n: int, d_queries: int, nq: int, k: int, d_idx: int, d_dist: int) -> None:
+        check(lib().ps_engine_hamming_topk(self.h, d_catalog, n, d_queries, nq, k, d_idx, d_dist))
+
+    def rerank_l2(self, d_qdesc: int, nq: int, d_cat_desc: int, n: int, d_shortlist: int, k: int, d_idx: int,
+                  d_d2: int) -> None:
+        check(lib().ps_engine_rerank_l2(self.h, d_qdesc, nq, d_cat_desc, n, d_shortlist, k, d_idx, d_d2))
 
     STAGES = ("pyramid", "extrema", "order", "describe", "match")
 

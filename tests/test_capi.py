@@ -95,3 +95,25 @@ def test_status_strings():
     for s in (ps.PS_OK, ps.PS_E_INVALID_ARG, ps.PS_E_CONFIG, ps.PS_E_DEVICE, ps.PS_E_CAPACITY, ps.PS_E_NO_DEVICE,
               ps.PS_E_OUT_OF_MEMORY, ps.PS_E_INJECTED):
         assert L.ps_status_string(s).decode() not in ("", "unknown status")
+
+
+@pytest.mark.parametrize("k", [0, 1025])
+def test_hamming_topk_rejects_k(k):
+    cat = np.zeros((4, 2), np.uint64)
+    with pytest.raises(ps.DeviceError) as ei:
+        ps.hamming_topk(cat, cat[:1], k)
+    assert ei.value.status == ps.PS_E_INVALID_ARG
+    with pytest.raises(ps.DeviceError) as ei:
+        ps.rerank_l2(np.zeros((1, 128), np.float32), np.zeros((4, 128), np.float32), np.zeros((1, max(k, 1)), np.uint32)
+                     if k else np.zeros((1, 0), np.uint32))
+    assert ei.value.status == ps.PS_E_INVALID_ARG
+
+
+def test_hamming_empty_inputs_need_no_device():
+    """catalog.rs:708-710: no entries -> no votes; an empty catalog gives an all-empty shortlist."""
+    q = np.ones((3, 2), np.uint64)
+    assert ps.count_coarse_hamming_votes(q, np.zeros((0, 2), np.uint64)) == 0
+    idx, dist = ps.hamming_topk(np.zeros((0, 2), np.uint64), q, 5)
+    assert np.all(idx == 0xFFFFFFFF) and np.all(dist == 255)
+    idx, dist = ps.hamming_topk(q, np.zeros((0, 2), np.uint64), 5)
+    assert idx.shape == (0, 5)

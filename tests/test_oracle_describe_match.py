@@ -281,3 +281,33 @@ def test_coarse_votes():
     v = psr.count_coarse_hamming_votes(qs, cat)
     exp = sum(1 for i in range(len(qs)) if np.bitwise_count(cat ^ qs[i]).sum(axis=1).min() <= 24)
     assert v == exp
+
+
+def test_rerank_known_answer():
+    """Rows at exactly representable offsets from the query: squared distances are exact,
+    equal distances order by index, invalid slots go last."""
+    q = np.zeros((1, 128), np.float32)
+    q[0, :] = 0.25
+    deltas = [0.5, 0.25, 1.0, 0.25, 0.125]
+    cd = np.tile(q, (len(deltas), 1))
+    for i, dl in enumerate(deltas):
+        cd[i, i] += dl
+    sl = np.array([[0, 1, 2, 3, 4, 0xFFFFFFFF, 9]], np.uint32)
+    idx, d2 = psr.rerank_l2(q, cd, sl)
+    assert idx[0].tolist() == [4, 1, 3, 0, 2, 0xFFFFFFFF, 0xFFFFFFFF]
+    assert d2[0, :5].tolist() == [0.015625, 0.0625, 0.0625, 0.25, 1.0]
+    assert np.all(np.isinf(d2[0, 5:]))
+
+
+def test_rerank_vs_float64():
+    q, db, _ = synth.unit_descriptors(20, 3000, seed=4)
+    rng = np.random.default_rng(4)
+    sl = rng.integers(0, 3000, (20, 64)).astype(np.uint32)
+    idx, d2 = psr.rerank_l2(q, db, sl)
+    ref = ((db[sl.astype(np.int64)].astype(np.float64) - q[:, None, :].astype(np.float64)) ** 2).sum(-1)
+    for i in range(20):
+        got = ((db[idx[i].astype(np.int64)].astype(np.float64) - q[i].astype(np.float64)) ** 2).sum(-1)
+        assert np.allclose(got, d2[i], rtol=1e-5, atol=1e-6)
+        assert np.all(np.diff(d2[i]) >= 0)
+        assert sorted(idx[i].tolist()) == sorted(sl[i].tolist())
+        assert np.allclose(np.sort(ref[i]), d2[i], rtol=1e-5, atol=1e-6)
