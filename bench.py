@@ -128,6 +128,85 @@ def bench_hamming(steps: int = 20, cpu_queries: int = 1000) -> dict:
     return out
 
 
+def bench_precision(steps: int = 5, cpu_queries: int = 200) -> dict:
+    """Precision form of matching: 50k query x 200k database unit descriptors (ReLU(N(0,1)),
+    L2-normalised, 20% of queries planted), ratio 0.8; BF16 and FP8-E4M3 MFMA distances with
+    the 8 best rescored in FP32, against the exact FP32 search on the same GPU."""
+    import numpy as np
+    import torch
+    import pure_sift as ps
+    import synth
+    nq, nt = 50_000, 200_000
+    q, db, planted = synth.unit_descriptors(nq, nt, seed=1)
+    dev = torch.device("cuda:0")
+    tq = torch.from_numpy(q).to(dev)
+    tt = torch.from_numpy(db).to(dev)
+    cnt = torch.tensor([nq, nt], dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    eng = ps.Engine(0)
+    st, e0, e1 = _stream_events(eng)
+    res, ms = {}, {}
+    for prec, name in ((ps.PRECISION_FP32, "fp32"), (ps.PRECISION_BF16, "bf16"), (ps.PRECISION_FP8, "fp8")):
+        bi = torch.empty(nq, dtype=torch.int32, device=dev)
+        b = torch.empty(nq, dtype=torch.float32, device=dev)
+        s2 = torch.empty(nq, dtype=torch.float32, device=dev)
+        ap = torch.empty(nq, dtype=torch.float32, device=dev)
+
+        def run():
+            eng.nn2(tq.data_ptr(), cnt.data_ptr(), nq, tt.data_ptr(), cnt.data_ptr() + 4, nt, prec, bi.data_ptr(),
+                    b.data_ptr(), s2.data_ptr(), ap.data_ptr() if prec else 0)
+
+        run()
+        eng.sync()
+        k = max(1, steps if prec else 2)
+        e0.record(st)
+        for _ in range(k):
+            run()
+        e1.record(st)
+        eng.sync()
+        ms[name] = e0.elapsed_time(e1) / k
+        res[name] = (bi.cpu().numpy().view(np.uint32), b.cpu().numpy(), s2.cpu().numpy(),
+                     ap.cpu().numpy() if prec else None)
+    eng.close()
+
+    def decide(r):
+        return (r[2] > 1e-12) & (np.sqrt(r[1]) / np.sqrt(r[2]) < 0.8)
+
+    ref_acc = decide(res["fp32"])
+    flop = 2.0 * nq * nt * 128
+    out = {"workload": "precision matcher: 50k query x 200k database 128-D unit descriptors (20% planted), "
+                       "ratio 0.8, top-8 rescored in FP32",
+           "ms_fp32_exact": ms["fp32"], "unit": "solves/s"}
+    sel = np.random.default_rng(0).choice(nq, 2000, replace=False)
+    for name, peak in (("bf16", 2500.0), ("fp8", 5000.0)):
+        r = res[name]
+        acc = decide(r)
+        dis = float(np.mean((acc != ref_acc) | (acc & (r[0] != res["fp32"][0]))))
+        d64 = ((q[sel].astype(np.float64) - db[r[0][sel]].astype(np.float64)) ** 2).sum(-1)
+        err = np.abs(r[3][sel].astype(np.float64) - d64)
+        tf = flop / (ms[name] * 1e-3) / 1e12
+        out[name] = {"ms": ms[name], "value": 1e3 / ms[name], "ratio_decision_disagreement_vs_fp32": dis,
+                     "best_index_differs_vs_fp32": float(np.mean(r[0] != res["fp32"][0])),
+                     "planted_recovered": float(np.mean(r[0][planted >= 0] == planted[planted >= 0])),
+                     "approx_d2_abs_err_vs_fp64": {"max": float(err.max()), "mean": float(err.mean()),
+                                                   "sample": len(sel)},
+                     "roofline": {"bound": "mfma", "achieved": tf, "peak": peak, "unit": "TFLOP/s",
+                                  "frac": tf / peak}}
+    out["value"] = out["bf16"]["value"]
+    if cpu_queries > 0:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import psr
+        cores = _host_cores()
+        psr.set_threads(cores)
+        t0 = time.perf_counter()
+        psr.nn2(q[:cpu_queries], db)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cpu_queries / nq / dt, "unit": "solves/s (exact f32)", "cores": cores,
+                               "kind": "port", "sample": f"{cpu_queries} of the 50k queries against all 200k "
+                                                         f"targets, {dt:.1f} s wall, scaled to a full solve"}
+    return out
+
+
 def cpu_baseline(a, b, frames: int):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import psr
@@ -160,7 +239,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-frames", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["frame", "hamming"], default="frame",
+    ap.add_argument("--workload", choices=["frame", "hamming", "precision"], default="frame",
                     help="frame: the 1080p extract+match headline (default); hamming: the catalog shortlist")
     ap.add_argument("--no-secondary", action="store_true",
                     help="frame workload only, without the secondary workloads in the same JSON line")
@@ -170,6 +249,10 @@ def main():
               f"(images shard across GPUs with no exchange)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "precision":
+        print(json.dumps(bench_precision(steps=min(args.steps, 10), cpu_queries=0 if args.no_cpu_baseline else 200)),
+              flush=True)
+        return
     if args.workload == "hamming":
         print(json.dumps(bench_hamming(steps=args.steps, cpu_queries=0 if args.no_cpu_baseline else 1000)), flush=True)
         return
@@ -258,7 +341,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(a, b, args.cpu_frames)
     if not args.no_secondary:
         sec = {}
-        for name, fn in (("hamming", lambda: bench_hamming(cpu_queries=0 if args.no_cpu_baseline else 1000)),):
+        for name, fn in (("hamming", lambda: bench_hamming(cpu_queries=0 if args.no_cpu_baseline else 1000)),
+                         ("precision", lambda: bench_precision(cpu_queries=0 if args.no_cpu_baseline else 200))):
             try:
                 sec[name] = fn()
             except Exception as ex:   # a secondary workload never hides the headline line

@@ -51,11 +51,18 @@ typedef struct {
     uint8_t _pad[2];
 } ps_sift_config;
 
-/* pipeline.rs:1720-1727 MatchOptions */
+/* pipeline.rs:1720-1727 MatchOptions. `precision` (in the reference struct's padding,
+ * 0 for reference callers) selects the distance engine: PS_PRECISION_FP32 is the reference's
+ * exact f32 search; BF16 / FP8 rank all targets with low-precision MFMA distances and
+ * rescore the 8 best with the exact f32 distance before the ratio test. */
+#define PS_PRECISION_FP32 0
+#define PS_PRECISION_BF16 1
+#define PS_PRECISION_FP8 2   /* E4M3 with a per-row power-of-two scale */
 typedef struct {
     float ratio;
     uint8_t mutual;
-    uint8_t _pad[3];
+    uint8_t precision;
+    uint8_t _pad[2];
 } ps_match_options;
 
 /* pipeline.rs:251-267 Keypoint */
@@ -167,6 +174,18 @@ ps_status ps_engine_check(ps_engine *e);
 #define PS_NUM_STAGES 5
 ps_status ps_engine_profile_begin(ps_engine *e, uint32_t max_calls);
 ps_status ps_engine_profile_end(ps_engine *e, float stage_ms[PS_NUM_STAGES], uint32_t *calls);
+
+/* Nearest / second-nearest search of ps_nn2 with a selectable distance engine
+ * (PS_PRECISION_*). approx_best_d2 (may be null) receives the low-precision squared
+ * distance of the returned best target, for error reporting. */
+ps_status ps_nn2_precision(const float *q_desc, size_t nq, const float *t_desc, size_t nt, int precision,
+                           uint32_t *best_idx, float *best_d2, float *second_d2, float *approx_best_d2);
+
+/* Device-resident form on an engine's stream; counts are device u32s bounded by the
+ * capacities nq_max / nt_max. d_approx may be null. */
+ps_status ps_engine_nn2(ps_engine *e, const float *d_q_desc, const uint32_t *d_nq, size_t nq_max,
+                        const float *d_t_desc, const uint32_t *d_nt, size_t nt_max, int precision,
+                        uint32_t *d_best_idx, float *d_best_d2, float *d_second_d2, float *d_approx);
 
 /* ---- 128-bit descriptor codes and the catalog shortlist ------------------------------
  * A code is two little-endian u64 words per descriptor: bit i (0..127) of the code is bit

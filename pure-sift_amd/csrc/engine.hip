@@ -270,6 +270,16 @@ ps_status validate_opt(const ps_match_options *o, char *msg, size_t cap) {
         }
         return set_err(PS_E_CONFIG, buf);
     }
+    if (o->precision > PS_PRECISION_FP8) {
+        char buf[160];
+        snprintf(buf, sizeof buf, "Invalid `precision`=%u: must be 0 (FP32), 1 (BF16) or 2 (FP8)",
+                 (unsigned)o->precision);
+        if (msg && cap) {
+            std::strncpy(msg, buf, cap - 1);
+            msg[cap - 1] = 0;
+        }
+        return set_err(PS_E_CONFIG, buf);
+    }
     if (msg && cap) msg[0] = 0;
     return PS_OK;
 }
@@ -391,7 +401,14 @@ ps_status enqueue_extract(ps_engine *e, const void *src, bool src_u8, size_t h, 
 
 // best/second/index per query into e->bidx / bdist / sdist (or the given outputs)
 ps_status enqueue_nn(ps_engine *e, const float *qd, const uint32_t *nq, size_t nqm, const float *td,
-                     const uint32_t *nt, size_t ntm, DevBuf &scratch, uint32_t *bidx, float *bd, float *sd) {
+                     const uint32_t *nt, size_t ntm, DevBuf &scratch, uint32_t *bidx, float *bd, float *sd,
+                     int precision = PS_PRECISION_FP32, float *approx = nullptr) {
+    if (precision == PS_PRECISION_BF16 || precision == PS_PRECISION_FP8) {
+        PS_TRY(ensure(scratch, ps::nn_precision_scratch_bytes(precision, nqm, ntm)));
+        PS_HIP(ps::launch_nn_precision(precision, qd, nq, nqm, td, nt, ntm, scratch.p, bidx, bd, sd, approx,
+                                       e->stream));
+        return PS_OK;
+    }
     if (e->match_valu) {
         int ns = ps::nn_splits(nqm, ntm);
         PS_TRY(ensure(scratch, (size_t)ns * nqm * sizeof(ps::NNPart)));
@@ -413,14 +430,14 @@ ps_status enqueue_match(ps_engine *e, const ps_feature *qf, const float *qd, con
     PS_TRY(ensure(e->bdist, nqm * sizeof(float)));
     PS_TRY(ensure(e->sdist, nqm * sizeof(float)));
     PS_TRY(enqueue_nn(e, qd, nq, nqm, td, nt, ntm, e->part, as<uint32_t>(e->bidx), as<float>(e->bdist),
-                      as<float>(e->sdist)));
+                      as<float>(e->sdist), opt.precision));
     uint32_t *rev = nullptr;
     if (opt.mutual) {
         // reverse pass (pipeline.rs:1815-1840): nearest query of every target, first index on ties
         PS_TRY(ensure(e->ridx, ntm * sizeof(uint32_t)));
         PS_TRY(ensure(e->ridx_d, 2 * ntm * sizeof(float)));
         PS_TRY(enqueue_nn(e, td, nt, ntm, qd, nq, nqm, e->part2, as<uint32_t>(e->ridx), as<float>(e->ridx_d),
-                          as<float>(e->ridx_d) + ntm));
+                          as<float>(e->ridx_d) + ntm, opt.precision));
         rev = as<uint32_t>(e->ridx);
     }
     if (nqm > 262144) return set_err(PS_E_INVALID_ARG, "at most 262144 query features per match call");
@@ -619,6 +636,42 @@ ps_status ps_nn2(const float *q_desc, size_t nq, const float *t_desc, size_t nt,
     return PS_OK;
 }
 
+ps_status ps_nn2_precision(const float *q_desc, size_t nq, const float *t_desc, size_t nt, int precision,
+                           uint32_t *best_idx, float *best_d2, float *second_d2, float *approx_best_d2) {
+    PS_TRY(check_fail_injection());
+    if (precision < PS_PRECISION_FP32 || precision > PS_PRECISION_FP8)
+        return set_err(PS_E_INVALID_ARG, "precision must be PS_PRECISION_FP32, _BF16 or _FP8");
+    if ((nq && (!q_desc || !best_idx || !best_d2 || !second_d2)) || (nt && !t_desc))
+        return set_err(PS_E_INVALID_ARG, "null argument");
+    if (nq == 0) return PS_OK;
+    if (nq >= 0xffffffffull || nt >= 0xffffffffull) return set_err(PS_E_INVALID_ARG, "too many descriptors");
+    Borrow b;
+    PS_TRY(b.st);
+    ps_engine *e = b.e;
+    PS_HIP(hipSetDevice(e->device));
+    PS_TRY(ensure(e->qd, nq * 128 * sizeof(float)));
+    PS_TRY(ensure(e->td, std::max<size_t>(nt, 1) * 128 * sizeof(float)));
+    PS_TRY(ensure(e->mcnt, 64));
+    e->host_counts[0] = (uint32_t)nq;
+    e->host_counts[1] = (uint32_t)nt;
+    uint32_t *dc = as<uint32_t>(e->mcnt);
+    PS_HIP(hipMemcpyAsync(dc, e->host_counts, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, e->stream));
+    PS_HIP(hipMemcpyAsync(e->qd.p, q_desc, nq * 128 * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    if (nt) PS_HIP(hipMemcpyAsync(e->td.p, t_desc, nt * 128 * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    PS_TRY(ensure(e->bidx, nq * sizeof(uint32_t)));
+    PS_TRY(ensure(e->bdist, 2 * nq * sizeof(float)));
+    PS_TRY(ensure(e->sdist, nq * sizeof(float)));
+    float *approx = approx_best_d2 ? as<float>(e->bdist) + nq : nullptr;
+    PS_TRY(enqueue_nn(e, as<float>(e->qd), dc, nq, as<float>(e->td), dc + 1, std::max<size_t>(nt, 1), e->part,
+                      as<uint32_t>(e->bidx), as<float>(e->bdist), as<float>(e->sdist), precision, approx));
+    PS_HIP(hipMemcpyAsync(best_idx, e->bidx.p, nq * sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream));
+    PS_HIP(hipMemcpyAsync(best_d2, e->bdist.p, nq * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    PS_HIP(hipMemcpyAsync(second_d2, e->sdist.p, nq * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    if (approx) PS_HIP(hipMemcpyAsync(approx_best_d2, approx, nq * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    PS_HIP(hipStreamSynchronize(e->stream));
+    return PS_OK;
+}
+
 ps_status ps_match(const ps_feature *q, const float *q_desc, size_t nq, const ps_feature *t, const float *t_desc,
                    size_t nt, const ps_match_options *opt, ps_feature_match *out, size_t cap, size_t *out_count) {
     PS_TRY(check_fail_injection());
@@ -706,6 +759,20 @@ ps_status ps_engine_match(ps_engine *e, const ps_feature *d_qf, const float *d_q
     PS_TRY(validate_opt(opt, nullptr, 0));
     PS_HIP(hipSetDevice(e->device));
     return enqueue_match(e, d_qf, d_qd, d_nq, nq_max, d_tf, d_td, d_nt, nt_max, *opt, d_out, d_out_count);
+}
+
+ps_status ps_engine_nn2(ps_engine *e, const float *d_q_desc, const uint32_t *d_nq, size_t nq_max,
+                        const float *d_t_desc, const uint32_t *d_nt, size_t nt_max, int precision,
+                        uint32_t *d_best_idx, float *d_best_d2, float *d_second_d2, float *d_approx) {
+    PS_TRY(check_fail_injection());
+    if (!e || !d_nq || !d_nt || !d_best_idx || !d_best_d2 || !d_second_d2)
+        return set_err(PS_E_INVALID_ARG, "null argument");
+    if (precision < PS_PRECISION_FP32 || precision > PS_PRECISION_FP8)
+        return set_err(PS_E_INVALID_ARG, "precision must be PS_PRECISION_FP32, _BF16 or _FP8");
+    if (nq_max == 0) return PS_OK;
+    PS_HIP(hipSetDevice(e->device));
+    return enqueue_nn(e, d_q_desc, d_nq, nq_max, d_t_desc, d_nt, std::max<size_t>(nt_max, 1), e->part, d_best_idx,
+                      d_best_d2, d_second_d2, precision, d_approx);
 }
 
 ps_status ps_engine_profile_begin(ps_engine *e, uint32_t max_calls) {

@@ -136,6 +136,12 @@ __device__ __forceinline__ float seq_dist128(const float *a, const float *b) {
     return sum;
 }
 
+// precision.hip: BF16 / FP8 MFMA distances with an exact f32 rescore of the top 8
+size_t nn_precision_scratch_bytes(int prec, size_t nq_max, size_t nt_max);
+hipError_t launch_nn_precision(int prec, const float *qd, const uint32_t *nq, size_t nq_max, const float *td,
+                               const uint32_t *nt, size_t nt_max, void *scratch, uint32_t *best_idx, float *best,
+                               float *second, float *approx_best, hipStream_t st);
+
 // hamming.hip: 128-bit codes (catalog.rs:37-45, :707-728) and the top-k shortlist
 constexpr uint32_t kHamMaxK = 1024;       // largest shortlist length
 constexpr uint32_t kHamCandCap = 4096;    // candidate slots per query in the fast path

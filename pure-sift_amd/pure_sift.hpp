@@ -97,6 +97,7 @@ struct FeatureMatch {        // pipeline.rs:1075-1091
 struct MatchOptions {        // pipeline.rs:1720-1736
     float ratio = DEFAULT_MATCH_RATIO;
     bool mutual = false;
+    uint8_t precision = PS_PRECISION_FP32;   // distance engine (PS_PRECISION_*)
 };
 
 // pipeline.rs:106-146
@@ -161,7 +162,7 @@ private:
 class Matcher {
 public:
     static Matcher create(const MatchOptions &o) {
-        ps_match_options c{o.ratio, (uint8_t)(o.mutual ? 1 : 0), {0, 0, 0}};
+        ps_match_options c{o.ratio, (uint8_t)(o.mutual ? 1 : 0), o.precision, {0, 0}};
         char msg[160];
         if (ps_validate_match_options(&c, msg, sizeof msg) != PS_OK) throw ConfigError(msg);
         return Matcher(o);
@@ -173,7 +174,7 @@ public:
         std::vector<float> qd, td;
         pack(query, qh, qd);
         pack(target, th, td);
-        ps_match_options c{opt_.ratio, (uint8_t)(opt_.mutual ? 1 : 0), {0, 0, 0}};
+        ps_match_options c{opt_.ratio, (uint8_t)(opt_.mutual ? 1 : 0), opt_.precision, {0, 0}};
         std::vector<ps_feature_match> buf(query.size() ? query.size() : 1);
         std::size_t n = 0;
         check(ps_match(qh.data(), qd.data(), qh.size(), th.data(), td.data(), th.size(), &c, buf.data(), buf.size(),

@@ -67,7 +67,7 @@ class _SiftConfigC(C.Structure):
 
 
 class _MatchOptionsC(C.Structure):
-    _fields_ = [("ratio", C.c_float), ("mutual", C.c_uint8), ("_pad", C.c_uint8 * 3)]
+    _fields_ = [("ratio", C.c_float), ("mutual", C.c_uint8), ("precision", C.c_uint8), ("_pad", C.c_uint8 * 2)]
 
 
 FEAT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("scale", "<f4"), ("response", "<f4"),
@@ -113,6 +113,8 @@ def lib() -> C.CDLL:
         L.ps_debug_fail_next.argtypes = []
         L.ps_engine_profile_begin.argtypes = [vp, C.c_uint32]
         L.ps_engine_profile_end.argtypes = [vp, C.POINTER(C.c_float), u32p]
+        L.ps_nn2_precision.argtypes = [vp, sz, vp, sz, C.c_int, vp, vp, vp, vp]
+        L.ps_engine_nn2.argtypes = [vp, vp, u32p, sz, vp, u32p, sz, C.c_int, vp, vp, vp, vp]
         L.ps_quantize_u128.argtypes = [vp, sz, vp]
         L.ps_count_coarse_hamming_votes.argtypes = [vp, sz, vp, sz, C.POINTER(sz)]
         L.ps_hamming_topk.argtypes = [vp, sz, vp, sz, C.c_uint32, vp, vp]
@@ -120,7 +122,7 @@ def lib() -> C.CDLL:
         L.ps_engine_quantize_u128.argtypes = [vp, vp, sz, vp]
         L.ps_engine_hamming_topk.argtypes = [vp, vp, sz, vp, sz, C.c_uint32, vp, vp]
         L.ps_engine_rerank_l2.argtypes = [vp, vp, sz, vp, sz, vp, C.c_uint32, vp, vp]
-        for name in ("ps_quantize_u128", "ps_count_coarse_hamming_votes", "ps_hamming_topk", "ps_rerank_l2",
+        for name in ("ps_nn2_precision", "ps_engine_nn2", "ps_quantize_u128", "ps_count_coarse_hamming_votes", "ps_hamming_topk", "ps_rerank_l2",
                      "ps_engine_quantize_u128", "ps_engine_hamming_topk", "ps_engine_rerank_l2","ps_init", "ps_validate_sift_config", "ps_validate_match_options", "ps_detect_and_compute",
                      "ps_detect_and_compute_u8", "ps_detect_keypoints", "ps_match", "ps_build_pyramid", "ps_nn2",
                      "ps_engine_create", "ps_engine_destroy", "ps_engine_sync", "ps_engine_check",
@@ -184,9 +186,10 @@ class MatchOptions:
     """pipeline.rs:1720-1736"""
     ratio: float = DEFAULT_MATCH_RATIO
     mutual: bool = False
+    precision: int = 0   # PRECISION_FP32 (the reference), PRECISION_BF16 or PRECISION_FP8
 
     def _c(self) -> _MatchOptionsC:
-        return _MatchOptionsC(self.ratio, 1 if self.mutual else 0)
+        return _MatchOptionsC(self.ratio, 1 if self.mutual else 0, int(self.precision))
 
 
 @dataclasses.dataclass
@@ -342,6 +345,23 @@ def nn2(q_desc: np.ndarray, t_desc: np.ndarray):
     return bi, b, s
 
 
+PRECISION_FP32, PRECISION_BF16, PRECISION_FP8 = 0, 1, 2
+
+
+def nn2_precision(q_desc: np.ndarray, t_desc: np.ndarray, precision: int, with_approx: bool = False):
+    """nn2 with a selectable distance engine: (best_idx, best_d2, second_d2[, approx_best_d2]).
+    BF16 / FP8 rank with MFMA distances and rescore the 8 best in exact f32."""
+    q = np.ascontiguousarray(q_desc, np.float32).reshape(-1, 128)
+    t = np.ascontiguousarray(t_desc, np.float32).reshape(-1, 128)
+    bi = np.zeros(len(q), np.uint32)
+    b = np.zeros(len(q), np.float32)
+    s = np.zeros(len(q), np.float32)
+    ap = np.zeros(len(q), np.float32) if with_approx else None
+    check(lib().ps_nn2_precision(q.ctypes.data, len(q), t.ctypes.data, len(t), precision, bi.ctypes.data,
+                                 b.ctypes.data, s.ctypes.data, ap.ctypes.data if with_approx else None))
+    return (bi, b, s, ap) if with_approx else (bi, b, s)
+
+
 HAMMING_MAX_K = 1024
 
 
@@ -425,6 +445,12 @@ class Engine:
 
     def sync(self) -> None:
         check(lib().ps_engine_sync(self.h))
+
+    def nn2(self, d_q: int, d_nq: int, nq_max: int, d_t: int, d_nt: int, nt_max: int, precision: int,
+            d_best_idx: int, d_best: int, d_second: int, d_approx: int = 0) -> None:
+        u = lambda p: C.cast(p, C.POINTER(C.c_uint32))
+        check(lib().ps_engine_nn2(self.h, d_q, u(d_nq), nq_max, d_t, u(d_nt), nt_max, precision, d_best_idx,
+                                  d_best, d_second, d_approx or None))
 
     def quantize_u128(self, d_desc: int, n: int, d_codes: int) -> None:
         check(lib().ps_engine_quantize_u128(self.h, d_desc, n, d_codes))

@@ -117,3 +117,11 @@ def test_hamming_empty_inputs_need_no_device():
     assert np.all(idx == 0xFFFFFFFF) and np.all(dist == 255)
     idx, dist = ps.hamming_topk(q, np.zeros((0, 2), np.uint64), 5)
     assert idx.shape == (0, 5)
+
+
+def test_precision_option_validation():
+    """The precision byte sits in MatchOptions' padding; values past FP8 are rejected."""
+    with pytest.raises(ValueError, match="precision"):
+        ps.Matcher.new(ps.MatchOptions(0.8, False, 3))
+    for p in (ps.PRECISION_FP32, ps.PRECISION_BF16, ps.PRECISION_FP8):
+        ps.Matcher.new(ps.MatchOptions(0.8, False, p))
