@@ -187,6 +187,64 @@ ps_status ps_engine_nn2(ps_engine *e, const float *d_q_desc, const uint32_t *d_n
                         const float *d_t_desc, const uint32_t *d_nt, size_t nt_max, int precision,
                         uint32_t *d_best_idx, float *d_best_d2, float *d_second_d2, float *d_approx);
 
+/* ---- homography RANSAC ---------------------------------------------------------------
+ * pipeline.rs:179-213 RansacOptions: reproj_threshold in [0.5, 100]; seed used when
+ * has_seed (else the reference's default seeds, pipeline.rs:173-175). */
+typedef struct {
+    float reproj_threshold;
+    uint8_t has_seed;
+    uint8_t _pad[3];
+    uint64_t seed;
+} ps_ransac_options;
+
+/* pipeline.rs:225-235 Homography::estimate -> estimate_homography_ransac(_with_seed)
+ * (:2635-2755): similarity-seeded LO-RANSAC. out_inliers (capacity n) receives the inlier
+ * indices into m in increasing order, *out_n their count; out_h the row-major homography
+ * when *out_has_model. Same inliers and model as the sequential algorithm. */
+ps_status ps_homography_estimate(const ps_feature_match *m, size_t n, const ps_ransac_options *opt,
+                                 uint32_t *out_inliers, size_t *out_n, float out_h[9], int32_t *out_has_model);
+
+/* Generalised batch form: many problems per launch (one workgroup each), a choice of
+ * sampler and termination rule. The reference configuration is PS_SAMPLER_REFERENCE,
+ * confidence 0.999, min 64, max 2000 iterations, local_opt = sim_seed_model = adaptive = 1.
+ * PS_SAMPLER_WEIGHTS samples proportionally to caller weights (e.g. a learned inlier score)
+ * and, when adaptive, stops when the sampled inlier mass reaches the confidence. */
+#define PS_SAMPLER_REFERENCE 0   /* 1 / clamp(ratio, 0.05, 1), pipeline.rs:2006-2028 */
+#define PS_SAMPLER_UNIFORM 1
+#define PS_SAMPLER_WEIGHTS 2
+typedef struct {
+    float reproj_threshold;
+    uint8_t sampler;
+    uint8_t local_opt;
+    uint8_t sim_seed_model;
+    uint8_t adaptive;
+    double confidence;
+    uint32_t min_iters;
+    uint32_t max_iters;
+} ps_ransac_params;
+typedef struct {
+    uint32_t offset;    /* first match of the problem in the batch array */
+    uint32_t n;         /* its match count */
+    uint64_t sim_seed;  /* similarity-stage stream */
+    uint64_t hom_seed;  /* homography-stage stream */
+} ps_ransac_problem;
+typedef struct {
+    float h[9];
+    uint32_t has_model, n_inliers, hyps;   /* hyps: hypotheses drawn before termination */
+    float sim[4];                          /* similarity seed model (a, b, tx, ty) */
+    uint32_t has_sim, n_sim;
+} ps_ransac_result;
+/* Device-resident batch on an engine's stream: d_inliers[offset .. offset + n_inliers) of
+ * each problem gets its inlier indices; d_weights (per match) is read for the weighted
+ * sampler only. */
+ps_status ps_engine_ransac_batch(ps_engine *e, const ps_feature_match *d_matches, const float *d_weights,
+                                 const ps_ransac_problem *d_problems, uint32_t n_problems, size_t total_matches,
+                                 const ps_ransac_params *params, uint32_t *d_inliers, ps_ransac_result *d_results);
+/* Host-pointer form of the batch call. */
+ps_status ps_ransac_batch(const ps_feature_match *matches, const float *weights, const ps_ransac_problem *problems,
+                          uint32_t n_problems, size_t total_matches, const ps_ransac_params *params,
+                          uint32_t *inliers, ps_ransac_result *results);
+
 /* ---- 128-bit descriptor codes and the catalog shortlist ------------------------------
  * A code is two little-endian u64 words per descriptor: bit i (0..127) of the code is bit
  * i % 64 of word i / 64, the layout of the reference's u128 signature (catalog.rs:436). */

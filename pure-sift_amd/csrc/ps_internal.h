@@ -142,6 +142,12 @@ hipError_t launch_nn_precision(int prec, const float *qd, const uint32_t *nq, si
                                const uint32_t *nt, size_t nt_max, void *scratch, uint32_t *best_idx, float *best,
                                float *second, float *approx_best, hipStream_t st);
 
+// ransac.hip: batched similarity-seeded LO-RANSAC, one workgroup per problem
+size_t ransac_work_bytes(size_t total_matches);
+hipError_t launch_ransac(const ps_feature_match *matches, const float *weights, const ps_ransac_problem *probs,
+                         uint32_t n_probs, const ps_ransac_params &p, size_t total_matches, void *work,
+                         uint32_t *inliers, ps_ransac_result *outs, hipStream_t st);
+
 // hamming.hip: 128-bit codes (catalog.rs:37-45, :707-728) and the top-k shortlist
 constexpr uint32_t kHamMaxK = 1024;       // largest shortlist length
 constexpr uint32_t kHamCandCap = 4096;    // candidate slots per query in the fast path

@@ -1,0 +1,984 @@
+// Homography LO-RANSAC (pipeline.rs:2663-2755) seeded by similarity RANSAC
+// (pipeline.rs:2055-2191), one workgroup per problem, for a batch of problems.
+//
+// Exactness: every f32 / f64 expression is evaluated in the reference's order with
+// separate roundings (no contraction), sums run sequentially in index order, and the
+// SplitMix64 stream (pipeline.rs:1862-1868) is reproduced draw for draw, so inliers and
+// models equal the sequential algorithm's. The adaptive iteration bound uses the device
+// f64 log (glibc's may differ in the last ulp, which moves ceil() only when the quotient
+// lies within ~1e-15 of an integer).
+//
+// Parallel structure. The sample stream never depends on the evaluation (draw k of the
+// stream is SplitMix64 at position k; which draws form an iteration depends only on the
+// draws), so hypotheses are evaluated in chunks of 256, one per thread (sample, degeneracy
+// test, 4-point normalised DLT with the cyclic Jacobi eigen-solver, sequential scoring over
+// all matches). The chunk is then replayed in order by the whole workgroup: the
+// accept rule, local optimisation on improvement (cooperative: ordered inlier compaction,
+// per-entry sequential AtA sums, sequential error sums) and the adaptive bound, exactly as
+// the sequential loop would, stopping at the bound.
+#include <algorithm>
+
+#include "ps_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace ps {
+
+namespace {
+
+constexpr int kRT = 256;            // threads per problem
+constexpr uint64_t kGamma = 0x9E3779B97F4A7C15ull;
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+// k-th output (k = 0, 1, ...) of the generator started at `seed`
+__device__ __forceinline__ uint64_t splitmix_at(uint64_t seed, uint64_t k) { return mix64(seed + (k + 1) * kGamma); }
+__device__ __forceinline__ double unit_draw(uint64_t z) { return (double)(z >> 11) * (1.0 / 9007199254740992.0); }
+
+// partition_point(c < u), clamped to n - 1 (pipeline.rs:2031-2036)
+__device__ uint32_t cdf_pick(const double *cdf, uint32_t n, double u) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (cdf[mid] < u) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < n - 1 ? lo : n - 1;
+}
+__device__ __forceinline__ uint32_t uniform_pick(uint32_t n, double u) {
+    uint64_t i = (uint64_t)(u * (double)n);
+    return i < n ? (uint32_t)i : n - 1;
+}
+
+// pipeline.rs:1979-2002
+__device__ uint32_t adaptive_iters(double inl, double total, int s, double conf, uint32_t min_it, uint32_t cap) {
+    if (total == 0.0 || inl == 0.0) return cap;
+    double w = inl / total;
+    if (w < 1e-3) w = 1e-3;
+    if (w > 1.0 - 1e-6) w = 1.0 - 1e-6;
+    double p = 1.0, a = w;
+    int b = s;
+    for (;;) {
+        if (b & 1) p *= a;
+        b /= 2;
+        if (b == 0) break;
+        a *= a;
+    }
+    if (p < 1e-12) p = 1e-12;
+    if (p > 1.0 - 1e-9) p = 1.0 - 1e-9;
+    double num = log(1.0 - conf), den = log(1.0 - p);
+    if (fabs(den) < 1e-12) return min_it;
+    double est = ceil(num / den);
+    if (!isfinite(est)) return cap;
+    double e = est <= 0.0 ? 0.0 : est;
+    if (e < (double)min_it) e = (double)min_it;
+    if (e > (double)cap) e = (double)cap;
+    return (uint32_t)e;
+}
+
+struct Sim {
+    float a, b, tx, ty;
+};
+
+// pipeline.rs:1873-1897 (sequential f32 error sum)
+__device__ uint32_t eval_sim(const ps_feature_match *m, uint32_t n, float a, float b, float tx, float ty, float tsq,
+                             float *err) {
+    uint32_t count = 0;
+    float es = 0.0f;
+    for (uint32_t i = 0; i < n; i++) {
+        float px = a * m[i].qx - b * m[i].qy + tx;
+        float py = b * m[i].qx + a * m[i].qy + ty;
+        float dx = px - m[i].tx, dy = py - m[i].ty;
+        float e = dx * dx + dy * dy;
+        if (e <= tsq) {
+            count++;
+            es += e;
+        }
+    }
+    *err = es;
+    return count;
+}
+
+__device__ __forceinline__ bool sim_inlier(const ps_feature_match &p, const Sim &s, float tsq) {
+    float dx = (s.a * p.qx - s.b * p.qy + s.tx) - p.tx, dy = (s.b * p.qx + s.a * p.qy + s.ty) - p.ty;
+    return dx * dx + dy * dy <= tsq;
+}
+
+// pipeline.rs:1900-1974 over m[idx[0..n)] (sequential sums)
+__device__ bool fit_procrustes(const ps_feature_match *m, const uint32_t *idx, uint32_t n, Sim *out) {
+    if (n == 0) return false;
+    if (n == 1) {
+        *out = Sim{1.0f, 0.0f, m[idx[0]].dx, m[idx[0]].dy};
+        return true;
+    }
+    float inv_n = 1.0f / (float)n;
+    float mqx = 0, mqy = 0, mtx = 0, mty = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const ps_feature_match &p = m[idx[i]];
+        mqx += p.qx;
+        mqy += p.qy;
+        mtx += p.tx;
+        mty += p.ty;
+    }
+    mqx *= inv_n;
+    mqy *= inv_n;
+    mtx *= inv_n;
+    mty *= inv_n;
+    float sqq = 0, na = 0, nb = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const ps_feature_match &p = m[idx[i]];
+        float qx = p.qx - mqx, qy = p.qy - mqy, tx = p.tx - mtx, ty = p.ty - mty;
+        sqq += qx * qx + qy * qy;
+        na += qx * tx + qy * ty;
+        nb += qx * ty - qy * tx;
+    }
+    float a = 1.0f, b = 0.0f;
+    if (sqq > 1e-5f) {
+        float af = na / sqq, bf = nb / sqq;
+        float s2 = af * af + bf * bf;
+        if (s2 >= 0.04f && s2 <= 25.0f) {
+            a = af;
+            b = bf;
+        }
+    }
+    out->a = a;
+    out->b = b;
+    out->tx = mtx - (a * mqx - b * mqy);
+    out->ty = mty - (b * mqx + a * mqy);
+    return true;
+}
+
+// pipeline.rs:2276-2353: cyclic Jacobi on the symmetric 9x9 AtA; eigenvector of the
+// smallest eigenvalue (first index on ties).
+__device__ bool smallest_eigvec9(double (&a)[9][9], double (&h)[9]) {
+    double v[9][9];
+    for (int i = 0; i < 9; i++)
+        for (int j = 0; j < 9; j++) v[i][j] = i == j ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 20; sweep++) {
+        double mo = 0.0;
+        for (int p = 0; p < 9; p++)
+            for (int q = p + 1; q < 9; q++) {
+                double av = fabs(a[p][q]);
+                if (av > mo) mo = av;
+            }
+        if (mo < 1e-13) break;
+        for (int p = 0; p < 8; p++)
+            for (int q = p + 1; q < 9; q++) {
+                double apq = a[p][q];
+                if (fabs(apq) < 1e-15) continue;
+                double app = a[p][p], aqq = a[q][q];
+                double tau = (aqq - app) / (2.0 * apq);
+                double t = tau >= 0.0 ? 1.0 / (tau + sqrt(1.0 + tau * tau)) : -1.0 / (-tau + sqrt(1.0 + tau * tau));
+                double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+                a[p][p] = app - t * apq;
+                a[q][q] = aqq + t * apq;
+                a[p][q] = 0.0;
+                a[q][p] = 0.0;
+                for (int r = 0; r < 9; r++) {
+                    if (r != p && r != q) {
+                        double arp = a[r][p], arq = a[r][q];
+                        a[r][p] = c * arp - s * arq;
+                        a[p][r] = a[r][p];
+                        a[r][q] = s * arp + c * arq;
+                        a[q][r] = a[r][q];
+                    }
+                    double vrp = v[r][p], vrq = v[r][q];
+                    v[r][p] = c * vrp - s * vrq;
+                    v[r][q] = s * vrp + c * vrq;
+                }
+            }
+    }
+    int mi = 0;
+    double me = a[0][0];
+    for (int i = 1; i < 9; i++)
+        if (a[i][i] < me) {
+            me = a[i][i];
+            mi = i;
+        }
+    for (int r = 0; r < 9; r++) {
+        h[r] = v[r][mi];
+        if (!isfinite(h[r])) return false;
+    }
+    return true;
+}
+
+// pipeline.rs:2357-2404
+__device__ bool degenerate_tri(const double (&px)[4], const double (&py)[4]) {
+    const int T[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+    for (int t = 0; t < 4; t++) {
+        int i = T[t][0], j = T[t][1], k = T[t][2];
+        if (!isfinite(px[i]) || !isfinite(py[i]) || !isfinite(px[j]) || !isfinite(py[j]) || !isfinite(px[k]) ||
+            !isfinite(py[k]))
+            return true;
+        double dx1 = px[j] - px[i], dy1 = py[j] - py[i], dx2 = px[k] - px[i], dy2 = py[k] - py[i];
+        double dx3 = px[k] - px[j], dy3 = py[k] - py[j];
+        double cross = fabs(dx1 * dy2 - dy1 * dx2);
+        double e1 = dx1 * dx1 + dy1 * dy1, e2 = dx2 * dx2 + dy2 * dy2, e3 = dx3 * dx3 + dy3 * dy3;
+        double me = fmax(fmax(e1, e2), e3);
+        if (cross < 4.0 || cross < 1e-3 * me) return true;
+    }
+    return false;
+}
+
+// Normalisation + AtA + eigenvector + denormalisation (pipeline.rs:2408-2528), for the
+// 4 points of one hypothesis (per thread).
+__device__ bool fit_dlt4(const ps_feature_match *m, const uint32_t (&idx)[4], double (&H)[3][3]) {
+    const double inv_n = 1.0 / 4.0;
+    double cxq = 0, cyq = 0, cxt = 0, cyt = 0;
+    for (int i = 0; i < 4; i++) {
+        const ps_feature_match &p = m[idx[i]];
+        double qx = p.qx, qy = p.qy, tx = p.tx, ty = p.ty;
+        if (!isfinite(qx) || !isfinite(qy) || !isfinite(tx) || !isfinite(ty)) return false;
+        cxq += qx;
+        cyq += qy;
+        cxt += tx;
+        cyt += ty;
+    }
+    cxq *= inv_n;
+    cyq *= inv_n;
+    cxt *= inv_n;
+    cyt *= inv_n;
+    double mdq = 0, mdt = 0;
+    for (int i = 0; i < 4; i++) {
+        const ps_feature_match &p = m[idx[i]];
+        double dxq = (double)p.qx - cxq, dyq = (double)p.qy - cyq, dxt = (double)p.tx - cxt, dyt = (double)p.ty - cyt;
+        mdq += sqrt(dxq * dxq + dyq * dyq);
+        mdt += sqrt(dxt * dxt + dyt * dyt);
+    }
+    mdq *= inv_n;
+    mdt *= inv_n;
+    if (!isfinite(mdq) || !isfinite(mdt) || mdq < 1e-6 || mdt < 1e-6) return false;
+    const double SQ2 = 1.41421356237309504880;
+    double sq = SQ2 / mdq, stt = SQ2 / mdt;
+    double ata[9][9];
+    for (int a = 0; a < 9; a++)
+        for (int b = 0; b < 9; b++) ata[a][b] = 0.0;
+    for (int i = 0; i < 4; i++) {
+        const ps_feature_match &p = m[idx[i]];
+        double x = ((double)p.qx - cxq) * sq, y = ((double)p.qy - cyq) * sq;
+        double xp = ((double)p.tx - cxt) * stt, yp = ((double)p.ty - cyt) * stt;
+        double r1[9] = {-x, -y, -1.0, 0.0, 0.0, 0.0, x * xp, y * xp, xp};
+        double r2[9] = {0.0, 0.0, 0.0, -x, -y, -1.0, x * yp, y * yp, yp};
+        for (int a = 0; a < 9; a++)
+            for (int b = a; b < 9; b++) ata[a][b] += r1[a] * r1[b] + r2[a] * r2[b];
+    }
+    for (int a = 0; a < 9; a++)
+        for (int b = 0; b < a; b++) ata[a][b] = ata[b][a];
+    double hv[9];
+    if (!smallest_eigvec9(ata, hv)) return false;
+    double hn[3][3] = {{hv[0], hv[1], hv[2]}, {hv[3], hv[4], hv[5]}, {hv[6], hv[7], hv[8]}};
+    double tq[3][3] = {{sq, 0.0, -sq * cxq}, {0.0, sq, -sq * cyq}, {0.0, 0.0, 1.0}};
+    double ist = 1.0 / stt;
+    double tti[3][3] = {{ist, 0.0, cxt}, {0.0, ist, cyt}, {0.0, 0.0, 1.0}};
+    double tmp[3][3];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) tmp[r][c] = hn[r][0] * tq[0][c] + hn[r][1] * tq[1][c] + hn[r][2] * tq[2][c];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) H[r][c] = tti[r][0] * tmp[0][c] + tti[r][1] * tmp[1][c] + tti[r][2] * tmp[2][c];
+    if (fabs(H[2][2]) > 1e-12) {
+        double inv = 1.0 / H[2][2];
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) H[r][c] *= inv;
+    }
+    return true;
+}
+
+// squared reprojection error of one match, or -1 when |w| < 1e-9 (pipeline.rs:2550-2576)
+__device__ __forceinline__ double reproj_err(const ps_feature_match &p, const double *H) {
+    double qx = p.qx, qy = p.qy;
+    double w = H[6] * qx + H[7] * qy + H[8];
+    if (fabs(w) < 1e-9) return -1.0;
+    double iw = 1.0 / w;
+    double px = (H[0] * qx + H[1] * qy + H[2]) * iw;
+    double py = (H[3] * qx + H[4] * qy + H[5]) * iw;
+    double dx = px - (double)p.tx, dy = py - (double)p.ty;
+    return dx * dx + dy * dy;
+}
+
+// sequential count / error / sampling-mass of one model (one thread)
+__device__ uint32_t eval_h_seq(const ps_feature_match *m, uint32_t n, const double *H, double tsq, double *err,
+                               const float *wts, double *mass) {
+    uint32_t c = 0;
+    double es = 0.0, ms = 0.0;
+    for (uint32_t i = 0; i < n; i++) {
+        double e = reproj_err(m[i], H);
+        if (e >= 0.0 && e <= tsq) {
+            c++;
+            es += e;
+            if (wts) ms += (double)wts[i];
+        }
+    }
+    *err = es;
+    if (mass) *mass = ms;
+    return c;
+}
+
+struct Hyp {
+    double H[9];
+    double err, mass;
+    uint32_t count, valid;
+};
+
+struct Shared {
+    Hyp hyp[kRT];
+    uint32_t quad[kRT][4];
+    uint32_t qvalid[kRT];
+    uint32_t draws[1024];
+    uint32_t wave_cnt[kRT / 64];
+    uint32_t n_quads, n_draws, draw_pos, total;
+    double red[kRT];
+};
+
+// Ordered compaction of the indices i in [0, n) whose predicate holds, into out[];
+// every thread calls it; returns the count.
+template <typename Pred>
+__device__ uint32_t ordered_compact(uint32_t n, uint32_t *out, Shared &S, Pred pred) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    uint32_t base = 0;
+    for (uint32_t c0 = 0; c0 < n; c0 += kRT) {
+        const uint32_t i = c0 + tid;
+        const bool p = i < n && pred(i);
+        const uint64_t mask = __ballot(p);
+        if (lane == 0) S.wave_cnt[w] = (uint32_t)__popcll(mask);
+        __syncthreads();
+        uint32_t off = base;
+        for (int v = 0; v < w; v++) off += S.wave_cnt[v];
+        off += (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+        if (p) out[off] = i;
+        uint32_t tot = 0;
+        for (int v = 0; v < kRT / 64; v++) tot += S.wave_cnt[v];
+        base += tot;
+        __syncthreads();
+    }
+    return base;
+}
+
+// Cooperative DLT over m[idx[0..n)] (n >= 4), same arithmetic as the sequential fit:
+// centroid / mean-distance sums by thread 0, the 45 AtA entries each summed by one thread
+// in point order, eigenvector by thread 0. Result broadcast through Hout (shared).
+__device__ bool fit_dlt_block(const ps_feature_match *m, const uint32_t *idx, uint32_t n, double *Hout, Shared &S) {
+    __shared__ double s_norm[6];
+    __shared__ double s_ata[45];
+    __shared__ int s_ok;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        double inv_n = 1.0 / (double)n, cxq = 0, cyq = 0, cxt = 0, cyt = 0;
+        int ok = 1;
+        for (uint32_t i = 0; i < n; i++) {
+            const ps_feature_match &p = m[idx[i]];
+            double qx = p.qx, qy = p.qy, tx = p.tx, ty = p.ty;
+            if (!isfinite(qx) || !isfinite(qy) || !isfinite(tx) || !isfinite(ty)) {
+                ok = 0;
+                break;
+            }
+            cxq += qx;
+            cyq += qy;
+            cxt += tx;
+            cyt += ty;
+        }
+        cxq *= inv_n;
+        cyq *= inv_n;
+        cxt *= inv_n;
+        cyt *= inv_n;
+        double mdq = 0, mdt = 0;
+        if (ok) {
+            for (uint32_t i = 0; i < n; i++) {
+                const ps_feature_match &p = m[idx[i]];
+                double dxq = (double)p.qx - cxq, dyq = (double)p.qy - cyq, dxt = (double)p.tx - cxt,
+                       dyt = (double)p.ty - cyt;
+                mdq += sqrt(dxq * dxq + dyq * dyq);
+                mdt += sqrt(dxt * dxt + dyt * dyt);
+            }
+            mdq *= inv_n;
+            mdt *= inv_n;
+            if (!isfinite(mdq) || !isfinite(mdt) || mdq < 1e-6 || mdt < 1e-6) ok = 0;
+        }
+        const double SQ2 = 1.41421356237309504880;
+        s_norm[0] = cxq;
+        s_norm[1] = cyq;
+        s_norm[2] = cxt;
+        s_norm[3] = cyt;
+        s_norm[4] = ok ? SQ2 / mdq : 0.0;
+        s_norm[5] = ok ? SQ2 / mdt : 0.0;
+        s_ok = ok;
+    }
+    __syncthreads();
+    if (!s_ok) {
+        __syncthreads();
+        return false;
+    }
+    if (tid < 45) {
+        // entry (a, b), a <= b, in row-major upper-triangle order
+        int a = 0, r = tid;
+        while (r >= 9 - a) {
+            r -= 9 - a;
+            a++;
+        }
+        const int b = a + r;
+        const double cxq = s_norm[0], cyq = s_norm[1], cxt = s_norm[2], cyt = s_norm[3], sq = s_norm[4],
+                     stt = s_norm[5];
+        double acc = 0.0;
+        for (uint32_t i = 0; i < n; i++) {
+            const ps_feature_match &p = m[idx[i]];
+            double x = ((double)p.qx - cxq) * sq, y = ((double)p.qy - cyq) * sq;
+            double xp = ((double)p.tx - cxt) * stt, yp = ((double)p.ty - cyt) * stt;
+            double r1[9] = {-x, -y, -1.0, 0.0, 0.0, 0.0, x * xp, y * xp, xp};
+            double r2[9] = {0.0, 0.0, 0.0, -x, -y, -1.0, x * yp, y * yp, yp};
+            acc += r1[a] * r1[b] + r2[a] * r2[b];
+        }
+        s_ata[tid] = acc;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double ata[9][9];
+        int k = 0;
+        for (int a = 0; a < 9; a++)
+            for (int b = a; b < 9; b++) ata[a][b] = s_ata[k++];
+        for (int a = 0; a < 9; a++)
+            for (int b = 0; b < a; b++) ata[a][b] = ata[b][a];
+        double hv[9];
+        int ok = smallest_eigvec9(ata, hv);
+        if (ok) {
+            const double cxq = s_norm[0], cyq = s_norm[1], cxt = s_norm[2], cyt = s_norm[3], sq = s_norm[4],
+                         stt = s_norm[5];
+            double hn[3][3] = {{hv[0], hv[1], hv[2]}, {hv[3], hv[4], hv[5]}, {hv[6], hv[7], hv[8]}};
+            double tq[3][3] = {{sq, 0.0, -sq * cxq}, {0.0, sq, -sq * cyq}, {0.0, 0.0, 1.0}};
+            double ist = 1.0 / stt;
+            double tti[3][3] = {{ist, 0.0, cxt}, {0.0, ist, cyt}, {0.0, 0.0, 1.0}};
+            double tmp[3][3], H[3][3];
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++)
+                    tmp[r][c] = hn[r][0] * tq[0][c] + hn[r][1] * tq[1][c] + hn[r][2] * tq[2][c];
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++)
+                    H[r][c] = tti[r][0] * tmp[0][c] + tti[r][1] * tmp[1][c] + tti[r][2] * tmp[2][c];
+            if (fabs(H[2][2]) > 1e-12) {
+                double inv = 1.0 / H[2][2];
+                for (int r = 0; r < 3; r++)
+                    for (int c = 0; c < 3; c++) H[r][c] *= inv;
+            }
+            for (int i = 0; i < 9; i++) Hout[i] = H[i / 3][i % 3];
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    const bool ok = s_ok;
+    __syncthreads();
+    return ok;
+}
+
+// Cooperative sequential-order scoring: per-match errors in parallel, then thread 0 adds
+// them in index order. Returns the count; *err / *mass broadcast through shared memory.
+__device__ uint32_t eval_h_block(const ps_feature_match *m, uint32_t n, const double *H, double tsq,
+                                 const float *wts, double *ebuf, double *err, double *mass) {
+    __shared__ uint32_t s_c;
+    __shared__ double s_e, s_m;
+    const int tid = threadIdx.x;
+    for (uint32_t i = tid; i < n; i += kRT) {
+        double e = reproj_err(m[i], H);
+        ebuf[i] = (e >= 0.0 && e <= tsq) ? e : -1.0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t c = 0;
+        double es = 0.0, ms = 0.0;
+        for (uint32_t i = 0; i < n; i++) {
+            double e = ebuf[i];
+            if (e >= 0.0) {
+                c++;
+                es += e;
+                if (wts) ms += (double)wts[i];
+            }
+        }
+        s_c = c;
+        s_e = es;
+        s_m = ms;
+    }
+    __syncthreads();
+    const uint32_t c = s_c;
+    *err = s_e;
+    if (mass) *mass = s_m;
+    __syncthreads();
+    return c;
+}
+
+// pipeline.rs:2605-2630 (two refits on the current inliers while they improve)
+__device__ void local_opt_block(const ps_feature_match *m, uint32_t n, double *H, uint32_t *bc, double *be,
+                                double tsq, uint32_t *ibuf, double *ebuf, Shared &S) {
+    __shared__ double s_R[9];
+    for (int it = 0; it < 2; it++) {
+        const uint32_t ni = ordered_compact(n, ibuf, S, [&](uint32_t i) {
+            double e = reproj_err(m[i], H);
+            return e >= 0.0 && e <= tsq;
+        });
+        if (ni < 4) break;
+        if (!fit_dlt_block(m, ibuf, ni, s_R, S)) break;
+        double re;
+        const uint32_t rc = eval_h_block(m, n, s_R, tsq, nullptr, ebuf, &re, nullptr);
+        if (rc > *bc || (rc == *bc && re < *be)) {
+            __syncthreads();
+            if (threadIdx.x < 9) H[threadIdx.x] = s_R[threadIdx.x];
+            *bc = rc;
+            *be = re;
+            __syncthreads();
+        } else {
+            break;
+        }
+    }
+}
+
+}  // namespace
+
+struct RansacParamsDev {
+    float reproj;
+    int sampler;          // 0: reference 1/ratio weights, 1: uniform, 2: caller weights
+    double confidence;
+    uint32_t min_iters, max_iters;
+    int local_opt, sim_seed_model, adaptive;
+};
+
+namespace {
+
+__global__ __launch_bounds__(kRT) void ransac_kernel(const ps_feature_match *matches, const float *weights,
+                                                     const ps_ransac_problem *probs, RansacParamsDev prm,
+                                                     double *work_d, uint32_t *work_u, uint32_t *inliers,
+                                                     ps_ransac_result *outs) {
+    __shared__ Shared S;
+    __shared__ double s_H[9];
+    __shared__ uint32_t s_flag;
+    const int tid = threadIdx.x;
+    const ps_ransac_problem P = probs[blockIdx.x];
+    const uint32_t n = P.n;
+    const ps_feature_match *m = matches + P.offset;
+    const float *wts = prm.sampler == 2 ? weights + P.offset : nullptr;
+    double *cdf = work_d + (size_t)P.offset * 2;           // n doubles
+    double *ebuf = cdf + n;                                 // n doubles
+    uint32_t *ibuf = work_u + (size_t)P.offset * 2;         // n u32
+    uint32_t *sim_inl = ibuf + n;                           // n u32
+    uint32_t *out_inl = inliers + P.offset;
+    ps_ransac_result &O = outs[blockIdx.x];
+
+    if (tid == 0) {
+        O.has_model = 0;
+        O.n_inliers = 0;
+        O.hyps = 0;
+        O.has_sim = 0;
+        O.n_sim = 0;
+    }
+    if (n == 0) return;
+
+    // ---------------- similarity RANSAC (pipeline.rs:2055-2191)
+    const float fth = fmaxf(prm.reproj, 0.5f), ftsq = fth * fth;
+    Sim best_sim{1.0f, 0.0f, 0.0f, 0.0f};
+    bool have_sim = false;
+    uint32_t n_sim = 0;
+    // reference weights 1 / clamp(ratio) (pipeline.rs:2006-2028), prefix sums in f64 in order
+    auto build_cdf = [&](bool ref_weights) {
+        if (tid == 0) {
+            double acc = 0.0;
+            for (uint32_t i = 0; i < n; i++) {
+                double wv;
+                if (!ref_weights) {
+                    wv = (double)wts[i];
+                } else {
+                    float rr = m[i].ratio;
+                    double r = isfinite(rr) ? (double)rr : 1.0;
+                    if (isfinite(rr)) {
+                        if (r < 0.05) r = 0.05;
+                        if (r > 1.0) r = 1.0;
+                    }
+                    wv = 1.0 / r;
+                }
+                acc += wv;
+                cdf[i] = acc;
+            }
+            if (acc > 0.0) {
+                double inv = 1.0 / acc;
+                for (uint32_t i = 0; i < n; i++) cdf[i] *= inv;
+            }
+            cdf[n - 1] = 1.0;
+        }
+        __syncthreads();
+    };
+
+    if (prm.sim_seed_model) {
+        if (n == 1) {
+            best_sim = Sim{1.0f, 0.0f, m[0].dx, m[0].dy};
+            have_sim = true;
+            n_sim = 1;
+            if (tid == 0) sim_inl[0] = 0;
+        } else {
+            // the 32 smallest (ratio, index): ratios order as their sign-adjusted bit patterns
+            __shared__ uint64_t s_sorted[32];
+            __shared__ uint64_t s_red[kRT];
+            const uint32_t ns = n < 32 ? n : 32;
+            uint64_t prev = 0;
+            for (uint32_t r = 0; r < ns; r++) {
+                uint64_t mine = ~0ull;
+                for (uint32_t i = tid; i < n; i += kRT) {
+                    uint32_t b = __float_as_uint(m[i].ratio);
+                    b = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+                    uint64_t key = ((uint64_t)b << 32) | i;
+                    if ((r == 0 || key > prev) && key < mine) mine = key;
+                }
+                s_red[tid] = mine;
+                __syncthreads();
+                for (int o = kRT / 2; o > 0; o >>= 1) {
+                    if (tid < o && s_red[tid + o] < s_red[tid]) s_red[tid] = s_red[tid + o];
+                    __syncthreads();
+                }
+                prev = s_red[0];
+                if (tid == 0) s_sorted[r] = prev;
+                __syncthreads();
+            }
+            // initial translation candidates (pipeline.rs:2080-2101), one per thread
+            __shared__ uint32_t s_cc[32];
+            __shared__ float s_ce[32];
+            if (tid < (int)ns) {
+                const uint32_t si = (uint32_t)s_sorted[tid];
+                float e;
+                s_cc[tid] = eval_sim(m, n, 1.0f, 0.0f, m[si].dx, m[si].dy, ftsq, &e);
+                s_ce[tid] = e;
+            }
+            __syncthreads();
+            {
+                const uint32_t s0 = (uint32_t)s_sorted[0];
+                best_sim = Sim{1.0f, 0.0f, m[s0].dx, m[s0].dy};
+            }
+            uint32_t bc = s_cc[0];
+            float berr = s_ce[0];
+            for (uint32_t t = 0; t < ns; t++) {
+                const uint32_t si = (uint32_t)s_sorted[t];
+                if (s_cc[t] > bc || (s_cc[t] == bc && s_ce[t] < berr)) {
+                    bc = s_cc[t];
+                    berr = s_ce[t];
+                    best_sim = Sim{1.0f, 0.0f, m[si].dx, m[si].dy};
+                }
+            }
+            build_cdf(true);
+            uint32_t max_it = adaptive_iters((double)bc, (double)n, 2, 0.999, 64, 1000), it = 0;
+            __shared__ uint32_t s_sc[kRT], s_sv[kRT];
+            __shared__ float s_se[kRT], s_sm[kRT][4];
+            while (it < max_it) {
+                // hypotheses it+1 .. it+kRT, one per thread
+                const uint32_t t = it + tid;   // 0-based iteration index
+                {
+                    uint32_t i1 = cdf_pick(cdf, n, unit_draw(splitmix_at(P.sim_seed, 2ull * t)));
+                    uint32_t i2 = cdf_pick(cdf, n, unit_draw(splitmix_at(P.sim_seed, 2ull * t + 1)));
+                    if (i2 == i1) i2 = (i1 + 1) % n;
+                    const ps_feature_match &m1 = m[i1], &m2 = m[i2];
+                    float dxq = m2.qx - m1.qx, dyq = m2.qy - m1.qy, dq = dxq * dxq + dyq * dyq;
+                    uint32_t valid = 0, c = 0;
+                    float e = 0.0f, a = 0.0f, b = 0.0f, tx = 0.0f, ty = 0.0f;
+                    if (!(dq < 16.0f)) {
+                        float dxt = m2.tx - m1.tx, dyt = m2.ty - m1.ty;
+                        a = (dxq * dxt + dyq * dyt) / dq;
+                        b = (dxq * dyt - dyq * dxt) / dq;
+                        float s2 = a * a + b * b;
+                        if (s2 >= 0.04f && s2 <= 25.0f) {
+                            tx = m1.tx - (a * m1.qx - b * m1.qy);
+                            ty = m1.ty - (b * m1.qx + a * m1.qy);
+                            c = eval_sim(m, n, a, b, tx, ty, ftsq, &e);
+                            valid = 1;
+                        }
+                    }
+                    s_sv[tid] = valid;
+                    s_sc[tid] = c;
+                    s_se[tid] = e;
+                    s_sm[tid][0] = a;
+                    s_sm[tid][1] = b;
+                    s_sm[tid][2] = tx;
+                    s_sm[tid][3] = ty;
+                }
+                __syncthreads();
+                // replay in order (every thread, identically)
+                for (int k = 0; k < kRT && it < max_it; k++) {
+                    it++;
+                    if (!s_sv[k]) continue;
+                    if (s_sc[k] > bc || (s_sc[k] == bc && s_se[k] < berr)) {
+                        bc = s_sc[k];
+                        berr = s_se[k];
+                        best_sim = Sim{s_sm[k][0], s_sm[k][1], s_sm[k][2], s_sm[k][3]};
+                        max_it = adaptive_iters((double)bc, (double)n, 2, 0.999, 64, 1000);
+                    }
+                }
+                __syncthreads();
+            }
+            // final inliers, refit, refine (pipeline.rs:2150-2191)
+            uint32_t ni = ordered_compact(n, sim_inl, S, [&](uint32_t i) { return sim_inlier(m[i], best_sim, ftsq); });
+            __shared__ float s_ref[4];
+            __shared__ int s_has_ref;
+            if (tid == 0) {
+                Sim ref;
+                s_has_ref = fit_procrustes(m, sim_inl, ni, &ref);
+                s_ref[0] = ref.a;
+                s_ref[1] = ref.b;
+                s_ref[2] = ref.tx;
+                s_ref[3] = ref.ty;
+            }
+            __syncthreads();
+            const bool has_ref = s_has_ref;
+            const Sim ref{s_ref[0], s_ref[1], s_ref[2], s_ref[3]};
+            if (has_ref) {
+                const uint32_t nr = ordered_compact(n, ibuf, S, [&](uint32_t i) { return sim_inlier(m[i], ref, ftsq); });
+                if (nr >= ni) {
+                    for (uint32_t i = tid; i < nr; i += kRT) sim_inl[i] = ibuf[i];
+                    ni = nr;
+                }
+                __syncthreads();
+            }
+            if (tid == 0) {
+                Sim fin;
+                int hf = fit_procrustes(m, sim_inl, ni, &fin);
+                if (hf) {
+                    s_ref[0] = fin.a;
+                    s_ref[1] = fin.b;
+                    s_ref[2] = fin.tx;
+                    s_ref[3] = fin.ty;
+                }
+                s_has_ref = hf || has_ref;
+            }
+            __syncthreads();
+            have_sim = s_has_ref;
+            best_sim = Sim{s_ref[0], s_ref[1], s_ref[2], s_ref[3]};
+            n_sim = ni;
+            __syncthreads();
+        }
+        if (tid == 0) {
+            O.has_sim = have_sim;
+            O.n_sim = n_sim;
+            O.sim[0] = best_sim.a;
+            O.sim[1] = best_sim.b;
+            O.sim[2] = best_sim.tx;
+            O.sim[3] = best_sim.ty;
+        }
+    }
+
+    // ---------------- homography LO-RANSAC (pipeline.rs:2663-2755)
+    const double thresh = (double)fth, tsq = thresh * thresh;
+    bool have = false;
+    uint32_t bc = 0;
+    double be = INFINITY;
+    if (prm.sim_seed_model && have_sim) {
+        if (tid < 9) {
+            const double Hs[9] = {(double)best_sim.a, -(double)best_sim.b, (double)best_sim.tx,
+                                  (double)best_sim.b, (double)best_sim.a, (double)best_sim.ty, 0.0, 0.0, 1.0};
+            s_H[tid] = Hs[tid];
+        }
+        __syncthreads();
+        have = true;
+        bc = eval_h_block(m, n, s_H, tsq, nullptr, ebuf, &be, nullptr);
+    }
+    if (n < 4) {
+        if (prm.sim_seed_model) {
+            for (uint32_t i = tid; i < n_sim; i += kRT) out_inl[i] = sim_inl[i];
+            if (tid == 0) {
+                O.n_inliers = n_sim;
+                if (have) {
+                    for (int i = 0; i < 9; i++) O.h[i] = (float)s_H[i];
+                    O.has_model = 1;
+                }
+            }
+        }
+        return;
+    }
+    const int sampler = prm.sampler;
+    if (sampler == 0) {
+        if (!prm.sim_seed_model || n == 1) build_cdf(true);   // otherwise the similarity stage built it
+    } else if (sampler == 2) {
+        build_cdf(false);
+    }
+    double wsum = 0.0;
+    if (sampler == 2) {
+        if (tid == 0) {
+            double acc = 0.0;
+            for (uint32_t i = 0; i < n; i++) acc += (double)wts[i];
+            S.red[0] = acc;
+        }
+        __syncthreads();
+        wsum = S.red[0];
+        __syncthreads();
+    }
+    const uint32_t min_it = prm.min_iters, cap = prm.max_iters;
+    uint32_t max_it;
+    auto bound_from = [&](uint32_t cnt, double bmass) -> uint32_t {
+        if (!prm.adaptive) return cap;
+        if (sampler == 2)
+            return wsum > 0 ? adaptive_iters((double)(uint64_t)(bmass / wsum * 1e9), 1e9, 4, prm.confidence, min_it, cap)
+                            : cap;
+        return adaptive_iters((double)cnt, (double)n, 4, prm.confidence, min_it, cap);
+    };
+    {
+        double bmass = 0.0;
+        if (prm.adaptive && sampler == 2 && have) {
+            double e2;
+            eval_h_block(m, n, s_H, tsq, wts, ebuf, &e2, &bmass);
+        }
+        if (!prm.adaptive) max_it = cap;
+        else if (sampler == 2) max_it = have && wsum > 0 ? bound_from(bc, bmass) : cap;
+        else max_it = bound_from(bc, 0.0);
+    }
+    uint32_t it = 0;
+    uint64_t draw = 0;   // next position in the homography SplitMix64 stream
+    while (it < max_it) {
+        // 1. group draws into the next kRT iterations (thread 0; draws computed in parallel)
+        if (tid == 0) {
+            S.n_quads = 0;
+            S.n_draws = 0;
+            S.draw_pos = 0;
+        }
+        __syncthreads();
+        {
+            uint32_t chosen = 0, att = 0;
+            uint32_t q[4] = {0, 0, 0, 0};
+            while (true) {
+                if (S.draw_pos >= S.n_draws) {
+                    __syncthreads();
+                    // refill 1024 draws
+                    for (int k = tid; k < 1024; k += kRT) {
+                        double u = unit_draw(splitmix_at(P.hom_seed, draw + (uint64_t)k));
+                        S.draws[k] = sampler == 1 ? uniform_pick(n, u) : cdf_pick(cdf, n, u);
+                    }
+                    __syncthreads();
+                    if (tid == 0) {
+                        S.n_draws = 1024;
+                        S.draw_pos = 0;
+                    }
+                    draw += 1024;
+                    __syncthreads();
+                }
+                if (tid == 0) {
+                    while (S.draw_pos < S.n_draws && S.n_quads < kRT) {
+                        const uint32_t c = S.draws[S.draw_pos++];
+                        att++;
+                        bool dup = false;
+                        for (uint32_t j = 0; j < chosen; j++) dup |= q[j] == c;
+                        if (!dup) q[chosen++] = c;
+                        if (chosen == 4 || att == 16) {
+                            const uint32_t k = S.n_quads++;
+                            S.qvalid[k] = chosen == 4;
+                            for (int j = 0; j < 4; j++) S.quad[k][j] = q[j];
+                            chosen = 0;
+                            att = 0;
+                        }
+                    }
+                }
+                __syncthreads();
+                if (S.n_quads >= (uint32_t)kRT) break;
+                __syncthreads();
+            }
+            // unconsumed draws of the buffer: rewind the stream position to the first one
+            __syncthreads();
+            draw -= (uint64_t)(S.n_draws - S.draw_pos);
+            __syncthreads();
+        }
+        // 2. evaluate the kRT hypotheses, one per thread
+        {
+            Hyp &h = S.hyp[tid];
+            h.valid = 0;
+            h.count = 0;
+            h.err = 0.0;
+            h.mass = 0.0;
+            if (S.qvalid[tid]) {
+                const uint32_t idx[4] = {S.quad[tid][0], S.quad[tid][1], S.quad[tid][2], S.quad[tid][3]};
+                double qx[4], qy[4], tx[4], ty[4];
+                for (int i = 0; i < 4; i++) {
+                    qx[i] = m[idx[i]].qx;
+                    qy[i] = m[idx[i]].qy;
+                    tx[i] = m[idx[i]].tx;
+                    ty[i] = m[idx[i]].ty;
+                }
+                if (!degenerate_tri(qx, qy) && !degenerate_tri(tx, ty)) {
+                    double Hc[3][3];
+                    if (fit_dlt4(m, idx, Hc)) {
+                        for (int i = 0; i < 9; i++) h.H[i] = Hc[i / 3][i % 3];
+                        double e, ms = 0.0;
+                        h.count = eval_h_seq(m, n, h.H, tsq, &e, wts, &ms);
+                        h.err = e;
+                        h.mass = ms;
+                        h.valid = 1;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // 3. replay in order
+        for (int k = 0; k < kRT && it < max_it; k++) {
+            it++;
+            const Hyp &h = S.hyp[k];
+            if (!h.valid) continue;
+            uint32_t c = h.count;
+            double e = h.err;
+            if (c > bc || (c == bc && e < be)) {
+                __syncthreads();
+                if (tid < 9) s_H[tid] = h.H[tid];
+                __syncthreads();
+                if (prm.local_opt) local_opt_block(m, n, s_H, &c, &e, tsq, ibuf, ebuf, S);
+                bc = c;
+                be = e;
+                have = true;
+                if (prm.adaptive) {
+                    if (sampler == 2) {
+                        double e2, bmass = 0.0;
+                        eval_h_block(m, n, s_H, tsq, wts, ebuf, &e2, &bmass);
+                        max_it = wsum > 0 ? bound_from(bc, bmass) : cap;
+                    } else {
+                        max_it = bound_from(bc, 0.0);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+    uint32_t ni;
+    if (!have) {
+        ni = n_sim;
+        for (uint32_t i = tid; i < ni; i += kRT) out_inl[i] = sim_inl[i];
+    } else {
+        if (prm.local_opt) local_opt_block(m, n, s_H, &bc, &be, tsq, ibuf, ebuf, S);
+        ni = ordered_compact(n, out_inl, S, [&](uint32_t i) {
+            double e = reproj_err(m[i], s_H);
+            return e >= 0.0 && e <= tsq;
+        });
+    }
+    if (tid == 0) {
+        O.hyps = it;
+        O.n_inliers = ni;
+        if (have) {
+            for (int i = 0; i < 9; i++) O.h[i] = (float)s_H[i];
+            O.has_model = 1;
+        }
+    }
+    (void)s_flag;
+}
+
+}  // namespace
+
+size_t ransac_work_bytes(size_t total_matches) {
+    return total_matches * 2 * sizeof(double) + total_matches * 2 * sizeof(uint32_t) + 512;
+}
+
+hipError_t launch_ransac(const ps_feature_match *matches, const float *weights, const ps_ransac_problem *probs,
+                         uint32_t n_probs, const ps_ransac_params &p, size_t total_matches, void *work,
+                         uint32_t *inliers, ps_ransac_result *outs, hipStream_t st) {
+    if (n_probs == 0) return hipSuccess;
+    RansacParamsDev prm;
+    prm.reproj = p.reproj_threshold;
+    prm.sampler = p.sampler;
+    prm.confidence = p.confidence;
+    prm.min_iters = p.min_iters;
+    prm.max_iters = p.max_iters;
+    prm.local_opt = p.local_opt;
+    prm.sim_seed_model = p.sim_seed_model;
+    prm.adaptive = p.adaptive;
+    double *wd = static_cast<double *>(work);
+    uint32_t *wu = reinterpret_cast<uint32_t *>(wd + total_matches * 2);
+    ransac_kernel<<<n_probs, kRT, 0, st>>>(matches, weights, probs, prm, wd, wu, inliers, outs);
+    return hipGetLastError();
+}
+
+}  // namespace ps

@@ -70,6 +70,22 @@ class _MatchOptionsC(C.Structure):
     _fields_ = [("ratio", C.c_float), ("mutual", C.c_uint8), ("precision", C.c_uint8), ("_pad", C.c_uint8 * 2)]
 
 
+class _RansacOptionsC(C.Structure):
+    _fields_ = [("reproj_threshold", C.c_float), ("has_seed", C.c_uint8), ("_pad", C.c_uint8 * 3),
+                ("seed", C.c_uint64)]
+
+
+class _RansacParamsC(C.Structure):
+    _fields_ = [("reproj_threshold", C.c_float), ("sampler", C.c_uint8), ("local_opt", C.c_uint8),
+                ("sim_seed_model", C.c_uint8), ("adaptive", C.c_uint8), ("confidence", C.c_double),
+                ("min_iters", C.c_uint32), ("max_iters", C.c_uint32)]
+
+
+RANSAC_PROBLEM_DTYPE = np.dtype([("offset", "<u4"), ("n", "<u4"), ("sim_seed", "<u8"), ("hom_seed", "<u8")])
+RANSAC_RESULT_DTYPE = np.dtype([("h", "<f4", (9,)), ("has_model", "<u4"), ("n_inliers", "<u4"), ("hyps", "<u4"),
+                                ("sim", "<f4", (4,)), ("has_sim", "<u4"), ("n_sim", "<u4")])
+SAMPLER_REFERENCE, SAMPLER_UNIFORM, SAMPLER_WEIGHTS = 0, 1, 2
+
 FEAT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("scale", "<f4"), ("response", "<f4"),
                        ("orientation", "<f4")])
 KP_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("scale", "<f4"), ("response", "<f4"),
@@ -113,6 +129,10 @@ def lib() -> C.CDLL:
         L.ps_debug_fail_next.argtypes = []
         L.ps_engine_profile_begin.argtypes = [vp, C.c_uint32]
         L.ps_engine_profile_end.argtypes = [vp, C.POINTER(C.c_float), u32p]
+        L.ps_homography_estimate.argtypes = [vp, sz, C.POINTER(_RansacOptionsC), vp, C.POINTER(sz), vp,
+                                             C.POINTER(C.c_int32)]
+        L.ps_ransac_batch.argtypes = [vp, vp, vp, C.c_uint32, sz, C.POINTER(_RansacParamsC), vp, vp]
+        L.ps_engine_ransac_batch.argtypes = [vp, vp, vp, vp, C.c_uint32, sz, C.POINTER(_RansacParamsC), vp, vp]
         L.ps_nn2_precision.argtypes = [vp, sz, vp, sz, C.c_int, vp, vp, vp, vp]
         L.ps_engine_nn2.argtypes = [vp, vp, u32p, sz, vp, u32p, sz, C.c_int, vp, vp, vp, vp]
         L.ps_quantize_u128.argtypes = [vp, sz, vp]
@@ -122,7 +142,7 @@ def lib() -> C.CDLL:
         L.ps_engine_quantize_u128.argtypes = [vp, vp, sz, vp]
         L.ps_engine_hamming_topk.argtypes = [vp, vp, sz, vp, sz, C.c_uint32, vp, vp]
         L.ps_engine_rerank_l2.argtypes = [vp, vp, sz, vp, sz, vp, C.c_uint32, vp, vp]
-        for name in ("ps_nn2_precision", "ps_engine_nn2", "ps_quantize_u128", "ps_count_coarse_hamming_votes", "ps_hamming_topk", "ps_rerank_l2",
+        for name in ("ps_homography_estimate", "ps_ransac_batch", "ps_engine_ransac_batch", "ps_nn2_precision", "ps_engine_nn2", "ps_quantize_u128", "ps_count_coarse_hamming_votes", "ps_hamming_topk", "ps_rerank_l2",
                      "ps_engine_quantize_u128", "ps_engine_hamming_topk", "ps_engine_rerank_l2","ps_init", "ps_validate_sift_config", "ps_validate_match_options", "ps_detect_and_compute",
                      "ps_detect_and_compute_u8", "ps_detect_keypoints", "ps_match", "ps_build_pyramid", "ps_nn2",
                      "ps_engine_create", "ps_engine_destroy", "ps_engine_sync", "ps_engine_check",
@@ -360,6 +380,48 @@ def nn2_precision(q_desc: np.ndarray, t_desc: np.ndarray, precision: int, with_a
     check(lib().ps_nn2_precision(q.ctypes.data, len(q), t.ctypes.data, len(t), precision, bi.ctypes.data,
                                  b.ctypes.data, s.ctypes.data, ap.ctypes.data if with_approx else None))
     return (bi, b, s, ap) if with_approx else (bi, b, s)
+
+
+@dataclasses.dataclass
+class RansacOptions:
+    """pipeline.rs:179-213"""
+    reproj_threshold: float = 3.0
+    seed: Optional[int] = None
+
+    def _c(self) -> _RansacOptionsC:
+        return _RansacOptionsC(self.reproj_threshold, 0 if self.seed is None else 1, (C.c_uint8 * 3)(),
+                               0 if self.seed is None else int(self.seed))
+
+
+def homography_estimate(matches: np.ndarray, options: RansacOptions = RansacOptions()):
+    """Homography::estimate (pipeline.rs:225-235): (inlier indices, 3x3 H or None)."""
+    m = np.ascontiguousarray(matches, MATCH_DTYPE)
+    inl = np.zeros(max(len(m), 1), np.uint32)
+    h = np.zeros(9, np.float32)
+    n = C.c_size_t(0)
+    has = C.c_int32(0)
+    c = options._c()
+    check(lib().ps_homography_estimate(m.ctypes.data, len(m), C.byref(c), inl.ctypes.data, C.byref(n), h.ctypes.data,
+                                       C.byref(has)))
+    return inl[:n.value].copy(), (h.reshape(3, 3).copy() if has.value else None)
+
+
+def ransac_params(sampler=SAMPLER_REFERENCE, reproj_threshold=3.0, confidence=0.999, min_iters=64, max_iters=2000,
+                  local_opt=True, sim_seed_model=True, adaptive=True) -> _RansacParamsC:
+    return _RansacParamsC(reproj_threshold, sampler, int(local_opt), int(sim_seed_model), int(adaptive), confidence,
+                          min_iters, max_iters)
+
+
+def ransac_batch(matches: np.ndarray, problems: np.ndarray, params: _RansacParamsC, weights=None):
+    """Batched RANSAC: (results [RANSAC_RESULT_DTYPE], inlier index array aligned with matches)."""
+    m = np.ascontiguousarray(matches, MATCH_DTYPE)
+    pr = np.ascontiguousarray(problems, RANSAC_PROBLEM_DTYPE)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+    res = np.zeros(len(pr), RANSAC_RESULT_DTYPE)
+    inl = np.zeros(max(len(m), 1), np.uint32)
+    check(lib().ps_ransac_batch(m.ctypes.data, None if w is None else w.ctypes.data, pr.ctypes.data, len(pr), len(m),
+                                C.byref(params), inl.ctypes.data, res.ctypes.data))
+    return res, inl
 
 
 HAMMING_MAX_K = 1024
