@@ -245,6 +245,37 @@ ps_status ps_ransac_batch(const ps_feature_match *matches, const float *weights,
                           uint32_t n_problems, size_t total_matches, const ps_ransac_params *params,
                           uint32_t *inliers, ps_ransac_result *results);
 
+/* ---- learned proposal ordering for RANSAC: the match-scoring MLP ----------------------
+ * No counterpart in the reference (it samples by 1 / ratio, pipeline.rs:2006-2028). An
+ * 8 -> 64 -> 64 -> 1 network scores each putative match; its sigmoid output is the sampling
+ * weight of PS_SAMPLER_WEIGHTS. Parameters: 4801 f32 masters laid out as W1[64][8], b1[64],
+ * W2[64][64], b2[64], w3[64], b3; the network computes with their BF16 values and FP32
+ * accumulation. Inputs (8 per match): ratio (NaN -> 1, capped at 2), log(scale_t / scale_q),
+ * sin and cos of (orientation_t - orientation_q), descriptor distance, query x / width,
+ * query y / height, rank of the match by (ratio, index) within its pair / match count. */
+#define PS_MLP_PARAMS 4801
+#define PS_MLP_FEATURES 8
+typedef struct {
+    uint32_t match_offset, n;        /* the pair's matches in the batch array */
+    uint32_t qfeat_offset, tfeat_offset;   /* its query / target features (indexed by the matches) */
+    float width, height;             /* query image size */
+} ps_mlp_pair;
+ps_status ps_engine_mlp_features(ps_engine *e, const ps_feature_match *d_matches, const ps_feature *d_qfeat,
+                                 const ps_feature *d_tfeat, const ps_mlp_pair *d_pairs, uint32_t n_pairs,
+                                 float *d_x);
+/* logits (and probabilities, when d_prob is not null) of n matches */
+ps_status ps_engine_mlp_forward(ps_engine *e, const float *d_params, const float *d_x, size_t n, float *d_logit,
+                                float *d_prob);
+/* mean binary cross-entropy (with logits) over n labelled matches and its gradient */
+ps_status ps_engine_mlp_grad(ps_engine *e, const float *d_params, const float *d_x, const float *d_y, size_t n,
+                             float *d_grad, float *d_loss);
+/* Adam step (beta 0.9 / 0.999, eps 1e-8) on the masters; step counts from 1 */
+ps_status ps_engine_mlp_adam(ps_engine *e, float *d_params, float *d_m, float *d_v, const float *d_grad,
+                             uint32_t step, float lr);
+/* host-pointer forms for checks */
+ps_status ps_mlp_forward(const float *params, const float *x, size_t n, float *logit);
+ps_status ps_mlp_loss_grad(const float *params, const float *x, const float *y, size_t n, float *grad, float *loss);
+
 /* ---- 128-bit descriptor codes and the catalog shortlist ------------------------------
  * A code is two little-endian u64 words per descriptor: bit i (0..127) of the code is bit
  * i % 64 of word i / 64, the layout of the reference's u128 signature (catalog.rs:436). */

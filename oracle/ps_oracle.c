@@ -1396,3 +1396,78 @@ size_t psr_homography_ransac_ref(const psr_match *m, size_t n, float thresh, uin
     else { sim_seed = seed; hom_seed = seed ^ 0x9E3779B97F4A7C15ull; }                         /* pipeline.rs:2650-2661 */
     return psr_homography_ransac(m, n, thresh, sim_seed, hom_seed, &prm, inl, h_out, has_model, NULL);
 }
+
+/* ---------------------------------------------------------------- match-scoring MLP
+ * 8 -> 64 (ReLU) -> 64 (ReLU) -> 1 logit; parameters (4801 f32) laid out as
+ * W1[64][8], b1[64], W2[64][64], b2[64], w3[64], b3. Weights are used as their BF16
+ * (round-to-nearest-even) values when bf16 != 0; arithmetic in f64. Loss: mean binary
+ * cross-entropy with logits. The gradient is with respect to the (rounded) weights. */
+static double bf16_round(float x) {
+    uint32_t b;
+    memcpy(&b, &x, 4);
+    b = (b + 0x7fffu + ((b >> 16) & 1u)) & 0xffff0000u;
+    float r;
+    memcpy(&r, &b, 4);
+    return (double)r;
+}
+
+void psr_mlp_forward(const float *params, int bf16, const float *x, size_t n, float *logit) {
+    double P[PSR_MLP_PARAMS];
+    for (int i = 0; i < PSR_MLP_PARAMS; i++) P[i] = bf16 ? bf16_round(params[i]) : (double)params[i];
+    const double *W1 = P, *b1 = P + 512, *W2 = P + 576, *b2 = P + 4672, *w3 = P + 4736, *b3 = P + 4800;
+    for (size_t s = 0; s < n; s++) {
+        double h1[64], z = b3[0];
+        for (int i = 0; i < 64; i++) {
+            double a = b1[i];
+            for (int k = 0; k < 8; k++) a += W1[i * 8 + k] * (double)x[s * 8 + k];
+            h1[i] = a > 0 ? a : 0;
+        }
+        for (int i = 0; i < 64; i++) {
+            double a = b2[i];
+            for (int j = 0; j < 64; j++) a += W2[i * 64 + j] * h1[j];
+            z += w3[i] * (a > 0 ? a : 0);
+        }
+        logit[s] = (float)z;
+    }
+}
+
+double psr_mlp_loss_grad(const float *params, int bf16, const float *x, const float *y, size_t n, double *grad) {
+    double P[PSR_MLP_PARAMS];
+    for (int i = 0; i < PSR_MLP_PARAMS; i++) { P[i] = bf16 ? bf16_round(params[i]) : (double)params[i]; grad[i] = 0.0; }
+    const double *W1 = P, *b1 = P + 512, *W2 = P + 576, *b2 = P + 4672, *w3 = P + 4736, *b3 = P + 4800;
+    double *gW1 = grad, *gb1 = grad + 512, *gW2 = grad + 576, *gb2 = grad + 4672, *gw3 = grad + 4736, *gb3 = grad + 4800;
+    double loss = 0.0, inv = n ? 1.0 / (double)n : 0.0;
+    for (size_t s = 0; s < n; s++) {
+        double z1[64], h1[64], z2[64], h2[64], z = b3[0];
+        for (int i = 0; i < 64; i++) {
+            double a = b1[i];
+            for (int k = 0; k < 8; k++) a += W1[i * 8 + k] * (double)x[s * 8 + k];
+            z1[i] = a; h1[i] = a > 0 ? a : 0;
+        }
+        for (int i = 0; i < 64; i++) {
+            double a = b2[i];
+            for (int j = 0; j < 64; j++) a += W2[i * 64 + j] * h1[j];
+            z2[i] = a; h2[i] = a > 0 ? a : 0;
+            z += w3[i] * h2[i];
+        }
+        double t = (double)y[s];
+        loss += (z > 0 ? z : 0) - z * t + log1p(exp(-fabs(z)));
+        double dz = (1.0 / (1.0 + exp(-z)) - t) * inv;
+        gb3[0] += dz;
+        double dz2[64], dh1[64];
+        for (int i = 0; i < 64; i++) {
+            gw3[i] += dz * h2[i];
+            dz2[i] = z2[i] > 0 ? dz * w3[i] : 0.0;
+            gb2[i] += dz2[i];
+        }
+        for (int j = 0; j < 64; j++) dh1[j] = 0.0;
+        for (int i = 0; i < 64; i++)
+            for (int j = 0; j < 64; j++) { gW2[i * 64 + j] += dz2[i] * h1[j]; dh1[j] += W2[i * 64 + j] * dz2[i]; }
+        for (int i = 0; i < 64; i++) {
+            double d1 = z1[i] > 0 ? dh1[i] : 0.0;
+            gb1[i] += d1;
+            for (int k = 0; k < 8; k++) gW1[i * 8 + k] += d1 * (double)x[s * 8 + k];
+        }
+    }
+    return loss * inv;
+}

@@ -132,6 +132,11 @@ int psr_fit_homography_dlt(const psr_match *m, size_t n, double h_out[9]);
 size_t psr_adaptive_ransac_iters(size_t inl, size_t total, int sample_size, double conf,
                                  size_t min_iters, size_t max_cap);
 
+/* match-scoring MLP 8-64-64-1 (4801 parameters), f64 arithmetic on (optionally BF16-rounded)
+ * weights; mean BCE-with-logits loss and its gradient */
+#define PSR_MLP_PARAMS 4801
+void psr_mlp_forward(const float *params, int bf16, const float *x, size_t n, float *logit);
+double psr_mlp_loss_grad(const float *params, int bf16, const float *x, const float *y, size_t n, double *grad);
 void psr_free(void *p);
 
 #ifdef __cplusplus

@@ -110,6 +110,11 @@ def lib():
                                        C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint8)]
         L.psr_rerank_l2.argtypes = [C.POINTER(C.c_float), C.c_size_t, C.POINTER(C.c_float), C.c_size_t,
                                     C.POINTER(C.c_uint32), C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]
+        L.psr_mlp_forward.argtypes = [C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float), C.c_size_t,
+                                      C.POINTER(C.c_float)]
+        L.psr_mlp_loss_grad.argtypes = [C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                        C.c_size_t, C.POINTER(C.c_double)]
+        L.psr_mlp_loss_grad.restype = C.c_double
         L.psr_splitmix64_next.argtypes = [C.POINTER(C.c_uint64)]
         L.psr_splitmix64_next.restype = C.c_uint64
         L.psr_filter_matches_ransac.argtypes = [C.POINTER(Match), C.c_size_t, C.c_float, C.c_uint64,
@@ -338,6 +343,49 @@ def rerank_l2(qd, cd, shortlist):
     lib().psr_rerank_l2(_fp(qd), nq, _fp(cd), len(cd), sl.ctypes.data_as(C.POINTER(C.c_uint32)), k,
                         idx.ctypes.data_as(C.POINTER(C.c_uint32)), _fp(d2))
     return idx, d2
+
+
+MLP_PARAMS = 4801
+
+
+def mlp_forward(params, x, bf16=True):
+    """Logits of the 8-64-64-1 match-scoring MLP (f64 arithmetic, BF16-rounded weights)."""
+    p = _f32(params)
+    x = _f32(x).reshape(-1, 8)
+    out = np.zeros(len(x), np.float32)
+    lib().psr_mlp_forward(_fp(p), int(bf16), _fp(x), len(x), _fp(out))
+    return out
+
+
+def mlp_loss_grad(params, x, y, bf16=True):
+    """(mean BCE-with-logits loss, gradient [4801] f64) of the match-scoring MLP."""
+    p = _f32(params)
+    x = _f32(x).reshape(-1, 8)
+    y = _f32(y)
+    g = np.zeros(MLP_PARAMS, np.float64)
+    loss = lib().psr_mlp_loss_grad(_fp(p), int(bf16), _fp(x), _fp(y), len(x), g.ctypes.data_as(C.POINTER(C.c_double)))
+    return loss, g
+
+
+def mlp_features(m, qfeat, tfeat, width, height):
+    """Match-scoring MLP inputs of one pair (float64 restatement of their definition):
+    ratio (NaN -> 1, capped at 2), log(scale_t / scale_q) (scales floored at 1e-6), sin and
+    cos of (orientation_t - orientation_q), distance, qx / width, qy / height, and the rank of
+    the match by (ratio, index) within the pair divided by the match count."""
+    m = np.asarray(m)
+    n = len(m)
+    r = np.where(np.isnan(m["ratio"]), 1.0, m["ratio"].astype(np.float64))
+    order = np.lexsort((np.arange(n), r))
+    rank = np.empty(n, np.float64)
+    rank[order] = np.arange(n)
+    qi = m["query_index"].astype(np.int64)
+    ti = m["target_index"].astype(np.int64)
+    sq = np.maximum(qfeat["scale"][qi].astype(np.float64), 1e-6)
+    st = np.maximum(tfeat["scale"][ti].astype(np.float64), 1e-6)
+    dth = tfeat["orientation"][ti].astype(np.float64) - qfeat["orientation"][qi].astype(np.float64)
+    x = np.stack([np.minimum(r, 2.0), np.log(st / sq), np.sin(dth), np.cos(dth), m["distance"].astype(np.float64),
+                  m["qx"] / width, m["qy"] / height, rank / max(n, 1)], 1)
+    return x
 
 
 def splitmix64(state: int, n: int):

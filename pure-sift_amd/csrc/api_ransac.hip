@@ -144,3 +144,98 @@ ps_status ps_engine_ransac_batch(ps_engine *e, const ps_feature_match *d_matches
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ match-scoring MLP
+extern "C" {
+
+ps_status ps_engine_mlp_features(ps_engine *e, const ps_feature_match *d_matches, const ps_feature *d_qfeat,
+                                 const ps_feature *d_tfeat, const ps_mlp_pair *d_pairs, uint32_t n_pairs,
+                                 float *d_x) {
+    PS_TRY(check_fail_injection());
+    if (!e || (n_pairs && (!d_matches || !d_qfeat || !d_tfeat || !d_pairs || !d_x)))
+        return set_err(PS_E_INVALID_ARG, "null argument");
+    PS_HIP(hipSetDevice(e->device));
+    PS_HIP(ps::launch_mlp_features(d_matches, d_qfeat, d_tfeat, d_pairs, n_pairs, d_x, e->stream));
+    return PS_OK;
+}
+
+ps_status ps_engine_mlp_forward(ps_engine *e, const float *d_params, const float *d_x, size_t n, float *d_logit,
+                                float *d_prob) {
+    PS_TRY(check_fail_injection());
+    if (!e || !d_params || (n && (!d_x || !d_logit))) return set_err(PS_E_INVALID_ARG, "null argument");
+    if (n >= 0xffffffffull) return set_err(PS_E_INVALID_ARG, "too many matches");
+    PS_HIP(hipSetDevice(e->device));
+    PS_HIP(ps::launch_mlp_forward(d_params, d_x, (uint32_t)n, d_logit, d_prob, e->stream));
+    return PS_OK;
+}
+
+ps_status ps_engine_mlp_grad(ps_engine *e, const float *d_params, const float *d_x, const float *d_y, size_t n,
+                             float *d_grad, float *d_loss) {
+    PS_TRY(check_fail_injection());
+    if (!e || !d_params || !d_grad || (n && (!d_x || !d_y))) return set_err(PS_E_INVALID_ARG, "null argument");
+    if (n >= 0xffffffffull) return set_err(PS_E_INVALID_ARG, "too many matches");
+    PS_HIP(hipSetDevice(e->device));
+    PS_TRY(ensure(e->aux[AUX_RS_6], ps::mlp_grad_scratch_bytes()));
+    PS_HIP(ps::launch_mlp_grad(d_params, d_x, d_y, (uint32_t)n, e->aux[AUX_RS_6].p, d_grad, d_loss, e->stream));
+    return PS_OK;
+}
+
+ps_status ps_engine_mlp_adam(ps_engine *e, float *d_params, float *d_m, float *d_v, const float *d_grad,
+                             uint32_t step, float lr) {
+    PS_TRY(check_fail_injection());
+    if (!e || !d_params || !d_m || !d_v || !d_grad) return set_err(PS_E_INVALID_ARG, "null argument");
+    if (step == 0) return set_err(PS_E_INVALID_ARG, "Adam steps count from 1");
+    PS_HIP(hipSetDevice(e->device));
+    PS_HIP(ps::launch_mlp_adam(d_params, d_m, d_v, d_grad, step, lr, e->stream));
+    return PS_OK;
+}
+
+ps_status ps_mlp_forward(const float *params, const float *x, size_t n, float *logit) {
+    PS_TRY(check_fail_injection());
+    if (!params || (n && (!x || !logit))) return set_err(PS_E_INVALID_ARG, "null argument");
+    if (n == 0) return PS_OK;
+    Borrow b;
+    PS_TRY(b.st);
+    ps_engine *e = b.e;
+    PS_HIP(hipSetDevice(e->device));
+    PS_TRY(ensure(e->aux[AUX_RS_7], PS_MLP_PARAMS * sizeof(float)));
+    PS_TRY(ensure(e->aux[AUX_PM_0], n * 8 * sizeof(float)));
+    PS_TRY(ensure(e->aux[AUX_PM_1], n * sizeof(float)));
+    PS_HIP(hipMemcpyAsync(e->aux[AUX_RS_7].p, params, PS_MLP_PARAMS * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    PS_HIP(hipMemcpyAsync(e->aux[AUX_PM_0].p, x, n * 8 * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    PS_TRY(ps_engine_mlp_forward(e, as<float>(e->aux[AUX_RS_7]), as<float>(e->aux[AUX_PM_0]), n,
+                                 as<float>(e->aux[AUX_PM_1]), nullptr));
+    PS_HIP(hipMemcpyAsync(logit, e->aux[AUX_PM_1].p, n * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    PS_HIP(hipStreamSynchronize(e->stream));
+    return PS_OK;
+}
+
+ps_status ps_mlp_loss_grad(const float *params, const float *x, const float *y, size_t n, float *grad, float *loss) {
+    PS_TRY(check_fail_injection());
+    if (!params || !grad || (n && (!x || !y))) return set_err(PS_E_INVALID_ARG, "null argument");
+    Borrow b;
+    PS_TRY(b.st);
+    ps_engine *e = b.e;
+    PS_HIP(hipSetDevice(e->device));
+    const size_t nn = n ? n : 1;
+    PS_TRY(ensure(e->aux[AUX_RS_7], PS_MLP_PARAMS * sizeof(float)));
+    PS_TRY(ensure(e->aux[AUX_PM_0], nn * 8 * sizeof(float)));
+    PS_TRY(ensure(e->aux[AUX_PM_1], nn * sizeof(float)));
+    PS_TRY(ensure(e->aux[AUX_PM_2], (PS_MLP_PARAMS + 1) * sizeof(float)));
+    PS_HIP(hipMemcpyAsync(e->aux[AUX_RS_7].p, params, PS_MLP_PARAMS * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    if (n) {
+        PS_HIP(hipMemcpyAsync(e->aux[AUX_PM_0].p, x, n * 8 * sizeof(float), hipMemcpyHostToDevice, e->stream));
+        PS_HIP(hipMemcpyAsync(e->aux[AUX_PM_1].p, y, n * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    }
+    float *g = as<float>(e->aux[AUX_PM_2]);
+    PS_TRY(ps_engine_mlp_grad(e, as<float>(e->aux[AUX_RS_7]), as<float>(e->aux[AUX_PM_0]), as<float>(e->aux[AUX_PM_1]),
+                              n, g, g + PS_MLP_PARAMS));
+    PS_HIP(hipMemcpyAsync(grad, g, PS_MLP_PARAMS * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    float l = 0.0f;
+    PS_HIP(hipMemcpyAsync(&l, g + PS_MLP_PARAMS, sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    PS_HIP(hipStreamSynchronize(e->stream));
+    if (loss) *loss = l;
+    return PS_OK;
+}
+
+}  // extern "C"

@@ -148,6 +148,17 @@ hipError_t launch_ransac(const ps_feature_match *matches, const float *weights, 
                          uint32_t n_probs, const ps_ransac_params &p, size_t total_matches, void *work,
                          uint32_t *inliers, ps_ransac_result *outs, hipStream_t st);
 
+// mlp.hip: match-scoring MLP (features, forward, gradient, Adam)
+size_t mlp_grad_scratch_bytes();
+hipError_t launch_mlp_forward(const float *params, const float *x, uint32_t n, float *logit, float *prob,
+                              hipStream_t st);
+hipError_t launch_mlp_grad(const float *params, const float *x, const float *y, uint32_t n, void *scratch,
+                           float *grad, float *loss, hipStream_t st);
+hipError_t launch_mlp_adam(float *params, float *m1, float *m2, const float *grad, uint32_t step, float lr,
+                           hipStream_t st);
+hipError_t launch_mlp_features(const ps_feature_match *m, const ps_feature *qf, const ps_feature *tf,
+                               const ps_mlp_pair *pairs, uint32_t n_pairs, float *x, hipStream_t st);
+
 // hamming.hip: 128-bit codes (catalog.rs:37-45, :707-728) and the top-k shortlist
 constexpr uint32_t kHamMaxK = 1024;       // largest shortlist length
 constexpr uint32_t kHamCandCap = 4096;    // candidate slots per query in the fast path

@@ -133,6 +133,12 @@ def lib() -> C.CDLL:
                                              C.POINTER(C.c_int32)]
         L.ps_ransac_batch.argtypes = [vp, vp, vp, C.c_uint32, sz, C.POINTER(_RansacParamsC), vp, vp]
         L.ps_engine_ransac_batch.argtypes = [vp, vp, vp, vp, C.c_uint32, sz, C.POINTER(_RansacParamsC), vp, vp]
+        L.ps_engine_mlp_features.argtypes = [vp, vp, vp, vp, vp, C.c_uint32, vp]
+        L.ps_engine_mlp_forward.argtypes = [vp, vp, vp, sz, vp, vp]
+        L.ps_engine_mlp_grad.argtypes = [vp, vp, vp, vp, sz, vp, vp]
+        L.ps_engine_mlp_adam.argtypes = [vp, vp, vp, vp, vp, C.c_uint32, C.c_float]
+        L.ps_mlp_forward.argtypes = [vp, vp, sz, vp]
+        L.ps_mlp_loss_grad.argtypes = [vp, vp, vp, sz, vp, vp]
         L.ps_nn2_precision.argtypes = [vp, sz, vp, sz, C.c_int, vp, vp, vp, vp]
         L.ps_engine_nn2.argtypes = [vp, vp, u32p, sz, vp, u32p, sz, C.c_int, vp, vp, vp, vp]
         L.ps_quantize_u128.argtypes = [vp, sz, vp]
@@ -142,7 +148,8 @@ def lib() -> C.CDLL:
         L.ps_engine_quantize_u128.argtypes = [vp, vp, sz, vp]
         L.ps_engine_hamming_topk.argtypes = [vp, vp, sz, vp, sz, C.c_uint32, vp, vp]
         L.ps_engine_rerank_l2.argtypes = [vp, vp, sz, vp, sz, vp, C.c_uint32, vp, vp]
-        for name in ("ps_homography_estimate", "ps_ransac_batch", "ps_engine_ransac_batch", "ps_nn2_precision", "ps_engine_nn2", "ps_quantize_u128", "ps_count_coarse_hamming_votes", "ps_hamming_topk", "ps_rerank_l2",
+        for name in ("ps_engine_mlp_features", "ps_engine_mlp_forward", "ps_engine_mlp_grad", "ps_engine_mlp_adam",
+                     "ps_mlp_forward", "ps_mlp_loss_grad", "ps_homography_estimate", "ps_ransac_batch", "ps_engine_ransac_batch", "ps_nn2_precision", "ps_engine_nn2", "ps_quantize_u128", "ps_count_coarse_hamming_votes", "ps_hamming_topk", "ps_rerank_l2",
                      "ps_engine_quantize_u128", "ps_engine_hamming_topk", "ps_engine_rerank_l2","ps_init", "ps_validate_sift_config", "ps_validate_match_options", "ps_detect_and_compute",
                      "ps_detect_and_compute_u8", "ps_detect_keypoints", "ps_match", "ps_build_pyramid", "ps_nn2",
                      "ps_engine_create", "ps_engine_destroy", "ps_engine_sync", "ps_engine_check",
@@ -424,6 +431,42 @@ def ransac_batch(matches: np.ndarray, problems: np.ndarray, params: _RansacParam
     return res, inl
 
 
+MLP_PARAMS, MLP_FEATURES = 4801, 8
+MLP_PAIR_DTYPE = np.dtype([("match_offset", "<u4"), ("n", "<u4"), ("qfeat_offset", "<u4"), ("tfeat_offset", "<u4"),
+                           ("width", "<f4"), ("height", "<f4")])
+
+
+def mlp_init(seed: int = 0) -> np.ndarray:
+    """He-uniform initial parameters of the 8-64-64-1 match-scoring MLP (f32 masters)."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(MLP_PARAMS, np.float32)
+    p[0:512] = rng.uniform(-1, 1, 512) * np.sqrt(6.0 / 8)
+    p[576:4672] = rng.uniform(-1, 1, 4096) * np.sqrt(6.0 / 64)
+    p[4736:4800] = rng.uniform(-1, 1, 64) * np.sqrt(6.0 / 64)
+    return p
+
+
+def mlp_forward(params: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """Logits of the match-scoring MLP (BF16 weights, FP32 accumulation) for n x 8 inputs."""
+    p = np.ascontiguousarray(params, np.float32)
+    xx = np.ascontiguousarray(x, np.float32).reshape(-1, 8)
+    out = np.zeros(len(xx), np.float32)
+    check(lib().ps_mlp_forward(p.ctypes.data, xx.ctypes.data, len(xx), out.ctypes.data))
+    return out
+
+
+def mlp_loss_grad(params: np.ndarray, x: np.ndarray, y: np.ndarray):
+    """(mean BCE-with-logits loss, gradient [4801]) computed by the GPU training kernels."""
+    p = np.ascontiguousarray(params, np.float32)
+    xx = np.ascontiguousarray(x, np.float32).reshape(-1, 8)
+    yy = np.ascontiguousarray(y, np.float32)
+    g = np.zeros(MLP_PARAMS, np.float32)
+    loss = C.c_float(0)
+    check(lib().ps_mlp_loss_grad(p.ctypes.data, xx.ctypes.data, yy.ctypes.data, len(xx), g.ctypes.data,
+                                 C.byref(loss)))
+    return loss.value, g
+
+
 HAMMING_MAX_K = 1024
 
 
@@ -513,6 +556,23 @@ class Engine:
         u = lambda p: C.cast(p, C.POINTER(C.c_uint32))
         check(lib().ps_engine_nn2(self.h, d_q, u(d_nq), nq_max, d_t, u(d_nt), nt_max, precision, d_best_idx,
                                   d_best, d_second, d_approx or None))
+
+    def ransac_batch(self, d_matches: int, d_weights: int, d_problems: int, n_problems: int, total_matches: int,
+                     params, d_inliers: int, d_results: int) -> None:
+        check(lib().ps_engine_ransac_batch(self.h, d_matches, d_weights or None, d_problems, n_problems, total_matches,
+                                           C.byref(params), d_inliers, d_results))
+
+    def mlp_features(self, d_matches: int, d_qfeat: int, d_tfeat: int, d_pairs: int, n_pairs: int, d_x: int) -> None:
+        check(lib().ps_engine_mlp_features(self.h, d_matches, d_qfeat, d_tfeat, d_pairs, n_pairs, d_x))
+
+    def mlp_forward(self, d_params: int, d_x: int, n: int, d_logit: int, d_prob: int = 0) -> None:
+        check(lib().ps_engine_mlp_forward(self.h, d_params, d_x, n, d_logit, d_prob or None))
+
+    def mlp_grad(self, d_params: int, d_x: int, d_y: int, n: int, d_grad: int, d_loss: int = 0) -> None:
+        check(lib().ps_engine_mlp_grad(self.h, d_params, d_x, d_y, n, d_grad, d_loss or None))
+
+    def mlp_adam(self, d_params: int, d_m: int, d_v: int, d_grad: int, step: int, lr: float) -> None:
+        check(lib().ps_engine_mlp_adam(self.h, d_params, d_m, d_v, d_grad, step, lr))
 
     def quantize_u128(self, d_desc: int, n: int, d_codes: int) -> None:
         check(lib().ps_engine_quantize_u128(self.h, d_desc, n, d_codes))

@@ -146,3 +146,33 @@ def test_variant_samplers_recover_model(sampler):
         _, _, hu = psr.homography_ransac(m, 3.0, 0, 0x5EED, sampler=1, confidence=0.99, min_iters=1,
                                          max_iters=100000, adaptive=True)
         assert hyps * 3 <= hu
+
+
+def test_mlp_gradient_finite_differences():
+    """The oracle's MLP gradient against central differences of its own loss (float64
+    arithmetic on f32 parameters; small steps stay off the ReLU kinks)."""
+    rng = np.random.default_rng(0)
+    p = (rng.standard_normal(psr.MLP_PARAMS) * 0.2).astype(np.float32)
+    x = rng.standard_normal((40, 8)).astype(np.float32)
+    y = (rng.random(40) < 0.5).astype(np.float32)
+    _, g = psr.mlp_loss_grad(p, x, y, bf16=False)
+    for i in [0, 77, 511, 512, 575, 600, 4000, 4671, 4672, 4700, 4736, 4799, 4800]:
+        lo, hi = p.copy(), p.copy()
+        hi[i] = np.float32(p[i] + 3e-5)
+        lo[i] = np.float32(p[i] - 3e-5)
+        fd = (psr.mlp_loss_grad(hi, x, y, bf16=False)[0] - psr.mlp_loss_grad(lo, x, y, bf16=False)[0]) / (
+            float(hi[i]) - float(lo[i]))
+        assert abs(fd - g[i]) <= 1e-4 * max(1.0, abs(g[i])) + 1e-7, (i, fd, g[i])
+
+
+def test_mlp_forward_known_answer():
+    """All-zero weights except b3 -> logit b3; one active path gives a closed form."""
+    p = np.zeros(psr.MLP_PARAMS, np.float32)
+    p[4800] = 0.75
+    x = np.ones((3, 8), np.float32)
+    assert np.all(psr.mlp_forward(p, x) == np.float32(0.75))
+    p[0] = 2.0          # W1[0][0]
+    p[576] = 3.0        # W2[0][0]
+    p[4736] = 0.5       # w3[0]
+    x = np.array([[1.0, 0, 0, 0, 0, 0, 0, 0], [-1.0, 0, 0, 0, 0, 0, 0, 0]], np.float32)
+    assert psr.mlp_forward(p, x).tolist() == [0.75 + 0.5 * 3.0 * 2.0, 0.75]
