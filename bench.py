@@ -207,6 +207,35 @@ def bench_precision(steps: int = 5, cpu_queries: int = 200) -> dict:
     return out
 
 
+def bench_ransac(n_pairs: int = 4096, n_base: int = 256, steps: int = 600, cpu_pairs: int = 64) -> dict:
+    """Learned-proposal RANSAC: synthetic 1024x768 warped pairs, GPU extraction + matching,
+    30-80% injected outliers, labels from a plain 10k-hypothesis RANSAC, MLP trained on the
+    GPU, then hypotheses to 99% confidence: MLP-guided vs uniform (and the reference's
+    1 / ratio sampler) on the held-out pairs; also with the outliers' distance and ratio
+    resampled from the putative matches (geometric cues only)."""
+    import numpy as np
+    import learned_ransac as lr
+    t0 = time.perf_counter()
+    log = lambda m: print(f"[ransac {time.perf_counter() - t0:6.1f}s] {m}", file=sys.stderr, flush=True)
+    ev, params = lr.run(n_pairs=n_pairs, n_base=n_base, steps=steps, log=log)
+    ev_hard, _ = lr.run(n_pairs=max(n_pairs // 4, 64), n_base=max(n_base // 4, 16), steps=steps,
+                        outlier_attrs="resampled", seed=1, log=log)
+    wall = time.perf_counter() - t0
+    out = {"workload": f"{n_pairs} synthetic 1024x768 pairs (random homographies), GPU SIFT + ratio-0.8 matches "
+                       f"+ 30-80% random point-pair outliers; labels from 10k-hypothesis uniform RANSAC; "
+                       f"MLP 8-64-64-1 trained on 75% of the pairs; confidence 0.99 on the rest",
+           "unit": "hypotheses/solve", "higher_is_better": False,
+           "hyps_guided": ev["guided"]["hyps_mean"], "hyps_uniform": ev["uniform"]["hyps_mean"],
+           "hyps_reference_sampler": ev["reference_sampler"]["hyps_mean"],
+           "uniform_over_guided": ev["hyps_ratio_uniform_over_guided"], "detail": ev,
+           "geometric_cues_only": {k: ev_hard[k] for k in ("guided", "uniform", "reference_sampler",
+                                                            "hyps_ratio_uniform_over_guided",
+                                                            "equal_inlier_count_frac", "mlp_test_accuracy")},
+           "wall_s": wall}
+    out["value"] = out["hyps_guided"]
+    return out
+
+
 def cpu_baseline(a, b, frames: int):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import psr
@@ -239,7 +268,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-frames", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["frame", "hamming", "precision"], default="frame",
+    ap.add_argument("--workload", choices=["frame", "hamming", "precision", "ransac"], default="frame",
                     help="frame: the 1080p extract+match headline (default); hamming: the catalog shortlist")
     ap.add_argument("--no-secondary", action="store_true",
                     help="frame workload only, without the secondary workloads in the same JSON line")
@@ -249,6 +278,9 @@ def main():
               f"(images shard across GPUs with no exchange)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "ransac":
+        print(json.dumps(bench_ransac(), default=float), flush=True)
+        return
     if args.workload == "precision":
         print(json.dumps(bench_precision(steps=min(args.steps, 10), cpu_queries=0 if args.no_cpu_baseline else 200)),
               flush=True)
@@ -342,7 +374,8 @@ def main():
     if not args.no_secondary:
         sec = {}
         for name, fn in (("hamming", lambda: bench_hamming(cpu_queries=0 if args.no_cpu_baseline else 1000)),
-                         ("precision", lambda: bench_precision(cpu_queries=0 if args.no_cpu_baseline else 200))):
+                         ("precision", lambda: bench_precision(cpu_queries=0 if args.no_cpu_baseline else 200)),
+                         ("ransac", bench_ransac)):
             try:
                 sec[name] = fn()
             except Exception as ex:   # a secondary workload never hides the headline line
