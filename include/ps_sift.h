@@ -187,6 +187,32 @@ ps_status ps_engine_nn2(ps_engine *e, const float *d_q_desc, const uint32_t *d_n
                         const float *d_t_desc, const uint32_t *d_nt, size_t nt_max, int precision,
                         uint32_t *d_best_idx, float *d_best_d2, float *d_second_d2, float *d_approx);
 
+/* ---- batched extraction ----------------------------------------------------------------
+ * Many uneven-size u8 images in one call: image i is d_pixels[offset ..] (height x width,
+ * row_stride bytes); its features / descriptors go to d_feat[i * cap ..] / d_desc[i * cap * 128 ..]
+ * and its count to d_counts[i]. Same results as one ps_engine_extract_u8 per image. The
+ * images run concurrently on up to 8 internal streams; the sequence is captured into a HIP
+ * graph on first use and replayed while images, buffers and config are unchanged
+ * (PS_BATCH_NO_GRAPH launches eagerly instead). No counterpart in the reference, whose
+ * callers loop over images (cli.rs:2093-2113, stitch.rs:493-499). */
+typedef struct {
+    uint64_t offset;
+    uint32_t height, width, row_stride, _pad;
+} ps_image_desc;
+#define PS_BATCH_NO_GRAPH 1u
+ps_status ps_engine_extract_batch_u8(ps_engine *e, const uint8_t *d_pixels, const ps_image_desc *images,
+                                     uint32_t n_images, const ps_sift_config *cfg, ps_feature *d_feat,
+                                     float *d_desc, size_t cap, uint32_t *d_counts, uint32_t flags);
+
+/* Blur tile autotuner: for the pyramid levels of the given image sizes (hw = height,
+ * width pairs), times each tile height (16, 32, 64 rows) on the device and keeps the
+ * fastest per (radius, source kind, level size class) for all later pyramid launches in
+ * the process. *n_tuned (may be null) gets the number of level classes tuned. */
+ps_status ps_engine_autotune_blur(ps_engine *e, const uint32_t *hw, uint32_t n_sizes, const ps_sift_config *cfg,
+                                  uint32_t *n_tuned);
+void ps_autotune_reset(void);
+uint32_t ps_autotune_entries(void);
+
 /* ---- homography RANSAC ---------------------------------------------------------------
  * pipeline.rs:179-213 RansacOptions: reproj_threshold in [0.5, 100]; seed used when
  * has_seed (else the reference's default seeds, pipeline.rs:173-175). */

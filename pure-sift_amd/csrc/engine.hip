@@ -204,6 +204,7 @@ void engine_free(ps_engine *e) {
         if (b->p) (void)hipFree(b->p);
     for (DevBuf &b : e->aux)
         if (b.p) (void)hipFree(b.p);
+    release_batch(e);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -319,7 +320,7 @@ ps_status enqueue_pyramid(ps_engine *e, const void *src, bool src_u8, size_t h, 
     a.dst_d = nullptr;
     a.dst_src = nullptr;
     int mode = src_u8 ? (p.upsample ? ps::SRC_U8_UP : ps::SRC_U8) : (p.upsample ? ps::SRC_F32_UP : ps::SRC_F32);
-    PS_HIP(ps::launch_blur(p.init_r, mode, a, p.init_taps, st));
+    PS_HIP(ps::launch_blur(p.init_r, mode, a, p.init_taps, blur_tile(p.init_r, mode, a.h, a.w), st));
     for (int o = 0; o < p.n_oct; o++) {
         for (int s = 1; s <= p.S + 2; s++) {
             ps::BlurArgs b;
@@ -342,7 +343,7 @@ ps_status enqueue_pyramid(ps_engine *e, const void *src, bool src_u8, size_t h, 
                 b.src_h = p.h[o];
                 b.src_w = p.w[o];
             }
-            PS_HIP(ps::launch_blur(p.lvl_r[s], m, b, p.lvl_taps[s], st));
+            PS_HIP(ps::launch_blur(p.lvl_r[s], m, b, p.lvl_taps[s], blur_tile(p.lvl_r[s], m, b.h, b.w), st));
         }
     }
     return PS_OK;
@@ -400,6 +401,24 @@ ps_status enqueue_extract(ps_engine *e, const void *src, bool src_u8, size_t h, 
 }
 
 // best/second/index per query into e->bidx / bdist / sdist (or the given outputs)
+// Grows every extraction buffer to what an h x w image needs (no allocation happens in a
+// later enqueue_extract of that size, so the sequence can be captured into a graph).
+ps_status prepare_extract(ps_engine *e, size_t h, size_t w, const ps_sift_config &cfg) {
+    ps::PyrPlan p;
+    if (!ps::make_plan((int)h, (int)w, cfg, p)) return set_err(PS_E_CONFIG, "unsupported blur radius");
+    const uint32_t kMax = PS_MAX_FEATURES;
+    PS_TRY(ensure(e->pyr, p.total_floats * sizeof(float)));
+    PS_TRY(ensure(e->cnt, 64));
+    PS_TRY(ensure(e->cand, (size_t)ps::kCandCap * sizeof(ps::Cand)));
+    PS_TRY(ensure(e->rank, ps::rank_scratch_bytes(ps::kCandCap)));
+    PS_TRY(ensure(e->sorted, (size_t)kMax * sizeof(ps::Cand)));
+    PS_TRY(ensure(e->ori, kMax * 4 * sizeof(float)));
+    PS_TRY(ensure(e->nori, kMax * sizeof(uint32_t)));
+    PS_TRY(ensure(e->feat_kp, kMax * sizeof(uint32_t)));
+    PS_TRY(ensure(e->feat_j, kMax * sizeof(uint32_t)));
+    return PS_OK;
+}
+
 ps_status enqueue_nn(ps_engine *e, const float *qd, const uint32_t *nq, size_t nqm, const float *td,
                      const uint32_t *nt, size_t ntm, DevBuf &scratch, uint32_t *bidx, float *bd, float *sd,
                      int precision = PS_PRECISION_FP32, float *approx = nullptr) {
@@ -814,6 +833,8 @@ ps_status ps_engine_profile_end(ps_engine *e, float stage_ms[PS_NUM_STAGES], uin
 ps_status ps_engine_check(ps_engine *e) {
     if (!e) return set_err(PS_E_INVALID_ARG, "null engine");
     PS_HIP(hipSetDevice(e->device));
+    for (ps_engine *c : e->children)
+        if (c->cnt.p) PS_TRY(check_err_word(c));
     if (!e->cnt.p) {
         PS_HIP(hipStreamSynchronize(e->stream));
         return PS_OK;

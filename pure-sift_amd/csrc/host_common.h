@@ -32,6 +32,13 @@ struct ps_engine {
     DevBuf part, part2, bidx, bdist, sdist, ridx, ridx_d, fin;  // matcher scratch
     DevBuf aux[AUX_COUNT];
     bool match_valu = false;                       // PS_MATCH_KERNEL=valu: direct f32 kernel
+    // batched extraction: child engines (own stream + workspace each) run images
+    // concurrently; the last batch is kept as an instantiated graph
+    std::vector<ps_engine *> children;
+    std::vector<hipEvent_t> batch_events;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uint64_t> graph_key;
     ps::PyrPlan plan;
     uint32_t host_counts[4] = {};
     // stage profiling: per recorded call, events [extract: 0..4][match: 5..6]
@@ -50,6 +57,17 @@ template <typename T>
 T *as(DevBuf &b) {
     return static_cast<T *>(b.p);
 }
+
+ps_status engine_new(int device, ps_engine **out);
+void engine_free(ps_engine *e);
+void release_batch(ps_engine *e);
+ps_status validate_cfg(const ps_sift_config *c, char *msg, size_t cap);
+ps_status prepare_extract(ps_engine *e, size_t h, size_t w, const ps_sift_config &cfg);
+ps_status enqueue_extract(ps_engine *e, const void *src, bool src_u8, size_t h, size_t w, size_t stride,
+                          const ps_sift_config &cfg, ps_feature *d_feat, float *d_desc, uint32_t cap,
+                          uint32_t *d_count);
+// blur tile height for a level (autotuned table, else 32)
+int blur_tile(int radius, int src_mode, int h, int w);
 
 // RAII borrow of a pooled engine for host-pointer calls
 struct Borrow {

@@ -73,7 +73,8 @@ struct BlurArgs {
 bool make_plan(int h, int w, const ps_sift_config &cfg, PyrPlan &p);
 
 // launches (pyramid.hip)
-hipError_t launch_blur(int radius, int src_mode, const BlurArgs &a, const Taps &t, hipStream_t st);
+// th: output tile height, 16, 32 or 64
+hipError_t launch_blur(int radius, int src_mode, const BlurArgs &a, const Taps &t, int th, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st);
 hipError_t launch_extrema(const float *dog_octave, int h, int w, int S, float thr, float mult,
                           const float *eff_scale, double log2k, int octave, Cand *out, uint32_t *count,

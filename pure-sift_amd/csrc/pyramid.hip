@@ -19,8 +19,9 @@ namespace ps {
 // pass writes (TH+2R) x TW partial rows to LDS, the vertical pass reads them back.
 // Clamping the staged window reproduces the reference's per-tap index clamp
 // (pipeline.rs:339, :401) because every staged element is the clamped source pixel.
+// Tile height TH in {16, 32, 64} is a template parameter (picked per level shape by the
+// engine's autotuner); the vertical pass gives each thread TH / 4 rows of one column.
 constexpr int kTW = 64;
-constexpr int kTH = 32;
 constexpr int kBlurThreads = 256;
 
 template <int SRC>
@@ -51,7 +52,7 @@ __device__ __forceinline__ float load_src(const BlurArgs &a, int y, int x) {
     }
 }
 
-template <int R, int SRC>
+template <int R, int SRC, int kTH>
 __global__ __launch_bounds__(kBlurThreads) void blur_kernel(BlurArgs a, Taps taps) {
     constexpr int IW = kTW + 2 * R;
     constexpr int IWP = (IW + 3) & ~3;
@@ -107,8 +108,8 @@ __global__ __launch_bounds__(kBlurThreads) void blur_kernel(BlurArgs a, Taps tap
     }
     __syncthreads();
 
-    // vertical: 8 consecutive rows of one column per thread (pipeline.rs:400-406)
-    constexpr int VR = 8;
+    // vertical: TH / 4 consecutive rows of one column per thread (pipeline.rs:400-406)
+    constexpr int VR = kTH / 4;
     for (int i = tid; i < (kTH / VR) * kTW; i += kBlurThreads) {
         int c = i % kTW, r0 = (i / kTW) * VR;
         float v[VR + NT - 1];
@@ -140,19 +141,19 @@ __global__ __launch_bounds__(kBlurThreads) void blur_kernel(BlurArgs a, Taps tap
     }
 }
 
-template <int R>
-static hipError_t blur_dispatch_src(int src_mode, const BlurArgs &a, const Taps &t, hipStream_t st) {
-    dim3 grid((a.w + kTW - 1) / kTW, (a.h + kTH - 1) / kTH);
+template <int R, int TH>
+static hipError_t blur_dispatch_th(int src_mode, const BlurArgs &a, const Taps &t, hipStream_t st) {
+    dim3 grid((a.w + kTW - 1) / kTW, (a.h + TH - 1) / TH);
     switch (src_mode) {
-    case SRC_F32: blur_kernel<R, SRC_F32><<<grid, kBlurThreads, 0, st>>>(a, t); break;
-    case SRC_DOWN2: blur_kernel<R, SRC_DOWN2><<<grid, kBlurThreads, 0, st>>>(a, t); break;
+    case SRC_F32: blur_kernel<R, SRC_F32, TH><<<grid, kBlurThreads, 0, st>>>(a, t); break;
+    case SRC_DOWN2: blur_kernel<R, SRC_DOWN2, TH><<<grid, kBlurThreads, 0, st>>>(a, t); break;
     case SRC_U8:
     case SRC_F32_UP:
     case SRC_U8_UP:
         if constexpr (R == 4 || R == 5) {
-            if (src_mode == SRC_U8) blur_kernel<R, SRC_U8><<<grid, kBlurThreads, 0, st>>>(a, t);
-            else if (src_mode == SRC_F32_UP) blur_kernel<R, SRC_F32_UP><<<grid, kBlurThreads, 0, st>>>(a, t);
-            else blur_kernel<R, SRC_U8_UP><<<grid, kBlurThreads, 0, st>>>(a, t);
+            if (src_mode == SRC_U8) blur_kernel<R, SRC_U8, TH><<<grid, kBlurThreads, 0, st>>>(a, t);
+            else if (src_mode == SRC_F32_UP) blur_kernel<R, SRC_F32_UP, TH><<<grid, kBlurThreads, 0, st>>>(a, t);
+            else blur_kernel<R, SRC_U8_UP, TH><<<grid, kBlurThreads, 0, st>>>(a, t);
             break;
         } else {
             return hipErrorInvalidValue;
@@ -162,21 +163,31 @@ static hipError_t blur_dispatch_src(int src_mode, const BlurArgs &a, const Taps 
     return hipGetLastError();
 }
 
-hipError_t launch_blur(int radius, int src_mode, const BlurArgs &a, const Taps &t, hipStream_t st) {
+template <int R>
+static hipError_t blur_dispatch_src(int src_mode, const BlurArgs &a, const Taps &t, int th, hipStream_t st) {
+    switch (th) {
+    case 16: return blur_dispatch_th<R, 16>(src_mode, a, t, st);
+    case 32: return blur_dispatch_th<R, 32>(src_mode, a, t, st);
+    case 64: return blur_dispatch_th<R, 64>(src_mode, a, t, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_blur(int radius, int src_mode, const BlurArgs &a, const Taps &t, int th, hipStream_t st) {
     if (a.h <= 0 || a.w <= 0) return hipSuccess;
     switch (radius) {
-    case 1: return blur_dispatch_src<1>(src_mode, a, t, st);
-    case 2: return blur_dispatch_src<2>(src_mode, a, t, st);
-    case 3: return blur_dispatch_src<3>(src_mode, a, t, st);
-    case 4: return blur_dispatch_src<4>(src_mode, a, t, st);
-    case 5: return blur_dispatch_src<5>(src_mode, a, t, st);
-    case 6: return blur_dispatch_src<6>(src_mode, a, t, st);
-    case 7: return blur_dispatch_src<7>(src_mode, a, t, st);
-    case 8: return blur_dispatch_src<8>(src_mode, a, t, st);
-    case 9: return blur_dispatch_src<9>(src_mode, a, t, st);
-    case 10: return blur_dispatch_src<10>(src_mode, a, t, st);
-    case 11: return blur_dispatch_src<11>(src_mode, a, t, st);
-    case 12: return blur_dispatch_src<12>(src_mode, a, t, st);
+    case 1: return blur_dispatch_src<1>(src_mode, a, t, th, st);
+    case 2: return blur_dispatch_src<2>(src_mode, a, t, th, st);
+    case 3: return blur_dispatch_src<3>(src_mode, a, t, th, st);
+    case 4: return blur_dispatch_src<4>(src_mode, a, t, th, st);
+    case 5: return blur_dispatch_src<5>(src_mode, a, t, th, st);
+    case 6: return blur_dispatch_src<6>(src_mode, a, t, th, st);
+    case 7: return blur_dispatch_src<7>(src_mode, a, t, th, st);
+    case 8: return blur_dispatch_src<8>(src_mode, a, t, th, st);
+    case 9: return blur_dispatch_src<9>(src_mode, a, t, th, st);
+    case 10: return blur_dispatch_src<10>(src_mode, a, t, th, st);
+    case 11: return blur_dispatch_src<11>(src_mode, a, t, th, st);
+    case 12: return blur_dispatch_src<12>(src_mode, a, t, th, st);
     default: return hipErrorInvalidValue;
     }
 }

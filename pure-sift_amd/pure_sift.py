@@ -81,6 +81,10 @@ class _RansacParamsC(C.Structure):
                 ("min_iters", C.c_uint32), ("max_iters", C.c_uint32)]
 
 
+IMAGE_DESC_DTYPE = np.dtype([("offset", "<u8"), ("height", "<u4"), ("width", "<u4"), ("row_stride", "<u4"),
+                             ("_pad", "<u4")])
+BATCH_NO_GRAPH = 1
+
 RANSAC_PROBLEM_DTYPE = np.dtype([("offset", "<u4"), ("n", "<u4"), ("sim_seed", "<u8"), ("hom_seed", "<u8")])
 RANSAC_RESULT_DTYPE = np.dtype([("h", "<f4", (9,)), ("has_model", "<u4"), ("n_inliers", "<u4"), ("hyps", "<u4"),
                                 ("sim", "<f4", (4,)), ("has_sim", "<u4"), ("n_sim", "<u4")])
@@ -139,6 +143,13 @@ def lib() -> C.CDLL:
         L.ps_engine_mlp_adam.argtypes = [vp, vp, vp, vp, vp, C.c_uint32, C.c_float]
         L.ps_mlp_forward.argtypes = [vp, vp, sz, vp]
         L.ps_mlp_loss_grad.argtypes = [vp, vp, vp, sz, vp, vp]
+        L.ps_engine_extract_batch_u8.argtypes = [vp, vp, vp, C.c_uint32, C.POINTER(_SiftConfigC), vp, vp, sz, vp,
+                                                 C.c_uint32]
+        L.ps_engine_autotune_blur.argtypes = [vp, vp, C.c_uint32, C.POINTER(_SiftConfigC), u32p]
+        L.ps_autotune_reset.argtypes = []
+        L.ps_autotune_reset.restype = None
+        L.ps_autotune_entries.argtypes = []
+        L.ps_autotune_entries.restype = C.c_uint32
         L.ps_nn2_precision.argtypes = [vp, sz, vp, sz, C.c_int, vp, vp, vp, vp]
         L.ps_engine_nn2.argtypes = [vp, vp, u32p, sz, vp, u32p, sz, C.c_int, vp, vp, vp, vp]
         L.ps_quantize_u128.argtypes = [vp, sz, vp]
@@ -148,7 +159,7 @@ def lib() -> C.CDLL:
         L.ps_engine_quantize_u128.argtypes = [vp, vp, sz, vp]
         L.ps_engine_hamming_topk.argtypes = [vp, vp, sz, vp, sz, C.c_uint32, vp, vp]
         L.ps_engine_rerank_l2.argtypes = [vp, vp, sz, vp, sz, vp, C.c_uint32, vp, vp]
-        for name in ("ps_engine_mlp_features", "ps_engine_mlp_forward", "ps_engine_mlp_grad", "ps_engine_mlp_adam",
+        for name in ("ps_engine_extract_batch_u8", "ps_engine_autotune_blur", "ps_engine_mlp_features", "ps_engine_mlp_forward", "ps_engine_mlp_grad", "ps_engine_mlp_adam",
                      "ps_mlp_forward", "ps_mlp_loss_grad", "ps_homography_estimate", "ps_ransac_batch", "ps_engine_ransac_batch", "ps_nn2_precision", "ps_engine_nn2", "ps_quantize_u128", "ps_count_coarse_hamming_votes", "ps_hamming_topk", "ps_rerank_l2",
                      "ps_engine_quantize_u128", "ps_engine_hamming_topk", "ps_engine_rerank_l2","ps_init", "ps_validate_sift_config", "ps_validate_match_options", "ps_detect_and_compute",
                      "ps_detect_and_compute_u8", "ps_detect_keypoints", "ps_match", "ps_build_pyramid", "ps_nn2",
@@ -550,6 +561,22 @@ class Engine:
 
     def sync(self) -> None:
         check(lib().ps_engine_sync(self.h))
+
+    def extract_batch_u8(self, d_pixels: int, images: np.ndarray, cfg: SiftConfig, d_feat: int, d_desc: int,
+                         cap: int, d_counts: int, flags: int = 0) -> None:
+        """images: IMAGE_DESC_DTYPE array (host); outputs at i * cap per image."""
+        im = np.ascontiguousarray(images, IMAGE_DESC_DTYPE)
+        c = cfg._c()
+        check(lib().ps_engine_extract_batch_u8(self.h, d_pixels, im.ctypes.data, len(im), C.byref(c), d_feat, d_desc,
+                                               cap, d_counts, flags))
+
+    def autotune_blur(self, sizes, cfg: SiftConfig) -> int:
+        """Times the blur tile heights for the pyramid levels of (height, width) sizes."""
+        hw = np.ascontiguousarray(np.asarray(sizes, np.uint32).reshape(-1, 2))
+        c = cfg._c()
+        n = C.c_uint32(0)
+        check(lib().ps_engine_autotune_blur(self.h, hw.ctypes.data, len(hw), C.byref(c), C.byref(n)))
+        return n.value
 
     def nn2(self, d_q: int, d_nq: int, nq_max: int, d_t: int, d_nt: int, nt_max: int, precision: int,
             d_best_idx: int, d_best: int, d_second: int, d_approx: int = 0) -> None:
