@@ -25,6 +25,9 @@ HBM_PEAK_GBS = 8000.0
 # 32-bit integer VALU issue rate: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 VALU_INT_PEAK_TOPS = 39.32
 ALG_BYTES_1080P = 707_097_600   # W*H + 4*P*11 + 4*P*5, P = 11,016,000 (4 octaves, upsampled)
+# HBM bytes per 1080p frame moved by the pyramid + extrema kernels, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes scaled by a 256 MiB copy (profiles/pyramid_extrema_1080p_hbm_traffic.md)
+TRAFFIC_BYTES_1080P = 1_529_587_947
 
 
 def alg_bytes(h, w, octaves=4, upsample=True):
@@ -360,6 +363,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=["frame", "hamming", "precision", "ransac", "batch"], default="frame",
                     help="frame: the 1080p extract+match headline (default); hamming: the catalog shortlist")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one 256 MiB device-to-device copy first (a known-size reference for HBM counters)")
     ap.add_argument("--no-secondary", action="store_true",
                     help="frame workload only, without the secondary workloads in the same JSON line")
     args = ap.parse_args()
@@ -386,6 +391,12 @@ def main():
     import torch
     import pure_sift as ps
     import synth
+
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        dst = src.clone()
+        torch.cuda.synchronize()
+        del src, dst
 
     W, H = 1920, 1080
     a, b, _ = synth.blob_pair(W, H, 2000, seed=1, band_noise=True)
@@ -458,7 +469,7 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "roofline": {"bound": "hbm", "kernel": "pyramid+extrema stage (blur_kernel x21, extrema_kernel x4)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES_1080P,
                      "alg_bytes_per_image": ALG_BYTES_1080P},
     }
     eng.close()
