@@ -171,9 +171,7 @@ hipError_t launch_nn_merge(const NNPart *part, int nsplit, const uint32_t *nq, s
 
 // Ratio test (pipeline.rs:1788-1805), optional mutual check (pipeline.rs:1815-1840) and
 // ordered compaction of the accepted matches in query order: match_flag_kernel decides every
-// query in parallel and counts per 256-query block; match_compact_kernel (one workgroup) scans
-// the block counts and writes the records.
-constexpr int kFinThreads = 1024;
+// query in parallel and counts per 256-query block; match_compact_kernel writes the records.
 __global__ __launch_bounds__(256) void match_flag_kernel(const uint32_t *nq_p, uint32_t nq_max, const uint32_t *nt_p,
                                                          const uint32_t *best_idx, const float *best,
                                                          const float *second, const uint32_t *rev_idx,
@@ -198,51 +196,49 @@ __global__ __launch_bounds__(256) void match_flag_kernel(const uint32_t *nq_p, u
     if (threadIdx.x == 0) block_count[blockIdx.x] = s_cnt;
 }
 
-__global__ __launch_bounds__(kFinThreads) void match_compact_kernel(const ps_feature *qf, const ps_feature *tf,
-                                                                    const uint32_t *nq_p, uint32_t nq_max,
-                                                                    const uint32_t *best_idx, const float *best,
-                                                                    const float *second, const uint8_t *flag,
-                                                                    const uint32_t *block_count, uint32_t nblocks,
-                                                                    ps_feature_match *out, uint32_t *out_count) {
-    __shared__ uint32_t s_off[1024];
-    __shared__ uint32_t s_total;
+// One workgroup per 256-query block: its output offset is the sum of the earlier blocks'
+// counts, the position inside the block a ballot prefix over the flags.
+__global__ __launch_bounds__(256) void match_compact_kernel(const ps_feature *qf, const ps_feature *tf,
+                                                            const uint32_t *nq_p, uint32_t nq_max,
+                                                            const uint32_t *best_idx, const float *best,
+                                                            const float *second, const uint8_t *flag,
+                                                            const uint32_t *block_count, uint32_t nblocks,
+                                                            ps_feature_match *out, uint32_t *out_count) {
+    __shared__ uint32_t s_red[256];
+    __shared__ uint32_t s_wave[4];
+    const uint32_t b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t nq = min(*nq_p, nq_max);
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (uint32_t b = 0; b < nblocks; b++) {
-            s_off[b] = acc;
-            acc += block_count[b];
-        }
-        s_total = acc;
-    }
+    // offset of this block (and, in block 0, the total)
+    uint32_t acc = 0;
+    for (uint32_t k = tid; k < nblocks; k += 256)
+        if (k < b || b == 0) acc += block_count[k];
+    s_red[tid] = acc;
     __syncthreads();
-    // each group of 256 threads owns one query block at a time: prefix over its flags by ballot
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;   // 16 waves
-    for (uint32_t b = wave >> 2; b < nblocks; b += kFinThreads / 256) {
-        const uint32_t sub = wave & 3;                                  // 64-query quarter of the block
-        // offset of this quarter inside the block
-        uint32_t before = 0;
-        for (uint32_t k = 0; k < sub; k++) {
-            uint32_t qq = b * 256 + k * 64 + lane;
-            before += __popcll(__ballot(qq < nq && flag[qq]));
-        }
-        const uint32_t q = b * 256 + sub * 64 + lane;
-        const bool f = q < nq && flag[q];
-        const uint64_t m = __ballot(f);
-        if (f) {
-            uint32_t pos = s_off[b] + before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-            float d1 = sqrtf(best[q]), d2 = sqrtf(second[q]);
-            uint32_t t = best_idx[q];
-            ps_feature a = qf[q], c = tf[t];
-            ps_feature_match r;
-            r.query_index = q; r.target_index = t;
-            r.qx = a.x; r.qy = a.y; r.tx = c.x; r.ty = c.y;
-            r.distance = d1; r.ratio = d1 / d2;
-            r.dx = c.x - a.x; r.dy = c.y - a.y;
-            out[pos] = r;
-        }
+    for (uint32_t o = 128; o > 0; o >>= 1) {
+        if (tid < o) s_red[tid] += s_red[tid + o];
+        __syncthreads();
     }
-    if (threadIdx.x == 0) *out_count = s_total;
+    const uint32_t base = b == 0 ? 0u : s_red[0];
+    if (b == 0 && tid == 0) *out_count = s_red[0];
+    const uint32_t q = b * 256 + tid;
+    const bool f = q < nq && flag[q];
+    const uint64_t m = __ballot(f);
+    if (lane == 0) s_wave[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = base;
+    for (uint32_t k = 0; k < wave; k++) before += s_wave[k];
+    if (f) {
+        const uint32_t pos = before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        float d1 = sqrtf(best[q]), d2 = sqrtf(second[q]);
+        uint32_t t = best_idx[q];
+        ps_feature a = qf[q], c = tf[t];
+        ps_feature_match r;
+        r.query_index = q; r.target_index = t;
+        r.qx = a.x; r.qy = a.y; r.tx = c.x; r.ty = c.y;
+        r.distance = d1; r.ratio = d1 / d2;
+        r.dx = c.x - a.x; r.dy = c.y - a.y;
+        out[pos] = r;
+    }
 }
 
 size_t match_finish_scratch_bytes(size_t nq_max) { return nq_max + ((nq_max + 255) / 256) * 4 + 64; }
@@ -257,8 +253,8 @@ hipError_t launch_match_finish(const ps_feature *qf, const ps_feature *tf, const
     uint8_t *flag = reinterpret_cast<uint8_t *>(block_count + nblocks);
     match_flag_kernel<<<nblocks, 256, 0, st>>>(nq, (uint32_t)nq_max, nt, best_idx, best, second, rev_idx, ratio, mutual,
                                                flag, block_count);
-    match_compact_kernel<<<1, kFinThreads, 0, st>>>(qf, tf, nq, (uint32_t)nq_max, best_idx, best, second, flag,
-                                                    block_count, nblocks, out, out_count);
+    match_compact_kernel<<<nblocks, 256, 0, st>>>(qf, tf, nq, (uint32_t)nq_max, best_idx, best, second, flag,
+                                                   block_count, nblocks, out, out_count);
     return hipGetLastError();
 }
 

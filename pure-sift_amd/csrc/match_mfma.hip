@@ -18,6 +18,7 @@ namespace ps {
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 constexpr int kMQ = 128;   // queries per workgroup (4 waves x 32 query columns)
+constexpr int kMaxNNSplit = 16;   // target splits (the rescore holds one entry per split per lane)
 constexpr int kMT = 64;    // targets per LDS tile (2 MFMA row blocks)
 constexpr int kMP = 132;   // LDS row stride in floats
 constexpr uint32_t kNoIdx = 0xffffffffu;
@@ -57,11 +58,50 @@ struct Cand4 {
     uint32_t i[4];
 };
 
+// |x|^2 of each row (any summation order: it only ranks and bounds); a workgroup covers 64
+// rows (each wave 16, one at a time); rows in [n, rows) get +inf so they never rank;
+// *vmax (when given) gets the largest norm of the valid rows as ordered bits (one atomic
+// per workgroup).
+__global__ __launch_bounds__(256) void row_norms_kernel(const float *x, const uint32_t *n_p, uint32_t n_max,
+                                                        uint32_t rows, float *out, uint32_t *vmax) {
+    __shared__ float s_max[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t n = min(*n_p, n_max);
+    float wmax = 0.0f;
+    float2 v[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {   // all loads first so their latencies overlap
+        const uint32_t r = blockIdx.x * 64 + w * 16 + k;
+        v[k] = r < n ? reinterpret_cast<const float2 *>(x + (size_t)r * 128)[lane] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const uint32_t r = blockIdx.x * 64 + w * 16 + k;
+        float s = v[k].x * v[k].x + v[k].y * v[k].y;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (lane == 0 && r < rows) out[r] = r < n ? s : INFINITY;
+        if (r < n) wmax = fmaxf(wmax, isfinite(s) ? s : INFINITY);
+    }
+    if (!vmax) return;
+    if (lane == 0) s_max[w] = wmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float m = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
+        if (m > 0.0f) atomicMax(vmax, __float_as_uint(m));
+    }
+}
+
+// Pass 1. Workgroup: 128 queries (4 waves x 32 B-operand columns, each lane holding half of
+// its query's 128 dims in VGPRs) against one split of the targets, 64 targets per LDS tile;
+// the next tile is loaded into registers while the current one is multiplied. Every lane
+// keeps the 4 smallest (|t|^2 - 2 q.t, index) of the targets it sees; both halves' lists are
+// written (8 candidates per query and split).
 __global__ __launch_bounds__(256) void nn_mfma_kernel(const float *qd, const uint32_t *nq_p, uint32_t nq_max,
                                                       const float *td, const uint32_t *nt_p, uint32_t nt_max,
-                                                      int nsplit, Cand4 *part, float *qn, uint32_t *tmax_bits) {
+                                                      const float *tn, int nsplit, Cand4 *part) {
     __shared__ __attribute__((aligned(16))) float s_t[kMT * kMP];
-    __shared__ float s_tn[kMT];
+    __shared__ __attribute__((aligned(16))) float s_tn[kMT];
     const uint32_t nq = min(*nq_p, nq_max), nt = min(*nt_p, nt_max);
     const uint32_t qb = blockIdx.x * kMQ;
     if (qb >= nq) return;
@@ -69,7 +109,6 @@ __global__ __launch_bounds__(256) void nn_mfma_kernel(const float *qd, const uin
     const int col = lane & 31, half = lane >> 5;
     const uint32_t q = qb + wave * 32 + col;
 
-    // B operand: this lane's query, dims half*64 .. half*64+63 (k order is free for a dot product)
     float b[64];
     if (q < nq) {
         const float4 *p = reinterpret_cast<const float4 *>(qd + (size_t)q * 128 + half * 64);
@@ -82,54 +121,36 @@ __global__ __launch_bounds__(256) void nn_mfma_kernel(const float *qd, const uin
 #pragma unroll
         for (int s = 0; s < 64; s++) b[s] = 0.0f;
     }
-    // |q|^2 (any order: it only feeds the error bound) for the rescoring pass
-    if (blockIdx.y == 0) {
-        float qq = 0.0f;
-#pragma unroll
-        for (int s = 0; s < 64; s++) qq += b[s] * b[s];
-        qq += __shfl_xor(qq, 32);
-        if (half == 0 && q < nq) qn[q] = qq;
-    }
     float kd[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
     uint32_t ki[4] = {kNoIdx, kNoIdx, kNoIdx, kNoIdx};
 
     const uint32_t ntiles = (nt + kMT - 1) / kMT;
     const uint32_t per = (ntiles + nsplit - 1) / nsplit;
     const uint32_t tb = blockIdx.y * per, te = min(tb + per, ntiles);
+    constexpr int kLd = kMT * 32 / 256;   // float4 loads per thread per tile
+    float4 v[kLd];
+    float vn = 0.0f;
+#define NN_LOAD(TILE)                                                                                 \
+    do {                                                                                              \
+        const uint32_t t0_ = (TILE) * kMT;                                                            \
+        _Pragma("unroll") for (int j = 0; j < kLd; j++) {                                             \
+            int i = tid + j * 256, r = i >> 5, c = (i & 31) * 4;                                     \
+            v[j] = t0_ + r < nt ? *reinterpret_cast<const float4 *>(td + (size_t)(t0_ + r) * 128 + c)  \
+                                : make_float4(0.f, 0.f, 0.f, 0.f);                                    \
+        }                                                                                             \
+        if (tid < kMT) vn = tn[t0_ + tid];                                                            \
+    } while (0)
+    if (tb < te) NN_LOAD(tb);
     for (uint32_t tile = tb; tile < te; tile++) {
-        const uint32_t t0 = tile * kMT;
-        float4 v[kMT * 32 / 256];
+        __syncthreads();   // previous tile's readers are done
 #pragma unroll
-        for (int j = 0; j < kMT * 32 / 256; j++) {
-            int i = tid + j * 256, r = i >> 5, c = (i & 31) * 4;
-            v[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (t0 + r < nt) v[j] = *reinterpret_cast<const float4 *>(td + (size_t)(t0 + r) * 128 + c);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < kMT * 32 / 256; j++) {
+        for (int j = 0; j < kLd; j++) {
             int i = tid + j * 256, r = i >> 5, c = (i & 31) * 4;
             *reinterpret_cast<float4 *>(s_t + r * kMP + c) = v[j];
         }
+        if (tid < kMT) s_tn[tid] = vn;
         __syncthreads();
-        {   // |t|^2 of the tile rows: 4 threads per row, combined by shuffles
-            const int r = tid >> 2, part = tid & 3;
-            const float *row = s_t + r * kMP + part * 32;
-            float tt = 0.0f;
-#pragma unroll
-            for (int c = 0; c < 32; c += 4) {
-                float4 x = *reinterpret_cast<const float4 *>(row + c);
-                tt += x.x * x.x + x.y * x.y + x.z * x.z + x.w * x.w;
-            }
-            tt += __shfl_xor(tt, 1);
-            tt += __shfl_xor(tt, 2);
-            if (part == 0) {
-                s_tn[r] = tt;
-                if (blockIdx.x == 0 && t0 + r < nt)
-                    atomicMax(tmax_bits, isfinite(tt) ? __float_as_uint(tt) : 0x7f800000u);
-            }
-        }
-        __syncthreads();
+        if (tile + 1 < te) NN_LOAD(tile + 1);
         floatx16 acc0 = {}, acc1 = {};
         const float *r0 = s_t + col * kMP + half * 64;
         const float *r1 = s_t + (32 + col) * kMP + half * 64;
@@ -146,105 +167,132 @@ __global__ __launch_bounds__(256) void nn_mfma_kernel(const float *qd, const uin
             acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, b[4 * m + 3], acc0, 0, 0, 0);
             acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, b[4 * m + 3], acc1, 0, 0, 0);
         }
-        // D layout: lane holds column (query) lane%32, rows 8*(r/4) + 4*(lane/32) + r%4
+        // D layout: lane holds column (query) lane%32, rows 8*(r/4) + 4*(lane/32) + r%4;
+        // padding rows have |t|^2 = +inf and never rank
+        const uint32_t t0 = tile * kMT;
+        float vmin = INFINITY;
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            int i = 8 * (r >> 2) + 4 * half + (r & 3);
-            uint32_t t = t0 + i;
-            if (t < nt) top4_insert(s_tn[i] - 2.0f * acc0[r], t, kd, ki);
+        for (int rr = 0; rr < 4; rr++) {
+            const float4 n0 = *reinterpret_cast<const float4 *>(&s_tn[8 * rr + 4 * half]);
+            const float4 n1 = *reinterpret_cast<const float4 *>(&s_tn[32 + 8 * rr + 4 * half]);
+            const float a[4] = {n0.x, n0.y, n0.z, n0.w}, c[4] = {n1.x, n1.y, n1.z, n1.w};
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                acc0[rr * 4 + u] = a[u] - 2.0f * acc0[rr * 4 + u];
+                acc1[rr * 4 + u] = c[u] - 2.0f * acc1[rr * 4 + u];
+                vmin = fminf(vmin, fminf(acc0[rr * 4 + u], acc1[rr * 4 + u]));
+            }
         }
+        if (vmin < kd[3]) {
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            int i = 32 + 8 * (r >> 2) + 4 * half + (r & 3);
-            uint32_t t = t0 + i;
-            if (t < nt) top4_insert(s_tn[i] - 2.0f * acc1[r], t, kd, ki);
+            for (int r = 0; r < 16; r++) top4_insert(acc0[r], t0 + 8 * (r >> 2) + 4 * half + (r & 3), kd, ki);
+#pragma unroll
+            for (int r = 0; r < 16; r++) top4_insert(acc1[r], t0 + 32 + 8 * (r >> 2) + 4 * half + (r & 3), kd, ki);
         }
     }
-    // merge the two lane halves that hold the same query
-    float od[4];
-    uint32_t oi[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        od[j] = __shfl_xor(kd[j], 32);
-        oi[j] = __shfl_xor(ki[j], 32);
-    }
-    if (half == 0 && q < nq) {
-#pragma unroll
-        for (int j = 0; j < 4; j++) top4_insert_lex(od[j], oi[j], kd, ki);
+#undef NN_LOAD
+    if (q < nq) {
         Cand4 c;
 #pragma unroll
         for (int j = 0; j < 4; j++) { c.d[j] = kd[j]; c.i[j] = ki[j]; }
-        part[(size_t)blockIdx.y * nq_max + q] = c;
+        part[((size_t)blockIdx.y * nq_max + q) * 2 + half] = c;
     }
 }
 
-// sequential f32 distance of pipeline.rs:1700-1717 (without the exactness-neutral early exit)
-// One thread per (query, candidate): the 4 exact distances of a query are computed by 4
-// adjacent lanes (each a 128-step sequential sum) and combined with shuffles.
+// Pass 2. Eight lanes per query: the 8 smallest (value, index) over the 2 * nsplit lists
+// (lane j holds entry j of every list), each rescored with the reference's sequential f32
+// sum (pipeline.rs:1700-1717), then best / second in (distance, index) order. Exactness
+// guard: with E a bound on |value + |q|^2 - exact| for any pair, every target that is not a
+// candidate has exact distance >= tau + |q|^2 - E, where tau is the smaller of the 8th
+// selected value and the 4th (last) entry of every list (a list drops only targets above
+// its own 4th); if that exceeds the exact second-best the result equals the full scan's,
+// otherwise the query is flagged for the exact scan.
 __global__ __launch_bounds__(256) void nn_rescore_kernel(const Cand4 *part, int nsplit, const float *qd, const float *qn,
                                                          const uint32_t *nq_p, uint32_t nq_max, const float *td,
                                                          const uint32_t *nt_p, const uint32_t *tmax_bits,
                                                          uint32_t *best_idx, float *best, float *second,
                                                          uint32_t *flag_list, uint32_t *flag_count) {
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t q = gid >> 2;
-    const int j = gid & 3;
+    const uint32_t q = blockIdx.x * 32 + (threadIdx.x >> 3);
+    const int j = threadIdx.x & 7;
     const bool active = q < min(*nq_p, nq_max);
     const uint32_t nt = *nt_p;
-    float kd[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
-    uint32_t ki[4] = {kNoIdx, kNoIdx, kNoIdx, kNoIdx};
-    if (active) {
-        // the per-split lists are loaded 8 at a time so their loads overlap
-        for (int s0 = 0; s0 < nsplit; s0 += 8) {
-            Cand4 c[8];
+    // lane j: entry j % 4 of the half-(j / 4) list of every split
+    float md[kMaxNNSplit];
+    uint32_t mi[kMaxNNSplit];
+    float last4 = INFINITY;
+    {
+        const int u = j & 3;
+        const uint32_t qs = active ? q : 0u;
 #pragma unroll
-            for (int v = 0; v < 8; v++)
-                if (s0 + v < nsplit) c[v] = part[(size_t)(s0 + v) * nq_max + q];
+        for (int s = 0; s < kMaxNNSplit; s++) {   // loads first, then the bookkeeping
+            const int ss = s < nsplit ? s : 0;
+            const float *e = reinterpret_cast<const float *>(part + ((size_t)ss * nq_max + qs) * 2 + (j >> 2));
+            md[s] = e[u];
+            mi[s] = reinterpret_cast<const uint32_t *>(e)[4 + u];
+        }
 #pragma unroll
-            for (int v = 0; v < 8; v++) {
-                if (s0 + v >= nsplit) break;
-#pragma unroll
-                for (int u = 0; u < 4; u++)
-                    if (c[v].i[u] != kNoIdx) top4_insert_lex(c[v].d[u], c[v].i[u], kd, ki);
+        for (int s = 0; s < kMaxNNSplit; s++) {
+            if (!active || s >= nsplit) {
+                md[s] = INFINITY;
+                mi[s] = kNoIdx;
             }
+            if (u == 3) last4 = fminf(last4, md[s]);
         }
     }
-    uint32_t my = kNoIdx;
 #pragma unroll
-    for (int u = 0; u < 4; u++)
-        if (u == j) my = ki[u];
+    for (int o = 1; o < 8; o <<= 1) last4 = fminf(last4, __shfl_xor(last4, o));
+    float sel_d = INFINITY, kth = INFINITY;
+    uint32_t sel_i = kNoIdx;
+    for (int round = 0; round < 8; round++) {
+        float m = INFINITY;
+        uint32_t mix = kNoIdx;
+        int ms = -1;
+#pragma unroll
+        for (int s = 0; s < kMaxNNSplit; s++)
+            if (lex_less(md[s], mi[s], m, mix)) { m = md[s]; mix = mi[s]; ms = s; }
+        float g = m;
+        uint32_t gi = mix;
+#pragma unroll
+        for (int o = 1; o < 8; o <<= 1) {
+            float ov = __shfl_xor(g, o);
+            uint32_t oi = __shfl_xor(gi, o);
+            if (lex_less(ov, oi, g, gi)) { g = ov; gi = oi; }
+        }
+        if (j == round) { sel_d = g; sel_i = gi; }
+        if (round == 7) kth = g;
+        if (ms >= 0 && mix == gi && m == g) {
+#pragma unroll
+            for (int s = 0; s < kMaxNNSplit; s++)
+                if (s == ms) { md[s] = INFINITY; mi[s] = kNoIdx; }
+        }
+    }
     float d = INFINITY;
-    if (active && my != kNoIdx) d = seq_dist128(qd + (size_t)q * 128, td + (size_t)my * 128);
-    // gather the 4 (d, idx) of this query into every one of its lanes
-    float dv[4];
-    uint32_t iv[4];
-    const int base = (threadIdx.x & 63) & ~3;
+    if (active && sel_i != kNoIdx) d = seq_dist128(qd + (size_t)q * 128, td + (size_t)sel_i * 128);
+    float bd = d;
+    uint32_t bi = sel_i;
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-        dv[u] = __shfl(d, base + u);
-        iv[u] = __shfl(my, base + u);
+    for (int o = 1; o < 8; o <<= 1) {
+        float od = __shfl_xor(bd, o);
+        uint32_t oi = __shfl_xor(bi, o);
+        if (lex_less(od, oi, bd, bi)) { bd = od; bi = oi; }
     }
+    float sd = (sel_i == bi) ? INFINITY : d;
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) sd = fminf(sd, __shfl_xor(sd, o));
+    int ncand = sel_i != kNoIdx;
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) ncand += __shfl_xor(ncand, o);
     if (!active || j != 0) return;
-    float b = INFINITY, sc = INFINITY;
-    uint32_t bi = 0;
-    int ncand = 0;
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-        if (iv[u] == kNoIdx) continue;
-        ncand++;
-        if (dv[u] < b || (dv[u] == b && iv[u] < bi)) { sc = b; b = dv[u]; bi = iv[u]; }
-        else if (dv[u] < sc) sc = dv[u];
-    }
-    // exactness guard (see header comment); all targets are candidates when nt <= 4
-    bool ok = ncand >= (int)min(nt, 4u);
-    if (ok && nt > 4) {
+    // all targets are candidates when nt <= 8
+    bool ok = ncand >= (int)min(nt, 8u);
+    if (ok && nt > 8) {
         const float qq = qn[q], tt = __uint_as_float(*tmax_bits);
         const float E = 1.0e-4f * (qq + tt + 2.0f * sqrtf(qq * tt)) + 1e-30f;
-        ok = (kd[3] + qq) - E > sc;
+        ok = (fminf(kth, last4) + qq) - E > sd;
     }
-    best_idx[q] = bi;
-    best[q] = b;
-    second[q] = sc;
+    best_idx[q] = bi == kNoIdx ? 0u : bi;
+    best[q] = bd;
+    second[q] = sd;
     if (!ok) flag_list[atomicAdd(flag_count, 1u)] = q;
 }
 
@@ -318,20 +366,22 @@ __global__ void nn_fallback_merge_kernel(const NNPart *fb, const uint32_t *flag_
 int nn_mfma_splits(size_t nq_max, size_t nt_max) {
     size_t qb = (nq_max + kMQ - 1) / kMQ, tt = (nt_max + kMT - 1) / kMT;
     if (qb == 0 || tt == 0) return 1;
-    size_t ns = (768 + qb - 1) / qb;
+    size_t ns = (512 + qb - 1) / qb;
     if (ns > tt) ns = tt;
-    if (ns > 1024) ns = 1024;
+    if (ns > (size_t)kMaxNNSplit) ns = kMaxNNSplit;
     return (int)(ns < 1 ? 1 : ns);
 }
 
+static size_t tn_rows(size_t nt_max) { return (nt_max + kMT - 1) / kMT * kMT; }
+
 size_t nn_mfma_scratch_bytes(size_t nq_max, size_t nt_max) {
     size_t ns = (size_t)nn_mfma_splits(nq_max, nt_max);
-    return ns * nq_max * sizeof(Cand4) + nq_max * kFbT * sizeof(NNPart) + nq_max * (sizeof(float) + sizeof(uint32_t)) +
-           256;
+    return ns * nq_max * 2 * sizeof(Cand4) + nq_max * kFbT * sizeof(NNPart) + nq_max * (sizeof(float) + sizeof(uint32_t)) +
+           tn_rows(nt_max) * sizeof(float) + 512;
 }
 
-// Scratch layout: [part: ns*nq_max Cand4][fb: nq_max*16 NNPart][qn: nq_max][flag_list: nq_max]
-// [counters: tmax, flag_count]
+// Scratch layout: [part: ns*nq_max*2 Cand4][fb: nq_max*16 NNPart][qn: nq_max][flag_list: nq_max]
+// [tn: padded nt_max][counters: tmax, flag_count]
 hipError_t launch_nn_mfma(const float *qd, const uint32_t *nq, size_t nq_max, const float *td, const uint32_t *nt,
                           size_t nt_max, void *scratch, uint32_t *best_idx, float *best, float *second,
                           hipStream_t st) {
@@ -339,22 +389,28 @@ hipError_t launch_nn_mfma(const float *qd, const uint32_t *nq, size_t nq_max, co
     const int ns = nn_mfma_splits(nq_max, nt_max);
     char *p = static_cast<char *>(scratch);
     Cand4 *part = reinterpret_cast<Cand4 *>(p);
-    p += (size_t)ns * nq_max * sizeof(Cand4);
+    p += (size_t)ns * nq_max * 2 * sizeof(Cand4);
     NNPart *fb = reinterpret_cast<NNPart *>(p);
     p += nq_max * kFbT * sizeof(NNPart);
     float *qn = reinterpret_cast<float *>(p);
     p += nq_max * sizeof(float);
     uint32_t *flag_list = reinterpret_cast<uint32_t *>(p);
     p += nq_max * sizeof(uint32_t);
+    float *tn = reinterpret_cast<float *>(p);
+    p += tn_rows(nt_max) * sizeof(float);
     uint32_t *tmax = reinterpret_cast<uint32_t *>(p);
     uint32_t *flag_count = tmax + 1;
     hipError_t e = launch_fill_u32(tmax, 0u, 2, st);
     if (e != hipSuccess) return e;
+    const uint32_t trows = (uint32_t)tn_rows(nt_max);
+    row_norms_kernel<<<(trows + 63) / 64, 256, 0, st>>>(td, nt, (uint32_t)nt_max, trows, tn, tmax);
+    row_norms_kernel<<<(unsigned)((nq_max + 63) / 64), 256, 0, st>>>(qd, nq, (uint32_t)nq_max, (uint32_t)nq_max, qn,
+                                                                      nullptr);
     dim3 grid((unsigned)((nq_max + kMQ - 1) / kMQ), (unsigned)ns);
-    nn_mfma_kernel<<<grid, 256, 0, st>>>(qd, nq, (uint32_t)nq_max, td, nt, (uint32_t)nt_max, ns, part, qn, tmax);
-    nn_rescore_kernel<<<(unsigned)((4 * nq_max + 255) / 256), 256, 0, st>>>(part, ns, qd, qn, nq, (uint32_t)nq_max, td,
-                                                                            nt, tmax, best_idx, best, second,
-                                                                            flag_list, flag_count);
+    nn_mfma_kernel<<<grid, 256, 0, st>>>(qd, nq, (uint32_t)nq_max, td, nt, (uint32_t)nt_max, tn, ns, part);
+    nn_rescore_kernel<<<(unsigned)((nq_max + 31) / 32), 256, 0, st>>>(part, ns, qd, qn, nq, (uint32_t)nq_max, td, nt,
+                                                                      tmax, best_idx, best, second, flag_list,
+                                                                      flag_count);
     nn_fallback_partial_kernel<<<dim3(kFbQ, kFbT), 256, 0, st>>>(qd, nq, (uint32_t)nq_max, td, nt, (uint32_t)nt_max,
                                                                  flag_list, flag_count, fb);
     nn_fallback_merge_kernel<<<(unsigned)((nq_max + 255) / 256), 256, 0, st>>>(fb, flag_list, flag_count, nq,

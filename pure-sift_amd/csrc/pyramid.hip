@@ -272,14 +272,15 @@ __device__ __forceinline__ void refine3d(const float *ps, const float *cs, const
     odx = 0.0f; ody = 0.0f; ods = 0.0f; oi = center;
 }
 
-// 64 x 8 interior pixels per 256-thread workgroup: all PL = S+2 DoG planes of the tile plus a
-// one-pixel halo are staged in LDS (each plane read from HBM once, every load of a thread in
-// flight together). Phase 1: each thread tests its 2 pixels x S slices against the contrast
-// threshold and the 8 in-plane neighbours (the cheap tests that reject almost everything) and
-// queues survivors in LDS. Phase 2: the queue is processed by all lanes in parallel -- the 18
-// cross-scale neighbours, the Hessian edge test and the Taylor refinement -- so no wave idles
-// while a few lanes run the expensive path.
-constexpr int kExRows = 8;
+// 64 x 16 interior pixels per 256-thread workgroup: the S searchable DoG planes of the tile
+// plus a one-pixel halo are staged in LDS (every load of a thread in flight together).
+// Phase 1: each thread tests its 4 pixels x S slices against the contrast threshold and the 8
+// in-plane neighbours (the cheap tests that reject almost everything) and queues survivors
+// in LDS. Phase 2: the queue is processed by all lanes in parallel -- the 18 cross-scale
+// neighbours (the outer DoG planes read from global memory, only for survivors), the Hessian
+// edge test and the Taylor refinement -- and accepted candidates reserve output slots with
+// one atomic per workgroup.
+constexpr int kExRows = 16;
 constexpr int kExW = 66;
 constexpr int kExH = kExRows + 2;
 template <int PL>
@@ -289,27 +290,31 @@ __global__ __launch_bounds__(256) void extrema_kernel(const float *dog, int h, i
     constexpr int S = PL - 2;
     constexpr int per_plane = kExH * kExW;
     constexpr int NL = (per_plane + 255) / 256;
-    __shared__ float s_dog[PL * per_plane];
-    __shared__ uint32_t s_q[64 * kExRows * S];
+    // only the S searchable DoG planes are staged; the outer planes 0 and S+1 are read from
+    // global memory for the rare survivors of the in-plane test
+    __shared__ float s_dog[S * per_plane];
+    // phase-1 survivors: strict in-plane extrema of one kind are never 8-adjacent, so at most
+    // one maximum and one minimum per 2x2 block: half of the tile's pixels per slice
+    __shared__ uint32_t s_q[64 * kExRows * S / 2];
     __shared__ uint32_t s_qn;
     const int x0 = 1 + blockIdx.x * 64, y0 = 1 + blockIdx.y * kExRows;
     const size_t P = (size_t)h * w;
     if (threadIdx.x == 0) s_qn = 0;
     {
-        float v[PL][NL];
+        float v[S][NL];
 #pragma unroll
         for (int j = 0; j < NL; j++) {
             int i = threadIdx.x + j * 256;
             int r = i / kExW, c = i - r * kExW;
             size_t o = (size_t)min(y0 - 1 + r, h - 1) * w + min(x0 - 1 + c, w - 1);
 #pragma unroll
-            for (int pl = 0; pl < PL; pl++) v[pl][j] = i < per_plane ? dog[(size_t)pl * P + o] : 0.0f;
+            for (int pl = 0; pl < S; pl++) v[pl][j] = i < per_plane ? dog[(size_t)(pl + 1) * P + o] : 0.0f;
         }
 #pragma unroll
         for (int j = 0; j < NL; j++) {
             int i = threadIdx.x + j * 256;
 #pragma unroll
-            for (int pl = 0; pl < PL; pl++)
+            for (int pl = 0; pl < S; pl++)
                 if (i < per_plane) s_dog[pl * per_plane + i] = v[pl][j];
         }
     }
@@ -324,7 +329,7 @@ __global__ __launch_bounds__(256) void extrema_kernel(const float *dog, int h, i
                 if (y0 + ly - 1 >= h - 1) break;
 #pragma unroll
                 for (int s = 1; s <= S; s++) {
-                    const float *cs = s_dog + s * per_plane;
+                    const float *cs = s_dog + (s - 1) * per_plane;
                     const float val = cs[ly * kExW + lx];
                     if (fabsf(val) <= thr) continue;
                     bool ext = true;
@@ -347,53 +352,79 @@ __global__ __launch_bounds__(256) void extrema_kernel(const float *dog, int h, i
         }
     }
     __syncthreads();
-    // phase 2 (pipeline.rs:852-893)
+    // phase 2 (pipeline.rs:852-893); accepted candidates of a round reserve their output
+    // slots with one global atomic per workgroup (ballot prefix inside the waves)
+    __shared__ uint32_t s_wcnt[4], s_base;
     const uint32_t nq = s_qn;
-    for (uint32_t qi = threadIdx.x; qi < nq; qi += 256) {
-        const uint32_t e = s_q[qi];
-        const int s = (int)(e >> 16), ly = (int)((e >> 8) & 255u), lx = (int)(e & 255u);
-        const int x = x0 + lx - 1, y = y0 + ly - 1;
-        const float *ps = s_dog + (s - 1) * per_plane, *cs = s_dog + s * per_plane, *ns = s_dog + (s + 1) * per_plane;
-        const float val = cs[ly * kExW + lx];
-        bool ext = true;
-        if (val > 0.0f) {
-#pragma unroll
-            for (int dy = -1; dy <= 1; dy++)
-#pragma unroll
-                for (int dx = -1; dx <= 1; dx++) {
-                    int o = (ly + dy) * kExW + lx + dx;
-                    ext = ext && (val > ps[o]) && (val > ns[o]);
-                }
-        } else {
-#pragma unroll
-            for (int dy = -1; dy <= 1; dy++)
-#pragma unroll
-                for (int dx = -1; dx <= 1; dx++) {
-                    int o = (ly + dy) * kExW + lx + dx;
-                    ext = ext && (val < ps[o]) && (val < ns[o]);
-                }
-        }
-        if (!ext || !hessian_edge_ok(cs, kExW, lx, ly, val)) continue;
-        float dx, dy, ds, interp;
-        refine3d(ps, cs, ns, kExW, lx, ly, dx, dy, ds, interp);
-        if (!(fabsf(interp) > thr)) continue;
-        // k.powf(ds): evaluated in f64 and rounded once (glibc powf agrees except in ~0.06% of
-        // arguments, where the two differ by one ulp)
-        float kpow = (float)exp2((double)ds * log2k);
-        float refined_scale = eff.v[s] * kpow;
-        float lf = roundf((float)s + ds);   // pipeline.rs:882-885
-        lf = lf < 1.0f ? 1.0f : (lf > (float)S ? (float)S : lf);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (uint32_t q0 = 0; q0 < nq; q0 += 256) {
+        const uint32_t qi = q0 + threadIdx.x;
+        bool accept = false;
         Cand c;
-        c.x = ((float)x + dx) * mult;
-        c.y = ((float)y + dy) * mult;
-        c.scale = fmaxf(refined_scale, 0.25f * 1.6f);
-        c.response = interp;
-        c.octave = (uint32_t)octave;
-        c.layer = (uint32_t)lf;
-        c.key = ((uint64_t)octave << 56) | ((uint64_t)s << 48) | ((uint64_t)y << 24) | (uint64_t)x;
-        uint32_t i = atomicAdd(count, 1u);
-        if (i < cap) out[i] = c;
-        else atomicOr(err, kErrCandOverflow);
+        if (qi < nq) {
+            const uint32_t e = s_q[qi];
+            const int s = (int)(e >> 16), ly = (int)((e >> 8) & 255u), lx = (int)(e & 255u);
+            const int x = x0 + lx - 1, y = y0 + ly - 1;
+            // 3 x 3 x 3 neighbourhood (slices s-1, s, s+1) from LDS, or from global memory for
+            // the outer planes; interior pixels need no clamping
+            float nb[3][9];
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+                const int pl = s - 1 + p;
+#pragma unroll
+                for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+                    for (int dx = 0; dx < 3; dx++)
+                        nb[p][dy * 3 + dx] = (pl >= 1 && pl <= S)
+                                                 ? s_dog[(pl - 1) * per_plane + (ly - 1 + dy) * kExW + lx - 1 + dx]
+                                                 : dog[(size_t)pl * P + (size_t)(y - 1 + dy) * w + (x - 1 + dx)];
+            }
+            const float *ps = nb[0], *cs = nb[1], *ns = nb[2];
+            const float val = cs[4];
+            bool ext = true;
+            if (val > 0.0f) {
+#pragma unroll
+                for (int o = 0; o < 9; o++) ext = ext && (val > ps[o]) && (val > ns[o]);
+            } else {
+#pragma unroll
+                for (int o = 0; o < 9; o++) ext = ext && (val < ps[o]) && (val < ns[o]);
+            }
+            if (ext && hessian_edge_ok(cs, 3, 1, 1, val)) {
+                float dx, dy, ds, interp;
+                refine3d(ps, cs, ns, 3, 1, 1, dx, dy, ds, interp);
+                if (fabsf(interp) > thr) {
+                    // k.powf(ds): evaluated in f64 and rounded once (glibc powf agrees except in
+                    // ~0.06% of arguments, where the two differ by one ulp)
+                    float kpow = (float)exp2((double)ds * log2k);
+                    float refined_scale = eff.v[s] * kpow;
+                    float lf = roundf((float)s + ds);   // pipeline.rs:882-885
+                    lf = lf < 1.0f ? 1.0f : (lf > (float)S ? (float)S : lf);
+                    c.x = ((float)x + dx) * mult;
+                    c.y = ((float)y + dy) * mult;
+                    c.scale = fmaxf(refined_scale, 0.25f * 1.6f);
+                    c.response = interp;
+                    c.octave = (uint32_t)octave;
+                    c.layer = (uint32_t)lf;
+                    c.key = ((uint64_t)octave << 56) | ((uint64_t)s << 48) | ((uint64_t)y << 24) | (uint64_t)x;
+                    accept = true;
+                }
+            }
+        }
+        const uint64_t m = __ballot(accept);
+        if (lane == 0) s_wcnt[wv] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t tot = s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
+            s_base = tot ? atomicAdd(count, tot) : 0u;
+        }
+        __syncthreads();
+        if (accept) {
+            uint32_t i = s_base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            for (int k = 0; k < wv; k++) i += s_wcnt[k];
+            if (i < cap) out[i] = c;
+            else atomicOr(err, kErrCandOverflow);
+        }
+        __syncthreads();
     }
 }
 
