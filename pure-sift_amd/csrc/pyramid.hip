@@ -455,22 +455,38 @@ hipError_t launch_extrema(const float *dog_octave, int h, int w, int S, float th
 // (|response| desc, scan key asc). For n > limit an exact radix select (four 8-bit passes
 // over the |response| bit patterns, which order like the positive floats) finds T, the
 // limit-th largest |response|; only candidates with |response| >= T (limit plus ties) are
-// ranked. rank[i] = #{j in subset : j precedes i}; a fixed 64x64 grid of workgroups walks
-// (i-tile, j-tile) pairs of the subset and adds integer partial counts, so the result is
+// ranked. rank[i] = #{j in subset : j precedes i}; a fixed grid of workgroups walks
+// (i-tile, j-chunk) pairs of the subset and adds integer partial counts, so the result is
 // exact and independent of scheduling.
 constexpr int kRankTile = 256;
-constexpr int kRankGrid = 64;
+constexpr int kRankGrid = 24;
 
 // scratch words: [0] subset size, [1] threshold bits T, [2] by-response flag
+constexpr uint32_t kSelLds = 12288;   // candidates whose |response| bits fit in LDS for the 4 passes
 __global__ __launch_bounds__(1024) void rank_select_kernel(const Cand *c, const uint32_t *count, uint32_t cap_in,
                                                            uint32_t limit, uint32_t *words) {
     __shared__ uint32_t hist[256];
     __shared__ uint32_t s_prefix, s_need;
+    __shared__ uint32_t s_bits[kSelLds];
     const uint32_t n = min(*count, cap_in);
     if (threadIdx.x == 0) words[0] = 0;   // subset counter for rank_gather_kernel
     if (n <= limit) {
         if (threadIdx.x == 0) { words[1] = 0; words[2] = 0; }
         return;
+    }
+    const bool in_lds = n <= kSelLds;
+    if (in_lds) {
+        for (uint32_t i0 = threadIdx.x; i0 < n; i0 += 4 * 1024) {
+            uint32_t v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                uint32_t i = i0 + u * 1024;
+                v[u] = i < n ? __float_as_uint(fabsf(c[i].response)) : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (i0 + u * 1024 < n) s_bits[i0 + u * 1024] = v[u];
+        }
     }
     if (threadIdx.x == 0) { s_prefix = 0; s_need = limit; }
     uint32_t mask = 0;
@@ -484,11 +500,29 @@ __global__ __launch_bounds__(1024) void rank_select_kernel(const Cand *c, const 
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 uint32_t i = i0 + u * 1024;
-                v[u] = i < n ? __float_as_uint(fabsf(c[i].response)) : 0u;
+                v[u] = i < n ? (in_lds ? s_bits[i] : __float_as_uint(fabsf(c[i].response))) : 0u;
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++)
-                if (i0 + u * 1024 < n && (v[u] & mask) == prefix) atomicAdd(&hist[(v[u] >> shift) & 255u], 1u);
+            for (int u = 0; u < 4; u++) {
+                // first pass (the exponent byte takes only a few values): wave-aggregated
+                // increments, one LDS atomic per distinct bin in the wave; later passes spread
+                // over many bins and use plain atomics
+                bool act = i0 + u * 1024 < n && (v[u] & mask) == prefix;
+                const uint32_t bin = (v[u] >> shift) & 255u;
+                if (pass > 0) {
+                    if (act) atomicAdd(&hist[bin], 1u);
+                    continue;
+                }
+                uint64_t pending = __ballot(act);
+                while (pending) {
+                    const int leader = __ffsll((unsigned long long)pending) - 1;
+                    const uint32_t lb = __shfl(bin, leader);
+                    const uint64_t same = __ballot(act && bin == lb);
+                    if ((int)(threadIdx.x & 63) == leader) atomicAdd(&hist[lb], (uint32_t)__popcll(same));
+                    pending &= ~same;
+                    act = act && bin != lb;
+                }
+            }
         }
         __syncthreads();
         if (threadIdx.x < 64) {
@@ -522,48 +556,83 @@ __global__ __launch_bounds__(1024) void rank_select_kernel(const Cand *c, const 
     if (threadIdx.x == 0) { words[1] = s_prefix; words[2] = 1; }
 }
 
-__global__ void rank_gather_kernel(const Cand *c, const uint32_t *count, uint32_t cap_in, uint32_t limit,
-                                   uint32_t *subset, uint32_t *rank, uint32_t *words) {
+// compact record of a subset member for the rank count
+struct RankRec {
+    float r;
+    uint32_t pad;
+    uint64_t key;
+};
+
+// one workgroup reservation per 256 candidates (ballot prefix), subset in an arbitrary but
+// complete order; the rank count does not depend on it
+__global__ __launch_bounds__(256) void rank_gather_kernel(const Cand *c, const uint32_t *count, uint32_t cap_in,
+                                                          uint32_t limit, uint32_t *subset, RankRec *rec,
+                                                          uint32_t *rank, uint32_t *words) {
+    __shared__ uint32_t s_w[4], s_base;
     const uint32_t n = min(*count, cap_in);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (n > limit && __float_as_uint(fabsf(c[i].response)) < words[1]) return;
-    const uint32_t pos = atomicAdd(&words[0], 1u);
+    if (blockIdx.x * blockDim.x >= n) return;
+    bool take = false;
+    Cand ci;
+    if (i < n) {
+        ci = c[i];
+        take = !(n > limit && __float_as_uint(fabsf(ci.response)) < words[1]);
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t m = __ballot(take);
+    if (lane == 0) s_w[wv] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        s_base = tot ? atomicAdd(&words[0], tot) : 0u;
+    }
+    __syncthreads();
+    if (!take) return;
+    uint32_t pos = s_base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    for (int k = 0; k < wv; k++) pos += s_w[k];
     subset[pos] = i;
     rank[pos] = 0;
+    RankRec r;
+    r.r = fabsf(ci.response);
+    r.pad = 0;
+    r.key = ci.key;
+    rec[pos] = r;
 }
 
-__global__ __launch_bounds__(kRankTile) void rank_count_kernel(const Cand *c, const uint32_t *subset,
-                                                               const uint32_t *words, uint32_t *rank) {
-    __shared__ float s_r[kRankTile];
-    __shared__ uint64_t s_k[kRankTile];
+// i-tiles of 256 (one per thread) against j-chunks of kRankJ staged in LDS; a fixed grid
+// walks the (i-tile, j-chunk) pairs
+constexpr int kRankJ = 64;
+__global__ __launch_bounds__(kRankTile) void rank_count_kernel(const RankRec *rec, const uint32_t *words,
+                                                               uint32_t *rank) {
+    __shared__ float s_r[kRankJ];
+    __shared__ uint64_t s_k[kRankJ];
     const uint32_t m = words[0];
     const bool by_resp = words[2] != 0;
-    const uint32_t ntile = (m + kRankTile - 1) / kRankTile;
-    for (uint32_t ti = blockIdx.x; ti < ntile; ti += kRankGrid) {
+    const uint32_t ntile = (m + kRankTile - 1) / kRankTile, nj = (m + kRankJ - 1) / kRankJ;
+    for (uint32_t ti = blockIdx.x; ti < ntile; ti += gridDim.x) {
         const uint32_t i = ti * kRankTile + threadIdx.x;
         float ri = 0.0f;
         uint64_t ki = 0;
-        if (i < m) { Cand ci = c[subset[i]]; ri = fabsf(ci.response); ki = ci.key; }
+        if (i < m) { RankRec a = rec[i]; ri = a.r; ki = a.key; }
         uint32_t acc = 0;
-        for (uint32_t tj = blockIdx.y; tj < ntile; tj += kRankGrid) {
-            const uint32_t j = tj * kRankTile + threadIdx.x;
-            float rj = 0.0f;
-            uint64_t kj = 0;
-            if (j < m) { Cand cj = c[subset[j]]; rj = fabsf(cj.response); kj = cj.key; }
+        for (uint32_t tj = blockIdx.y; tj < nj; tj += gridDim.y) {
             __syncthreads();
-            s_r[threadIdx.x] = rj;
-            s_k[threadIdx.x] = kj;
+            if (threadIdx.x < kRankJ) {
+                const uint32_t j = tj * kRankJ + threadIdx.x;
+                RankRec b = j < m ? rec[j] : RankRec{0.0f, 0u, ~0ull};
+                s_r[threadIdx.x] = b.r;
+                s_k[threadIdx.x] = b.key;
+            }
             __syncthreads();
-            const uint32_t mm = min((uint32_t)kRankTile, m - tj * kRankTile);
+            const uint32_t mm = min((uint32_t)kRankJ, m - tj * kRankJ);
             if (by_resp) {
-#pragma unroll 8
+#pragma unroll 16
                 for (uint32_t t = 0; t < mm; t++) {
                     float r = s_r[t];
                     acc += (r > ri || (r == ri && s_k[t] < ki)) ? 1u : 0u;
                 }
             } else {
-#pragma unroll 8
+#pragma unroll 16
                 for (uint32_t t = 0; t < mm; t++) acc += s_k[t] < ki ? 1u : 0u;
             }
         }
@@ -579,17 +648,20 @@ __global__ void rank_scatter_kernel(const Cand *c, const uint32_t *subset, const
     if (i < m && rank[i] < limit) out[rank[i]] = c[subset[i]];
 }
 
-size_t rank_scratch_bytes(uint32_t cap_in) { return ((size_t)2 * cap_in + 64) * sizeof(uint32_t); }
+size_t rank_scratch_bytes(uint32_t cap_in) {
+    return ((size_t)2 * cap_in + 64) * sizeof(uint32_t) + (size_t)cap_in * sizeof(RankRec);
+}
 
 hipError_t launch_rank(const Cand *cand, const uint32_t *count, uint32_t cap_in, uint32_t limit, Cand *out,
                        uint32_t *out_count, void *scratch, hipStream_t st) {
     uint32_t *rank = static_cast<uint32_t *>(scratch);
     uint32_t *subset = rank + cap_in;
     uint32_t *words = subset + cap_in;
+    RankRec *rec = reinterpret_cast<RankRec *>(words + 64);
     const unsigned blocks = (cap_in + 255) / 256;
     rank_select_kernel<<<1, 1024, 0, st>>>(cand, count, cap_in, limit, words);
-    rank_gather_kernel<<<blocks, 256, 0, st>>>(cand, count, cap_in, limit, subset, rank, words);
-    rank_count_kernel<<<dim3(kRankGrid, kRankGrid), kRankTile, 0, st>>>(cand, subset, words, rank);
+    rank_gather_kernel<<<blocks, 256, 0, st>>>(cand, count, cap_in, limit, subset, rec, rank, words);
+    rank_count_kernel<<<dim3(kRankGrid, 4 * kRankGrid), kRankTile, 0, st>>>(rec, words, rank);
     rank_scatter_kernel<<<blocks, 256, 0, st>>>(cand, subset, words, limit, rank, out, out_count);
     return hipGetLastError();
 }
