@@ -169,8 +169,10 @@ ps_status ps_engine_check(ps_engine *e);
 /* Stage timing with HIP events recorded on the engine's stream (profiling hook; off by
  * default). Between begin and end, every engine extract/match call records its stage
  * boundaries; end synchronises and returns per-stage totals in milliseconds:
- * [0] pyramid (blur + DoG), [1] extrema, [2] order + cap, [3] orientation + descriptors,
- * [4] match; *calls gets the number of extract calls recorded. */
+ * [0] octave 0's Gaussian + DoG levels (engine stream), [1] the later octaves (side streams
+ * forked during octave 0) and every octave's extrema, up to the join -- [0] + [1] is the
+ * whole scale space --, [2] order + cap, [3] orientation + descriptors, [4] match; *calls
+ * gets the number of extract calls recorded. */
 #define PS_NUM_STAGES 5
 ps_status ps_engine_profile_begin(ps_engine *e, uint32_t max_calls);
 ps_status ps_engine_profile_end(ps_engine *e, float stage_ms[PS_NUM_STAGES], uint32_t *calls);

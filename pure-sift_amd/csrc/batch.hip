@@ -7,8 +7,9 @@
 // the host launch cost of the ~25 kernels per image.
 //
 // Autotuner: for every (radius, source mode, level size class) of the given image sizes,
-// each blur tile height (16, 32, 64 rows) is timed on the device and the fastest is kept
-// in a process-wide table that the pyramid launcher consults.
+// each blur launch shape (tile kernel with 16, 32 or 64 rows; sliding-window kernel with
+// 64-, 128- or 192-row chunks) is timed on the device and the fastest is kept in a
+// process-wide table that the pyramid launcher consults.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -37,10 +38,23 @@ void key_push(std::vector<uint64_t> &k, uint64_t v) { k.push_back(v); }
 
 namespace pshost {
 
+// Untuned default launch shape of a blur level, from device timings of every shape on the
+// levels of a 1920x1080 pyramid: levels of >= 3M pixels take the sliding-window kernel with
+// 128-row chunks (radius <= 8; the radius-10 level and the upsampling first level stay on
+// 32- / 64-row tiles), levels of >= 1M pixels 32-row tiles, smaller ones 16-row tiles (more
+// workgroups for a launch that cannot fill the device).
+static int default_shape(int radius, int src_mode, int h, int w) {
+    const long px = (long)h * w;
+    const bool up = src_mode == ps::SRC_U8_UP || src_mode == ps::SRC_F32_UP;
+    if (px >= (3L << 20)) return up ? 64 : (radius <= 8 ? (ps::kBlurSlide | 128) : 32);
+    if (px >= (1L << 20)) return 32;
+    return 16;
+}
+
 int blur_tile(int radius, int src_mode, int h, int w) {
     std::lock_guard<std::mutex> lk(g_tile_mu);
     auto it = g_tiles.find(std::make_tuple(radius, src_mode, size_class(h), size_class(w)));
-    return it == g_tiles.end() ? 32 : it->second;
+    return it == g_tiles.end() ? default_shape(radius, src_mode, h, w) : it->second;
 }
 
 void release_batch(ps_engine *e) {
@@ -241,11 +255,11 @@ ps_status ps_engine_autotune_blur(ps_engine *e, const uint32_t *hw, uint32_t n_s
         args.w = L.w;
         float *d = as<float>(e->aux[AUX_PM_7]);
         args.dst_g = d;
-        args.dst_d = L.mode == ps::SRC_U8_UP || L.mode == ps::SRC_U8 ? nullptr : d + max_dst / 4;
+        args.dst_d = nullptr;   // extraction writes no DoG planes
         args.dst_src = L.mode == ps::SRC_DOWN2 ? d + 2 * (max_dst / 4) : nullptr;
         float best_ms = 1e30f;
         int best_th = 32;
-        for (int th : {16, 32, 64}) {
+        for (int th : {16, 32, 64, ps::kBlurSlide | 64, ps::kBlurSlide | 128, ps::kBlurSlide | 192}) {
             PS_HIP(ps::launch_blur(L.r, L.mode, args, *L.taps, th, e->stream));   // warm-up
             PS_HIP(hipEventRecord(a, e->stream));
             for (int rep = 0; rep < 3; rep++) PS_HIP(ps::launch_blur(L.r, L.mode, args, *L.taps, th, e->stream));

@@ -161,14 +161,31 @@ ps_status init_locked(int device) {
     return PS_OK;
 }
 
+static void release_streams(ps_engine *e) {
+    for (hipEvent_t ev : e->fork_ev)
+        if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : e->join_ev)
+        if (ev) (void)hipEventDestroy(ev);
+    for (hipStream_t s : e->side)
+        if (s) (void)hipStreamDestroy(s);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+}
+
 ps_status engine_new(int device, ps_engine **out) {
     auto *e = new ps_engine();
     e->device = device;
     hipError_t err = hipSetDevice(device);
     if (err == hipSuccess) err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    for (int i = 0; i < ps_engine::kSides && err == hipSuccess; i++) {
+        err = hipStreamCreateWithFlags(&e->side[i], hipStreamNonBlocking);
+        if (err == hipSuccess) err = hipEventCreateWithFlags(&e->join_ev[i], hipEventDisableTiming);
+    }
+    for (int i = 0; i < ps::kMaxOct && err == hipSuccess; i++)
+        err = hipEventCreateWithFlags(&e->fork_ev[i], hipEventDisableTiming);
     const char *mk = std::getenv("PS_MATCH_KERNEL");
     e->match_valu = mk && std::strcmp(mk, "valu") == 0;
     if (err != hipSuccess) {
+        release_streams(e);
         delete e;
         return set_err(PS_E_DEVICE, std::string("engine stream: ") + hipGetErrorString(err));
     }
@@ -185,12 +202,13 @@ void prof_release(ps_engine *e) {
     e->prof_on = false;
 }
 
-// records stage boundary `slot` of the current extract (0..4) or match (5..6) call
-void prof_mark(ps_engine *e, int slot) {
+// records stage boundary `slot` of the current extract (0..4) or match (5..6) call on `st`
+// (the engine stream unless given)
+void prof_mark(ps_engine *e, int slot, hipStream_t st = nullptr) {
     if (!e->prof_on) return;
     uint32_t call = slot < 5 ? e->prof_ext : e->prof_mat;
     if (call >= e->prof_max) return;
-    (void)hipEventRecord(e->prof_ev[(size_t)call * kEvPerCall + slot], e->stream);
+    (void)hipEventRecord(e->prof_ev[(size_t)call * kEvPerCall + slot], st ? st : e->stream);
 }
 
 void engine_free(ps_engine *e) {
@@ -205,7 +223,7 @@ void engine_free(ps_engine *e) {
     for (DevBuf &b : e->aux)
         if (b.p) (void)hipFree(b.p);
     release_batch(e);
-    if (e->stream) (void)hipStreamDestroy(e->stream);
+    release_streams(e);
     delete e;
 }
 
@@ -300,15 +318,61 @@ ps::PyrView pyr_view(ps_engine *e) {
     return v;
 }
 
-// Gaussian + DoG pyramid (pipeline.rs:550-638) into e->pyr; src is device memory.
-ps_status enqueue_pyramid(ps_engine *e, const void *src, bool src_u8, size_t h, size_t w, size_t stride,
-                          const ps_sift_config &cfg) {
+// Gaussian + DoG pyramid (pipeline.rs:550-638) into e->pyr (src is device memory) and, with
+// `extrema`, every octave's extrema appended to e->cand (cnt[0] candidates, cnt[1] error bits).
+// Octave o+1 depends only on level S of octave o, so once that level is written octave o+1
+// (the next, four times smaller and launch-latency bound) starts on a side stream while
+// octave o's last two levels and its extrema finish on its own stream; every side stream is
+// joined back into the engine stream before this returns. Candidate order is fixed later by
+// the ordering stage, so it does not matter which stream appends first. No fork happens while
+// the engine stream is being captured: a capturing caller (the batched path) already overlaps
+// whole images on separate streams.
+ps_status enqueue_scale_space(ps_engine *e, const void *src, bool src_u8, size_t h, size_t w, size_t stride,
+                              const ps_sift_config &cfg, bool extrema) {
     ps::PyrPlan &p = e->plan;
     if (!ps::make_plan((int)h, (int)w, cfg, p)) return set_err(PS_E_CONFIG, "unsupported blur radius");
     PS_TRY(ensure(e->pyr, p.total_floats * sizeof(float)));
-    if (p.n_oct == 0) return PS_OK;
+    if (extrema) PS_TRY(ensure(e->cand, (size_t)ps::kCandCap * sizeof(ps::Cand)));
+    if (p.n_oct == 0) {
+        prof_mark(e, 1);
+        return PS_OK;
+    }
     float *P = as<float>(e->pyr);
-    hipStream_t st = e->stream;
+    uint32_t *cnt = as<uint32_t>(e->cnt);
+    auto level = [&](int o, int s, hipStream_t st) -> hipError_t {
+        ps::BlurArgs b;
+        b.h = p.h[o];
+        b.w = p.w[o];
+        b.dst_g = P + p.gauss(o, s);
+        // extraction forms DoG from the Gaussian planes inside the extrema kernel; only the
+        // pyramid dump stores the DoG planes
+        b.dst_d = extrema ? nullptr : P + p.dog(o, s - 1);
+        b.dst_src = nullptr;
+        int m = ps::SRC_F32;
+        if (s == 1 && o > 0) {   // octave base = downsample_2x(gaussians[S]) (pipeline.rs:624-626)
+            b.src = P + p.gauss(o - 1, p.S);
+            b.src_stride = (size_t)p.w[o - 1];
+            b.src_h = p.h[o - 1];
+            b.src_w = p.w[o - 1];
+            b.dst_src = P + p.gauss(o, 0);
+            m = ps::SRC_DOWN2;
+        } else {
+            b.src = P + p.gauss(o, s - 1);
+            b.src_stride = (size_t)p.w[o];
+            b.src_h = p.h[o];
+            b.src_w = p.w[o];
+        }
+        return ps::launch_blur(p.lvl_r[s], m, b, p.lvl_taps[s], blur_tile(p.lvl_r[s], m, b.h, b.w), st);
+    };
+    auto octave_extrema = [&](int o, hipStream_t st) -> hipError_t {
+        return ps::launch_extrema(P + p.gauss(o, 0), p.h[o], p.w[o], p.S, cfg.contrast_threshold, p.scale_factor[o],
+                                  p.eff_scale[o], p.log2k, o, as<ps::Cand>(e->cand), cnt + 0, ps::kCandCap, cnt + 1,
+                                  st);
+    };
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    PS_HIP(hipStreamIsCapturing(e->stream, &cap));
+    const bool fork = cap == hipStreamCaptureStatusNone;
+    auto lane = [&](int o) { return fork && o > 0 ? e->side[(o - 1) % ps_engine::kSides] : e->stream; };
     ps::BlurArgs a;
     a.src = src;
     a.src_stride = stride;
@@ -320,32 +384,23 @@ ps_status enqueue_pyramid(ps_engine *e, const void *src, bool src_u8, size_t h, 
     a.dst_d = nullptr;
     a.dst_src = nullptr;
     int mode = src_u8 ? (p.upsample ? ps::SRC_U8_UP : ps::SRC_U8) : (p.upsample ? ps::SRC_F32_UP : ps::SRC_F32);
-    PS_HIP(ps::launch_blur(p.init_r, mode, a, p.init_taps, blur_tile(p.init_r, mode, a.h, a.w), st));
+    PS_HIP(ps::launch_blur(p.init_r, mode, a, p.init_taps, blur_tile(p.init_r, mode, a.h, a.w), e->stream));
     for (int o = 0; o < p.n_oct; o++) {
-        for (int s = 1; s <= p.S + 2; s++) {
-            ps::BlurArgs b;
-            b.h = p.h[o];
-            b.w = p.w[o];
-            b.dst_g = P + p.gauss(o, s);
-            b.dst_d = P + p.dog(o, s - 1);
-            b.dst_src = nullptr;
-            int m = ps::SRC_F32;
-            if (s == 1 && o > 0) {   // octave base = downsample_2x(gaussians[S]) (pipeline.rs:624-626)
-                b.src = P + p.gauss(o - 1, p.S);
-                b.src_stride = (size_t)p.w[o - 1];
-                b.src_h = p.h[o - 1];
-                b.src_w = p.w[o - 1];
-                b.dst_src = P + p.gauss(o, 0);
-                m = ps::SRC_DOWN2;
-            } else {
-                b.src = P + p.gauss(o, s - 1);
-                b.src_stride = (size_t)p.w[o];
-                b.src_h = p.h[o];
-                b.src_w = p.w[o];
-            }
-            PS_HIP(ps::launch_blur(p.lvl_r[s], m, b, p.lvl_taps[s], blur_tile(p.lvl_r[s], m, b.h, b.w), st));
+        hipStream_t st = lane(o);
+        for (int s = 1; s <= p.S; s++) PS_HIP(level(o, s, st));
+        if (fork && o + 1 < p.n_oct) {
+            PS_HIP(hipEventRecord(e->fork_ev[o], st));
+            PS_HIP(hipStreamWaitEvent(lane(o + 1), e->fork_ev[o], 0));
         }
+        for (int s = p.S + 1; s <= p.S + 2; s++) PS_HIP(level(o, s, st));
+        if (o == 0) prof_mark(e, 1);
+        if (extrema) PS_HIP(octave_extrema(o, st));
     }
+    if (fork)
+        for (int i = 0; i < ps_engine::kSides && i + 1 < p.n_oct; i++) {
+            PS_HIP(hipEventRecord(e->join_ev[i], e->side[i]));
+            PS_HIP(hipStreamWaitEvent(e->stream, e->join_ev[i], 0));
+        }
     return PS_OK;
 }
 
@@ -355,20 +410,11 @@ ps_status enqueue_keypoints(ps_engine *e, const void *src, bool src_u8, size_t h
                             const ps_sift_config &cfg, uint32_t limit) {
     PS_TRY(ensure(e->cnt, 64));
     uint32_t *cnt = as<uint32_t>(e->cnt);
-    PS_HIP(ps::launch_fill_u32(cnt, 0u, 16, e->stream));
-    prof_mark(e, 0);
-    PS_TRY(enqueue_pyramid(e, src, src_u8, h, w, stride, cfg));
-    prof_mark(e, 1);
-    PS_TRY(ensure(e->cand, (size_t)ps::kCandCap * sizeof(ps::Cand)));
     PS_TRY(ensure(e->rank, ps::rank_scratch_bytes(ps::kCandCap)));
     PS_TRY(ensure(e->sorted, (size_t)limit * sizeof(ps::Cand)));
-    ps::PyrPlan &p = e->plan;
-    float *P = as<float>(e->pyr);
-    for (int o = 0; o < p.n_oct; o++) {
-        PS_HIP(ps::launch_extrema(P + p.dog(o, 0), p.h[o], p.w[o], p.S, cfg.contrast_threshold, p.scale_factor[o],
-                                  p.eff_scale[o], p.log2k, o, as<ps::Cand>(e->cand), cnt + 0, ps::kCandCap, cnt + 1,
-                                  e->stream));
-    }
+    PS_HIP(ps::launch_fill_u32(cnt, 0u, 16, e->stream));
+    prof_mark(e, 0);
+    PS_TRY(enqueue_scale_space(e, src, src_u8, h, w, stride, cfg, true));
     prof_mark(e, 2);
     PS_HIP(ps::launch_rank(as<ps::Cand>(e->cand), cnt + 0, ps::kCandCap, limit, as<ps::Cand>(e->sorted), cnt + 2,
                            e->rank.p, e->stream));
@@ -617,7 +663,8 @@ ps_status ps_build_pyramid(const float *img, size_t h, size_t w, size_t row_stri
     if (h && w)
         PS_HIP(hipMemcpy2DAsync(e->img.p, w * sizeof(float), img, row_stride * sizeof(float), w * sizeof(float), h,
                                 hipMemcpyHostToDevice, e->stream));
-    PS_TRY(enqueue_pyramid(e, e->img.p, false, h, w, w, *cfg));
+    PS_TRY(ensure(e->cnt, 64));
+    PS_TRY(enqueue_scale_space(e, e->img.p, false, h, w, w, *cfg, false));
     if (p.total_floats)
         PS_HIP(hipMemcpyAsync(out, e->pyr.p, p.total_floats * sizeof(float), hipMemcpyDeviceToHost, e->stream));
     PS_HIP(hipStreamSynchronize(e->stream));

@@ -26,6 +26,12 @@ enum AuxSlot {
 struct ps_engine {
     int device = 0;
     hipStream_t stream = nullptr;
+    // octave o >= 1 of an extraction runs on side[(o - 1) % kSides], forked once level S of
+    // octave o - 1 is written and joined back into `stream` at the end of the scale space
+    static constexpr int kSides = 3;
+    hipStream_t side[kSides] = {};
+    hipEvent_t fork_ev[ps::kMaxOct] = {};
+    hipEvent_t join_ev[kSides] = {};
     DevBuf pyr, cand, rank, sorted, ori, nori, feat_kp, feat_j, cnt;
     DevBuf img, feat, desc, kpout;                 // host-API staging
     DevBuf qf, qd, tf, td, mcnt, mout;             // host-API match staging
@@ -66,7 +72,7 @@ ps_status prepare_extract(ps_engine *e, size_t h, size_t w, const ps_sift_config
 ps_status enqueue_extract(ps_engine *e, const void *src, bool src_u8, size_t h, size_t w, size_t stride,
                           const ps_sift_config &cfg, ps_feature *d_feat, float *d_desc, uint32_t cap,
                           uint32_t *d_count);
-// blur tile height for a level (autotuned table, else 32)
+// blur launch shape for a level (autotuned table, else a size-based default; see launch_blur)
 int blur_tile(int radius, int src_mode, int h, int w);
 
 // RAII borrow of a pooled engine for host-pointer calls

@@ -73,10 +73,14 @@ struct BlurArgs {
 bool make_plan(int h, int w, const ps_sift_config &cfg, PyrPlan &p);
 
 // launches (pyramid.hip)
-// th: output tile height, 16, 32 or 64
-hipError_t launch_blur(int radius, int src_mode, const BlurArgs &a, const Taps &t, int th, hipStream_t st);
+// Blur launch shape: 16, 32 or 64 selects the tile kernel with that output tile height;
+// kBlurSlide | rows selects the sliding-window kernel with `rows` output rows per workgroup
+// (a positive multiple of kBlurSlideStep).
+constexpr int kBlurSlide = 1 << 16;
+constexpr int kBlurSlideStep = 16;
+hipError_t launch_blur(int radius, int src_mode, const BlurArgs &a, const Taps &t, int shape, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st);
-hipError_t launch_extrema(const float *dog_octave, int h, int w, int S, float thr, float mult,
+hipError_t launch_extrema(const float *gauss_octave, int h, int w, int S, float thr, float mult,
                           const float *eff_scale, double log2k, int octave, Cand *out, uint32_t *count,
                           uint32_t cap, uint32_t *err, hipStream_t st);
 // Orders candidates: scan order if n <= limit, else (|response| desc, scan order); writes the

@@ -141,6 +141,142 @@ __global__ __launch_bounds__(kBlurThreads) void blur_kernel(BlurArgs a, Taps tap
     }
 }
 
+// Sliding-window form of the same blur for large levels. A workgroup owns a 64-column strip
+// and `chunk` output rows of it and walks down in steps of kSlideTH rows: each step stages
+// kSlideTH source rows (with the 2R-column halo), runs the horizontal pass once for them into
+// a ring of horizontal-pass rows in LDS, and emits the kSlideTH output rows whose vertical
+// window is complete. The horizontal pass therefore runs once per source row instead of once
+// per tile (plus P steps of prologue per chunk), staging uses loop-invariant clamped columns
+// and wave-uniform rows, and the next step's global loads are in flight during the vertical
+// pass. The ring keeps RING = (P + 1) * kSlideTH rows and is written twice (at slot and at
+// slot + RING) so any window of up to RING consecutive rows is contiguous in LDS. Per output
+// the arithmetic is that of blur_kernel (same taps, same order, same clamping): the two
+// kernels are bit-identical.
+constexpr int kSlideTH = kBlurSlideStep;
+
+template <int R, int SRC>
+__global__ __launch_bounds__(kBlurThreads) void blur_slide_kernel(BlurArgs a, Taps taps, int chunk) {
+    constexpr int NT = 2 * R + 1;
+    constexpr int IW = kTW + 2 * R;
+    constexpr int IWP = (IW + 3) & ~3;
+    constexpr int TH = kSlideTH;
+    constexpr int P = (2 * R + TH - 1) / TH;   // steps before the first output block
+    constexpr int RING = (P + 1) * TH;         // horizontal-pass rows kept in LDS
+    constexpr int VR = TH / 4;                 // output rows per thread per step
+    static_assert(IW <= 2 * kTW, "two staged columns per lane");
+    static_assert(VR + 2 * R <= RING, "vertical window fits the ring");
+    static_assert(kBlurThreads == 4 * 64 && TH == 4 * VR && TH * (kTW / 4) == kBlurThreads, "step shape");
+    __shared__ __attribute__((aligned(16))) float s_in[TH * IWP];
+    __shared__ __attribute__((aligned(16))) float s_h[2 * RING * kTW];
+    __shared__ __attribute__((aligned(16))) float s_c[RING * kTW];
+    // the wave index is uniform: readfirstlane lets row arithmetic stay scalar
+    const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int h = a.h, w = a.w;
+    const int x0 = blockIdx.x * kTW;
+    const int y_start = blockIdx.y * chunk;
+    const int y_end = min(y_start + chunk, h);
+    const int n_steps = (y_end - y_start + TH - 1) / TH + P;
+    const int hs = y_start - R;   // source row of horizontal-pass row 0
+    // staged columns of this lane: lane and lane + 64 of the (kTW + 2R)-wide window
+    const int gxa = min(max(x0 - R + lane, 0), w - 1);
+    const int gxb = min(max(x0 - R + kTW + lane, 0), w - 1);
+    const bool has_b = lane + kTW < IW;
+    float ra[4], rb[4];
+    auto fetch = [&](int k) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int gy = min(max(hs + k * TH + 4 * wv + i, 0), h - 1);
+            ra[i] = load_src<SRC>(a, gy, gxa);
+            rb[i] = has_b ? load_src<SRC>(a, gy, gxb) : 0.0f;
+        }
+    };
+    fetch(0);
+    const int hr = tid >> 4, hc = (tid & 15) * 4;   // this thread's horizontal-pass item
+    for (int k = 0; k < n_steps; k++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            s_in[(4 * wv + i) * IWP + lane] = ra[i];
+            if (has_b) s_in[(4 * wv + i) * IWP + kTW + lane] = rb[i];
+        }
+        __syncthreads();
+        {   // horizontal pass of row k*TH + hr, taps in k order (pipeline.rs:351-354)
+            const float *src = s_in + hr * IWP + hc;
+            float v[NT + 3];
+#pragma unroll
+            for (int j = 0; j < NT + 3; j++) v[j] = src[j];
+            float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+#pragma unroll
+            for (int q = 0; q < NT; q++) {
+                float wk = taps.w[q];
+                a0 += v[q] * wk;
+                a1 += v[q + 1] * wk;
+                a2 += v[q + 2] * wk;
+                a3 += v[q + 3] * wk;
+            }
+            const int slot = (k % (P + 1)) * TH + hr;
+            const float4 o = make_float4(a0, a1, a2, a3);
+            *reinterpret_cast<float4 *>(s_h + slot * kTW + hc) = o;
+            *reinterpret_cast<float4 *>(s_h + (slot + RING) * kTW + hc) = o;
+            *reinterpret_cast<float4 *>(s_c + slot * kTW + hc) = make_float4(v[R], v[R + 1], v[R + 2], v[R + 3]);
+        }
+        if (k + 1 < n_steps) fetch(k + 1);
+        __syncthreads();
+        if (k >= P) {   // vertical pass of output rows (k-P)*TH + 4*wv + [0, VR) (pipeline.rs:400-406)
+            const int r0 = (k - P) * TH + 4 * wv;   // relative to y_start
+            const int base = r0 % RING;
+            float v[VR + 2 * R];
+#pragma unroll
+            for (int t = 0; t < VR + 2 * R; t++) v[t] = s_h[(base + t) * kTW + lane];
+            float acc[VR];
+#pragma unroll
+            for (int j = 0; j < VR; j++) acc[j] = 0.0f;
+#pragma unroll
+            for (int q = 0; q < NT; q++) {
+                float wk = taps.w[q];
+#pragma unroll
+                for (int j = 0; j < VR; j++) acc[j] += v[j + q] * wk;
+            }
+            const int x = x0 + lane;
+#pragma unroll
+            for (int j = 0; j < VR; j++) {
+                const int y = y_start + r0 + j;
+                if (y < y_end && x < w) {
+                    int cs = base + R + j;
+                    if (cs >= RING) cs -= RING;
+                    const float center = s_c[cs * kTW + lane];
+                    const size_t o = (size_t)y * w + x;
+                    if (a.dst_g) a.dst_g[o] = acc[j];
+                    if (a.dst_d) a.dst_d[o] = acc[j] - center;   // pipeline.rs:611
+                    if (a.dst_src) a.dst_src[o] = center;
+                }
+            }
+        }
+    }
+}
+
+template <int R>
+static hipError_t blur_slide_dispatch(int src_mode, const BlurArgs &a, const Taps &t, int chunk, hipStream_t st) {
+    if (chunk <= 0 || chunk % kSlideTH) return hipErrorInvalidValue;
+    dim3 grid((a.w + kTW - 1) / kTW, (a.h + chunk - 1) / chunk);
+    switch (src_mode) {
+    case SRC_F32: blur_slide_kernel<R, SRC_F32><<<grid, kBlurThreads, 0, st>>>(a, t, chunk); break;
+    case SRC_DOWN2: blur_slide_kernel<R, SRC_DOWN2><<<grid, kBlurThreads, 0, st>>>(a, t, chunk); break;
+    case SRC_U8:
+    case SRC_F32_UP:
+    case SRC_U8_UP:
+        if constexpr (R == 4 || R == 5) {
+            if (src_mode == SRC_U8) blur_slide_kernel<R, SRC_U8><<<grid, kBlurThreads, 0, st>>>(a, t, chunk);
+            else if (src_mode == SRC_F32_UP) blur_slide_kernel<R, SRC_F32_UP><<<grid, kBlurThreads, 0, st>>>(a, t, chunk);
+            else blur_slide_kernel<R, SRC_U8_UP><<<grid, kBlurThreads, 0, st>>>(a, t, chunk);
+            break;
+        } else {
+            return hipErrorInvalidValue;
+        }
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 template <int R, int TH>
 static hipError_t blur_dispatch_th(int src_mode, const BlurArgs &a, const Taps &t, hipStream_t st) {
     dim3 grid((a.w + kTW - 1) / kTW, (a.h + TH - 1) / TH);
@@ -165,6 +301,7 @@ static hipError_t blur_dispatch_th(int src_mode, const BlurArgs &a, const Taps &
 
 template <int R>
 static hipError_t blur_dispatch_src(int src_mode, const BlurArgs &a, const Taps &t, int th, hipStream_t st) {
+    if (th & kBlurSlide) return blur_slide_dispatch<R>(src_mode, a, t, th & (kBlurSlide - 1), st);
     switch (th) {
     case 16: return blur_dispatch_th<R, 16>(src_mode, a, t, st);
     case 32: return blur_dispatch_th<R, 32>(src_mode, a, t, st);
@@ -273,7 +410,10 @@ __device__ __forceinline__ void refine3d(const float *ps, const float *cs, const
 }
 
 // 64 x 16 interior pixels per 256-thread workgroup: the S searchable DoG planes of the tile
-// plus a one-pixel halo are staged in LDS (every load of a thread in flight together).
+// plus a one-pixel halo are staged in LDS (every load of a thread in flight together). The
+// DoG planes are not stored in HBM on the extraction path: DoG s = G(s+1) - G(s) is formed
+// here from the Gaussian planes (the same single f32 subtraction the blur kernel performs,
+// pipeline.rs:611), so the blur writes one plane per level instead of two.
 // Phase 1: each thread tests its 4 pixels x S slices against the contrast threshold and the 8
 // in-plane neighbours (the cheap tests that reject almost everything) and queues survivors
 // in LDS. Phase 2: the queue is processed by all lanes in parallel -- the 18 cross-scale
@@ -284,7 +424,7 @@ constexpr int kExRows = 16;
 constexpr int kExW = 66;
 constexpr int kExH = kExRows + 2;
 template <int PL>
-__global__ __launch_bounds__(256) void extrema_kernel(const float *dog, int h, int w, float thr, float mult,
+__global__ __launch_bounds__(256) void extrema_kernel(const float *gauss, int h, int w, float thr, float mult,
                                                       EffScale eff, double log2k, int octave, Cand *out,
                                                       uint32_t *count, uint32_t cap, uint32_t *err) {
     constexpr int S = PL - 2;
@@ -301,21 +441,22 @@ __global__ __launch_bounds__(256) void extrema_kernel(const float *dog, int h, i
     const size_t P = (size_t)h * w;
     if (threadIdx.x == 0) s_qn = 0;
     {
-        float v[S][NL];
+        // Gaussian planes 1..S+1 give DoG planes 1..S
+        float v[S + 1][NL];
 #pragma unroll
         for (int j = 0; j < NL; j++) {
             int i = threadIdx.x + j * 256;
             int r = i / kExW, c = i - r * kExW;
             size_t o = (size_t)min(y0 - 1 + r, h - 1) * w + min(x0 - 1 + c, w - 1);
 #pragma unroll
-            for (int pl = 0; pl < S; pl++) v[pl][j] = i < per_plane ? dog[(size_t)(pl + 1) * P + o] : 0.0f;
+            for (int pl = 0; pl <= S; pl++) v[pl][j] = i < per_plane ? gauss[(size_t)(pl + 1) * P + o] : 0.0f;
         }
 #pragma unroll
         for (int j = 0; j < NL; j++) {
             int i = threadIdx.x + j * 256;
 #pragma unroll
             for (int pl = 0; pl < S; pl++)
-                if (i < per_plane) s_dog[pl * per_plane + i] = v[pl][j];
+                if (i < per_plane) s_dog[pl * per_plane + i] = v[pl + 1][j] - v[pl][j];
         }
     }
     __syncthreads();
@@ -365,8 +506,8 @@ __global__ __launch_bounds__(256) void extrema_kernel(const float *dog, int h, i
             const uint32_t e = s_q[qi];
             const int s = (int)(e >> 16), ly = (int)((e >> 8) & 255u), lx = (int)(e & 255u);
             const int x = x0 + lx - 1, y = y0 + ly - 1;
-            // 3 x 3 x 3 neighbourhood (slices s-1, s, s+1) from LDS, or from global memory for
-            // the outer planes; interior pixels need no clamping
+            // 3 x 3 x 3 neighbourhood (slices s-1, s, s+1) from LDS, or for the outer planes
+            // formed from the Gaussian planes in global memory; interior pixels need no clamping
             float nb[3][9];
 #pragma unroll
             for (int p = 0; p < 3; p++) {
@@ -375,9 +516,12 @@ __global__ __launch_bounds__(256) void extrema_kernel(const float *dog, int h, i
                 for (int dy = 0; dy < 3; dy++)
 #pragma unroll
                     for (int dx = 0; dx < 3; dx++)
+                    {
+                        const size_t go = (size_t)(y - 1 + dy) * w + (x - 1 + dx);
                         nb[p][dy * 3 + dx] = (pl >= 1 && pl <= S)
                                                  ? s_dog[(pl - 1) * per_plane + (ly - 1 + dy) * kExW + lx - 1 + dx]
-                                                 : dog[(size_t)pl * P + (size_t)(y - 1 + dy) * w + (x - 1 + dx)];
+                                                 : gauss[(size_t)(pl + 1) * P + go] - gauss[(size_t)pl * P + go];
+                    }
             }
             const float *ps = nb[0], *cs = nb[1], *ns = nb[2];
             const float val = cs[4];
@@ -428,7 +572,7 @@ __global__ __launch_bounds__(256) void extrema_kernel(const float *dog, int h, i
     }
 }
 
-hipError_t launch_extrema(const float *dog_octave, int h, int w, int S, float thr, float mult, const float *eff_scale,
+hipError_t launch_extrema(const float *gauss_octave, int h, int w, int S, float thr, float mult, const float *eff_scale,
                           double log2k, int octave, Cand *out, uint32_t *count, uint32_t cap, uint32_t *err,
                           hipStream_t st) {
     if (h < 3 || w < 3 || S < 1) return hipSuccess;
@@ -437,7 +581,7 @@ hipError_t launch_extrema(const float *dog_octave, int h, int w, int S, float th
     dim3 grid((w - 2 + 63) / 64, (h - 2 + kExRows - 1) / kExRows);
 #define PS_EXTREMA_CASE(SS)                                                                                   \
     case SS:                                                                                                  \
-        extrema_kernel<SS + 2><<<grid, 256, 0, st>>>(dog_octave, h, w, thr, mult, e, log2k, octave, out, count, \
+        extrema_kernel<SS + 2><<<grid, 256, 0, st>>>(gauss_octave, h, w, thr, mult, e, log2k, octave, out, count, \
                                                      cap, err);                                                \
         break;
     switch (S) {

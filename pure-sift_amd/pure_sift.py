@@ -611,7 +611,10 @@ class Engine:
                   d_d2: int) -> None:
         check(lib().ps_engine_rerank_l2(self.h, d_qdesc, nq, d_cat_desc, n, d_shortlist, k, d_idx, d_d2))
 
-    STAGES = ("pyramid", "extrema", "order", "describe", "match")
+    # "octave0" ends when octave 0's levels are written on the engine stream; "octaves_extrema"
+    # covers the later octaves (on side streams, started during octave 0) and every octave's
+    # extrema, up to the join -- their sum is the whole scale-space stage
+    STAGES = ("octave0", "octaves_extrema", "order", "describe", "match")
 
     def profile_begin(self, max_calls: int) -> None:
         check(lib().ps_engine_profile_begin(self.h, max_calls))

@@ -60,6 +60,20 @@ def test_pyramid_bit_exact_scales(gpu, scales):
         assert np.array_equal(d, r["dog"])
 
 
+@pytest.mark.parametrize("shape", [(800, 1000), (799, 1001)])
+def test_pyramid_bit_exact_large(gpu, shape):
+    """Upsampled first octave of >= 3M pixels: its levels take the sliding-window blur
+    (128-row chunks, ragged last chunk and strip for the odd shape)."""
+    u8, img = _blob_f32(shape[1], shape[0], 200, seed=shape[1])
+    cfg = gpu.SiftConfig().octaves(3).scales(3)
+    dev = gpu.Sift(cfg).build_pyramid(img)
+    ref = psr.build_pyramid(img, 3, 3, True)
+    assert len(dev) == len(ref)
+    for (g, d), r in zip(dev, ref):
+        assert np.array_equal(g, r["gauss"])
+        assert np.array_equal(d, r["dog"])
+
+
 def _cmp_keypoints(dev, ref):
     assert len(dev) == len(ref), (len(dev), len(ref))
     for f in ("x", "y", "response", "octave", "layer"):
