@@ -409,33 +409,36 @@ __device__ __forceinline__ void refine3d(const float *ps, const float *cs, const
     odx = 0.0f; ody = 0.0f; ods = 0.0f; oi = center;
 }
 
-// 64 x 16 interior pixels per 256-thread workgroup: the S searchable DoG planes of the tile
+// 64 x ROWS interior pixels per 256-thread workgroup (ROWS = 32 for large octaves, 16 for
+// small ones, whose launches are latency bound): the S searchable DoG planes of the tile
 // plus a one-pixel halo are staged in LDS (every load of a thread in flight together). The
 // DoG planes are not stored in HBM on the extraction path: DoG s = G(s+1) - G(s) is formed
 // here from the Gaussian planes (the same single f32 subtraction the blur kernel performs,
 // pipeline.rs:611), so the blur writes one plane per level instead of two.
-// Phase 1: each thread tests its 4 pixels x S slices against the contrast threshold and the 8
+// Phase 1: each thread tests its ROWS/4 pixels x S slices against the contrast threshold and the 8
 // in-plane neighbours (the cheap tests that reject almost everything) and queues survivors
 // in LDS. Phase 2: the queue is processed by all lanes in parallel -- the 18 cross-scale
 // neighbours (the outer DoG planes read from global memory, only for survivors), the Hessian
 // edge test and the Taylor refinement -- and accepted candidates reserve output slots with
 // one atomic per workgroup.
-constexpr int kExRows = 16;
 constexpr int kExW = 66;
-constexpr int kExH = kExRows + 2;
-template <int PL>
+constexpr long kExBigLevel = 1L << 20;   // levels of at least this many pixels use 32-row tiles
+template <int PL, int kExRows>
 __global__ __launch_bounds__(256) void extrema_kernel(const float *gauss, int h, int w, float thr, float mult,
                                                       EffScale eff, double log2k, int octave, Cand *out,
                                                       uint32_t *count, uint32_t cap, uint32_t *err) {
     constexpr int S = PL - 2;
+    constexpr int kExH = kExRows + 2;
     constexpr int per_plane = kExH * kExW;
     constexpr int NL = (per_plane + 255) / 256;
     // only the S searchable DoG planes are staged; the outer planes 0 and S+1 are read from
     // global memory for the rare survivors of the in-plane test
     __shared__ float s_dog[S * per_plane];
-    // phase-1 survivors: strict in-plane extrema of one kind are never 8-adjacent, so at most
-    // one maximum and one minimum per 2x2 block: half of the tile's pixels per slice
-    __shared__ uint32_t s_q[64 * kExRows * S / 2];
+    // phase-1 survivors ((slice-1) << 11 | (row-1) << 6 | (column-1)): strict in-plane extrema
+    // of one kind are never 8-adjacent, so at most one maximum and one minimum per 2x2 block:
+    // half of the tile's pixels per slice
+    static_assert(S <= 16 && kExRows <= 32 && kExW - 2 <= 64, "queue entry fields");
+    __shared__ uint16_t s_q[64 * kExRows * S / 2];
     __shared__ uint32_t s_qn;
     const int x0 = 1 + blockIdx.x * 64, y0 = 1 + blockIdx.y * kExRows;
     const size_t P = (size_t)h * w;
@@ -460,35 +463,63 @@ __global__ __launch_bounds__(256) void extrema_kernel(const float *gauss, int h,
         }
     }
     __syncthreads();
-    // phase 1
+    // phase 1 (pipeline.rs:800-850, the threshold and in-plane part): each thread owns RT
+    // consecutive rows of one column; per slice it reads the (RT+2) x 3 window around them
+    // once and evaluates all 8 comparisons of each pixel without branches, collecting
+    // survivors in a bit mask (bit (s - s0) * RT + j for up to 64 / RT slices from s0); the
+    // wave then queues them with one LDS atomic.
     {
+        constexpr int RT = kExRows / 4;
+        constexpr int SC = 64 / RT;   // slices per mask
         const int tx = threadIdx.x & 63, lx = tx + 1;
-        const int ly0 = 1 + (threadIdx.x >> 6) * (kExRows / 4);
-        if (x0 + tx < w - 1) {
-            for (int j = 0; j < kExRows / 4; j++) {
-                const int ly = ly0 + j;
-                if (y0 + ly - 1 >= h - 1) break;
+        const int ly0 = 1 + (threadIdx.x >> 6) * RT;
+        const bool col_ok = x0 + tx < w - 1;
+        auto flush = [&](uint64_t found, int s0) {
+            const uint32_t nf = (uint32_t)__popcll(found);
+            uint32_t incl = nf;
 #pragma unroll
-                for (int s = 1; s <= S; s++) {
-                    const float *cs = s_dog + (s - 1) * per_plane;
-                    const float val = cs[ly * kExW + lx];
-                    if (fabsf(val) <= thr) continue;
-                    bool ext = true;
-                    if (val > 0.0f) {
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t o = __shfl_up(incl, d);
+                if ((threadIdx.x & 63) >= d) incl += o;
+            }
+            const uint32_t wtot = __shfl(incl, 63);
+            uint32_t pos = 0;
+            if ((threadIdx.x & 63) == 63 && wtot) pos = atomicAdd(&s_qn, wtot);
+            pos = __shfl(pos, 63) + incl - nf;
+            while (found) {
+                const int b = __ffsll((unsigned long long)found) - 1;
+                found &= found - 1;
+                const int sl = s0 + b / RT, j = b % RT;
+                s_q[pos++] = (uint16_t)(((sl - 1) << 11) | ((ly0 + j - 1) << 6) | (lx - 1));
+            }
+        };
+        uint64_t found = 0;
 #pragma unroll
-                        for (int dy = -1; dy <= 1; dy++)
+        for (int s = 1; s <= S; s++) {
+            const float *cs = s_dog + (s - 1) * per_plane + (ly0 - 1) * kExW + lx - 1;
+            float r[RT + 2][3];
 #pragma unroll
-                            for (int dx = -1; dx <= 1; dx++)
-                                if (dx != 0 || dy != 0) ext = ext && (val > cs[(ly + dy) * kExW + lx + dx]);
-                    } else {
+            for (int i = 0; i < RT + 2; i++)
 #pragma unroll
-                        for (int dy = -1; dy <= 1; dy++)
+                for (int c = 0; c < 3; c++) r[i][c] = cs[i * kExW + c];
 #pragma unroll
-                            for (int dx = -1; dx <= 1; dx++)
-                                if (dx != 0 || dy != 0) ext = ext && (val < cs[(ly + dy) * kExW + lx + dx]);
-                    }
-                    if (ext) s_q[atomicAdd(&s_qn, 1u)] = ((uint32_t)s << 16) | ((uint32_t)ly << 8) | (uint32_t)lx;
-                }
+            for (int j = 0; j < RT; j++) {
+                const float val = r[j + 1][1];
+                bool gt = true, lt = true;
+#pragma unroll
+                for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+                    for (int dx = 0; dx < 3; dx++)
+                        if (dy != 1 || dx != 1) {
+                            gt &= val > r[j + dy][dx];
+                            lt &= val < r[j + dy][dx];
+                        }
+                const bool ok = col_ok && (y0 + ly0 + j - 1 < h - 1) && fabsf(val) > thr && (val > 0.0f ? gt : lt);
+                found |= (uint64_t)ok << (((s - 1) % SC) * RT + j);
+            }
+            if (s % SC == 0 || s == S) {
+                flush(found, s - (s - 1) % SC);
+                found = 0;
             }
         }
     }
@@ -499,12 +530,13 @@ __global__ __launch_bounds__(256) void extrema_kernel(const float *gauss, int h,
     const uint32_t nq = s_qn;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (uint32_t q0 = 0; q0 < nq; q0 += 256) {
-        const uint32_t qi = q0 + threadIdx.x;
+        // queue entry q0 + 4*lane + wave: a short queue is spread over all four waves
+        const uint32_t qi = q0 + 4 * (uint32_t)lane + (uint32_t)wv;
         bool accept = false;
         Cand c;
         if (qi < nq) {
             const uint32_t e = s_q[qi];
-            const int s = (int)(e >> 16), ly = (int)((e >> 8) & 255u), lx = (int)(e & 255u);
+            const int s = (int)(e >> 11) + 1, ly = (int)((e >> 6) & 31u) + 1, lx = (int)(e & 63u) + 1;
             const int x = x0 + lx - 1, y = y0 + ly - 1;
             // 3 x 3 x 3 neighbourhood (slices s-1, s, s+1) from LDS, or for the outer planes
             // formed from the Gaussian planes in global memory; interior pixels need no clamping
@@ -512,16 +544,26 @@ __global__ __launch_bounds__(256) void extrema_kernel(const float *gauss, int h,
 #pragma unroll
             for (int p = 0; p < 3; p++) {
                 const int pl = s - 1 + p;
+                if (pl >= 1 && pl <= S) {
+                    const float *q = s_dog + (pl - 1) * per_plane + (ly - 1) * kExW + lx - 1;
 #pragma unroll
-                for (int dy = 0; dy < 3; dy++)
+                    for (int dy = 0; dy < 3; dy++)
 #pragma unroll
-                    for (int dx = 0; dx < 3; dx++)
-                    {
-                        const size_t go = (size_t)(y - 1 + dy) * w + (x - 1 + dx);
-                        nb[p][dy * 3 + dx] = (pl >= 1 && pl <= S)
-                                                 ? s_dog[(pl - 1) * per_plane + (ly - 1 + dy) * kExW + lx - 1 + dx]
-                                                 : gauss[(size_t)(pl + 1) * P + go] - gauss[(size_t)pl * P + go];
-                    }
+                        for (int dx = 0; dx < 3; dx++) nb[p][dy * 3 + dx] = q[dy * kExW + dx];
+                } else {
+                    const float *g0 = gauss + (size_t)pl * P + (size_t)(y - 1) * w + (x - 1);
+                    const float *g1 = g0 + P;
+                    float a[9], b[9];
+#pragma unroll
+                    for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+                        for (int dx = 0; dx < 3; dx++) {
+                            a[dy * 3 + dx] = g1[(size_t)dy * w + dx];
+                            b[dy * 3 + dx] = g0[(size_t)dy * w + dx];
+                        }
+#pragma unroll
+                    for (int k = 0; k < 9; k++) nb[p][k] = a[k] - b[k];
+                }
             }
             const float *ps = nb[0], *cs = nb[1], *ns = nb[2];
             const float val = cs[4];
@@ -578,11 +620,17 @@ hipError_t launch_extrema(const float *gauss_octave, int h, int w, int S, float 
     if (h < 3 || w < 3 || S < 1) return hipSuccess;
     EffScale e;
     for (int i = 0; i < kMaxLevels; i++) e.v[i] = eff_scale[i];
-    dim3 grid((w - 2 + 63) / 64, (h - 2 + kExRows - 1) / kExRows);
+    const bool big = (long)h * w >= kExBigLevel;
+    const int rows = big ? 32 : 16;
+    dim3 grid((w - 2 + 63) / 64, (h - 2 + rows - 1) / rows);
 #define PS_EXTREMA_CASE(SS)                                                                                   \
     case SS:                                                                                                  \
-        extrema_kernel<SS + 2><<<grid, 256, 0, st>>>(gauss_octave, h, w, thr, mult, e, log2k, octave, out, count, \
-                                                     cap, err);                                                \
+        if (big)                                                                                              \
+            extrema_kernel<SS + 2, 32><<<grid, 256, 0, st>>>(gauss_octave, h, w, thr, mult, e, log2k, octave,  \
+                                                             out, count, cap, err);                           \
+        else                                                                                                  \
+            extrema_kernel<SS + 2, 16><<<grid, 256, 0, st>>>(gauss_octave, h, w, thr, mult, e, log2k, octave,  \
+                                                             out, count, cap, err);                           \
         break;
     switch (S) {
         PS_EXTREMA_CASE(3) PS_EXTREMA_CASE(4) PS_EXTREMA_CASE(5) PS_EXTREMA_CASE(6)

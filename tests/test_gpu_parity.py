@@ -93,6 +93,19 @@ def test_keypoints_exact_640x480(gpu, upsample):
     _cmp_keypoints(dev, ref)
 
 
+@pytest.mark.parametrize("scales", [3, 5, 8, 9, 10])
+def test_keypoints_exact_scales(gpu, scales):
+    """Slice counts whose survivor masks fill one or two 64-bit words in the extremum
+    kernel's in-plane pass: the upsampled first octave (1200 x 900) takes 32-row tiles (8 rows
+    per thread: S = 8 fills one word, S > 8 flushes twice), the later octaves 16-row tiles."""
+    u8, img = _blob_f32(600, 450, 200, seed=scales)
+    cfg = gpu.SiftConfig().octaves(3).scales(scales)
+    dev = gpu.Sift(cfg).detect_keypoints(img)
+    ref = psr.detect_keypoints(img, 3, scales, 0.015, True)
+    assert len(ref) > 10
+    _cmp_keypoints(dev, ref)
+
+
 def _align_features(dh, dd, rh, rd):
     """Pairs device and oracle features by (x, y, response) in order; returns index pairs
     and the number of keypoints whose orientation count differs."""
