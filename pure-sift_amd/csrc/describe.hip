@@ -254,11 +254,13 @@ hipError_t launch_feature_scan(const uint32_t *nkp, const uint32_t *nori, uint32
 // the 4 neighbouring cells then forms each of the 128 bins.
 constexpr int kLanesPerCell = 10;
 constexpr int kPrivStride = 33;   // 32 accumulators + 1 pad
+constexpr int kListCap = 12;      // box pixels tested per lane before the recorded ones are processed
 __global__ __launch_bounds__(kDescThreads) void descriptor_kernel(PyrView pv, const Cand *kps, const float *ori,
                                                                   const uint32_t *feat_kp, const uint32_t *feat_j,
                                                                   const uint32_t *nfeat, uint32_t cap, int kind,
                                                                   ps_feature *out_f, float *out_d) {
     __shared__ float s_priv[kDescThreads * kPrivStride];
+    __shared__ uint16_t s_list[kDescThreads * kListCap];
     __shared__ float s_desc[128];
     const uint32_t f = blockIdx.x;
     const int lane = threadIdx.x;
@@ -297,51 +299,72 @@ __global__ __launch_bounds__(kDescThreads) void descriptor_kernel(PyrView pv, co
         const int xa = max(x_min, (int)floorf(l.x + bx0) - 1), xb = min(x_max, (int)ceilf(l.x + bx1) + 1);
         const int ya = max(y_min, (int)floorf(l.y + by0) - 1), yb = min(y_max, (int)ceilf(l.y + by1) + 1);
         const int ww = xb - xa + 1, hh = yb - ya + 1;
-        const int total = (ww > 0 && hh > 0) ? ww * hh : 0;
+        // a cell box spans about 1.42 hist_width + 5 pixels; a box of 256 or more (a keypoint
+        // scale beyond any the detector produces) is skipped rather than mis-packed
+        const int total = (ww > 0 && hh > 0 && ww <= 256 && hh <= 256) ? ww * hh : 0;
         const float inv_ww = 1.0f / (float)max(ww, 1);
-        for (int p0 = sub; p0 < total; p0 += kBatch * kLanesPerCell) {
-            float gxa[kBatch], gxb[kBatch], gya[kBatch], gyb[kBatch], rb_[kBatch], cb_[kBatch], rr_[kBatch], cc_[kBatch];
-            bool ok[kBatch];
-#pragma unroll
-            for (int u = 0; u < kBatch; u++) {
-                int p = p0 + u * kLanesPerCell;
+        // window test and bin coordinates of box pixel (qy, qx) (pipeline.rs:1370-1393)
+        auto bins = [&](int qy, int qx, float &r_rot, float &c_rot, float &r_bin, float &c_bin) {
+            const float dy = (float)(ya + qy) - l.y, dx = (float)(xa + qx) - l.x;
+            r_rot = (dx * cos_t + dy * sin_t) * inv_hw;
+            c_rot = (-dx * sin_t + dy * cos_t) * inv_hw;
+            r_bin = r_rot + 4.0f * 0.5f - 0.5f;
+            c_bin = c_rot + 4.0f * 0.5f - 0.5f;
+        };
+        // Two phases per chunk of kListCap box pixels so the expensive part runs on dense lanes:
+        // (A) the lane tests the chunk's pixels and records those of its cell in a lane-private
+        // LDS list; (B) it processes the list in batches whose gradient loads are issued
+        // together.
+        // entry i at list[i * kDescThreads], packed (qy << 8 | qx); the box sides are below
+        // 256 pixels (checked above)
+        uint16_t *list = s_list + lane;
+        for (int chunk = sub; chunk < total; chunk += kListCap * kLanesPerCell) {
+            const int chunk_end = min(chunk + kListCap * kLanesPerCell, total);
+            int n = 0;
+            for (int p = chunk; p < chunk_end; p += kLanesPerCell) {
                 int qy, qx;
                 divmod_small(p, ww, inv_ww, qy, qx);
-                int yy = ya + qy, xx = xa + qx;
-                float dy = (float)yy - l.y, dx = (float)xx - l.x;
-                float r_rot = (dx * cos_t + dy * sin_t) * inv_hw;
-                float c_rot = (-dx * sin_t + dy * cos_t) * inv_hw;
-                float r_bin = r_rot + 4.0f * 0.5f - 0.5f;
-                float c_bin = c_rot + 4.0f * 0.5f - 0.5f;
-                rr_[u] = r_rot; cc_[u] = c_rot; rb_[u] = r_bin; cb_[u] = c_bin;
-                ok[u] = p < total && !(r_bin <= -1.0f || r_bin >= 4.0f || c_bin <= -1.0f || c_bin >= 4.0f) &&
-                        (int)floorf(r_bin) == r0 && (int)floorf(c_bin) == c0;
-                const float *row = l.g + (size_t)(ok[u] ? yy : y_min) * l.W + (ok[u] ? xx : x_min);
-                gxa[u] = row[1]; gxb[u] = row[-1]; gyb[u] = row[l.W]; gya[u] = row[-l.W];
+                float r_rot, c_rot, r_bin, c_bin;
+                bins(qy, qx, r_rot, c_rot, r_bin, c_bin);
+                const bool ok = !(r_bin <= -1.0f || r_bin >= 4.0f || c_bin <= -1.0f || c_bin >= 4.0f) &&
+                                (int)floorf(r_bin) == r0 && (int)floorf(c_bin) == c0;
+                if (ok) list[(n++) * kDescThreads] = (uint16_t)((qy << 8) | qx);
             }
+            for (int i0 = 0; i0 < n; i0 += kBatch) {
+                float gxa[kBatch], gxb[kBatch], gya[kBatch], gyb[kBatch];
+                int pq[kBatch];
 #pragma unroll
-            for (int u = 0; u < kBatch; u++) {
-                if (!ok[u]) continue;
-                float gdx = gxa[u] - gxb[u], gdy = gyb[u] - gya[u];   // pipeline.rs:1186-1189
-                float m = sqrtf(gdx * gdx + gdy * gdy);
-                if (m <= 1e-7f) continue;
-                float a = rem_euclid(grad_atan2(gdy, gdx), kTwoPi);
-                float r_rot = rr_[u], c_rot = cc_[u], r_bin = rb_[u], c_bin = cb_[u];
-                float wm = m * expf((r_rot * r_rot + c_rot * c_rot) * exp_scale);
-                float o_bin = rem_euclid(a - theta, kTwoPi) * bin_per_rad;
-                int o0 = (int)floorf(o_bin);
-                float dr = r_bin - (float)r0, dc = c_bin - (float)c0, dof = o_bin - (float)o0;
-                const int ob0 = o0 & 7, ob1 = (o0 + 1) & 7;
+                for (int u = 0; u < kBatch; u++) {
+                    const int i = min(i0 + u, n - 1);
+                    pq[u] = list[i * kDescThreads];
+                    const float *row = l.g + (size_t)(ya + (pq[u] >> 8)) * l.W + (xa + (pq[u] & 255));
+                    gxa[u] = row[1]; gxb[u] = row[-1]; gyb[u] = row[l.W]; gya[u] = row[-l.W];
+                }
 #pragma unroll
-                for (int ir = 0; ir <= 1; ir++) {
-                    float wr = ir == 0 ? 1.0f - dr : dr;
+                for (int u = 0; u < kBatch; u++) {
+                    if (i0 + u >= n) continue;
+                    float gdx = gxa[u] - gxb[u], gdy = gyb[u] - gya[u];   // pipeline.rs:1186-1189
+                    float m = sqrtf(gdx * gdx + gdy * gdy);
+                    if (m <= 1e-7f) continue;
+                    float a = rem_euclid(grad_atan2(gdy, gdx), kTwoPi);
+                    float r_rot, c_rot, r_bin, c_bin;
+                    bins(pq[u] >> 8, pq[u] & 255, r_rot, c_rot, r_bin, c_bin);
+                    float wm = m * expf((r_rot * r_rot + c_rot * c_rot) * exp_scale);
+                    float o_bin = rem_euclid(a - theta, kTwoPi) * bin_per_rad;
+                    int o0 = (int)floorf(o_bin);
+                    float dr = r_bin - (float)r0, dc = c_bin - (float)c0, dof = o_bin - (float)o0;
+                    const int ob0 = o0 & 7, ob1 = (o0 + 1) & 7;
 #pragma unroll
-                    for (int ic = 0; ic <= 1; ic++) {
-                        float wc = ic == 0 ? 1.0f - dc : dc;
-                        float w0 = wm * wr * wc;
-                        float *acc = priv + (ir * 2 + ic) * 8;
-                        acc[ob0] += w0 * (1.0f - dof);
-                        acc[ob1] += w0 * dof;
+                    for (int ir = 0; ir <= 1; ir++) {
+                        float wr = ir == 0 ? 1.0f - dr : dr;
+#pragma unroll
+                        for (int ic = 0; ic <= 1; ic++) {
+                            float wc = ic == 0 ? 1.0f - dc : dc;
+                            float w0 = wm * wr * wc;
+                            float *acc = priv + (ir * 2 + ic) * 8;
+                            acc[ob0] += w0 * (1.0f - dof);
+                            acc[ob1] += w0 * dof;
+                        }
                     }
                 }
             }
