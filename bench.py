@@ -6,9 +6,11 @@ octave upsampled, contrast 0.015, capped at 4096 features) and ratio-0.8 match i
 against the pre-extracted warped partner. Inputs are resident in HBM before the
 timed region; the whole step runs on one engine stream through the C ABI.
 
-Prints one JSON line. `roofline` covers the HBM-bound pyramid + extrema stage
-(algorithmic bytes per image W*H + 4*P*(6+5) + 4*P*5 = 707,097,600 B at 1080p),
-timed with HIP events on the engine's stream. `cpu_baseline` is the CPU oracle
+Prints one JSON line. `roofline` covers the scale-space stage (pyramid + extrema, HBM bound):
+algorithmic bytes per image W*H + 4*P*(6 + 5 + 4) = 663,033,600 B at 1080p -- the u8 input, the
+6 Gaussian planes written per octave, the 5 source planes the blur levels read and the 4
+Gaussian planes the extrema pass reads (P = pixels summed over octaves) -- timed with HIP events
+on the engine's stream. `cpu_baseline` is the CPU oracle
 (the C restatement of the reference's Rayon path, parallelised at the same sites)
 timed on a bounded sample on this host.
 """
@@ -24,10 +26,10 @@ sys.path.insert(0, os.path.join(REPO, "pure-sift_amd"))
 HBM_PEAK_GBS = 8000.0
 # 32-bit integer VALU issue rate: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 VALU_INT_PEAK_TOPS = 39.32
-ALG_BYTES_1080P = 707_097_600   # W*H + 4*P*11 + 4*P*5, P = 11,016,000 (4 octaves, upsampled)
-# HBM bytes per 1080p frame moved by the pyramid + extrema kernels, from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE passes scaled by a 256 MiB copy (profiles/pyramid_extrema_1080p_hbm_traffic.md)
-TRAFFIC_BYTES_1080P = 1_529_587_947
+ALG_BYTES_1080P = 663_033_600   # W*H + 4*P*(6 + 5 + 4), P = 11,016,000 (4 octaves, upsampled)
+# HBM bytes per 1080p frame moved by the scale-space kernels (blur + extrema), from rocprofv3
+# FETCH_SIZE and WRITE_SIZE passes scaled by a 256 MiB copy (profiles/pyramid_extrema_1080p_hbm_traffic.md)
+TRAFFIC_BYTES_1080P = 819_377_923
 
 
 def alg_bytes(h, w, octaves=4, upsample=True):
@@ -39,7 +41,7 @@ def alg_bytes(h, w, octaves=4, upsample=True):
         P += hh * ww
         hh //= 2
         ww //= 2
-    return h * w + 4 * P * 11 + 4 * P * 5
+    return h * w + 4 * P * (6 + 5 + 4)
 
 
 def _host_cores() -> int:
@@ -473,7 +475,8 @@ def main():
                                "match against the pre-extracted partner",
                    "features": [n_q, n_t], "matches": n_m, "parallelism": "single GPU"},
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
-        "roofline": {"bound": "hbm", "kernel": "pyramid+extrema stage (blur_kernel x21, extrema_kernel x4)",
+        "roofline": {"bound": "hbm", "kernel": "scale-space stage: 21 blur launches (tile / sliding-window kernels) + 4 extrema launches, "
+                               "octaves 1-3 overlapped on side streams",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES_1080P,
                      "alg_bytes_per_image": ALG_BYTES_1080P},

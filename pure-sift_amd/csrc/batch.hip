@@ -39,14 +39,17 @@ void key_push(std::vector<uint64_t> &k, uint64_t v) { k.push_back(v); }
 namespace pshost {
 
 // Untuned default launch shape of a blur level, from device timings of every shape on the
-// levels of a 1920x1080 pyramid: levels of >= 3M pixels take the sliding-window kernel with
-// 128-row chunks (radius <= 8; the radius-10 level and the upsampling first level stay on
-// 32- / 64-row tiles), levels of >= 1M pixels 32-row tiles, smaller ones 16-row tiles (more
-// workgroups for a launch that cannot fill the device).
+// levels of a 1920x1080 pyramid. Levels of >= 3M pixels: radius 5..8 take the sliding-window
+// kernel with 128-row chunks, radius <= 4 32-row tiles, larger radii and the upsampling first
+// level 64-row tiles; levels of >= 1M pixels take 32-row tiles, smaller ones 16-row tiles
+// (more workgroups for a launch that cannot fill the device).
 static int default_shape(int radius, int src_mode, int h, int w) {
     const long px = (long)h * w;
     const bool up = src_mode == ps::SRC_U8_UP || src_mode == ps::SRC_F32_UP;
-    if (px >= (3L << 20)) return up ? 64 : (radius <= 8 ? (ps::kBlurSlide | 128) : 32);
+    if (px >= (3L << 20)) {
+        if (up || radius > 8) return 64;
+        return radius <= 4 ? 32 : (ps::kBlurSlide | 128);
+    }
     if (px >= (1L << 20)) return 32;
     return 16;
 }

@@ -24,6 +24,17 @@ namespace ps {
 constexpr int kTW = 64;
 constexpr int kBlurThreads = 256;
 
+// Workgroups are dispatched to the 8 XCDs round-robin by linear workgroup ID, and each XCD has
+// its own L2. xcd_tile maps the linear ID b (of nb) to a tile index so that every XCD owns one
+// contiguous, row-major range of tiles: neighbouring tiles, whose halos overlap, then share an
+// L2 instead of fetching the shared cache lines from HBM once per XCD. The map is a bijection
+// on [0, nb) (XCD k owns nb/8 + (k < nb%8) consecutive tiles).
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
+    const uint32_t q = nb / kXcds, r = nb % kXcds, k = b % kXcds;
+    return k * q + min(k, r) + b / kXcds;
+}
+
 template <int SRC>
 __device__ __forceinline__ float load_src(const BlurArgs &a, int y, int x) {
     const float inv255 = 1.0f / 255.0f;   // luma_to_array2, pipeline.rs:961
@@ -61,7 +72,8 @@ __global__ __launch_bounds__(kBlurThreads) void blur_kernel(BlurArgs a, Taps tap
     __shared__ __attribute__((aligned(16))) float s_in[IH * IWP];
     __shared__ __attribute__((aligned(16))) float s_tmp[IH * kTW];
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * kTW, y0 = blockIdx.y * kTH;
+    const uint32_t tile = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int x0 = (int)(tile % gridDim.x) * kTW, y0 = (int)(tile / gridDim.x) * kTH;
     const int h = a.h, w = a.w;
 
     // stage the clamped window: all loads of a thread are issued before its LDS stores so
@@ -172,8 +184,9 @@ __global__ __launch_bounds__(kBlurThreads) void blur_slide_kernel(BlurArgs a, Ta
     // the wave index is uniform: readfirstlane lets row arithmetic stay scalar
     const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int h = a.h, w = a.w;
-    const int x0 = blockIdx.x * kTW;
-    const int y_start = blockIdx.y * chunk;
+    const uint32_t tile = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int x0 = (int)(tile % gridDim.x) * kTW;
+    const int y_start = (int)(tile / gridDim.x) * chunk;
     const int y_end = min(y_start + chunk, h);
     const int n_steps = (y_end - y_start + TH - 1) / TH + P;
     const int hs = y_start - R;   // source row of horizontal-pass row 0
