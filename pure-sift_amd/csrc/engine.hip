@@ -482,7 +482,8 @@ ps_status enqueue_nn(ps_engine *e, const float *qd, const uint32_t *nq, size_t n
         return PS_OK;
     }
     PS_TRY(ensure(scratch, ps::nn_mfma_scratch_bytes(nqm, ntm)));
-    PS_HIP(ps::launch_nn_mfma(qd, nq, nqm, td, nt, ntm, scratch.p, bidx, bd, sd, e->stream));
+    if (++e->nn_gen == 0) e->nn_gen = 1;
+    PS_HIP(ps::launch_nn_mfma(qd, nq, nqm, td, nt, ntm, scratch.p, bidx, bd, sd, e->nn_gen, e->stream));
     return PS_OK;
 }
 
@@ -494,6 +495,7 @@ ps_status enqueue_match(ps_engine *e, const ps_feature *qf, const float *qd, con
     PS_TRY(ensure(e->bidx, nqm * sizeof(uint32_t)));
     PS_TRY(ensure(e->bdist, nqm * sizeof(float)));
     PS_TRY(ensure(e->sdist, nqm * sizeof(float)));
+    if (nqm > 262144) return set_err(PS_E_INVALID_ARG, "at most 262144 query features per match call");
     PS_TRY(enqueue_nn(e, qd, nq, nqm, td, nt, ntm, e->part, as<uint32_t>(e->bidx), as<float>(e->bdist),
                       as<float>(e->sdist), opt.precision));
     uint32_t *rev = nullptr;
@@ -505,7 +507,6 @@ ps_status enqueue_match(ps_engine *e, const ps_feature *qf, const float *qd, con
                           as<float>(e->ridx_d) + ntm, opt.precision));
         rev = as<uint32_t>(e->ridx);
     }
-    if (nqm > 262144) return set_err(PS_E_INVALID_ARG, "at most 262144 query features per match call");
     PS_TRY(ensure(e->fin, ps::match_finish_scratch_bytes(nqm)));
     PS_HIP(ps::launch_match_finish(qf, tf, nq, nqm, nt, as<uint32_t>(e->bidx), as<float>(e->bdist),
                                    as<float>(e->sdist), rev, opt.ratio, opt.mutual ? 1 : 0, out, out_count, e->fin.p,

@@ -38,6 +38,7 @@ struct ps_engine {
     DevBuf part, part2, bidx, bdist, sdist, ridx, ridx_d, fin;  // matcher scratch
     DevBuf aux[AUX_COUNT];
     bool match_valu = false;                       // PS_MATCH_KERNEL=valu: direct f32 kernel
+    uint32_t nn_gen = 0;                           // per-call tag of the exact matcher's scratch words
     // batched extraction: child engines (own stream + workspace each) run images
     // concurrently; the last batch is kept as an instantiated graph
     std::vector<ps_engine *> children;

@@ -1,8 +1,14 @@
-// Exact Lowe-ratio matching through an FP32 MFMA distance GEMM.
+// Exact Lowe-ratio matching through a split-BF16 MFMA distance GEMM.
 //
-// Pass 1 (nn_mfma_kernel): d'(q,t) = |t|^2 - 2 q.t on v_mfma_f32_32x32x2f32 (the per-query
-// constant |q|^2 does not change the ranking; |t|^2 comes from the LDS tile); every lane keeps
-// the 4 smallest (d', index) of one query over its targets in registers.
+// Pass 1 (nn_mfma_kernel): d'(q,t) = |t|^2 - 2 q.t with q.t from three bf16 MFMA products:
+// every f32 element x is split as x = hi + lo + e with hi = bf16(x), lo = bf16(x - hi)
+// (round to nearest even; x - hi is exact in f32), |e| <= 2^-16 |x|, and
+// q.t ~ qh.th + qh.tl + ql.th on v_mfma_f32_32x32x16_bf16 (products exact in f32, f32
+// accumulation). The dropped terms are below 3.1 * 2^-16 |q||t| and the accumulation of 384
+// terms below 384 * 2^-24 |q||t|, so |error of q.t| < 7e-5 |q||t| -- about 5x the cheapest
+// exact alternative (v_mfma_f32_32x32x2f32) in matrix throughput. The per-query constant
+// |q|^2 does not change the ranking; |t|^2 comes from the LDS tile; every lane keeps the 4
+// smallest (d', index) of one query over its targets in registers.
 // Pass 2 (nn_rescore_kernel): the per-split lists are merged into the global top-4, each
 // candidate is rescored with the reference's sequential f32 sum (pipeline.rs:1700-1717),
 // and best/second/index follow the reference's rule. Exactness guard: with E a bound on
@@ -58,49 +64,76 @@ struct Cand4 {
     uint32_t i[4];
 };
 
-// |x|^2 of each row (any summation order: it only ranks and bounds); a workgroup covers 64
-// rows (each wave 16, one at a time); rows in [n, rows) get +inf so they never rank;
-// *vmax (when given) gets the largest norm of the valid rows as ordered bits (one atomic
-// per workgroup).
-__global__ __launch_bounds__(256) void row_norms_kernel(const float *x, const uint32_t *n_p, uint32_t n_max,
-                                                        uint32_t rows, float *out, uint32_t *vmax) {
+__device__ __forceinline__ uint32_t bf16_bits_rne(float x) {
+    const uint32_t b = __float_as_uint(x);
+    return (b + 0x7fffu + ((b >> 16) & 1u)) >> 16;
+}
+
+// Matcher prep, one launch for both sides: |x|^2 of each row (any summation order: it only
+// ranks and bounds) and the row's bf16 split (128 words: hi pairs, then lo pairs; elements 2j
+// and 2j+1 in word j). Workgroups [0, tblocks) take target rows, the rest query rows, 64 rows
+// each (each wave 16, one at a time); rows in [n, rows) get norm +inf, so they never rank,
+// and a zero split. The largest target norm goes to *tmax as (gen << 32 | ordered bits) by
+// one 64-bit atomicMax per workgroup: values left by earlier calls carry a smaller
+// generation, so the word needs no clearing.
+struct PrepSide {
+    const float *x;
+    const uint32_t *n_p;
+    uint32_t n_max, rows;
+    float *norm;
+    uint32_t *split;
+};
+__global__ __launch_bounds__(256) void nn_prep_kernel(PrepSide t, PrepSide q, uint32_t tblocks,
+                                                      unsigned long long *tmax, uint32_t gen) {
     __shared__ float s_max[4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t n = min(*n_p, n_max);
+    const bool tside = blockIdx.x < tblocks;
+    const PrepSide &sd = tside ? t : q;
+    const uint32_t blk = tside ? blockIdx.x : blockIdx.x - tblocks;
+    const uint32_t n = min(*sd.n_p, sd.n_max);
     float wmax = 0.0f;
     float2 v[16];
 #pragma unroll
     for (int k = 0; k < 16; k++) {   // all loads first so their latencies overlap
-        const uint32_t r = blockIdx.x * 64 + w * 16 + k;
-        v[k] = r < n ? reinterpret_cast<const float2 *>(x + (size_t)r * 128)[lane] : make_float2(0.f, 0.f);
+        const uint32_t r = blk * 64 + w * 16 + k;
+        v[k] = r < n ? reinterpret_cast<const float2 *>(sd.x + (size_t)r * 128)[lane] : make_float2(0.f, 0.f);
     }
 #pragma unroll
     for (int k = 0; k < 16; k++) {
-        const uint32_t r = blockIdx.x * 64 + w * 16 + k;
+        const uint32_t r = blk * 64 + w * 16 + k;
         float s = v[k].x * v[k].x + v[k].y * v[k].y;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-        if (lane == 0 && r < rows) out[r] = r < n ? s : INFINITY;
+        if (lane == 0 && r < sd.rows) sd.norm[r] = r < n ? s : INFINITY;
         if (r < n) wmax = fmaxf(wmax, isfinite(s) ? s : INFINITY);
+        if (r < sd.rows) {
+            const uint32_t hx = bf16_bits_rne(v[k].x), hy = bf16_bits_rne(v[k].y);
+            const float rx = v[k].x - __uint_as_float(hx << 16), ry = v[k].y - __uint_as_float(hy << 16);
+            sd.split[(size_t)r * 128 + lane] = hx | (hy << 16);
+            sd.split[(size_t)r * 128 + 64 + lane] = bf16_bits_rne(rx) | (bf16_bits_rne(ry) << 16);
+        }
     }
-    if (!vmax) return;
+    if (!tside) return;
     if (lane == 0) s_max[w] = wmax;
     __syncthreads();
     if (threadIdx.x == 0) {
         const float m = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
-        if (m > 0.0f) atomicMax(vmax, __float_as_uint(m));
+        atomicMax(tmax, ((unsigned long long)gen << 32) | __float_as_uint(m));
     }
 }
 
-// Pass 1. Workgroup: 128 queries (4 waves x 32 B-operand columns, each lane holding half of
-// its query's 128 dims in VGPRs) against one split of the targets, 64 targets per LDS tile;
-// the next tile is loaded into registers while the current one is multiplied. Every lane
-// keeps the 4 smallest (|t|^2 - 2 q.t, index) of the targets it sees; both halves' lists are
-// written (8 candidates per query and split).
-__global__ __launch_bounds__(256) void nn_mfma_kernel(const float *qd, const uint32_t *nq_p, uint32_t nq_max,
-                                                      const float *td, const uint32_t *nt_p, uint32_t nt_max,
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+// Pass 1. Workgroup: 128 queries (4 waves x 32 B-operand columns; lane (col, kh) holds the
+// hi and lo words of its query for k = 16s + 8kh .. +7, s = 0..7, in VGPRs) against one split of
+// the targets, 64 split target rows per LDS tile; the next tile is loaded into registers while
+// the current one is multiplied. Every lane keeps the 4 smallest (|t|^2 - 2 q.t, index) of
+// the targets it sees; both halves' lists are written (8 candidates per query and split).
+__global__ __launch_bounds__(256) void nn_mfma_kernel(const uint32_t *qs, const uint32_t *nq_p, uint32_t nq_max,
+                                                      const uint32_t *ts, const uint32_t *nt_p, uint32_t nt_max,
                                                       const float *tn, int nsplit, Cand4 *part) {
-    __shared__ __attribute__((aligned(16))) float s_t[kMT * kMP];
+    __shared__ __attribute__((aligned(16))) uint32_t s_t[kMT * kMP];
     __shared__ __attribute__((aligned(16))) float s_tn[kMT];
     const uint32_t nq = min(*nq_p, nq_max), nt = min(*nt_p, nt_max);
     const uint32_t qb = blockIdx.x * kMQ;
@@ -109,17 +142,16 @@ __global__ __launch_bounds__(256) void nn_mfma_kernel(const float *qd, const uin
     const int col = lane & 31, half = lane >> 5;
     const uint32_t q = qb + wave * 32 + col;
 
-    float b[64];
-    if (q < nq) {
-        const float4 *p = reinterpret_cast<const float4 *>(qd + (size_t)q * 128 + half * 64);
+    uint4 bh[8], bl[8];
 #pragma unroll
-        for (int m = 0; m < 16; m++) {
-            float4 v = p[m];
-            b[4 * m] = v.x; b[4 * m + 1] = v.y; b[4 * m + 2] = v.z; b[4 * m + 3] = v.w;
+    for (int s = 0; s < 8; s++) {
+        if (q < nq) {
+            bh[s] = *reinterpret_cast<const uint4 *>(qs + (size_t)q * 128 + 8 * s + 4 * half);
+            bl[s] = *reinterpret_cast<const uint4 *>(qs + (size_t)q * 128 + 64 + 8 * s + 4 * half);
+        } else {
+            bh[s] = make_uint4(0u, 0u, 0u, 0u);
+            bl[s] = make_uint4(0u, 0u, 0u, 0u);
         }
-    } else {
-#pragma unroll
-        for (int s = 0; s < 64; s++) b[s] = 0.0f;
     }
     float kd[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
     uint32_t ki[4] = {kNoIdx, kNoIdx, kNoIdx, kNoIdx};
@@ -127,16 +159,16 @@ __global__ __launch_bounds__(256) void nn_mfma_kernel(const float *qd, const uin
     const uint32_t ntiles = (nt + kMT - 1) / kMT;
     const uint32_t per = (ntiles + nsplit - 1) / nsplit;
     const uint32_t tb = blockIdx.y * per, te = min(tb + per, ntiles);
-    constexpr int kLd = kMT * 32 / 256;   // float4 loads per thread per tile
-    float4 v[kLd];
+    constexpr int kLd = kMT * 32 / 256;   // 16-byte loads per thread per tile
+    uint4 v[kLd];
     float vn = 0.0f;
 #define NN_LOAD(TILE)                                                                                 \
     do {                                                                                              \
         const uint32_t t0_ = (TILE) * kMT;                                                            \
         _Pragma("unroll") for (int j = 0; j < kLd; j++) {                                             \
             int i = tid + j * 256, r = i >> 5, c = (i & 31) * 4;                                     \
-            v[j] = t0_ + r < nt ? *reinterpret_cast<const float4 *>(td + (size_t)(t0_ + r) * 128 + c)  \
-                                : make_float4(0.f, 0.f, 0.f, 0.f);                                    \
+            v[j] = t0_ + r < nt ? *reinterpret_cast<const uint4 *>(ts + (size_t)(t0_ + r) * 128 + c)   \
+                                : make_uint4(0u, 0u, 0u, 0u);                                         \
         }                                                                                             \
         if (tid < kMT) vn = tn[t0_ + tid];                                                            \
     } while (0)
@@ -146,26 +178,29 @@ __global__ __launch_bounds__(256) void nn_mfma_kernel(const float *qd, const uin
 #pragma unroll
         for (int j = 0; j < kLd; j++) {
             int i = tid + j * 256, r = i >> 5, c = (i & 31) * 4;
-            *reinterpret_cast<float4 *>(s_t + r * kMP + c) = v[j];
+            *reinterpret_cast<uint4 *>(s_t + r * kMP + c) = v[j];
         }
         if (tid < kMT) s_tn[tid] = vn;
         __syncthreads();
         if (tile + 1 < te) NN_LOAD(tile + 1);
         floatx16 acc0 = {}, acc1 = {};
-        const float *r0 = s_t + col * kMP + half * 64;
-        const float *r1 = s_t + (32 + col) * kMP + half * 64;
+        // A operand: target row col (block 0) / 32 + col (block 1), words 8s + 4kh (hi) and
+        // 64 + 8s + 4kh (lo)
+        const uint32_t *r0 = s_t + col * kMP + 4 * half;
+        const uint32_t *r1 = s_t + (32 + col) * kMP + 4 * half;
 #pragma unroll
-        for (int m = 0; m < 16; m++) {
-            float4 a0 = *reinterpret_cast<const float4 *>(r0 + 4 * m);
-            float4 a1 = *reinterpret_cast<const float4 *>(r1 + 4 * m);
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b[4 * m], acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, b[4 * m], acc1, 0, 0, 0);
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b[4 * m + 1], acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, b[4 * m + 1], acc1, 0, 0, 0);
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, b[4 * m + 2], acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, b[4 * m + 2], acc1, 0, 0, 0);
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, b[4 * m + 3], acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, b[4 * m + 3], acc1, 0, 0, 0);
+        for (int s = 0; s < 8; s++) {
+            const bf16x8 ah0 = as_bf16x8(*reinterpret_cast<const uint4 *>(r0 + 8 * s));
+            const bf16x8 al0 = as_bf16x8(*reinterpret_cast<const uint4 *>(r0 + 64 + 8 * s));
+            const bf16x8 ah1 = as_bf16x8(*reinterpret_cast<const uint4 *>(r1 + 8 * s));
+            const bf16x8 al1 = as_bf16x8(*reinterpret_cast<const uint4 *>(r1 + 64 + 8 * s));
+            const bf16x8 qh = as_bf16x8(bh[s]), ql = as_bf16x8(bl[s]);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, qh, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, qh, acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, ql, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, ql, acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al0, qh, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al1, qh, acc1, 0, 0, 0);
         }
         // D layout: lane holds column (query) lane%32, rows 8*(r/4) + 4*(lane/32) + r%4;
         // padding rows have |t|^2 = +inf and never rank
@@ -206,12 +241,19 @@ __global__ __launch_bounds__(256) void nn_mfma_kernel(const float *qd, const uin
 // candidate has exact distance >= tau + |q|^2 - E, where tau is the smaller of the 8th
 // selected value and the 4th (last) entry of every list (a list drops only targets above
 // its own 4th); if that exceeds the exact second-best the result equals the full scan's,
-// otherwise the query is flagged for the exact scan.
+// otherwise the query is flagged and the workgroup scans all targets for it exactly
+// (pipeline.rs:1700-1740 order rules).
 __global__ __launch_bounds__(256) void nn_rescore_kernel(const Cand4 *part, int nsplit, const float *qd, const float *qn,
                                                          const uint32_t *nq_p, uint32_t nq_max, const float *td,
-                                                         const uint32_t *nt_p, const uint32_t *tmax_bits,
-                                                         uint32_t *best_idx, float *best, float *second,
-                                                         uint32_t *flag_list, uint32_t *flag_count) {
+                                                         const uint32_t *nt_p, uint32_t nt_max,
+                                                         const unsigned long long *tmax, uint32_t gen,
+                                                         uint32_t *best_idx, float *best, float *second) {
+    __shared__ uint32_t s_flag[32];
+    __shared__ uint32_t s_nflag;
+    __shared__ float sb[256], ss[256];
+    __shared__ uint32_t si[256];
+    if (threadIdx.x == 0) s_nflag = 0;
+    __syncthreads();
     const uint32_t q = blockIdx.x * 32 + (threadIdx.x >> 3);
     const int j = threadIdx.x & 7;
     const bool active = q < min(*nq_p, nq_max);
@@ -282,85 +324,63 @@ __global__ __launch_bounds__(256) void nn_rescore_kernel(const Cand4 *part, int 
     int ncand = sel_i != kNoIdx;
 #pragma unroll
     for (int o = 1; o < 8; o <<= 1) ncand += __shfl_xor(ncand, o);
-    if (!active || j != 0) return;
+    if (active && j == 0) {
     // all targets are candidates when nt <= 8
     bool ok = ncand >= (int)min(nt, 8u);
     if (ok && nt > 8) {
-        const float qq = qn[q], tt = __uint_as_float(*tmax_bits);
-        const float E = 1.0e-4f * (qq + tt + 2.0f * sqrtf(qq * tt)) + 1e-30f;
+        // the largest target norm of this call (a word of another generation: +inf, which
+        // flags the query -- always safe)
+        const unsigned long long tw = *tmax;
+        const float qq = qn[q], tt = (uint32_t)(tw >> 32) == gen ? __uint_as_float((uint32_t)tw) : INFINITY;
+        // E bounds |value + |q|^2 - exact| for every pair: twice the pass-1 dot-product error
+        // (< 7e-5 |q||t|) plus the f32 roundings of |q|^2, |t|^2 and the sequential exact
+        // distance (each < 128 * 2^-24 of its magnitude) stay below 5e-5 (|q| + |t|)^2; the
+        // factor 2e-4 leaves a 4x margin (a flagged query only costs the exact scan)
+        const float E = 2.0e-4f * (qq + tt + 2.0f * sqrtf(qq * tt)) + 1e-30f;
         ok = (fminf(kth, last4) + qq) - E > sd;
     }
     best_idx[q] = bi == kNoIdx ? 0u : bi;
     best[q] = bd;
     second[q] = sd;
-    if (!ok) flag_list[atomicAdd(flag_count, 1u)] = q;
-}
-
-// Exact scan for flagged queries over a fixed 64 x 16 grid: workgroup (a, b) takes flagged
-// queries a, a+64, ... and targets b*256 + tid (+ 4096 strides), one target per thread, then
-// reduces its 256 partial states; nn_fallback_merge_kernel combines the 16 splits in order.
-constexpr int kFbQ = 64, kFbT = 16;
-__global__ __launch_bounds__(256) void nn_fallback_partial_kernel(const float *qd, const uint32_t *nq_p,
-                                                                  uint32_t nq_max, const float *td,
-                                                                  const uint32_t *nt_p, uint32_t nt_max,
-                                                                  const uint32_t *flag_list,
-                                                                  const uint32_t *flag_count, NNPart *fb) {
-    __shared__ float sb[256], ss[256];
-    __shared__ uint32_t si[256];
-    const uint32_t nflag = min(*flag_count, min(*nq_p, nq_max));
-    const uint32_t nt = min(*nt_p, nt_max);
-    for (uint32_t f = blockIdx.x; f < nflag; f += kFbQ) {
-        const uint32_t q = flag_list[f];
-        float b = INFINITY, s = INFINITY;
-        uint32_t bi = 0;
-        for (uint32_t t = blockIdx.y * 256 + threadIdx.x; t < nt; t += kFbT * 256) {
-            float d = seq_dist128(qd + (size_t)q * 128, td + (size_t)t * 128);
-            if (d < b) { s = b; b = d; bi = t; }
-            else if (d < s) s = d;
+    if (!ok) s_flag[atomicAdd(&s_nflag, 1u)] = q;
+    }
+    __syncthreads();
+    // exact scan of every target for this workgroup's flagged queries: per thread the
+    // reference's sequential rule over its targets, then a tree reduction with the
+    // order-independent rule ((value, index) best; second = smallest of the rest)
+    const uint32_t nflag = s_nflag;
+    const uint32_t nt_scan = min(nt, nt_max);
+    for (uint32_t f = 0; f < nflag; f++) {
+        const uint32_t fq = s_flag[f];
+        float b = INFINITY, s2 = INFINITY;
+        uint32_t bix = 0;
+        for (uint32_t t = threadIdx.x; t < nt_scan; t += 256) {
+            const float d2 = seq_dist128(qd + (size_t)fq * 128, td + (size_t)t * 128);
+            if (d2 < b) { s2 = b; b = d2; bix = t; }
+            else if (d2 < s2) s2 = d2;
         }
+        sb[threadIdx.x] = b; ss[threadIdx.x] = s2; si[threadIdx.x] = bix;
         __syncthreads();
-        sb[threadIdx.x] = b; ss[threadIdx.x] = s; si[threadIdx.x] = bi;
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {   // tree reduction with the order-independent rule
+        for (int w = 128; w > 0; w >>= 1) {
             if (threadIdx.x < (unsigned)w) {
                 float B = sb[threadIdx.x], S = ss[threadIdx.x];
                 uint32_t BI = si[threadIdx.x];
-                float b2 = sb[threadIdx.x + w], s2 = ss[threadIdx.x + w];
-                uint32_t i2 = si[threadIdx.x + w];
+                const float b2 = sb[threadIdx.x + w], sv2 = ss[threadIdx.x + w];
+                const uint32_t i2 = si[threadIdx.x + w];
                 float other;
                 if (b2 < B || (b2 == B && i2 < BI)) { other = B; B = b2; BI = i2; }
                 else other = b2;
-                sb[threadIdx.x] = B; ss[threadIdx.x] = fminf(fminf(S, s2), other); si[threadIdx.x] = BI;
+                sb[threadIdx.x] = B; ss[threadIdx.x] = fminf(fminf(S, sv2), other); si[threadIdx.x] = BI;
             }
             __syncthreads();
         }
         if (threadIdx.x == 0) {
-            NNPart p;
-            p.best = sb[0]; p.second = ss[0]; p.idx = si[0]; p.pad = 0;
-            fb[(size_t)f * kFbT + blockIdx.y] = p;
+            best_idx[fq] = si[0];
+            best[fq] = sb[0];
+            second[fq] = ss[0];
         }
+        __syncthreads();
     }
-}
-
-__global__ void nn_fallback_merge_kernel(const NNPart *fb, const uint32_t *flag_list, const uint32_t *flag_count,
-                                         const uint32_t *nq_p, uint32_t nq_max, uint32_t *best_idx, float *best,
-                                         float *second) {
-    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= min(*flag_count, min(*nq_p, nq_max))) return;
-    NNPart p = fb[(size_t)f * kFbT];
-    float B = p.best, S = p.second;
-    uint32_t BI = p.idx;
-    for (int k = 1; k < kFbT; k++) {
-        NNPart o = fb[(size_t)f * kFbT + k];
-        float other;
-        if (o.best < B || (o.best == B && o.idx < BI)) { other = B; B = o.best; BI = o.idx; }
-        else other = o.best;
-        S = fminf(fminf(S, o.second), other);
-    }
-    const uint32_t q = flag_list[f];
-    best_idx[q] = BI;
-    best[q] = B;
-    second[q] = S;
 }
 
 int nn_mfma_splits(size_t nq_max, size_t nt_max) {
@@ -376,46 +396,41 @@ static size_t tn_rows(size_t nt_max) { return (nt_max + kMT - 1) / kMT * kMT; }
 
 size_t nn_mfma_scratch_bytes(size_t nq_max, size_t nt_max) {
     size_t ns = (size_t)nn_mfma_splits(nq_max, nt_max);
-    return ns * nq_max * 2 * sizeof(Cand4) + nq_max * kFbT * sizeof(NNPart) + nq_max * (sizeof(float) + sizeof(uint32_t)) +
-           tn_rows(nt_max) * sizeof(float) + 512;
+    return ns * nq_max * 2 * sizeof(Cand4) + nq_max * sizeof(float) + tn_rows(nt_max) * sizeof(float) +
+           (nq_max + tn_rows(nt_max)) * 128 * sizeof(uint32_t) + 512;
 }
 
-// Scratch layout: [part: ns*nq_max*2 Cand4][fb: nq_max*16 NNPart][qn: nq_max][flag_list: nq_max]
-// [tn: padded nt_max][counters: tmax, flag_count]
+// Scratch layout: [part: ns*nq_max*2 Cand4][qn: nq_max][tn: padded nt_max]
+// [q split: nq_max*128 words][t split: padded nt_max*128 words][tmax: u64 (8-aligned)].
+// Three launches: prep, MFMA distances, rescore (+ exact scans of flagged queries).
 hipError_t launch_nn_mfma(const float *qd, const uint32_t *nq, size_t nq_max, const float *td, const uint32_t *nt,
-                          size_t nt_max, void *scratch, uint32_t *best_idx, float *best, float *second,
+                          size_t nt_max, void *scratch, uint32_t *best_idx, float *best, float *second, uint32_t gen,
                           hipStream_t st) {
     if (nq_max == 0) return hipSuccess;
     const int ns = nn_mfma_splits(nq_max, nt_max);
     char *p = static_cast<char *>(scratch);
     Cand4 *part = reinterpret_cast<Cand4 *>(p);
     p += (size_t)ns * nq_max * 2 * sizeof(Cand4);
-    NNPart *fb = reinterpret_cast<NNPart *>(p);
-    p += nq_max * kFbT * sizeof(NNPart);
     float *qn = reinterpret_cast<float *>(p);
     p += nq_max * sizeof(float);
-    uint32_t *flag_list = reinterpret_cast<uint32_t *>(p);
-    p += nq_max * sizeof(uint32_t);
     float *tn = reinterpret_cast<float *>(p);
     p += tn_rows(nt_max) * sizeof(float);
-    uint32_t *tmax = reinterpret_cast<uint32_t *>(p);
-    uint32_t *flag_count = tmax + 1;
-    hipError_t e = launch_fill_u32(tmax, 0u, 2, st);
-    if (e != hipSuccess) return e;
+    uint32_t *qsplit = reinterpret_cast<uint32_t *>(p);
+    p += nq_max * 128 * sizeof(uint32_t);
+    uint32_t *tsplit = reinterpret_cast<uint32_t *>(p);
+    p += tn_rows(nt_max) * 128 * sizeof(uint32_t);
+    p += (8 - reinterpret_cast<uintptr_t>(p) % 8) % 8;
+    unsigned long long *tmax = reinterpret_cast<unsigned long long *>(p);
     const uint32_t trows = (uint32_t)tn_rows(nt_max);
-    row_norms_kernel<<<(trows + 63) / 64, 256, 0, st>>>(td, nt, (uint32_t)nt_max, trows, tn, tmax);
-    row_norms_kernel<<<(unsigned)((nq_max + 63) / 64), 256, 0, st>>>(qd, nq, (uint32_t)nq_max, (uint32_t)nq_max, qn,
-                                                                      nullptr);
+    const uint32_t tblocks = (trows + 63) / 64, qblocks = (uint32_t)((nq_max + 63) / 64);
+    const PrepSide ts{td, nt, (uint32_t)nt_max, trows, tn, tsplit};
+    const PrepSide qs{qd, nq, (uint32_t)nq_max, (uint32_t)nq_max, qn, qsplit};
+    nn_prep_kernel<<<tblocks + qblocks, 256, 0, st>>>(ts, qs, tblocks, tmax, gen);
     dim3 grid((unsigned)((nq_max + kMQ - 1) / kMQ), (unsigned)ns);
-    nn_mfma_kernel<<<grid, 256, 0, st>>>(qd, nq, (uint32_t)nq_max, td, nt, (uint32_t)nt_max, tn, ns, part);
+    nn_mfma_kernel<<<grid, 256, 0, st>>>(qsplit, nq, (uint32_t)nq_max, tsplit, nt, (uint32_t)nt_max, tn, ns, part);
     nn_rescore_kernel<<<(unsigned)((nq_max + 31) / 32), 256, 0, st>>>(part, ns, qd, qn, nq, (uint32_t)nq_max, td, nt,
-                                                                      tmax, best_idx, best, second, flag_list,
-                                                                      flag_count);
-    nn_fallback_partial_kernel<<<dim3(kFbQ, kFbT), 256, 0, st>>>(qd, nq, (uint32_t)nq_max, td, nt, (uint32_t)nt_max,
-                                                                 flag_list, flag_count, fb);
-    nn_fallback_merge_kernel<<<(unsigned)((nq_max + 255) / 256), 256, 0, st>>>(fb, flag_list, flag_count, nq,
-                                                                               (uint32_t)nq_max, best_idx, best,
-                                                                               second);
+                                                                      (uint32_t)nt_max, tmax, gen, best_idx, best,
+                                                                      second);
     return hipGetLastError();
 }
 

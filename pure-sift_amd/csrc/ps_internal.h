@@ -120,8 +120,9 @@ hipError_t launch_match_finish(const ps_feature *qf, const ps_feature *tf, const
 // match_mfma.hip: exact NN/2NN through an FP32 MFMA distance GEMM + exact rescoring
 int nn_mfma_splits(size_t nq_max, size_t nt_max);
 size_t nn_mfma_scratch_bytes(size_t nq_max, size_t nt_max);
+// gen: a per-call number, distinct from the previous call's on the same scratch
 hipError_t launch_nn_mfma(const float *qd, const uint32_t *nq, size_t nq_max, const float *td, const uint32_t *nt,
-                          size_t nt_max, void *scratch, uint32_t *best_idx, float *best, float *second,
+                          size_t nt_max, void *scratch, uint32_t *best_idx, float *best, float *second, uint32_t gen,
                           hipStream_t st);
 
 // Squared L2 distance of two 128-D f32 rows summed sequentially in element order
