@@ -119,6 +119,20 @@ ps_status ps_detect_keypoints(const float *img, size_t h, size_t w, size_t row_s
                               const ps_sift_config *cfg, ps_keypoint *out, size_t cap,
                               size_t *out_count);
 
+/* pipeline.rs:1581-1620 compute_sift_descriptors_with_kind (the uncapped CLI / HTTP extract path
+ * of cli.rs:174-212 pairs it with ps_detect_keypoints): orientations and 128-D descriptors for
+ * the given keypoints. The pyramid is rebuilt with the parameters the reference infers from the
+ * keypoints -- scales = max(max layer, 3); the first octave upsampled when the first keypoint
+ * with layer > 0 has scale / (1.6 * 2^octave * k^layer) < 0.707; max octave + 1 octaves, a
+ * layer-0 keypoint counting as octave floor(log2(max(scale / 1.6, 1)))) -- and each keypoint's
+ * level is resolve_octave_and_layer (pipeline.rs:1206-1227). Features follow keypoint order, each
+ * keypoint's orientations in peak order; kind is PS_DESCRIPTOR_SIFT or PS_DESCRIPTOR_ROOTSIFT.
+ * Inferred pyramids beyond 8 octaves or 10 scales return PS_E_CONFIG. If more than `cap`
+ * features result, returns PS_E_CAPACITY and sets *out_count to the needed count. */
+ps_status ps_compute_descriptors(const float *img, size_t h, size_t w, size_t row_stride,
+                                 const ps_keypoint *kps, size_t n, uint8_t kind, ps_feature *out_feat,
+                                 float *out_desc, size_t cap, size_t *out_count);
+
 /* pipeline.rs:167-169 Matcher::match_ -> match_descriptors_with_options (pipeline.rs:1744-1841).
  * Descriptors are [n][128] f32. out needs capacity >= nq. */
 ps_status ps_match(const ps_feature *q, const float *q_desc, size_t nq,

@@ -694,6 +694,35 @@ size_t psr_descriptors_from_pyramid(const psr_pyramid *p, const psr_keypoint *kp
     return total;
 }
 
+/* pipeline.rs:1581-1620 compute_sift_descriptors_with_kind: pyramid parameters inferred from
+ * the keypoints, then the features of every keypoint (no truncation) */
+size_t psr_compute_descriptors(const float *img, int h, int w, const psr_keypoint *kps, size_t n, int kind,
+                               psr_feature **feats_out, float **descs_out) {
+    *feats_out = NULL; *descs_out = NULL;
+    if (h < 3 || w < 3 || n == 0) return 0;
+    uint32_t max_layer = 0;
+    for (size_t i = 0; i < n; i++) if (kps[i].layer > max_layer) max_layer = kps[i].layer;
+    size_t scales = max_layer > 3 ? max_layer : 3;
+    float k = powf(2.0f, 1.0f / (float)scales);
+    int upsample = 0;
+    for (size_t i = 0; i < n; i++)
+        if (kps[i].layer > 0) {   /* the first keypoint with a layer decides */
+            float expected = BASE_SIGMA * (float)((size_t)1 << kps[i].octave) * powi_rs(k, (int)kps[i].layer);
+            upsample = (kps[i].scale / expected) < 0.707f;
+            break;
+        }
+    size_t max_oct = 0;
+    for (size_t i = 0; i < n; i++) {
+        size_t o = kps[i].layer > 0 ? (size_t)kps[i].octave : f2usize(floorf(log2f(fmaxf(kps[i].scale / BASE_SIGMA, 1.0f))));
+        if (o > max_oct) max_oct = o;
+    }
+    psr_pyramid p;
+    psr_build_pyramid(img, h, w, (int)(max_oct + 1), (int)scales, upsample, &p);
+    size_t nf = psr_descriptors_from_pyramid(&p, kps, n, kind, feats_out, descs_out);
+    psr_free_pyramid(&p);
+    return nf;
+}
+
 /* stable merge sort of keypoints by |response| descending (pipeline.rs:1643-1651) */
 static int resp_before(const psr_keypoint *a, const psr_keypoint *b) {
     /* true if a must come strictly before b */

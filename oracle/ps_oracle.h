@@ -82,6 +82,10 @@ void psr_rootsift(const float *in, float *out);
 /* pipeline.rs:1482-1577: features for an ordered keypoint list (no 4096 truncation) */
 size_t psr_descriptors_from_pyramid(const psr_pyramid *p, const psr_keypoint *kps, size_t n,
                                     int kind, psr_feature **feats, float **descs);
+/* pipeline.rs:1581-1620 (compute_sift_descriptors_with_kind): pyramid parameters inferred from
+ * the keypoints, features for every keypoint in order (no truncation) */
+size_t psr_compute_descriptors(const float *img, int h, int w, const psr_keypoint *kps, size_t n, int kind,
+                               psr_feature **feats, float **descs);
 /* pipeline.rs:1633-1658 (extract_sift_features_with_kind); cap + truncate to 4096 */
 size_t psr_extract_features(const float *img, int h, int w, int octaves, int scales,
                             float threshold, int upsample, int kind,

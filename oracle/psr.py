@@ -95,6 +95,9 @@ def lib():
         L.psr_descriptors_from_pyramid.argtypes = [C.POINTER(Pyramid), C.POINTER(Keypoint), C.c_size_t, C.c_int,
                                                    C.POINTER(C.POINTER(Feature)), C.POINTER(fp)]
         L.psr_descriptors_from_pyramid.restype = C.c_size_t
+        L.psr_compute_descriptors.argtypes = [fp, C.c_int, C.c_int, C.POINTER(Keypoint), C.c_size_t, C.c_int,
+                                              C.POINTER(C.POINTER(Feature)), C.POINTER(C.POINTER(C.c_float))]
+        L.psr_compute_descriptors.restype = C.c_size_t
         L.psr_extract_features.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int,
                                            C.POINTER(C.POINTER(Feature)), C.POINTER(fp)]
         L.psr_extract_features.restype = C.c_size_t
@@ -282,6 +285,18 @@ def descriptors_for_keypoints(img, kps, octaves, scales, upsample, kind=0):
     n = lib().psr_descriptors_from_pyramid(C.byref(p), kps.ctypes.data_as(C.POINTER(Keypoint)), len(kps), int(kind),
                                            C.byref(fptr), C.byref(dptr))
     lib().psr_free_pyramid(C.byref(p))
+    return _take_feats(fptr, dptr, n)
+
+
+def compute_descriptors(img, kps, kind=0):
+    """compute_sift_descriptors_with_kind (pipeline.rs:1581-1620): pyramid parameters inferred
+    from the keypoints, features for every keypoint in order."""
+    img = _f32(img)
+    kps = np.ascontiguousarray(kps, KP_DTYPE)
+    fptr = C.POINTER(Feature)()
+    dptr = C.POINTER(C.c_float)()
+    n = lib().psr_compute_descriptors(_fp(img), img.shape[0], img.shape[1], kps.ctypes.data_as(C.POINTER(Keypoint)),
+                                      len(kps), int(kind), C.byref(fptr), C.byref(dptr))
     return _take_feats(fptr, dptr, n)
 
 

@@ -299,9 +299,7 @@ __global__ __launch_bounds__(kDescThreads) void descriptor_kernel(PyrView pv, co
         const int xa = max(x_min, (int)floorf(l.x + bx0) - 1), xb = min(x_max, (int)ceilf(l.x + bx1) + 1);
         const int ya = max(y_min, (int)floorf(l.y + by0) - 1), yb = min(y_max, (int)ceilf(l.y + by1) + 1);
         const int ww = xb - xa + 1, hh = yb - ya + 1;
-        // a cell box spans about 1.42 hist_width + 5 pixels; a box of 256 or more (a keypoint
-        // scale beyond any the detector produces) is skipped rather than mis-packed
-        const int total = (ww > 0 && hh > 0 && ww <= 256 && hh <= 256) ? ww * hh : 0;
+        const int total = (ww > 0 && hh > 0) ? ww * hh : 0;
         const float inv_ww = 1.0f / (float)max(ww, 1);
         // window test and bin coordinates of box pixel (qy, qx) (pipeline.rs:1370-1393)
         auto bins = [&](int qy, int qx, float &r_rot, float &c_rot, float &r_bin, float &c_bin) {
@@ -311,62 +309,78 @@ __global__ __launch_bounds__(kDescThreads) void descriptor_kernel(PyrView pv, co
             r_bin = r_rot + 4.0f * 0.5f - 0.5f;
             c_bin = c_rot + 4.0f * 0.5f - 0.5f;
         };
-        // Two phases per chunk of kListCap box pixels so the expensive part runs on dense lanes:
-        // (A) the lane tests the chunk's pixels and records those of its cell in a lane-private
-        // LDS list; (B) it processes the list in batches whose gradient loads are issued
-        // together.
-        // entry i at list[i * kDescThreads], packed (qy << 8 | qx); the box sides are below
-        // 256 pixels (checked above)
-        uint16_t *list = s_list + lane;
-        for (int chunk = sub; chunk < total; chunk += kListCap * kLanesPerCell) {
-            const int chunk_end = min(chunk + kListCap * kLanesPerCell, total);
-            int n = 0;
-            for (int p = chunk; p < chunk_end; p += kLanesPerCell) {
-                int qy, qx;
-                divmod_small(p, ww, inv_ww, qy, qx);
+        auto in_cell = [&](float r_bin, float c_bin) {
+            return !(r_bin <= -1.0f || r_bin >= 4.0f || c_bin <= -1.0f || c_bin >= 4.0f) &&
+                   (int)floorf(r_bin) == r0 && (int)floorf(c_bin) == c0;
+        };
+        // gradient sample of box pixel (qy, qx): weight, orientation and trilinear accumulation
+        // into the lane's 4 corners x 8 orientations (pipeline.rs:1186-1189, :1394-1440)
+        auto add_pixel = [&](float gxa, float gxb, float gya, float gyb, int qy, int qx) {
+            const float gdx = gxa - gxb, gdy = gyb - gya;
+            const float m = sqrtf(gdx * gdx + gdy * gdy);
+            if (m <= 1e-7f) return;
+            const float a = rem_euclid(grad_atan2(gdy, gdx), kTwoPi);
+            float r_rot, c_rot, r_bin, c_bin;
+            bins(qy, qx, r_rot, c_rot, r_bin, c_bin);
+            const float wm = m * expf((r_rot * r_rot + c_rot * c_rot) * exp_scale);
+            const float o_bin = rem_euclid(a - theta, kTwoPi) * bin_per_rad;
+            const int o0 = (int)floorf(o_bin);
+            const float dr = r_bin - (float)r0, dc = c_bin - (float)c0, dof = o_bin - (float)o0;
+            const int ob0 = o0 & 7, ob1 = (o0 + 1) & 7;
+#pragma unroll
+            for (int ir = 0; ir <= 1; ir++) {
+                const float wr = ir == 0 ? 1.0f - dr : dr;
+#pragma unroll
+                for (int ic = 0; ic <= 1; ic++) {
+                    const float wc = ic == 0 ? 1.0f - dc : dc;
+                    const float w0 = wm * wr * wc;
+                    float *acc = priv + (ir * 2 + ic) * 8;
+                    acc[ob0] += w0 * (1.0f - dof);
+                    acc[ob1] += w0 * dof;
+                }
+            }
+        };
+        if (ww <= 256 && hh <= 256) {
+            // Two phases per chunk of kListCap box pixels so the expensive part runs on dense
+            // lanes: (A) the lane tests the chunk's pixels and records those of its cell in a
+            // lane-private LDS list (entry i at list[i * kDescThreads], packed qy << 8 | qx);
+            // (B) it processes the list in batches whose gradient loads are issued together.
+            uint16_t *list = s_list + lane;
+            for (int chunk = sub; chunk < total; chunk += kListCap * kLanesPerCell) {
+                const int chunk_end = min(chunk + kListCap * kLanesPerCell, total);
+                int n = 0;
+                for (int p = chunk; p < chunk_end; p += kLanesPerCell) {
+                    int qy, qx;
+                    divmod_small(p, ww, inv_ww, qy, qx);
+                    float r_rot, c_rot, r_bin, c_bin;
+                    bins(qy, qx, r_rot, c_rot, r_bin, c_bin);
+                    if (in_cell(r_bin, c_bin)) list[(n++) * kDescThreads] = (uint16_t)((qy << 8) | qx);
+                }
+                for (int i0 = 0; i0 < n; i0 += kBatch) {
+                    float gxa[kBatch], gxb[kBatch], gya[kBatch], gyb[kBatch];
+                    int pq[kBatch];
+#pragma unroll
+                    for (int u = 0; u < kBatch; u++) {
+                        const int i = min(i0 + u, n - 1);
+                        pq[u] = list[i * kDescThreads];
+                        const float *row = l.g + (size_t)(ya + (pq[u] >> 8)) * l.W + (xa + (pq[u] & 255));
+                        gxa[u] = row[1]; gxb[u] = row[-1]; gyb[u] = row[l.W]; gya[u] = row[-l.W];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kBatch; u++)
+                        if (i0 + u < n) add_pixel(gxa[u], gxb[u], gya[u], gyb[u], pq[u] >> 8, pq[u] & 255);
+                }
+            }
+        } else {
+            // boxes of 256+ pixels a side (keypoint scales far beyond what detection produces):
+            // the same pixels, tested and accumulated one at a time
+            for (int p = sub; p < total; p += kLanesPerCell) {
+                const int qy = p / ww, qx = p - qy * ww;
                 float r_rot, c_rot, r_bin, c_bin;
                 bins(qy, qx, r_rot, c_rot, r_bin, c_bin);
-                const bool ok = !(r_bin <= -1.0f || r_bin >= 4.0f || c_bin <= -1.0f || c_bin >= 4.0f) &&
-                                (int)floorf(r_bin) == r0 && (int)floorf(c_bin) == c0;
-                if (ok) list[(n++) * kDescThreads] = (uint16_t)((qy << 8) | qx);
-            }
-            for (int i0 = 0; i0 < n; i0 += kBatch) {
-                float gxa[kBatch], gxb[kBatch], gya[kBatch], gyb[kBatch];
-                int pq[kBatch];
-#pragma unroll
-                for (int u = 0; u < kBatch; u++) {
-                    const int i = min(i0 + u, n - 1);
-                    pq[u] = list[i * kDescThreads];
-                    const float *row = l.g + (size_t)(ya + (pq[u] >> 8)) * l.W + (xa + (pq[u] & 255));
-                    gxa[u] = row[1]; gxb[u] = row[-1]; gyb[u] = row[l.W]; gya[u] = row[-l.W];
-                }
-#pragma unroll
-                for (int u = 0; u < kBatch; u++) {
-                    if (i0 + u >= n) continue;
-                    float gdx = gxa[u] - gxb[u], gdy = gyb[u] - gya[u];   // pipeline.rs:1186-1189
-                    float m = sqrtf(gdx * gdx + gdy * gdy);
-                    if (m <= 1e-7f) continue;
-                    float a = rem_euclid(grad_atan2(gdy, gdx), kTwoPi);
-                    float r_rot, c_rot, r_bin, c_bin;
-                    bins(pq[u] >> 8, pq[u] & 255, r_rot, c_rot, r_bin, c_bin);
-                    float wm = m * expf((r_rot * r_rot + c_rot * c_rot) * exp_scale);
-                    float o_bin = rem_euclid(a - theta, kTwoPi) * bin_per_rad;
-                    int o0 = (int)floorf(o_bin);
-                    float dr = r_bin - (float)r0, dc = c_bin - (float)c0, dof = o_bin - (float)o0;
-                    const int ob0 = o0 & 7, ob1 = (o0 + 1) & 7;
-#pragma unroll
-                    for (int ir = 0; ir <= 1; ir++) {
-                        float wr = ir == 0 ? 1.0f - dr : dr;
-#pragma unroll
-                        for (int ic = 0; ic <= 1; ic++) {
-                            float wc = ic == 0 ? 1.0f - dc : dc;
-                            float w0 = wm * wr * wc;
-                            float *acc = priv + (ir * 2 + ic) * 8;
-                            acc[ob0] += w0 * (1.0f - dof);
-                            acc[ob1] += w0 * dof;
-                        }
-                    }
-                }
+                if (!in_cell(r_bin, c_bin)) continue;
+                const float *row = l.g + (size_t)(ya + qy) * l.W + (xa + qx);
+                add_pixel(row[1], row[-1], row[-l.W], row[l.W], qy, qx);
             }
         }
     }

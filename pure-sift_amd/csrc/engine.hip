@@ -639,6 +639,120 @@ ps_status ps_detect_keypoints(const float *img, size_t h, size_t w, size_t row_s
     return PS_OK;
 }
 
+// Rust `f32 as usize`: saturating, NaN -> 0
+static size_t f32_to_usize(float v) {
+    if (!(v > 0.0f)) return 0;
+    if (v >= 18446744073709551615.0f) return SIZE_MAX;
+    return (size_t)v;
+}
+
+ps_status ps_compute_descriptors(const float *img, size_t h, size_t w, size_t row_stride, const ps_keypoint *kps,
+                                 size_t n, uint8_t kind, ps_feature *out_feat, float *out_desc, size_t cap,
+                                 size_t *out_count) {
+    PS_TRY(check_fail_injection());
+    if (!img || !out_count || (n && !kps) || (cap && (!out_feat || !out_desc)))
+        return set_err(PS_E_INVALID_ARG, "null argument");
+    if (row_stride < w) return set_err(PS_E_INVALID_ARG, "row_stride < width");
+    if (kind != PS_DESCRIPTOR_SIFT && kind != PS_DESCRIPTOR_ROOTSIFT)
+        return set_err(PS_E_INVALID_ARG, "descriptor kind must be PS_DESCRIPTOR_SIFT or PS_DESCRIPTOR_ROOTSIFT");
+    *out_count = 0;
+    if (h < 3 || w < 3 || n == 0) return PS_OK;   // pipeline.rs:1587-1589
+    // pyramid parameters (pipeline.rs:1591-1617), host f32 with the reference's libm calls
+    uint32_t max_layer = 0;
+    for (size_t i = 0; i < n; i++) max_layer = std::max(max_layer, kps[i].layer);
+    const uint32_t scales = std::max<uint32_t>(max_layer, 3);
+    if (scales > (uint32_t)ps::kMaxScales) return set_err(PS_E_CONFIG, "keypoint layers need more than 10 scales");
+    const float k = std::pow(2.0f, 1.0f / (float)scales);
+    bool upsample = false;
+    size_t max_oct = 0;
+    bool have_layer = false;
+    for (size_t i = 0; i < n; i++) {
+        const ps_keypoint &kp = kps[i];
+        size_t o;
+        if (kp.layer > 0) {
+            o = kp.octave;
+            if (!have_layer) {
+                if (kp.octave >= 31) return set_err(PS_E_CONFIG, "keypoint octave out of range");
+                const float expected = kBaseSigma * (float)(1u << kp.octave) * powi_rs(k, (int)kp.layer);
+                upsample = (kp.scale / expected) < 0.707f;
+                have_layer = true;
+            }
+        } else {
+            o = f32_to_usize(std::floor(std::log2(std::max(kp.scale / kBaseSigma, 1.0f))));
+        }
+        max_oct = std::max(max_oct, o);
+    }
+    if (max_oct + 1 > (size_t)ps::kMaxOct) return set_err(PS_E_CONFIG, "keypoints need more than 8 octaves");
+    ps_sift_config cfg{};
+    cfg.octaves = (uint32_t)(max_oct + 1);
+    cfg.scales = scales;
+    cfg.contrast_threshold = 0.015f;   // unused: no extremum search on this path
+    cfg.upsample_first_octave = upsample ? 1 : 0;
+    cfg.descriptor_kind = kind;
+    Borrow b;
+    PS_TRY(b.st);
+    ps_engine *e = b.e;
+    PS_HIP(hipSetDevice(e->device));
+    PS_TRY(ensure(e->img, h * w * sizeof(float)));
+    PS_HIP(hipMemcpy2DAsync(e->img.p, w * sizeof(float), img, row_stride * sizeof(float), w * sizeof(float), h,
+                            hipMemcpyHostToDevice, e->stream));
+    PS_TRY(ensure(e->cnt, 64));
+    PS_TRY(enqueue_scale_space(e, e->img.p, false, h, w, w, cfg, false));
+    const ps::PyrPlan &p = e->plan;
+    if (p.n_oct == 0) return PS_OK;
+    // per-keypoint level, resolve_octave_and_layer (pipeline.rs:1206-1227)
+    const int g_len = p.S + 3;
+    const int searchable = std::max(g_len - 3, 1);
+    std::vector<ps::Cand> c(n);
+    for (size_t i = 0; i < n; i++) {
+        const ps_keypoint &kp = kps[i];
+        size_t oct, layer;
+        if (kp.layer > 0) {
+            oct = std::min<size_t>(kp.octave, (size_t)p.n_oct - 1);
+            layer = std::min<size_t>(kp.layer, (size_t)g_len - 1);
+        } else {
+            oct = std::min(f32_to_usize(std::floor(std::log2(std::max(kp.scale / kBaseSigma, 1.0f)))),
+                           (size_t)p.n_oct - 1);
+            const float local_sigma = std::max(kp.scale / (kBaseSigma * (float)(1u << oct)), 1.0f);
+            const size_t v = f32_to_usize(std::round(std::log2(local_sigma) * (float)searchable));
+            layer = std::min(std::max<size_t>(v, 1), (size_t)std::max(g_len - 2, 1));
+        }
+        c[i] = ps::Cand{kp.x, kp.y, kp.scale, kp.response, (uint32_t)oct, (uint32_t)layer, 0};
+    }
+    const size_t fmax = 4 * n;
+    if (fmax > UINT32_MAX / 2) return set_err(PS_E_INVALID_ARG, "too many keypoints");
+    PS_TRY(ensure(e->sorted, n * sizeof(ps::Cand)));
+    PS_TRY(ensure(e->ori, n * 4 * sizeof(float)));
+    PS_TRY(ensure(e->nori, n * sizeof(uint32_t)));
+    PS_TRY(ensure(e->feat_kp, fmax * sizeof(uint32_t)));
+    PS_TRY(ensure(e->feat_j, fmax * sizeof(uint32_t)));
+    PS_TRY(ensure(e->feat, fmax * sizeof(ps_feature)));
+    PS_TRY(ensure(e->desc, fmax * 128 * sizeof(float)));
+    PS_TRY(ensure(e->mcnt, 64));
+    uint32_t *cnt = as<uint32_t>(e->cnt), *dcount = as<uint32_t>(e->mcnt);
+    PS_HIP(hipMemcpyAsync(e->sorted.p, c.data(), n * sizeof(ps::Cand), hipMemcpyHostToDevice, e->stream));
+    PS_HIP(ps::launch_fill_u32(cnt + 2, (uint32_t)n, 1, e->stream));
+    ps::PyrView pv = pyr_view(e);
+    PS_HIP(ps::launch_orientations(pv, as<ps::Cand>(e->sorted), cnt + 2, (uint32_t)n, as<float>(e->ori),
+                                   as<uint32_t>(e->nori), e->stream));
+    PS_HIP(ps::launch_feature_scan(cnt + 2, as<uint32_t>(e->nori), (uint32_t)n, (uint32_t)fmax,
+                                   as<uint32_t>(e->feat_kp), as<uint32_t>(e->feat_j), dcount, e->stream));
+    PS_HIP(ps::launch_descriptors(pv, as<ps::Cand>(e->sorted), as<float>(e->ori), as<uint32_t>(e->feat_kp),
+                                  as<uint32_t>(e->feat_j), dcount, (uint32_t)fmax, kind, as<ps_feature>(e->feat),
+                                  as<float>(e->desc), e->stream));
+    PS_HIP(hipMemcpyAsync(&e->host_counts[0], dcount, sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream));
+    PS_HIP(hipStreamSynchronize(e->stream));
+    const size_t nf = e->host_counts[0];
+    *out_count = nf;
+    if (nf > cap) return set_err(PS_E_CAPACITY, "output capacity smaller than the feature count");
+    if (nf) {
+        PS_HIP(hipMemcpyAsync(out_feat, e->feat.p, nf * sizeof(ps_feature), hipMemcpyDeviceToHost, e->stream));
+        PS_HIP(hipMemcpyAsync(out_desc, e->desc.p, nf * 128 * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+        PS_HIP(hipStreamSynchronize(e->stream));
+    }
+    return PS_OK;
+}
+
 ps_status ps_build_pyramid(const float *img, size_t h, size_t w, size_t row_stride, const ps_sift_config *cfg,
                            float *out, size_t out_cap, size_t *out_len, int32_t *dims, int32_t *n_octaves) {
     PS_TRY(check_fail_injection());

@@ -131,6 +131,28 @@ public:
                                                 std::size_t stride) const {
         return run(img, h, w, stride, false);
     }
+
+    // pipeline.rs:1581-1620 compute_sift_descriptors_with_kind (this config's descriptor kind):
+    // uncapped features for the given keypoints, pyramid parameters inferred from them
+    std::vector<SiftFeature> compute_descriptors(const float *img, std::size_t h, std::size_t w, std::size_t stride,
+                                                 const std::vector<Keypoint> &kps) const {
+        std::vector<ps_keypoint> k(kps.size());
+        for (std::size_t i = 0; i < kps.size(); i++)
+            k[i] = {kps[i].x, kps[i].y, kps[i].scale, kps[i].response, (uint32_t)kps[i].octave, (uint32_t)kps[i].layer};
+        const std::size_t cap = 4 * kps.size();
+        std::vector<ps_feature> hdr(cap ? cap : 1);
+        std::vector<float> desc(hdr.size() * SIFT_DESCRIPTOR_LEN);
+        std::size_t n = 0;
+        check(ps_compute_descriptors(img, h, w, stride, k.data(), k.size(), (uint8_t)config_.descriptor_kind_,
+                                     hdr.data(), desc.data(), cap, &n));
+        std::vector<SiftFeature> out(n);
+        for (std::size_t i = 0; i < n; i++) {
+            out[i] = {hdr[i].x, hdr[i].y, hdr[i].scale, hdr[i].response, hdr[i].orientation,
+                      std::vector<float>(desc.begin() + i * SIFT_DESCRIPTOR_LEN,
+                                         desc.begin() + (i + 1) * SIFT_DESCRIPTOR_LEN)};
+        }
+        return out;
+    }
     std::vector<SiftFeature> detect_and_compute_u8(const uint8_t *img, std::size_t h, std::size_t w,
                                                    std::size_t stride) const {
         return run(img, h, w, stride, true);

@@ -118,6 +118,7 @@ def lib() -> C.CDLL:
         L.ps_detect_and_compute.argtypes = [vp, sz, sz, sz, C.POINTER(_SiftConfigC), vp, vp, sz, C.POINTER(sz)]
         L.ps_detect_and_compute_u8.argtypes = [vp, sz, sz, sz, C.POINTER(_SiftConfigC), vp, vp, sz, C.POINTER(sz)]
         L.ps_detect_keypoints.argtypes = [vp, sz, sz, sz, C.POINTER(_SiftConfigC), vp, sz, C.POINTER(sz)]
+        L.ps_compute_descriptors.argtypes = [vp, sz, sz, sz, vp, sz, C.c_uint8, vp, vp, sz, C.POINTER(sz)]
         L.ps_match.argtypes = [vp, vp, sz, vp, vp, sz, C.POINTER(_MatchOptionsC), vp, sz, C.POINTER(sz)]
         L.ps_build_pyramid.argtypes = [vp, sz, sz, sz, C.POINTER(_SiftConfigC), vp, sz, C.POINTER(sz),
                                        C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
@@ -314,6 +315,25 @@ class Sift:
         fn = lib().ps_detect_and_compute_u8 if is_u8 else lib().ps_detect_and_compute
         check(fn(a.ctypes.data, a.shape[0], a.shape[1], stride, C.byref(self._c), hdr.ctypes.data, desc.ctypes.data,
                  MAX_FEATURES_PER_IMAGE, C.byref(n)))
+        return Features(hdr[:n.value].copy(), desc[:n.value].copy())
+
+    def compute_descriptors(self, img: np.ndarray, keypoints: np.ndarray) -> Features:
+        """compute_sift_descriptors_with_kind (pipeline.rs:1581-1620) with this config's descriptor
+        kind: orientations + descriptors for the given keypoints (KP_DTYPE records), uncapped; the
+        pyramid parameters are inferred from the keypoints as the reference does. Paired with
+        detect_keypoints it is the reference's CLI / HTTP extract path (cli.rs:174-212)."""
+        a, stride, is_u8 = _img_arg(img)
+        if is_u8:
+            a = a.astype(np.float32) * np.float32(1.0 / 255.0)
+            stride = a.strides[0] // 4
+        kps = np.ascontiguousarray(keypoints, KP_DTYPE)
+        cap = max(4 * len(kps), 1)
+        hdr = np.zeros(cap, FEAT_DTYPE)
+        desc = np.zeros((cap, 128), np.float32)
+        n = C.c_size_t(0)
+        check(lib().ps_compute_descriptors(a.ctypes.data, a.shape[0], a.shape[1], stride, kps.ctypes.data, len(kps),
+                                           int(self._config.descriptor_kind_), hdr.ctypes.data, desc.ctypes.data,
+                                           cap, C.byref(n)))
         return Features(hdr[:n.value].copy(), desc[:n.value].copy())
 
     def build_pyramid(self, img: np.ndarray):

@@ -1,5 +1,6 @@
 // Exercises the C++ host mirror (pure-sift_amd/pure_sift.hpp) end to end: validation
-// errors, detect_keypoints, detect_and_compute (f32 and u8), match_ (plain and mutual).
+// errors, detect_keypoints, detect_and_compute (f32 and u8), compute_descriptors, match_ (plain
+// and mutual).
 // Reads a raw u8 image (h, w from argv) and prints a one-line summary the GPU test checks.
 #include <cstdio>
 #include <fstream>
@@ -42,9 +43,12 @@ int main(int argc, char **argv) {
     if (fq.size() != fq32.size()) bad++;
     for (std::size_t i = 0; i < fq.size() && i < fq32.size(); i++)
         if (fq[i].descriptor != fq32[i].descriptor || fq[i].orientation != fq32[i].orientation) bad++;
+    // uncapped CLI extract path: detect_keypoints + compute_descriptors on the same keypoints
+    auto fk = sift.compute_descriptors(qf.data(), h, w, w, kps);
+    if (fk.size() < kps.size()) bad++;
     auto m = pure_sift::Matcher::create({0.8f, false}).match_(fq, ft);
     auto mm = pure_sift::Matcher::create({0.8f, true}).match_(fq, ft);
-    std::printf("keypoints=%zu features=%zu,%zu matches=%zu mutual=%zu bad=%d\n", kps.size(), fq.size(), ft.size(),
-                m.size(), mm.size(), bad);
+    std::printf("keypoints=%zu features=%zu,%zu described=%zu matches=%zu mutual=%zu bad=%d\n", kps.size(), fq.size(),
+                ft.size(), fk.size(), m.size(), mm.size(), bad);
     return bad ? 1 : 0;
 }

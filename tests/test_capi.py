@@ -125,3 +125,20 @@ def test_precision_option_validation():
         ps.Matcher.new(ps.MatchOptions(0.8, False, 3))
     for p in (ps.PRECISION_FP32, ps.PRECISION_BF16, ps.PRECISION_FP8):
         ps.Matcher.new(ps.MatchOptions(0.8, False, p))
+
+
+def test_compute_descriptors_trivial_inputs_need_no_device():
+    """pipeline.rs:1587-1589: an image under 3x3 or no keypoints gives no features; an unknown
+    descriptor kind is rejected before any device work."""
+    L = ps.lib()
+    n = C.c_size_t(7)
+    kp = np.zeros(1, ps.KP_DTYPE)
+    small = np.zeros((2, 2), np.float32)
+    assert L.ps_compute_descriptors(small.ctypes.data, 2, 2, 2, kp.ctypes.data, 1, 0, None, None, 0, C.byref(n)) == ps.PS_OK
+    assert n.value == 0
+    img = np.zeros((16, 16), np.float32)
+    n.value = 7
+    assert L.ps_compute_descriptors(img.ctypes.data, 16, 16, 16, kp.ctypes.data, 0, 0, None, None, 0, C.byref(n)) == ps.PS_OK
+    assert n.value == 0
+    assert L.ps_compute_descriptors(img.ctypes.data, 16, 16, 16, kp.ctypes.data, 1, 2, None, None, 0,
+                                    C.byref(n)) == ps.PS_E_INVALID_ARG

@@ -171,6 +171,55 @@ def test_detect_and_compute_capped_1080p(gpu):
     assert flips <= 8
 
 
+@pytest.mark.parametrize("upsample", [True, False])
+@pytest.mark.parametrize("kind", [0, 1])
+def test_compute_descriptors_for_detected_keypoints(gpu, upsample, kind):
+    """The uncapped CLI extract path (cli.rs:174-212): detect_keypoints, then
+    compute_sift_descriptors_with_kind (pipeline.rs:1581-1620), whose pyramid parameters are
+    inferred from the keypoints; compared with the oracle's restatement at descriptor tolerance."""
+    u8, img = _blob_f32(640, 480, 200, seed=11)
+    cfg = gpu.SiftConfig().octaves(4).scales(3).upsample_first_octave(upsample).descriptor_kind(kind)
+    sift = gpu.Sift(cfg)
+    kps = sift.detect_keypoints(img)
+    assert len(kps) > 50
+    dev = sift.compute_descriptors(img, kps)
+    rh, rd = psr.compute_descriptors(img, kps, kind)
+    assert len(rh) >= len(kps)
+    flips, _ = _check_features(dev, rh, rd)
+
+
+def test_compute_descriptors_layer0_keypoints(gpu):
+    """Keypoints without octave/layer (layer 0): octave floor(log2(max(scale/1.6, 1))), layer from
+    the local sigma (resolve_octave_and_layer, pipeline.rs:1206-1227), no upsampling."""
+    u8, img = _blob_f32(400, 300, 120, seed=5)
+    rng = np.random.default_rng(3)
+    k = np.zeros(300, gpu.KP_DTYPE)
+    k["x"] = rng.uniform(2, 397, 300).astype(np.float32)
+    k["y"] = rng.uniform(2, 297, 300).astype(np.float32)
+    k["scale"] = rng.uniform(0.5, 20.0, 300).astype(np.float32)
+    k["response"] = rng.uniform(-0.1, 0.1, 300).astype(np.float32)
+    dev = gpu.Sift(gpu.SiftConfig()).compute_descriptors(img, k)
+    rh, rd = psr.compute_descriptors(img, k, 0)
+    assert len(rh) >= 300
+    _check_features(dev, rh, rd)
+
+
+def test_compute_descriptors_large_scales(gpu):
+    """Keypoints at scales far beyond detection's (octave-local sigma up to 90): descriptor cell
+    boxes of 256+ pixels take the kernel's direct path instead of the packed pixel lists."""
+    u8, img = _blob_f32(1100, 1000, 300, seed=8)
+    k = np.zeros(6, gpu.KP_DTYPE)
+    k["x"] = [550, 500, 600, 300, 800, 550]
+    k["y"] = [500, 450, 550, 300, 700, 520]
+    k["scale"] = [8, 40, 60, 70, 80, 90]
+    k["octave"] = 0
+    k["layer"] = 1
+    dev = gpu.Sift(gpu.SiftConfig()).compute_descriptors(img, k)
+    rh, rd = psr.compute_descriptors(img, k, 0)
+    assert len(rh) >= 6
+    _check_features(dev, rh, rd, max_flip_frac=0.2)
+
+
 def test_determinism(gpu):
     u8, img = _blob_f32(640, 480, 200, seed=3)
     cfg = gpu.SiftConfig().octaves(4)
