@@ -311,3 +311,35 @@ def test_rerank_vs_float64():
         assert np.all(np.diff(d2[i]) >= 0)
         assert sorted(idx[i].tolist()) == sorted(sl[i].tolist())
         assert np.allclose(np.sort(ref[i]), d2[i], rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("upsample", [True, False])
+def test_compute_descriptors_infers_the_detection_parameters(upsample):
+    """pipeline.rs:1591-1617: for keypoints detected with (4 octaves, 3 scales, upsample) the
+    inferred pyramid is that one (scales from the largest layer, upsampling from the first
+    keypoint's scale / (1.6 * 2^octave * k^layer) < 0.707, octaves from the largest octave), so
+    the features equal those computed on the explicit pyramid."""
+    img = psr.luma_to_f32(synth.blob_image(320, 240, 60, 3))
+    kps = psr.detect_keypoints(img, 4, 3, 0.015, upsample)
+    assert len(kps) > 20
+    f1, d1 = psr.compute_descriptors(img, kps, 0)
+    f2, d2 = psr.descriptors_for_keypoints(img, kps, int(kps["octave"].max()) + 1, 3, upsample, 0)
+    assert np.array_equal(f1, f2) and np.array_equal(d1, d2)
+
+
+def test_compute_descriptors_layer0_resolution():
+    """resolve_octave_and_layer (pipeline.rs:1206-1227): a layer-0 keypoint of scale
+    1.6 * 2^o * 2^(l/3) takes octave o and layer l; the explicit (o, l) keypoint of the same
+    scale (no upsampling: scale / expected = 1) gives the same feature."""
+    img = psr.luma_to_f32(synth.blob_image(320, 240, 60, 4))
+    for o, l in [(0, 1), (1, 2), (2, 1), (1, 1)]:
+        s = np.float32(1.6 * 2 ** o * 2 ** (l / 3))
+        k0 = np.zeros(2, psr.KP_DTYPE)
+        k0["x"], k0["y"], k0["scale"], k0["response"] = [120, 200], [100, 150], s, 0.02
+        k0[1]["scale"] = np.float32(1.6 * 2 ** 2)   # same octave count for both calls
+        k1 = k0.copy()
+        k1[0]["octave"], k1[0]["layer"] = o, l
+        k1[1]["octave"], k1[1]["layer"] = 2, 1   # scale 6.4: octave floor(log2 4) = 2, local sigma 1 -> layer 1
+        f0, d0 = psr.compute_descriptors(img, k0, 0)
+        f1, d1 = psr.compute_descriptors(img, k1, 0)
+        assert np.array_equal(d0, d1), (o, l)
