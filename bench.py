@@ -331,6 +331,105 @@ def bench_batch(n_images: int = 256, reps: int = 5, cpu_images: int = 4) -> dict
     return out
 
 
+def bench_catalog(n_entries: int = 1000, steps: int = 20, cpu_searches: int = 1) -> dict:
+    """GPU-resident catalog search (catalog.rs:576-702): a PSIFTCAT catalog of 1000 entries,
+    each the features of a seeded 640x480 blob image (4 octaves x 3 scales, upsampled, cap
+    4096), searched with the features of a homography-warped view of entry 517; top_k 5,
+    shortlist 16, ratio 0.75, RANSAC 3 px (the reference defaults). Query features are
+    device-resident before the timed region; each search returns its hits to the host."""
+    import numpy as np
+    import torch
+    import pure_sift as ps
+    import learned_ransac as lr
+    import synth
+    dev = torch.device("cuda:0")
+    W, H = 640, 480
+    cfg = ps.SiftConfig().octaves(4).scales(3).contrast_threshold(0.015)
+    cap = ps.MAX_FEATURES_PER_IMAGE
+    fsz = ps.FEAT_DTYPE.itemsize
+    eng = ps.Engine(0)
+    chunk = 200
+    feat = torch.empty((chunk, cap * fsz), dtype=torch.uint8, device=dev)
+    desc = torch.empty((chunk, cap, 128), dtype=torch.float32, device=dev)
+    cnt = torch.zeros(chunk, dtype=torch.int32, device=dev)
+    entries = []
+    for c0 in range(0, n_entries, chunk):
+        imgs = lr.render_blobs(chunk, 9000 + c0, dev, W, H, 200)
+        im = np.array([(i * W * H, H, W, W, 0) for i in range(chunk)], ps.IMAGE_DESC_DTYPE)
+        eng.extract_batch_u8(imgs.data_ptr(), im, cfg, feat.data_ptr(), desc.data_ptr(), cap, cnt.data_ptr())
+        eng.sync()
+        counts = cnt.cpu().numpy()
+        fh, dh = feat.cpu().numpy(), desc.cpu().numpy()
+        for i in range(chunk):
+            n = int(counts[i])
+            entries.append((f"img_{c0 + i:04d}.png", W, H,
+                            ps.Features(fh[i, :n * fsz].view(ps.FEAT_DTYPE).copy(), dh[i, :n].copy())))
+        if c0 <= 517 < c0 + chunk:
+            target = imgs[517 - c0].clone()
+    data = ps.FeatureCatalog.binary_bytes(entries, 4, 3, 0.015)
+    t0 = time.perf_counter()
+    cat = ps.FeatureCatalog.from_binary_bytes(data)
+    load_s = time.perf_counter() - t0
+    n_feat = cat.n_features
+    rng = np.random.default_rng(5)
+    Hm = synth.random_homography(W, H, rng)
+    query = lr.warp(target, Hm)
+    qf = torch.empty(cap * fsz, dtype=torch.uint8, device=dev)
+    qd = torch.empty((cap, 128), dtype=torch.float32, device=dev)
+    qc = torch.zeros(1, dtype=torch.int32, device=dev)
+    eng.extract_u8(query.data_ptr(), H, W, W, cfg, qf.data_ptr(), qd.data_ptr(), cap, qc.data_ptr())
+    eng.sync()
+    nq = int(qc.item())
+
+    def search():
+        return cat.search_raw_device(eng, qf.data_ptr(), qd.data_ptr(), nq)
+
+    for _ in range(2):
+        hits = search()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        hits = search()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    qdh = qd[:nq].cpu().numpy()
+    cat.coarse_votes(qdh)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        cat.coarse_votes(qdh)
+    vote_ms = (time.perf_counter() - t0) / steps * 1e3
+    top = cat.summary(int(hits[0]["entry"])).id if len(hits) else None
+    pairs = nq * n_feat
+    ops = 10 * pairs   # per (query, signature): 4 xor, 4 popcount, add3, compare (the byte test only on near codes)
+    out = {"workload": f"{n_entries}-entry PSIFTCAT catalog ({n_feat} features, 640x480 blob images), "
+                       f"query {nq} features of a warped view of entry 517; top 5, shortlist 16, ratio 0.75, 3 px",
+           "value": 1e3 / ms, "unit": "searches/s", "ms_per_search": ms, "load_s": load_s,
+           "vote_pass_ms_host_timed": vote_ms, "top_hit": top, "top_hit_correct": top == "img_0517.png",
+           "top_hit_match_count": int(hits[0]["match_count"]) if len(hits) else 0,
+           "roofline": {"bound": "valu", "kernel": "catalog_vote_kernel (inverted-index predicate over every "
+                        "catalog signature)", "achieved": ops / (vote_ms * 1e-3) / 1e12,
+                        "peak": VALU_INT_PEAK_TOPS, "unit": "TOP/s", "frac": ops / (vote_ms * 1e-3) / 1e12 / VALU_INT_PEAK_TOPS,
+                        "pairs": pairs, "note": "host-timed vote call (includes the query upload, launch "
+                        "and readback); 10 integer VALU ops per (query, signature) pair"}}
+    if cpu_searches > 0:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import psr
+        cores = _host_cores()
+        psr.set_threads(cores)
+        ref = psr.catalog_parse(data)
+        ref.build_index()   # built once at load, as the reference's rebuild_indices (catalog.rs:284-297)
+        qfh = qf[:nq * fsz].cpu().numpy().view(psr.FEAT_DTYPE).copy()
+        t0 = time.perf_counter()
+        for _ in range(cpu_searches):
+            rh = psr.catalog_search(ref, qfh, qdh, 5, 16, 0.75, 3.0)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cpu_searches / dt, "unit": "searches/s", "cores": cores, "kind": "port",
+                               "sample": f"{cpu_searches} search(es) of the same query and catalog, {dt:.1f} s wall"}
+        out["hits_equal_oracle"] = bool(len(rh) == len(hits) and all(
+            int(a["entry"]) == int(b["entry"]) and int(a["match_count"]) == int(b["match_count"])
+            for a, b in zip(rh, hits)))
+    eng.close()
+    return out
+
+
 def cpu_baseline(a, b, frames: int):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import psr
@@ -363,7 +462,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-frames", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["frame", "hamming", "precision", "ransac", "batch"], default="frame",
+    ap.add_argument("--workload", choices=["frame", "hamming", "precision", "ransac", "batch", "catalog"],
+                    default="frame",
                     help="frame: the 1080p extract+match headline (default); hamming: the catalog shortlist")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device-to-device copy first (a known-size reference for HBM counters)")
@@ -375,6 +475,10 @@ def main():
               f"(images shard across GPUs with no exchange)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "catalog":
+        print(json.dumps(bench_catalog(steps=min(args.steps, 20), cpu_searches=0 if args.no_cpu_baseline else 1),
+                         default=float), flush=True)
+        return
     if args.workload == "batch":
         print(json.dumps(bench_batch(cpu_images=0 if args.no_cpu_baseline else 4), default=float), flush=True)
         return
@@ -489,7 +593,8 @@ def main():
         for name, fn in (("hamming", lambda: bench_hamming(cpu_queries=0 if args.no_cpu_baseline else 1000)),
                          ("precision", lambda: bench_precision(cpu_queries=0 if args.no_cpu_baseline else 200)),
                          ("ransac", bench_ransac),
-                         ("batch", lambda: bench_batch(cpu_images=0 if args.no_cpu_baseline else 4))):
+                         ("batch", lambda: bench_batch(cpu_images=0 if args.no_cpu_baseline else 4)),
+                         ("catalog", lambda: bench_catalog(cpu_searches=0 if args.no_cpu_baseline else 1))):
             try:
                 sec[name] = fn()
             except Exception as ex:   # a secondary workload never hides the headline line

@@ -34,6 +34,8 @@ typedef int32_t ps_status;
 #define PS_E_NO_DEVICE (-5)     /* no usable GPU */
 #define PS_E_OUT_OF_MEMORY (-6) /* device allocation failed */
 #define PS_E_INJECTED (-7)      /* failure forced by ps_debug_fail_next (tests only) */
+#define PS_E_IO (-8)            /* catalog file could not be read or written */
+#define PS_E_FORMAT (-9)        /* malformed catalog bytes (message as catalog.rs:447-540) */
 
 #define PS_DESCRIPTOR_LEN 128      /* pipeline.rs:998 SIFT_DESCRIPTOR_LEN */
 #define PS_MAX_FEATURES 4096       /* pipeline.rs:1054 MAX_FEATURES_PER_IMAGE */
@@ -354,6 +356,84 @@ ps_status ps_engine_hamming_topk(ps_engine *e, const uint64_t *d_catalog, size_t
 ps_status ps_engine_rerank_l2(ps_engine *e, const float *d_q_desc, size_t nq, const float *d_cat_desc,
                               size_t n, const uint32_t *d_shortlist, uint32_t k, uint32_t *d_idx,
                               float *d_d2);
+
+/* ---- GPU-resident feature catalog (catalog.rs) ----------------------------------------
+ * FeatureCatalog (catalog.rs:179-197) with every entry's features, stored 128-bit signatures
+ * and f32 descriptors resident in device memory of the library's device. Entry order, id
+ * lookup and removal follow the reference (upsert replaces in place, remove swaps the last
+ * entry in). Calls that change a catalog need exclusive access to it; searches may run
+ * concurrently. */
+typedef struct ps_catalog ps_catalog;
+typedef struct {
+    uint64_t n_entries, n_features;
+    uint32_t octaves, scales;
+    float contrast_threshold;
+    uint8_t descriptor_kind;
+    uint8_t _pad[3];
+} ps_catalog_info;
+typedef struct {
+    const char *id;          /* UTF-8, not NUL-terminated; valid until the catalog changes */
+    uint64_t id_len;
+    uint32_t width, height;
+    uint64_t feature_count;
+} ps_catalog_entry_info;
+/* catalog.rs:158-177 CatalogSearchHit with its TrackingSummary (pipeline.rs:1137-1153);
+ * the entry's id, width, height and feature count come from ps_catalog_entry. */
+typedef struct {
+    uint32_t entry;          /* catalog index */
+    uint32_t coarse_votes;
+    uint64_t raw_match_count, match_count;
+    float inlier_ratio;
+    float mean_dx, mean_dy, median_dx, median_dy;
+    float mean_scale_ratio, mean_rotation_deg;
+    uint32_t has_transform, has_homography;
+    float transform[6];      /* SimilarityTransform2D a, b, tx, ty, scale, rotation_deg */
+    float homography[9];     /* Homography2D, row-major */
+} ps_catalog_hit;
+
+/* catalog.rs:272-282 FeatureCatalog::new (descriptor_kind as the catalog field, :186-188) */
+ps_status ps_catalog_create(uint32_t octaves, uint32_t scales, float contrast_threshold, uint8_t descriptor_kind,
+                            ps_catalog **out);
+/* catalog.rs:447-540 load_binary from the bytes of a PSIFTCAT file: signatures kept as
+ * stored, u8 descriptors dequantised on the device (pipeline.rs:1034-1044). PS_E_FORMAT
+ * with the reference's message for a malformed file. */
+ps_status ps_catalog_load_binary(const uint8_t *bytes, size_t len, ps_catalog **out);
+/* catalog.rs:447 load_binary(path) */
+ps_status ps_catalog_load_file(const char *path, ps_catalog **out);
+/* catalog.rs:400-445 save_binary: u8-quantised descriptors (pipeline.rs:1024-1030), written
+ * to a temporary file then renamed (catalog.rs:204-244). */
+ps_status ps_catalog_save_binary(const ps_catalog *c, const char *path);
+ps_status ps_catalog_destroy(ps_catalog *c);
+ps_status ps_catalog_get_info(const ps_catalog *c, ps_catalog_info *out);
+/* catalog.rs:347-357 list_summaries, one entry */
+ps_status ps_catalog_entry(const ps_catalog *c, size_t idx, ps_catalog_entry_info *out);
+/* An entry's features as stored on the device (any output may be null; cap rows each). */
+ps_status ps_catalog_entry_features(const ps_catalog *c, size_t idx, ps_feature *out_feat, float *out_desc,
+                                    uint64_t *out_sig, size_t cap);
+/* catalog.rs:309-328 upsert_entry: signatures quantised from desc on the device
+ * (catalog.rs:37-45); id must be valid UTF-8. */
+ps_status ps_catalog_upsert(ps_catalog *c, const char *id, size_t id_len, uint32_t width, uint32_t height,
+                            const ps_feature *feat, const float *desc, size_t n);
+/* catalog.rs:331-339 remove; *removed = 1 when the id existed */
+ps_status ps_catalog_remove(ps_catalog *c, const char *id, size_t id_len, int32_t *removed);
+/* catalog.rs:595-633: the coarse vote of every entry (out_votes holds n_entries) for the
+ * query descriptors, by the reference's path choice: more than 32 entries -> a vote needs a
+ * signature within 24 bits that also shares a byte position with the query (the inverted
+ * index), else within 24 bits (count_coarse_hamming_votes). At most 65535 * 256 queries. */
+ps_status ps_catalog_coarse_votes(const ps_catalog *c, const float *q_desc, size_t nq, uint32_t *out_votes);
+/* catalog.rs:576-702 search_features: coarse votes, shortlist by (votes desc, id asc) cut to
+ * max(shortlist_k, 1), then per shortlisted entry the exact ratio match (pipeline.rs:1845-1858),
+ * summarize_tracking (:2210-2272), similarity and homography RANSAC at ransac_threshold; hits
+ * ordered by (match_count desc, inlier_ratio desc, coarse_votes desc, id asc), cut to top_k.
+ * *out_count = the hit count (PS_E_CAPACITY when above cap). At most 262144 queries. */
+ps_status ps_catalog_search(const ps_catalog *c, const ps_feature *q, const float *q_desc, size_t nq,
+                            size_t top_k, size_t shortlist_k, float ratio, float ransac_threshold,
+                            ps_catalog_hit *out, size_t cap, size_t *out_count);
+/* The same with device-resident query features (e.g. from ps_engine_extract_u8), run on the
+ * engine's stream; returns after the hits are on the host. */
+ps_status ps_engine_catalog_search(ps_engine *e, const ps_catalog *c, const ps_feature *d_q, const float *d_q_desc,
+                                   size_t nq, size_t top_k, size_t shortlist_k, float ratio,
+                                   float ransac_threshold, ps_catalog_hit *out, size_t cap, size_t *out_count);
 
 /* Tests only: the next compute call on this thread fails with PS_E_INJECTED before
  * touching the device. */

@@ -998,7 +998,7 @@ static int fit_procrustes(const psr_match *m, const uint32_t *idx, size_t n, psr
     out->tx = mtx - (a * mqx - b * mqy);
     out->ty = mty - (b * mqx + a * mqy);
     out->scale = sqrtf(a * a + b * b);
-    out->rotation_deg = atan2f(b, a) * (180.0f / PI_F);
+    out->rotation_deg = atan2f(b, a) * 57.2957795130823208767981548141051703f;   /* f32::to_degrees */
     return 1;
 }
 

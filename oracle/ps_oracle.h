@@ -143,6 +143,59 @@ void psr_mlp_forward(const float *params, int bf16, const float *x, size_t n, fl
 double psr_mlp_loss_grad(const float *params, int bf16, const float *x, const float *y, size_t n, double *grad);
 void psr_free(void *p);
 
+/* ---- feature catalog (ps_oracle_catalog.c; catalog.rs) ---- */
+/* pipeline.rs:1024-1030 / 1034-1044, n rows of 128 */
+void psr_quantize_u8(const float *desc, size_t n, uint8_t *out);
+void psr_dequantize_u8(const uint8_t *q, size_t n, float *out);
+/* A catalog as flat arrays: entry e has id id_blob[id_off[e] .. id_off[e+1]) and the
+ * features off[e] .. off[e+1] (records, 2-word signatures, 128-float descriptors). */
+typedef struct {
+    uint32_t octaves, scales;
+    float threshold;
+    uint32_t kind;
+    size_t n_entries, n_features;
+    const char *id_blob;
+    const uint64_t *id_off;
+    const uint32_t *width, *height;
+    const uint64_t *off;
+    const psr_feature *feat;
+    const uint64_t *sig;
+    const float *desc;
+    const void *mih;    /* optional inverted index from psr_catalog_index (else built per call) */
+} psr_catalog;
+/* catalog.rs:280-297 the inverted index of a catalog (free with psr_catalog_index_free) */
+void *psr_catalog_index(const psr_catalog *c);
+void psr_catalog_index_free(void *mih);
+/* catalog.rs:447-540 over an in-memory file; 0 or -1 with the message in err */
+int psr_catalog_parse(const uint8_t *buf, size_t len, psr_catalog *out, char *err, size_t cap);
+void psr_catalog_free(psr_catalog *c);
+/* catalog.rs:400-445; returns the size (writes only when cap suffices) */
+size_t psr_catalog_write(const psr_catalog *c, uint8_t *buf, size_t cap);
+/* catalog.rs:595-633 (mih: inverted-index path) */
+void psr_catalog_votes(const psr_catalog *c, const uint64_t *q, size_t nq, int mih, uint32_t *votes);
+size_t psr_catalog_shortlist(const psr_catalog *c, const uint32_t *votes, int mih, size_t shortlist_k,
+                             uint32_t *cand_entry, uint32_t *cand_votes);
+/* pipeline.rs:1137-1153 TrackingSummary / :2210-2272 summarize_tracking */
+typedef struct {
+    uint64_t raw_match_count, match_count;
+    float inlier_ratio, mean_dx, mean_dy, median_dx, median_dy, mean_scale_ratio, mean_rotation_deg;
+} psr_tracking;
+void psr_summarize_tracking(const psr_feature *q, const psr_feature *t, const psr_match *m, size_t n,
+                            psr_tracking *out);
+/* catalog.rs:158-177 CatalogSearchHit, the layout of the product's ps_catalog_hit */
+typedef struct {
+    uint32_t entry, coarse_votes;
+    uint64_t raw_match_count, match_count;
+    float inlier_ratio, mean_dx, mean_dy, median_dx, median_dy, mean_scale_ratio, mean_rotation_deg;
+    uint32_t has_transform, has_homography;
+    float transform[6];   /* a, b, tx, ty, scale, rotation_deg */
+    float homography[9];
+} psr_catalog_hit;
+/* catalog.rs:576-702; out capacity min(top_k, n_entries); returns the hit count */
+size_t psr_catalog_search(const psr_catalog *c, const psr_feature *qf, const float *qd, size_t nq,
+                          size_t top_k, size_t shortlist_k, float ratio, float ransac_thresh,
+                          psr_catalog_hit *out);
+
 #ifdef __cplusplus
 }
 #endif

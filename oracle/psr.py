@@ -60,6 +60,22 @@ MATCH_DTYPE = np.dtype([("query_index", "<u8"), ("target_index", "<u8"), ("qx", 
                         ("tx", "<f4"), ("ty", "<f4"), ("distance", "<f4"), ("ratio", "<f4"),
                         ("dx", "<f4"), ("dy", "<f4")])
 
+class CatalogC(C.Structure):
+    _fields_ = [("octaves", C.c_uint32), ("scales", C.c_uint32), ("threshold", C.c_float), ("kind", C.c_uint32),
+                ("n_entries", C.c_size_t), ("n_features", C.c_size_t), ("id_blob", C.c_void_p),
+                ("id_off", C.c_void_p), ("width", C.c_void_p), ("height", C.c_void_p), ("off", C.c_void_p),
+                ("feat", C.c_void_p), ("sig", C.c_void_p), ("desc", C.c_void_p), ("mih", C.c_void_p)]
+
+
+TRACKING_DTYPE = np.dtype([("raw_match_count", "<u8"), ("match_count", "<u8"), ("inlier_ratio", "<f4"),
+                           ("mean_dx", "<f4"), ("mean_dy", "<f4"), ("median_dx", "<f4"), ("median_dy", "<f4"),
+                           ("mean_scale_ratio", "<f4"), ("mean_rotation_deg", "<f4")], align=True)
+HIT_DTYPE = np.dtype([("entry", "<u4"), ("coarse_votes", "<u4"), ("raw_match_count", "<u8"),
+                      ("match_count", "<u8"), ("inlier_ratio", "<f4"), ("mean_dx", "<f4"), ("mean_dy", "<f4"),
+                      ("median_dx", "<f4"), ("median_dy", "<f4"), ("mean_scale_ratio", "<f4"),
+                      ("mean_rotation_deg", "<f4"), ("has_transform", "<u4"), ("has_homography", "<u4"),
+                      ("transform", "<f4", (6,)), ("homography", "<f4", (9,))], align=True)
+
 _lib = None
 
 
@@ -134,6 +150,24 @@ def lib():
         L.psr_fit_homography_dlt.restype = C.c_int
         L.psr_adaptive_ransac_iters.argtypes = [C.c_size_t, C.c_size_t, C.c_int, C.c_double, C.c_size_t, C.c_size_t]
         L.psr_adaptive_ransac_iters.restype = C.c_size_t
+        L.psr_quantize_u8.argtypes = [fp, C.c_size_t, C.c_void_p]
+        L.psr_dequantize_u8.argtypes = [C.c_void_p, C.c_size_t, fp]
+        L.psr_catalog_parse.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(CatalogC), C.c_char_p, C.c_size_t]
+        L.psr_catalog_parse.restype = C.c_int
+        L.psr_catalog_free.argtypes = [C.POINTER(CatalogC)]
+        L.psr_catalog_index.argtypes = [C.POINTER(CatalogC)]
+        L.psr_catalog_index.restype = C.c_void_p
+        L.psr_catalog_index_free.argtypes = [C.c_void_p]
+        L.psr_catalog_write.argtypes = [C.POINTER(CatalogC), C.c_void_p, C.c_size_t]
+        L.psr_catalog_write.restype = C.c_size_t
+        L.psr_catalog_votes.argtypes = [C.POINTER(CatalogC), C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+        L.psr_catalog_shortlist.argtypes = [C.POINTER(CatalogC), C.c_void_p, C.c_int, C.c_size_t, C.c_void_p,
+                                            C.c_void_p]
+        L.psr_catalog_shortlist.restype = C.c_size_t
+        L.psr_summarize_tracking.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
+        L.psr_catalog_search.argtypes = [C.POINTER(CatalogC), C.c_void_p, fp, C.c_size_t, C.c_size_t, C.c_size_t,
+                                         C.c_float, C.c_float, C.c_void_p]
+        L.psr_catalog_search.restype = C.c_size_t
         L.psr_set_threads.argtypes = [C.c_int]
         L.psr_get_threads.restype = C.c_int
         L.psr_free.argtypes = [C.c_void_p]
@@ -478,3 +512,142 @@ def fit_homography_dlt(m):
 
 def adaptive_ransac_iters(inl, total, sample_size, conf, min_iters, max_cap) -> int:
     return lib().psr_adaptive_ransac_iters(inl, total, sample_size, conf, min_iters, max_cap)
+
+
+# ---------------------------------------------------------------- feature catalog
+
+def quantize_u8(desc) -> np.ndarray:
+    desc = _f32(desc).reshape(-1, 128)
+    out = np.zeros(desc.shape, np.uint8)
+    lib().psr_quantize_u8(_fp(desc), len(desc), out.ctypes.data)
+    return out
+
+
+def dequantize_u8(q) -> np.ndarray:
+    q = np.ascontiguousarray(q, np.uint8).reshape(-1, 128)
+    out = np.zeros(q.shape, np.float32)
+    lib().psr_dequantize_u8(q.ctypes.data, len(q), _fp(out))
+    return out
+
+
+class Catalog:
+    """A catalog as flat arrays (the oracle's psr_catalog view): entry e owns the features
+    off[e]:off[e+1] of feat (FEAT_DTYPE), sig (n x 2 u64) and desc (n x 128 f32)."""
+
+    def __init__(self, ids, width, height, off, feat, sig, desc, octaves=3, scales=3, threshold=0.015, kind=0):
+        self.ids = list(ids)
+        self.width = np.ascontiguousarray(width, np.uint32)
+        self.height = np.ascontiguousarray(height, np.uint32)
+        self.off = np.ascontiguousarray(off, np.uint64)
+        self.feat = np.ascontiguousarray(feat, FEAT_DTYPE)
+        self.sig = np.ascontiguousarray(sig, np.uint64).reshape(-1, 2)
+        self.desc = _f32(desc).reshape(-1, 128)
+        self.octaves, self.scales, self.threshold, self.kind = octaves, scales, threshold, kind
+        blobs = [i.encode("utf-8") if isinstance(i, str) else bytes(i) for i in self.ids]
+        self._blob = np.frombuffer(b"".join(blobs) or b"\0", np.uint8).copy()
+        self._id_off = np.zeros(len(blobs) + 1, np.uint64)
+        self._id_off[1:] = np.cumsum([len(b) for b in blobs]) if blobs else []
+        self._mih = None
+
+    def build_index(self) -> None:
+        """Builds the inverted index once (catalog.rs:280-297), as the reference keeps it."""
+        if self._mih is None:
+            cc = self.c()
+            self._mih = lib().psr_catalog_index(C.byref(cc))
+
+    def __del__(self):
+        if getattr(self, "_mih", None) and _lib is not None:
+            _lib.psr_catalog_index_free(self._mih)
+            self._mih = None
+
+    @staticmethod
+    def from_entries(entries, octaves=3, scales=3, threshold=0.015, kind=0, sigs=None):
+        """entries: list of (id, width, height, feat, desc); signatures quantised from the
+        descriptors (catalog.rs:124-132) unless given."""
+        counts = [len(e[3]) for e in entries]
+        off = np.zeros(len(entries) + 1, np.uint64)
+        off[1:] = np.cumsum(counts) if entries else []
+        feat = np.concatenate([np.asarray(e[3], FEAT_DTYPE) for e in entries]) if entries else np.zeros(0, FEAT_DTYPE)
+        desc = (np.concatenate([_f32(e[4]).reshape(-1, 128) for e in entries]) if entries
+                else np.zeros((0, 128), np.float32))
+        sig = quantize_u128(desc) if sigs is None else sigs
+        return Catalog([e[0] for e in entries], [e[1] for e in entries], [e[2] for e in entries], off, feat,
+                       sig, desc, octaves, scales, threshold, kind)
+
+    def c(self) -> CatalogC:
+        return CatalogC(self.octaves, self.scales, self.threshold, self.kind, len(self.ids), len(self.feat),
+                        self._blob.ctypes.data, self._id_off.ctypes.data, self.width.ctypes.data,
+                        self.height.ctypes.data, self.off.ctypes.data, self.feat.ctypes.data,
+                        self.sig.ctypes.data, self.desc.ctypes.data, self._mih)
+
+
+def catalog_parse(data: bytes) -> Catalog:
+    """catalog.rs:447-540 load_binary; raises ValueError with the reference's message."""
+    cc = CatalogC()
+    err = C.create_string_buffer(256)
+    if lib().psr_catalog_parse(data, len(data), C.byref(cc), err, 256) != 0:
+        raise ValueError(err.value.decode())
+    try:
+        ne, nf = cc.n_entries, cc.n_features
+
+        def arr(ptr, dtype, count):
+            if count == 0:
+                return np.zeros(0, dtype)
+            return np.frombuffer((C.c_char * (count * np.dtype(dtype).itemsize)).from_address(ptr), dtype).copy()
+
+        id_off = arr(cc.id_off, np.uint64, ne + 1)
+        blob = arr(cc.id_blob, np.uint8, int(id_off[-1])).tobytes()
+        ids = [blob[id_off[i]:id_off[i + 1]].decode("utf-8") for i in range(ne)]
+        return Catalog(ids, arr(cc.width, np.uint32, ne), arr(cc.height, np.uint32, ne), arr(cc.off, np.uint64, ne + 1),
+                       arr(cc.feat, FEAT_DTYPE, nf), arr(cc.sig, np.uint64, 2 * nf).reshape(-1, 2),
+                       arr(cc.desc, np.float32, 128 * nf).reshape(-1, 128), cc.octaves, cc.scales, cc.threshold,
+                       cc.kind)
+    finally:
+        lib().psr_catalog_free(C.byref(cc))
+
+
+def catalog_write(cat: Catalog) -> bytes:
+    """catalog.rs:400-445 save_binary (u8-quantised descriptors, stored signatures)."""
+    cc = cat.c()
+    need = lib().psr_catalog_write(C.byref(cc), None, 0)
+    buf = np.zeros(need, np.uint8)
+    lib().psr_catalog_write(C.byref(cc), buf.ctypes.data, need)
+    return buf.tobytes()
+
+
+def catalog_votes(cat: Catalog, q_codes, mih: bool) -> np.ndarray:
+    q = np.ascontiguousarray(q_codes, np.uint64).reshape(-1, 2)
+    votes = np.zeros(max(len(cat.ids), 1), np.uint32)
+    cc = cat.c()
+    lib().psr_catalog_votes(C.byref(cc), q.ctypes.data, len(q), int(mih), votes.ctypes.data)
+    return votes[:len(cat.ids)]
+
+
+def catalog_shortlist(cat: Catalog, votes, mih: bool, shortlist_k: int):
+    v = np.ascontiguousarray(votes, np.uint32)
+    ce = np.zeros(max(len(cat.ids), 1), np.uint32)
+    cv = np.zeros(max(len(cat.ids), 1), np.uint32)
+    cc = cat.c()
+    n = lib().psr_catalog_shortlist(C.byref(cc), v.ctypes.data, int(mih), shortlist_k, ce.ctypes.data,
+                                    cv.ctypes.data)
+    return ce[:n].copy(), cv[:n].copy()
+
+
+def summarize_tracking(qf, tf, m):
+    qf = np.ascontiguousarray(qf, FEAT_DTYPE)
+    tf = np.ascontiguousarray(tf, FEAT_DTYPE)
+    m = np.ascontiguousarray(m, MATCH_DTYPE)
+    out = np.zeros(1, TRACKING_DTYPE)
+    lib().psr_summarize_tracking(qf.ctypes.data, tf.ctypes.data, m.ctypes.data, len(m), out.ctypes.data)
+    return out[0]
+
+
+def catalog_search(cat: Catalog, qf, qd, top_k=5, shortlist_k=16, ratio=0.75, ransac_threshold=3.0):
+    """catalog.rs:576-702 search_features -> HIT_DTYPE records (entry = catalog index)."""
+    qf = np.ascontiguousarray(qf, FEAT_DTYPE)
+    qd = _f32(qd).reshape(-1, 128)
+    out = np.zeros(max(min(top_k, len(cat.ids)), 1), HIT_DTYPE)
+    cc = cat.c()
+    n = lib().psr_catalog_search(C.byref(cc), qf.ctypes.data, _fp(qd), len(qf), top_k, shortlist_k,
+                                 C.c_float(ratio), C.c_float(ransac_threshold), out.ctypes.data)
+    return out[:n].copy()

@@ -581,6 +581,8 @@ const char *ps_status_string(ps_status s) {
     case PS_E_NO_DEVICE: return "no usable device";
     case PS_E_OUT_OF_MEMORY: return "device out of memory";
     case PS_E_INJECTED: return "injected failure";
+    case PS_E_IO: return "catalog file error";
+    case PS_E_FORMAT: return "malformed catalog";
     default: return "unknown status";
     }
 }

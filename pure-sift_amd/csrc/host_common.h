@@ -20,6 +20,8 @@ enum AuxSlot {
     AUX_RR_QD, AUX_RR_CD, AUX_RR_IDX, AUX_RR_DIST,
     AUX_PM_0, AUX_PM_1, AUX_PM_2, AUX_PM_3, AUX_PM_4, AUX_PM_5, AUX_PM_6, AUX_PM_7,
     AUX_RS_0, AUX_RS_1, AUX_RS_2, AUX_RS_3, AUX_RS_4, AUX_RS_5, AUX_RS_6, AUX_RS_7,
+    AUX_CS_0, AUX_CS_1, AUX_CS_2, AUX_CS_3, AUX_CS_4, AUX_CS_5, AUX_CS_6, AUX_CS_7, AUX_CS_8, AUX_CS_9,
+    AUX_CS_10, AUX_CS_11,
     AUX_COUNT
 };
 
@@ -70,6 +72,10 @@ void engine_free(ps_engine *e);
 void release_batch(ps_engine *e);
 ps_status validate_cfg(const ps_sift_config *c, char *msg, size_t cap);
 ps_status prepare_extract(ps_engine *e, size_t h, size_t w, const ps_sift_config &cfg);
+// exact ratio match (pipeline.rs:1744-1841) of device features, counts as device u32s
+ps_status enqueue_match(ps_engine *e, const ps_feature *qf, const float *qd, const uint32_t *nq, size_t nq_max,
+                        const ps_feature *tf, const float *td, const uint32_t *nt, size_t nt_max,
+                        const ps_match_options &opt, ps_feature_match *out, uint32_t *out_count);
 ps_status enqueue_extract(ps_engine *e, const void *src, bool src_u8, size_t h, size_t w, size_t stride,
                           const ps_sift_config &cfg, ps_feature *d_feat, float *d_desc, uint32_t cap,
                           uint32_t *d_count);

@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256) void nn_rescore_kernel(const Cand4 *part, int 
     }
 #pragma unroll
     for (int o = 1; o < 8; o <<= 1) last4 = fminf(last4, __shfl_xor(last4, o));
-    float sel_d = INFINITY, kth = INFINITY;
+    float kth = INFINITY;
     uint32_t sel_i = kNoIdx;
     for (int round = 0; round < 8; round++) {
         float m = INFINITY;
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(256) void nn_rescore_kernel(const Cand4 *part, int 
             uint32_t oi = __shfl_xor(gi, o);
             if (lex_less(ov, oi, g, gi)) { g = ov; gi = oi; }
         }
-        if (j == round) { sel_d = g; sel_i = gi; }
+        if (j == round) sel_i = gi;
         if (round == 7) kth = g;
         if (ms >= 0 && mix == gi && m == g) {
 #pragma unroll

@@ -150,9 +150,34 @@ hipError_t launch_nn_precision(int prec, const float *qd, const uint32_t *nq, si
 
 // ransac.hip: batched similarity-seeded LO-RANSAC, one workgroup per problem
 size_t ransac_work_bytes(size_t total_matches);
+// sim_only: run the similarity stage alone (sim_seed_model must be set); after either form,
+// ransac_sim_inliers() points at a problem's similarity inliers (result.n_sim of them) in work.
 hipError_t launch_ransac(const ps_feature_match *matches, const float *weights, const ps_ransac_problem *probs,
                          uint32_t n_probs, const ps_ransac_params &p, size_t total_matches, void *work,
-                         uint32_t *inliers, ps_ransac_result *outs, hipStream_t st);
+                         uint32_t *inliers, ps_ransac_result *outs, hipStream_t st, bool sim_only = false);
+__host__ __device__ inline const uint32_t *ransac_sim_inliers(const void *work, size_t total_matches,
+                                                              uint32_t offset, uint32_t n) {
+    // workspace: [2 * total f64 (CDF, errors)][per problem at 2 * offset u32: n scratch, n similarity inliers]
+    const uint32_t *wu = reinterpret_cast<const uint32_t *>(static_cast<const double *>(work) + total_matches * 2);
+    return wu + (size_t)offset * 2 + n;
+}
+
+// catalog.hip: GPU-resident feature catalog
+struct TrackJob {
+    uint32_t moff, n, tfoff, pad;   // matches at moff (n of them), target features at tfoff
+};
+struct TrackOut {
+    float mean_dx, mean_dy, median_dx, median_dy, mean_scale_ratio, mean_rotation_deg;
+    uint32_t active, pad;
+};
+hipError_t launch_dequant_u8(const uint8_t *q, uint32_t n, float *out, hipStream_t st);
+// votes[e] for entries [0, n_entries): off has n_entries + 1 u32 feature offsets
+hipError_t launch_catalog_votes(const uint64_t *sig, const uint32_t *off, uint32_t n_entries, const uint64_t *q,
+                                uint32_t nq, bool mih, uint32_t *votes, hipStream_t st);
+// sim_res / sim_work: results and workspace of a RANSAC launch over the same matches at 3 px
+hipError_t launch_tracking(const ps_feature_match *matches, const TrackJob *jobs, uint32_t n_jobs,
+                           const ps_ransac_result *sim_res, const void *sim_work, size_t sim_total,
+                           const ps_feature *qf, const ps_feature *tf, TrackOut *out, hipStream_t st);
 
 // mlp.hip: match-scoring MLP (features, forward, gradient, Adam)
 size_t mlp_grad_scratch_bytes();

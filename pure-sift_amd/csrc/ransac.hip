@@ -538,6 +538,7 @@ struct RansacParamsDev {
     double confidence;
     uint32_t min_iters, max_iters;
     int local_opt, sim_seed_model, adaptive;
+    int sim_only;         // stop after the similarity stage (its inliers stay in the workspace)
 };
 
 namespace {
@@ -756,6 +757,7 @@ __global__ __launch_bounds__(kRT) void ransac_kernel(const ps_feature_match *mat
             O.sim[3] = best_sim.ty;
         }
     }
+    if (prm.sim_only) return;
 
     // ---------------- homography LO-RANSAC (pipeline.rs:2663-2755)
     const double thresh = (double)fth, tsq = thresh * thresh;
@@ -962,9 +964,10 @@ size_t ransac_work_bytes(size_t total_matches) {
     return total_matches * 2 * sizeof(double) + total_matches * 2 * sizeof(uint32_t) + 512;
 }
 
+
 hipError_t launch_ransac(const ps_feature_match *matches, const float *weights, const ps_ransac_problem *probs,
                          uint32_t n_probs, const ps_ransac_params &p, size_t total_matches, void *work,
-                         uint32_t *inliers, ps_ransac_result *outs, hipStream_t st) {
+                         uint32_t *inliers, ps_ransac_result *outs, hipStream_t st, bool sim_only) {
     if (n_probs == 0) return hipSuccess;
     RansacParamsDev prm;
     prm.reproj = p.reproj_threshold;
@@ -975,6 +978,7 @@ hipError_t launch_ransac(const ps_feature_match *matches, const float *weights, 
     prm.local_opt = p.local_opt;
     prm.sim_seed_model = p.sim_seed_model;
     prm.adaptive = p.adaptive;
+    prm.sim_only = sim_only ? 1 : 0;
     double *wd = static_cast<double *>(work);
     uint32_t *wu = reinterpret_cast<uint32_t *>(wd + total_matches * 2);
     ransac_kernel<<<n_probs, kRT, 0, st>>>(matches, weights, probs, prm, wd, wu, inliers, outs);

@@ -48,6 +48,8 @@ PS_E_CAPACITY = -4
 PS_E_NO_DEVICE = -5
 PS_E_OUT_OF_MEMORY = -6
 PS_E_INJECTED = -7
+PS_E_IO = -8
+PS_E_FORMAT = -9
 
 
 class DeviceError(RuntimeError):
@@ -79,6 +81,17 @@ class _RansacParamsC(C.Structure):
     _fields_ = [("reproj_threshold", C.c_float), ("sampler", C.c_uint8), ("local_opt", C.c_uint8),
                 ("sim_seed_model", C.c_uint8), ("adaptive", C.c_uint8), ("confidence", C.c_double),
                 ("min_iters", C.c_uint32), ("max_iters", C.c_uint32)]
+
+
+class _CatalogInfoC(C.Structure):
+    _fields_ = [("n_entries", C.c_uint64), ("n_features", C.c_uint64), ("octaves", C.c_uint32),
+                ("scales", C.c_uint32), ("contrast_threshold", C.c_float), ("descriptor_kind", C.c_uint8),
+                ("_pad", C.c_uint8 * 3)]
+
+
+class _CatalogEntryC(C.Structure):
+    _fields_ = [("id", C.c_void_p), ("id_len", C.c_uint64), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("feature_count", C.c_uint64)]
 
 
 IMAGE_DESC_DTYPE = np.dtype([("offset", "<u8"), ("height", "<u4"), ("width", "<u4"), ("row_stride", "<u4"),
@@ -160,13 +173,30 @@ def lib() -> C.CDLL:
         L.ps_engine_quantize_u128.argtypes = [vp, vp, sz, vp]
         L.ps_engine_hamming_topk.argtypes = [vp, vp, sz, vp, sz, C.c_uint32, vp, vp]
         L.ps_engine_rerank_l2.argtypes = [vp, vp, sz, vp, sz, vp, C.c_uint32, vp, vp]
+        L.ps_catalog_create.argtypes = [C.c_uint32, C.c_uint32, C.c_float, C.c_uint8, C.POINTER(vp)]
+        L.ps_catalog_load_binary.argtypes = [C.c_char_p, sz, C.POINTER(vp)]
+        L.ps_catalog_load_file.argtypes = [C.c_char_p, C.POINTER(vp)]
+        L.ps_catalog_save_binary.argtypes = [vp, C.c_char_p]
+        L.ps_catalog_destroy.argtypes = [vp]
+        L.ps_catalog_get_info.argtypes = [vp, C.POINTER(_CatalogInfoC)]
+        L.ps_catalog_entry.argtypes = [vp, sz, C.POINTER(_CatalogEntryC)]
+        L.ps_catalog_entry_features.argtypes = [vp, sz, vp, vp, vp, sz]
+        L.ps_catalog_upsert.argtypes = [vp, C.c_char_p, sz, C.c_uint32, C.c_uint32, vp, vp, sz]
+        L.ps_catalog_remove.argtypes = [vp, C.c_char_p, sz, C.POINTER(C.c_int32)]
+        L.ps_catalog_coarse_votes.argtypes = [vp, vp, sz, vp]
+        L.ps_catalog_search.argtypes = [vp, vp, vp, sz, sz, sz, C.c_float, C.c_float, vp, sz, C.POINTER(sz)]
+        L.ps_engine_catalog_search.argtypes = [vp, vp, vp, vp, sz, sz, sz, C.c_float, C.c_float, vp, sz,
+                                               C.POINTER(sz)]
         for name in ("ps_engine_extract_batch_u8", "ps_engine_autotune_blur", "ps_engine_mlp_features", "ps_engine_mlp_forward", "ps_engine_mlp_grad", "ps_engine_mlp_adam",
                      "ps_mlp_forward", "ps_mlp_loss_grad", "ps_homography_estimate", "ps_ransac_batch", "ps_engine_ransac_batch", "ps_nn2_precision", "ps_engine_nn2", "ps_quantize_u128", "ps_count_coarse_hamming_votes", "ps_hamming_topk", "ps_rerank_l2",
                      "ps_engine_quantize_u128", "ps_engine_hamming_topk", "ps_engine_rerank_l2","ps_init", "ps_validate_sift_config", "ps_validate_match_options", "ps_detect_and_compute",
                      "ps_detect_and_compute_u8", "ps_detect_keypoints", "ps_match", "ps_build_pyramid", "ps_nn2",
                      "ps_engine_create", "ps_engine_destroy", "ps_engine_sync", "ps_engine_check",
                      "ps_engine_extract_u8", "ps_engine_match", "ps_engine_profile_begin",
-                     "ps_engine_profile_end"):
+                     "ps_engine_profile_end", "ps_catalog_create", "ps_catalog_load_binary", "ps_catalog_load_file",
+                     "ps_catalog_save_binary", "ps_catalog_destroy", "ps_catalog_get_info", "ps_catalog_entry",
+                     "ps_catalog_entry_features", "ps_catalog_upsert", "ps_catalog_remove",
+                     "ps_catalog_coarse_votes", "ps_catalog_search", "ps_engine_catalog_search"):
             getattr(L, name).restype = i32
         _lib = L
     return _lib
@@ -175,8 +205,10 @@ def lib() -> C.CDLL:
 def check(status: int) -> None:
     if status != PS_OK:
         msg = lib().ps_last_error().decode() or lib().ps_status_string(status).decode()
-        if status == PS_E_CONFIG:
+        if status in (PS_E_CONFIG, PS_E_FORMAT):
             raise ValueError(msg)
+        if status == PS_E_IO:
+            raise OSError(msg)
         raise DeviceError(status, msg)
 
 
@@ -659,3 +691,280 @@ class Engine:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- feature catalog (catalog.rs)
+
+DEFAULT_CATALOG_SHORTLIST = 16   # catalog.rs:21
+DEFAULT_CATALOG_TOP_K = 5        # catalog.rs:24
+
+CATALOG_HIT_DTYPE = np.dtype([("entry", "<u4"), ("coarse_votes", "<u4"), ("raw_match_count", "<u8"),
+                              ("match_count", "<u8"), ("inlier_ratio", "<f4"), ("mean_dx", "<f4"),
+                              ("mean_dy", "<f4"), ("median_dx", "<f4"), ("median_dy", "<f4"),
+                              ("mean_scale_ratio", "<f4"), ("mean_rotation_deg", "<f4"), ("has_transform", "<u4"),
+                              ("has_homography", "<u4"), ("transform", "<f4", (6,)), ("homography", "<f4", (9,))],
+                             align=True)
+
+
+@dataclasses.dataclass
+class TrackingSummary:
+    """pipeline.rs:1137-1153"""
+    raw_match_count: int
+    match_count: int
+    inlier_ratio: float
+    mean_displacement: tuple
+    median_displacement: tuple
+    mean_scale_ratio: float
+    mean_rotation_deg: float
+
+
+@dataclasses.dataclass
+class SimilarityTransform2D:
+    """pipeline.rs:1096-1109"""
+    a: float
+    b: float
+    tx: float
+    ty: float
+    scale: float
+    rotation_deg: float
+
+
+@dataclasses.dataclass
+class CatalogEntrySummary:
+    """catalog.rs:47-57"""
+    id: str
+    width: int
+    height: int
+    feature_count: int
+
+
+@dataclasses.dataclass
+class CatalogSearchHit:
+    """catalog.rs:158-177 (homography as a row-major 3x3 array)"""
+    id: str
+    width: int
+    height: int
+    feature_count: int
+    coarse_votes: int
+    tracking: TrackingSummary
+    transform: Optional[SimilarityTransform2D]
+    homography: Optional[np.ndarray]
+
+
+class FeatureCatalog:
+    """catalog.rs:179-197 FeatureCatalog, resident in device memory (libps_sift's ps_catalog).
+
+    Mutating calls (upsert_entry, remove) need exclusive access; searches may run concurrently."""
+
+    def __init__(self, octaves: int = 3, scales: int = 3, threshold: float = DEFAULT_CONTRAST_THRESHOLD,
+                 descriptor_kind: int = DescriptorKind.Sift, _handle=None):
+        if _handle is None:
+            _handle = C.c_void_p()
+            check(lib().ps_catalog_create(octaves, scales, threshold, descriptor_kind, C.byref(_handle)))
+        self.h = _handle
+
+    @staticmethod
+    def new(octaves: int, scales: int, threshold: float) -> "FeatureCatalog":
+        return FeatureCatalog(octaves, scales, threshold)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.ps_catalog_destroy(h)
+            self.h = None
+
+    # -- persistence (catalog.rs:361-570)
+    @staticmethod
+    def from_binary_bytes(data: bytes) -> "FeatureCatalog":
+        h = C.c_void_p()
+        check(lib().ps_catalog_load_binary(data, len(data), C.byref(h)))
+        return FeatureCatalog(_handle=h)
+
+    @staticmethod
+    def binary_bytes(entries, octaves: int = 3, scales: int = 3, threshold: float = DEFAULT_CONTRAST_THRESHOLD,
+                     descriptor_kind: int = DescriptorKind.Sift) -> bytes:
+        """PSIFTCAT bytes (catalog.rs:400-445) for entries (id, width, height, Features): u8
+        descriptors (pipeline.rs:1024-1030) and signatures quantised on the device from the f32
+        descriptors -- a built catalog as save_binary writes it."""
+        rec_t = np.dtype([("f", "<f4", (5,)), ("sig", "<u8", (2,)), ("q", "u1", (128,))])
+        assert rec_t.itemsize == 164
+        parts = [b"PSIFTCAT", np.array([1, octaves, scales], "<u4").tobytes(), np.float32(threshold).tobytes(),
+                 bytes([descriptor_kind]), np.uint64(len(entries)).tobytes()]
+        for ident, w, h, feats in entries:
+            b = ident.encode("utf-8")
+            hdr = np.ascontiguousarray(feats.header, FEAT_DTYPE)
+            d = np.ascontiguousarray(feats.descriptors, np.float32).reshape(-1, 128)
+            rec = np.zeros(len(hdr), rec_t)
+            rec["f"] = hdr.view("<f4").reshape(-1, 5)
+            if len(hdr):
+                rec["sig"] = quantize_descriptor_u128(d)
+                x = (d * np.float32(512)).astype(np.float32).astype(np.float64)   # f32 product, then exact
+                r = np.sign(x) * np.floor(np.abs(x) + 0.5)                         # half away from zero
+                rec["q"] = np.where(np.isnan(r), 0, np.clip(r, 0, 255)).astype(np.uint8)
+            parts += [np.uint32(len(b)).tobytes(), b, np.array([w, h], "<u4").tobytes(),
+                      np.uint64(len(hdr)).tobytes(), rec.tobytes()]
+        return b"".join(parts)
+
+    @staticmethod
+    def load_binary(path: str) -> "FeatureCatalog":
+        h = C.c_void_p()
+        check(lib().ps_catalog_load_file(os.fsencode(path), C.byref(h)))
+        return FeatureCatalog(_handle=h)
+
+    @staticmethod
+    def load_json(path: str) -> "FeatureCatalog":
+        """catalog.rs:386-395: serde JSON; signatures recomputed from the f32 descriptors."""
+        import json
+        with open(path, "r", encoding="utf-8") as f:
+            raw = json.load(f)
+        kind = raw.get("descriptor_kind", "Sift")
+        cat = FeatureCatalog(int(raw["octaves"]), int(raw["scales"]), float(raw["threshold"]),
+                             DescriptorKind.RootSift if kind == "RootSift" else DescriptorKind.Sift)
+        for e in raw.get("entries", []):
+            feats = e.get("features", [])
+            hdr = np.zeros(len(feats), FEAT_DTYPE)
+            desc = np.zeros((len(feats), 128), np.float32)
+            for i, ft in enumerate(feats):
+                hdr[i] = (ft["x"], ft["y"], ft["scale"], ft["response"], ft["orientation"])
+                d = np.asarray(ft["descriptor"], np.float32)
+                desc[i, :len(d)] = d[:128]
+            cat.upsert_entry(e["id"], int(e["width"]), int(e["height"]), Features(hdr, desc))
+        return cat
+
+    @staticmethod
+    def load_auto(path: str) -> "FeatureCatalog":
+        """catalog.rs:555-570: PSIFTCAT by magic bytes, else JSON."""
+        with open(path, "rb") as f:
+            head = f.read(8)
+        if head == b"PSIFTCAT":
+            return FeatureCatalog.load_binary(path)
+        return FeatureCatalog.load_json(path)
+
+    def save_binary(self, path: str) -> None:
+        check(lib().ps_catalog_save_binary(self.h, os.fsencode(path)))
+
+    def save_json(self, path: str, pretty: bool = False) -> None:
+        """catalog.rs:374-383 (written through a temporary file and a rename)."""
+        import json
+        info = self._info()
+        out = {"octaves": info.octaves, "scales": info.scales, "threshold": float(np.float32(info.contrast_threshold)),
+               "descriptor_kind": "RootSift" if info.descriptor_kind == DescriptorKind.RootSift else "Sift",
+               "entries": []}
+        for i in range(len(self)):
+            s = self.summary(i)
+            hdr, desc, _ = self.entry_features(i)
+            out["entries"].append({"id": s.id, "width": s.width, "height": s.height, "features": [
+                {"x": float(h["x"]), "y": float(h["y"]), "scale": float(h["scale"]),
+                 "response": float(h["response"]), "orientation": float(h["orientation"]),
+                 "descriptor": [float(v) for v in d]} for h, d in zip(hdr, desc)]})
+        parent = os.path.dirname(path)
+        if parent:
+            os.makedirs(parent, exist_ok=True)
+        tmp = os.path.join(parent, f"{os.path.basename(path) or 'catalog'}.tmp.{os.getpid()}")
+        with open(tmp, "w", encoding="utf-8") as f:
+            json.dump(out, f, indent=2 if pretty else None)
+            f.write("\n")
+        os.replace(tmp, path)
+
+    def save_auto(self, path: str, pretty: bool = False) -> None:
+        """catalog.rs:543-552: JSON when the extension is .json (any case), else binary."""
+        if path.lower().endswith(".json"):
+            self.save_json(path, pretty)
+        else:
+            self.save_binary(path)
+
+    # -- contents
+    def _info(self) -> _CatalogInfoC:
+        info = _CatalogInfoC()
+        check(lib().ps_catalog_get_info(self.h, C.byref(info)))
+        return info
+
+    def __len__(self) -> int:
+        return int(self._info().n_entries)
+
+    @property
+    def n_features(self) -> int:
+        return int(self._info().n_features)
+
+    def summary(self, idx: int) -> CatalogEntrySummary:
+        e = _CatalogEntryC()
+        check(lib().ps_catalog_entry(self.h, idx, C.byref(e)))
+        id_ = C.string_at(e.id, e.id_len).decode("utf-8") if e.id_len else ""
+        return CatalogEntrySummary(id_, int(e.width), int(e.height), int(e.feature_count))
+
+    def list_summaries(self) -> List[CatalogEntrySummary]:
+        """catalog.rs:347-357"""
+        return [self.summary(i) for i in range(len(self))]
+
+    def entry_features(self, idx: int):
+        """(FEAT_DTYPE records, [n, 128] f32 descriptors, [n, 2] u64 signatures) as stored."""
+        n = self.summary(idx).feature_count
+        hdr = np.zeros(n, FEAT_DTYPE)
+        desc = np.zeros((n, 128), np.float32)
+        sig = np.zeros((n, 2), np.uint64)
+        check(lib().ps_catalog_entry_features(self.h, idx, hdr.ctypes.data, desc.ctypes.data, sig.ctypes.data, n))
+        return hdr, desc, sig
+
+    def upsert_entry(self, id: str, width: int, height: int, features: Features) -> None:
+        """catalog.rs:309-328"""
+        b = id.encode("utf-8")
+        hdr = np.ascontiguousarray(features.header, FEAT_DTYPE)
+        desc = np.ascontiguousarray(features.descriptors, np.float32).reshape(-1, 128)
+        check(lib().ps_catalog_upsert(self.h, b, len(b), width, height, hdr.ctypes.data, desc.ctypes.data,
+                                      len(hdr)))
+
+    def remove(self, id: str) -> bool:
+        """catalog.rs:331-339"""
+        b = id.encode("utf-8")
+        r = C.c_int32(0)
+        check(lib().ps_catalog_remove(self.h, b, len(b), C.byref(r)))
+        return bool(r.value)
+
+    # -- search (catalog.rs:576-702)
+    def coarse_votes(self, query_descriptors: np.ndarray) -> np.ndarray:
+        """The vote of every entry (catalog.rs:595-633), as u32 [n_entries]."""
+        q = np.ascontiguousarray(query_descriptors, np.float32).reshape(-1, 128)
+        out = np.zeros(max(len(self), 1), np.uint32)
+        check(lib().ps_catalog_coarse_votes(self.h, q.ctypes.data, len(q), out.ctypes.data))
+        return out[:len(self)]
+
+    def search_raw(self, query: Features, top_k: int = DEFAULT_CATALOG_TOP_K,
+                   shortlist_k: int = DEFAULT_CATALOG_SHORTLIST, ratio_threshold: float = DEFAULT_MATCH_RATIO,
+                   ransac_threshold: float = DEFAULT_RANSAC_REPROJ_THRESHOLD) -> np.ndarray:
+        """search_features as CATALOG_HIT_DTYPE records (entry = catalog index)."""
+        hdr = np.ascontiguousarray(query.header, FEAT_DTYPE)
+        desc = np.ascontiguousarray(query.descriptors, np.float32).reshape(-1, 128)
+        cap = max(min(top_k, len(self)), 1)
+        out = np.zeros(cap, CATALOG_HIT_DTYPE)
+        n = C.c_size_t(0)
+        check(lib().ps_catalog_search(self.h, hdr.ctypes.data, desc.ctypes.data, len(hdr), top_k, shortlist_k,
+                                      ratio_threshold, ransac_threshold, out.ctypes.data, cap, C.byref(n)))
+        return out[:n.value].copy()
+
+    def search_raw_device(self, engine: Engine, d_q: int, d_q_desc: int, nq: int, top_k: int = DEFAULT_CATALOG_TOP_K,
+                          shortlist_k: int = DEFAULT_CATALOG_SHORTLIST, ratio_threshold: float = DEFAULT_MATCH_RATIO,
+                          ransac_threshold: float = DEFAULT_RANSAC_REPROJ_THRESHOLD) -> np.ndarray:
+        """search_raw with device-resident query features on an engine's stream."""
+        cap = max(min(top_k, len(self)), 1)
+        out = np.zeros(cap, CATALOG_HIT_DTYPE)
+        n = C.c_size_t(0)
+        check(lib().ps_engine_catalog_search(engine.h, self.h, d_q, d_q_desc, nq, top_k, shortlist_k,
+                                             ratio_threshold, ransac_threshold, out.ctypes.data, cap, C.byref(n)))
+        return out[:n.value].copy()
+
+    def search_features(self, query: Features, top_k: int = DEFAULT_CATALOG_TOP_K,
+                        shortlist_k: int = DEFAULT_CATALOG_SHORTLIST, ratio_threshold: float = DEFAULT_MATCH_RATIO,
+                        ransac_threshold: float = DEFAULT_RANSAC_REPROJ_THRESHOLD) -> List[CatalogSearchHit]:
+        """catalog.rs:576-702"""
+        hits = []
+        for r in self.search_raw(query, top_k, shortlist_k, ratio_threshold, ransac_threshold):
+            s = self.summary(int(r["entry"]))
+            tr = TrackingSummary(int(r["raw_match_count"]), int(r["match_count"]), float(r["inlier_ratio"]),
+                                 (float(r["mean_dx"]), float(r["mean_dy"])),
+                                 (float(r["median_dx"]), float(r["median_dy"])), float(r["mean_scale_ratio"]),
+                                 float(r["mean_rotation_deg"]))
+            tf = SimilarityTransform2D(*[float(v) for v in r["transform"]]) if r["has_transform"] else None
+            hm = r["homography"].reshape(3, 3).copy() if r["has_homography"] else None
+            hits.append(CatalogSearchHit(s.id, s.width, s.height, s.feature_count, int(r["coarse_votes"]), tr, tf,
+                                         hm))
+        return hits

@@ -142,3 +142,51 @@ def test_compute_descriptors_trivial_inputs_need_no_device():
     assert n.value == 0
     assert L.ps_compute_descriptors(img.ctypes.data, 16, 16, 16, kp.ctypes.data, 1, 2, None, None, 0,
                                     C.byref(n)) == ps.PS_E_INVALID_ARG
+
+
+def test_catalog_layouts():
+    assert ps.CATALOG_HIT_DTYPE.itemsize == 120
+    assert C.sizeof(ps._CatalogInfoC) == 32
+    assert C.sizeof(ps._CatalogEntryC) == 32
+    assert ps.CATALOG_HIT_DTYPE.fields["transform"][1] == 60
+    assert ps.CATALOG_HIT_DTYPE.fields["homography"][1] == 84
+
+
+@pytest.mark.parametrize("data,msg", [
+    (b"PSIFT", "Unexpected EOF while reading binary catalog"),
+    (b"NOTACAT!" + b"\0" * 40, "Invalid binary catalog magic header"),
+    (b"PSIFTCAT" + bytes([2, 0, 0, 0]) + b"\0" * 25, "Unsupported binary catalog version 2 (expected 1)"),
+    (b"PSIFTCAT" + bytes([1, 0, 0, 0]) + b"\0" * 12 + b"\x05" + b"\0" * 8, "Unknown descriptor kind 5 in binary catalog"),
+    (b"PSIFTCAT" + bytes([1, 0, 0, 0]) + b"\0" * 13 + bytes([1] + [0] * 7) + bytes([3, 0, 0, 0]) + b"ab",
+     "Truncated entry id in binary catalog"),
+    (b"PSIFTCAT" + bytes([1, 0, 0, 0]) + b"\0" * 13 + bytes([1] + [0] * 7) + bytes([1, 0, 0, 0]) + b"\xc0" + b"\0" * 16,
+     "Invalid UTF-8 entry id in binary catalog"),
+    (b"PSIFTCAT" + bytes([1, 0, 0, 0]) + b"\0" * 13 + bytes([1] + [0] * 7) + bytes([1, 0, 0, 0]) + b"a" + b"\0" * 8
+     + bytes([1] + [0] * 7) + b"\0" * 163, "Unexpected EOF while reading binary catalog"),
+])
+def test_catalog_format_errors_need_no_device(data, msg):
+    """catalog.rs:447-540: a malformed PSIFTCAT file is rejected with the reference's message
+    before any device work."""
+    with pytest.raises(ValueError) as ei:
+        ps.FeatureCatalog.from_binary_bytes(data)
+    assert str(ei.value) == msg
+
+
+def test_catalog_file_errors():
+    with pytest.raises(OSError):
+        ps.FeatureCatalog.load_binary("/nonexistent/dir/catalog.psift")
+    L = ps.lib()
+    for s in (ps.PS_E_IO, ps.PS_E_FORMAT):
+        assert L.ps_status_string(s).decode() not in ("", "unknown status")
+
+
+def test_catalog_needs_a_device():
+    """A well-formed catalog still needs the GPU: no host copy is searched instead."""
+    if has_gpu():
+        pytest.skip("a GPU is present")
+    empty = b"PSIFTCAT" + bytes([1, 0, 0, 0]) + b"\0" * 21
+    with pytest.raises(ps.DeviceError) as ei:
+        ps.FeatureCatalog.from_binary_bytes(empty)
+    assert ei.value.status == ps.PS_E_NO_DEVICE
+    with pytest.raises(ps.DeviceError):
+        ps.FeatureCatalog(3, 3, 0.015)
