@@ -152,57 +152,77 @@ __device__ bool fit_procrustes(const ps_feature_match *m, const uint32_t *idx, u
 }
 
 // pipeline.rs:2276-2353: cyclic Jacobi on the symmetric 9x9 AtA; eigenvector of the
-// smallest eigenvalue (first index on ties).
+// smallest eigenvalue (first index on ties). Every loop has a constant trip count and is
+// unrolled, so all indices are compile-time constants and the matrix (diagonal + upper
+// triangle: the reference writes a[p][r] and a[r][p] with the same value) and the
+// eigenvectors stay in registers instead of per-lane scratch memory. Same operations in
+// the same order as the reference.
+__device__ constexpr int tri9(int p, int q) { return p * 9 - p * (p + 1) / 2 + (q - p - 1); }   // p < q
+
 __device__ bool smallest_eigvec9(double (&a)[9][9], double (&h)[9]) {
-    double v[9][9];
-    for (int i = 0; i < 9; i++)
+    double d[9], u[36], v[9][9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        d[i] = a[i][i];
+#pragma unroll
         for (int j = 0; j < 9; j++) v[i][j] = i == j ? 1.0 : 0.0;
+#pragma unroll
+        for (int j = i + 1; j < 9; j++) u[tri9(i, j)] = a[i][j];
+    }
     for (int sweep = 0; sweep < 20; sweep++) {
         double mo = 0.0;
-        for (int p = 0; p < 9; p++)
-            for (int q = p + 1; q < 9; q++) {
-                double av = fabs(a[p][q]);
-                if (av > mo) mo = av;
-            }
+#pragma unroll
+        for (int k = 0; k < 36; k++) {
+            const double av = fabs(u[k]);
+            if (av > mo) mo = av;
+        }
         if (mo < 1e-13) break;
+#pragma unroll
         for (int p = 0; p < 8; p++)
+#pragma unroll
             for (int q = p + 1; q < 9; q++) {
-                double apq = a[p][q];
+                const double apq = u[tri9(p, q)];
                 if (fabs(apq) < 1e-15) continue;
-                double app = a[p][p], aqq = a[q][q];
-                double tau = (aqq - app) / (2.0 * apq);
-                double t = tau >= 0.0 ? 1.0 / (tau + sqrt(1.0 + tau * tau)) : -1.0 / (-tau + sqrt(1.0 + tau * tau));
-                double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
-                a[p][p] = app - t * apq;
-                a[q][q] = aqq + t * apq;
-                a[p][q] = 0.0;
-                a[q][p] = 0.0;
+                const double app = d[p], aqq = d[q];
+                const double tau = (aqq - app) / (2.0 * apq);
+                const double t = tau >= 0.0 ? 1.0 / (tau + sqrt(1.0 + tau * tau))
+                                            : -1.0 / (-tau + sqrt(1.0 + tau * tau));
+                const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+                d[p] = app - t * apq;
+                d[q] = aqq + t * apq;
+                u[tri9(p, q)] = 0.0;
+#pragma unroll
                 for (int r = 0; r < 9; r++) {
                     if (r != p && r != q) {
-                        double arp = a[r][p], arq = a[r][q];
-                        a[r][p] = c * arp - s * arq;
-                        a[p][r] = a[r][p];
-                        a[r][q] = s * arp + c * arq;
-                        a[q][r] = a[r][q];
+                        const int ip = r < p ? tri9(r, p) : tri9(p, r), iq = r < q ? tri9(r, q) : tri9(q, r);
+                        const double arp = u[ip], arq = u[iq];
+                        u[ip] = c * arp - s * arq;
+                        u[iq] = s * arp + c * arq;
                     }
-                    double vrp = v[r][p], vrq = v[r][q];
+                    const double vrp = v[r][p], vrq = v[r][q];
                     v[r][p] = c * vrp - s * vrq;
                     v[r][q] = s * vrp + c * vrq;
                 }
             }
     }
     int mi = 0;
-    double me = a[0][0];
+    double me = d[0];
+#pragma unroll
     for (int i = 1; i < 9; i++)
-        if (a[i][i] < me) {
-            me = a[i][i];
+        if (d[i] < me) {
+            me = d[i];
             mi = i;
         }
+    bool ok = true;
+#pragma unroll
     for (int r = 0; r < 9; r++) {
-        h[r] = v[r][mi];
-        if (!isfinite(h[r])) return false;
+        double x = v[r][0];
+#pragma unroll
+        for (int i = 1; i < 9; i++) x = mi == i ? v[r][i] : x;   // select, no dynamic index
+        h[r] = x;
+        ok = ok && isfinite(x);
     }
-    return true;
+    return ok;
 }
 
 // pipeline.rs:2357-2404
