@@ -10,8 +10,12 @@
 // reference's infallible signatures become a panic in the Rust wrapper).
 #pragma once
 
+#include <algorithm>
+#include <array>
 #include <cstddef>
 #include <cstdint>
+#include <memory>
+#include <optional>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -41,7 +45,7 @@ inline void check(ps_status s) {
     if (s == PS_OK) return;
     std::string msg = ps_last_error();
     if (msg.empty()) msg = ps_status_string(s);
-    if (s == PS_E_CONFIG) throw ConfigError(msg);
+    if (s == PS_E_CONFIG || s == PS_E_FORMAT) throw ConfigError(msg);
     throw DeviceError(s, msg);
 }
 
@@ -223,6 +227,131 @@ private:
         }
     }
     MatchOptions opt_;
+};
+
+// ---- feature catalog (src/catalog.rs) over ps_catalog_* --------------------------------
+
+constexpr std::size_t DEFAULT_CATALOG_SHORTLIST = 16;   // catalog.rs:21
+constexpr std::size_t DEFAULT_CATALOG_TOP_K = 5;        // catalog.rs:24
+constexpr float DEFAULT_RANSAC_REPROJ_THRESHOLD = 3.0f; // pipeline.rs:1050
+
+struct TrackingSummary {       // pipeline.rs:1137-1153
+    std::size_t raw_match_count, match_count;
+    float inlier_ratio;
+    std::pair<float, float> mean_displacement, median_displacement;
+    float mean_scale_ratio, mean_rotation_deg;
+};
+struct SimilarityTransform2D { float a, b, tx, ty, scale, rotation_deg; };   // pipeline.rs:1096-1109
+struct CatalogEntrySummary {   // catalog.rs:47-57
+    std::string id;
+    std::size_t width, height, feature_count;
+};
+struct CatalogSearchHit {      // catalog.rs:158-177
+    std::string id;
+    std::size_t width, height, feature_count, coarse_votes;
+    TrackingSummary tracking;
+    std::optional<SimilarityTransform2D> transform;
+    std::optional<std::array<std::array<float, 3>, 3>> homography;
+};
+
+// catalog.rs:179-197, resident in device memory. Mutation needs exclusive access.
+class FeatureCatalog {
+public:
+    FeatureCatalog(std::size_t octaves, std::size_t scales, float threshold) {   // catalog.rs:272-282
+        ps_catalog *c = nullptr;
+        check(ps_catalog_create((uint32_t)octaves, (uint32_t)scales, threshold, PS_DESCRIPTOR_SIFT, &c));
+        h_.reset(c);
+    }
+    static FeatureCatalog load_binary(const std::string &path) {              // catalog.rs:447
+        ps_catalog *c = nullptr;
+        check(ps_catalog_load_file(path.c_str(), &c));
+        return FeatureCatalog(c);
+    }
+    static FeatureCatalog from_binary_bytes(const std::vector<uint8_t> &bytes) {
+        ps_catalog *c = nullptr;
+        check(ps_catalog_load_binary(bytes.data(), bytes.size(), &c));
+        return FeatureCatalog(c);
+    }
+    void save_binary(const std::string &path) const { check(ps_catalog_save_binary(h_.get(), path.c_str())); }
+
+    std::size_t size() const { return info().n_entries; }
+    std::vector<CatalogEntrySummary> list_summaries() const {                    // catalog.rs:347-357
+        std::vector<CatalogEntrySummary> out;
+        for (std::size_t i = 0; i < size(); i++) out.push_back(summary(i));
+        return out;
+    }
+    void upsert_entry(const std::string &id, std::size_t width, std::size_t height,
+                      const std::vector<SiftFeature> &features) {                 // catalog.rs:309-328
+        std::vector<ps_feature> h;
+        std::vector<float> d;
+        pack(features, h, d);
+        check(ps_catalog_upsert(h_.get(), id.data(), id.size(), (uint32_t)width, (uint32_t)height, h.data(),
+                                d.data(), h.size()));
+    }
+    bool remove(const std::string &id) {                                          // catalog.rs:331-339
+        int32_t r = 0;
+        check(ps_catalog_remove(h_.get(), id.data(), id.size(), &r));
+        return r != 0;
+    }
+    std::vector<CatalogSearchHit> search_features(const std::vector<SiftFeature> &query,
+                                                  std::size_t top_k = DEFAULT_CATALOG_TOP_K,
+                                                  std::size_t shortlist_k = DEFAULT_CATALOG_SHORTLIST,
+                                                  float ratio_threshold = DEFAULT_MATCH_RATIO,
+                                                  float ransac_threshold = DEFAULT_RANSAC_REPROJ_THRESHOLD) const {
+        std::vector<ps_feature> h;
+        std::vector<float> d;
+        pack(query, h, d);
+        std::vector<ps_catalog_hit> buf(std::max<std::size_t>(std::min(top_k, size()), 1));
+        std::size_t n = 0;
+        check(ps_catalog_search(h_.get(), h.data(), d.data(), h.size(), top_k, shortlist_k, ratio_threshold,
+                                ransac_threshold, buf.data(), buf.size(), &n));
+        std::vector<CatalogSearchHit> out;
+        for (std::size_t i = 0; i < n; i++) {
+            const ps_catalog_hit &r = buf[i];
+            const CatalogEntrySummary s = summary(r.entry);
+            CatalogSearchHit hit{s.id, s.width, s.height, s.feature_count, r.coarse_votes,
+                                 {(std::size_t)r.raw_match_count, (std::size_t)r.match_count, r.inlier_ratio,
+                                  {r.mean_dx, r.mean_dy}, {r.median_dx, r.median_dy}, r.mean_scale_ratio,
+                                  r.mean_rotation_deg},
+                                 std::nullopt, std::nullopt};
+            if (r.has_transform)
+                hit.transform = SimilarityTransform2D{r.transform[0], r.transform[1], r.transform[2],
+                                                      r.transform[3], r.transform[4], r.transform[5]};
+            if (r.has_homography) {
+                std::array<std::array<float, 3>, 3> m{};
+                for (int k = 0; k < 9; k++) m[k / 3][k % 3] = r.homography[k];
+                hit.homography = m;
+            }
+            out.push_back(std::move(hit));
+        }
+        return out;
+    }
+
+private:
+    struct Del {
+        void operator()(ps_catalog *c) const { ps_catalog_destroy(c); }
+    };
+    explicit FeatureCatalog(ps_catalog *c) : h_(c) {}
+    ps_catalog_info info() const {
+        ps_catalog_info i;
+        check(ps_catalog_get_info(h_.get(), &i));
+        return i;
+    }
+    CatalogEntrySummary summary(std::size_t idx) const {
+        ps_catalog_entry_info e;
+        check(ps_catalog_entry(h_.get(), idx, &e));
+        return {std::string(e.id, e.id_len), e.width, e.height, (std::size_t)e.feature_count};
+    }
+    static void pack(const std::vector<SiftFeature> &f, std::vector<ps_feature> &h, std::vector<float> &d) {
+        h.resize(f.size());
+        d.resize(f.size() * SIFT_DESCRIPTOR_LEN);
+        for (std::size_t i = 0; i < f.size(); i++) {
+            if (f[i].descriptor.size() != SIFT_DESCRIPTOR_LEN) throw ConfigError("descriptor length must be 128");
+            h[i] = {f[i].x, f[i].y, f[i].scale, f[i].response, f[i].orientation};
+            std::copy(f[i].descriptor.begin(), f[i].descriptor.end(), d.begin() + i * SIFT_DESCRIPTOR_LEN);
+        }
+    }
+    std::unique_ptr<ps_catalog, Del> h_;
 };
 
 }  // namespace pure_sift

@@ -3,6 +3,7 @@
 // and mutual).
 // Reads a raw u8 image (h, w from argv) and prints a one-line summary the GPU test checks.
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <vector>
 
@@ -48,7 +49,22 @@ int main(int argc, char **argv) {
     if (fk.size() < kps.size()) bad++;
     auto m = pure_sift::Matcher::create({0.8f, false}).match_(fq, ft);
     auto mm = pure_sift::Matcher::create({0.8f, true}).match_(fq, ft);
-    std::printf("keypoints=%zu features=%zu,%zu described=%zu matches=%zu mutual=%zu bad=%d\n", kps.size(), fq.size(),
-                ft.size(), fk.size(), m.size(), mm.size(), bad);
+    // feature catalog: the target under two ids, then search with the query; save / load round trip
+    pure_sift::FeatureCatalog cat(4, 3, 0.015f);
+    cat.upsert_entry("decoy", w, h, std::vector<pure_sift::SiftFeature>(ft.begin(), ft.begin() + ft.size() / 10));
+    cat.upsert_entry("target", w, h, ft);
+    auto hits = cat.search_features(fq);
+    if (hits.empty() || hits[0].id != "target" || hits[0].tracking.match_count == 0) bad++;
+    const char *dir = std::getenv("MIRROR_DEMO_TMP");
+    if (dir) {
+        const std::string path = std::string(dir) + "/demo.psift";
+        cat.save_binary(path);
+        auto back = pure_sift::FeatureCatalog::load_binary(path);
+        auto names = back.list_summaries();
+        if (names.size() != 2 || names[1].id != "target" || names[1].feature_count != ft.size()) bad++;
+    }
+    if (!cat.remove("decoy") || cat.remove("decoy") || cat.size() != 1) bad++;
+    std::printf("keypoints=%zu features=%zu,%zu described=%zu matches=%zu mutual=%zu catalog_hits=%zu bad=%d\n",
+                kps.size(), fq.size(), ft.size(), fk.size(), m.size(), mm.size(), hits.size(), bad);
     return bad ? 1 : 0;
 }

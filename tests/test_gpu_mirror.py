@@ -1,7 +1,8 @@
 """The C++ host mirror (pure-sift_amd/pure_sift.hpp) end to end: tests/cpp/mirror_demo (built
 with the library) runs validation, detect_keypoints, detect_and_compute (f32 and u8),
-compute_descriptors and match_ (plain and mutual) through the C ABI in its own process; its
-counts must equal the Python mirror's on the same images."""
+compute_descriptors, match_ (plain and mutual) and the feature catalog (upsert, search, save /
+load, remove) through the C ABI in its own process; its counts must equal the Python mirror's on
+the same images."""
 import os
 import re
 import subprocess
@@ -23,7 +24,8 @@ def test_cpp_mirror_demo(gpu, tmp_path):
     qa, ta = tmp_path / "q.u8", tmp_path / "t.u8"
     qa.write_bytes(np.ascontiguousarray(a).tobytes())
     ta.write_bytes(np.ascontiguousarray(b).tobytes())
-    r = subprocess.run([DEMO, str(qa), str(ta), "480", "640"], capture_output=True, text=True, timeout=120)
+    env = dict(os.environ, MIRROR_DEMO_TMP=str(tmp_path))
+    r = subprocess.run([DEMO, str(qa), str(ta), "480", "640"], capture_output=True, text=True, timeout=120, env=env)
     assert r.returncode == 0, r.stdout + r.stderr
     got = dict((k, v) for k, v in re.findall(r"(\w+)=([\d,]+)", r.stdout))
     assert got["bad"] == "0"
@@ -36,3 +38,5 @@ def test_cpp_mirror_demo(gpu, tmp_path):
     assert int(got["described"]) == len(sift.compute_descriptors(img, kps))
     assert int(got["matches"]) == len(gpu.Matcher(gpu.MatchOptions(0.8)).match_(fa, fb))
     assert int(got["mutual"]) == len(gpu.Matcher(gpu.MatchOptions(0.8, True)).match_(fa, fb))
+    assert int(got["catalog_hits"]) >= 1
+    assert (tmp_path / "demo.psift").exists()
