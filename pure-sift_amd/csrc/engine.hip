@@ -481,8 +481,11 @@ ps_status enqueue_nn(ps_engine *e, const float *qd, const uint32_t *nq, size_t n
         PS_HIP(ps::launch_nn_merge(as<ps::NNPart>(scratch), ns, nq, nqm, bidx, bd, sd, e->stream));
         return PS_OK;
     }
+    void *const before = scratch.p;
     PS_TRY(ensure(scratch, ps::nn_mfma_scratch_bytes(nqm, ntm)));
     if (++e->nn_gen == 0) e->nn_gen = 1;
+    // fresh memory: clear the two norm-bound words at its head (later calls clear them in turn)
+    if (scratch.p != before) PS_HIP(ps::launch_fill_u32(static_cast<uint32_t *>(scratch.p), 0u, 4, e->stream));
     PS_HIP(ps::launch_nn_mfma(qd, nq, nqm, td, nt, ntm, scratch.p, bidx, bd, sd, e->nn_gen, e->stream));
     return PS_OK;
 }

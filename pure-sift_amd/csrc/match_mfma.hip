@@ -73,9 +73,11 @@ __device__ __forceinline__ uint32_t bf16_bits_rne(float x) {
 // ranks and bounds) and the row's bf16 split (128 words: hi pairs, then lo pairs; elements 2j
 // and 2j+1 in word j). Workgroups [0, tblocks) take target rows, the rest query rows, 64 rows
 // each (each wave 16, one at a time); rows in [n, rows) get norm +inf, so they never rank,
-// and a zero split. The largest target norm goes to *tmax as (gen << 32 | ordered bits) by
-// one 64-bit atomicMax per workgroup: values left by earlier calls carry a smaller
-// generation, so the word needs no clearing.
+// and a zero split. The largest target norm goes to tmax[gen & 1] as (gen << 32 | ordered
+// bits) by one 64-bit atomicMax per workgroup, and workgroup 0 zeroes tmax[(gen + 1) & 1],
+// the word the next call (next generation) will take: stream order puts that store after
+// this generation's readers of the other word. A word of another generation (fresh memory
+// before its first clearing, a wrapped counter) reads as +inf, which is safe.
 struct PrepSide {
     const float *x;
     const uint32_t *n_p;
@@ -113,12 +115,13 @@ __global__ __launch_bounds__(256) void nn_prep_kernel(PrepSide t, PrepSide q, ui
             sd.split[(size_t)r * 128 + 64 + lane] = bf16_bits_rne(rx) | (bf16_bits_rne(ry) << 16);
         }
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) tmax[(gen + 1) & 1] = 0ull;
     if (!tside) return;
     if (lane == 0) s_max[w] = wmax;
     __syncthreads();
     if (threadIdx.x == 0) {
         const float m = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
-        atomicMax(tmax, ((unsigned long long)gen << 32) | __float_as_uint(m));
+        atomicMax(tmax + (gen & 1), ((unsigned long long)gen << 32) | __float_as_uint(m));
     }
 }
 
@@ -330,7 +333,7 @@ __global__ __launch_bounds__(256) void nn_rescore_kernel(const Cand4 *part, int 
     if (ok && nt > 8) {
         // the largest target norm of this call (a word of another generation: +inf, which
         // flags the query -- always safe)
-        const unsigned long long tw = *tmax;
+        const unsigned long long tw = tmax[gen & 1];
         const float qq = qn[q], tt = (uint32_t)(tw >> 32) == gen ? __uint_as_float((uint32_t)tw) : INFINITY;
         // E bounds |value + |q|^2 - exact| for every pair: twice the pass-1 dot-product error
         // (< 7e-5 |q||t|) plus the f32 roundings of |q|^2, |t|^2 and the sequential exact
@@ -400,8 +403,10 @@ size_t nn_mfma_scratch_bytes(size_t nq_max, size_t nt_max) {
            (nq_max + tn_rows(nt_max)) * 128 * sizeof(uint32_t) + 512;
 }
 
-// Scratch layout: [part: ns*nq_max*2 Cand4][qn: nq_max][tn: padded nt_max]
-// [q split: nq_max*128 words][t split: padded nt_max*128 words][tmax: u64 (8-aligned)].
+// Scratch layout: [tmax: 2 x u64, padded to 256 B][part: ns*nq_max*2 Cand4][qn: nq_max]
+// [tn: padded nt_max][q split: nq_max*128 words][t split: padded nt_max*128 words]. The
+// tmax words sit at a fixed offset so that calls of any size find the previous call's
+// clearing there.
 // Three launches: prep, MFMA distances, rescore (+ exact scans of flagged queries).
 hipError_t launch_nn_mfma(const float *qd, const uint32_t *nq, size_t nq_max, const float *td, const uint32_t *nt,
                           size_t nt_max, void *scratch, uint32_t *best_idx, float *best, float *second, uint32_t gen,
@@ -409,6 +414,8 @@ hipError_t launch_nn_mfma(const float *qd, const uint32_t *nq, size_t nq_max, co
     if (nq_max == 0) return hipSuccess;
     const int ns = nn_mfma_splits(nq_max, nt_max);
     char *p = static_cast<char *>(scratch);
+    unsigned long long *tmax = reinterpret_cast<unsigned long long *>(p);
+    p += 256;
     Cand4 *part = reinterpret_cast<Cand4 *>(p);
     p += (size_t)ns * nq_max * 2 * sizeof(Cand4);
     float *qn = reinterpret_cast<float *>(p);
@@ -418,9 +425,6 @@ hipError_t launch_nn_mfma(const float *qd, const uint32_t *nq, size_t nq_max, co
     uint32_t *qsplit = reinterpret_cast<uint32_t *>(p);
     p += nq_max * 128 * sizeof(uint32_t);
     uint32_t *tsplit = reinterpret_cast<uint32_t *>(p);
-    p += tn_rows(nt_max) * 128 * sizeof(uint32_t);
-    p += (8 - reinterpret_cast<uintptr_t>(p) % 8) % 8;
-    unsigned long long *tmax = reinterpret_cast<unsigned long long *>(p);
     const uint32_t trows = (uint32_t)tn_rows(nt_max);
     const uint32_t tblocks = (trows + 63) / 64, qblocks = (uint32_t)((nq_max + 63) / 64);
     const PrepSide ts{td, nt, (uint32_t)nt_max, trows, tn, tsplit};

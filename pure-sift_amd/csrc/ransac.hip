@@ -88,6 +88,7 @@ __device__ uint32_t eval_sim(const ps_feature_match *m, uint32_t n, float a, flo
                              float *err) {
     uint32_t count = 0;
     float es = 0.0f;
+#pragma unroll 8
     for (uint32_t i = 0; i < n; i++) {
         float px = a * m[i].qx - b * m[i].qy + tx;
         float py = b * m[i].qx + a * m[i].qy + ty;
@@ -116,6 +117,7 @@ __device__ bool fit_procrustes(const ps_feature_match *m, const uint32_t *idx, u
     }
     float inv_n = 1.0f / (float)n;
     float mqx = 0, mqy = 0, mtx = 0, mty = 0;
+#pragma unroll 8
     for (uint32_t i = 0; i < n; i++) {
         const ps_feature_match &p = m[idx[i]];
         mqx += p.qx;
@@ -128,6 +130,7 @@ __device__ bool fit_procrustes(const ps_feature_match *m, const uint32_t *idx, u
     mtx *= inv_n;
     mty *= inv_n;
     float sqq = 0, na = 0, nb = 0;
+#pragma unroll 8
     for (uint32_t i = 0; i < n; i++) {
         const ps_feature_match &p = m[idx[i]];
         float qx = p.qx - mqx, qy = p.qy - mqy, tx = p.tx - mtx, ty = p.ty - mty;
@@ -323,6 +326,7 @@ __device__ uint32_t eval_h_seq(const ps_feature_match *m, uint32_t n, const doub
                                const float *wts, double *mass) {
     uint32_t c = 0;
     double es = 0.0, ms = 0.0;
+#pragma unroll 8
     for (uint32_t i = 0; i < n; i++) {
         double e = reproj_err(m[i], H);
         if (e >= 0.0 && e <= tsq) {
@@ -387,6 +391,7 @@ __device__ bool fit_dlt_block(const ps_feature_match *m, const uint32_t *idx, ui
     if (tid == 0) {
         double inv_n = 1.0 / (double)n, cxq = 0, cyq = 0, cxt = 0, cyt = 0;
         int ok = 1;
+#pragma unroll 8
         for (uint32_t i = 0; i < n; i++) {
             const ps_feature_match &p = m[idx[i]];
             double qx = p.qx, qy = p.qy, tx = p.tx, ty = p.ty;
@@ -405,6 +410,7 @@ __device__ bool fit_dlt_block(const ps_feature_match *m, const uint32_t *idx, ui
         cyt *= inv_n;
         double mdq = 0, mdt = 0;
         if (ok) {
+#pragma unroll 8
             for (uint32_t i = 0; i < n; i++) {
                 const ps_feature_match &p = m[idx[i]];
                 double dxq = (double)p.qx - cxq, dyq = (double)p.qy - cyq, dxt = (double)p.tx - cxt,
@@ -441,6 +447,7 @@ __device__ bool fit_dlt_block(const ps_feature_match *m, const uint32_t *idx, ui
         const double cxq = s_norm[0], cyq = s_norm[1], cxt = s_norm[2], cyt = s_norm[3], sq = s_norm[4],
                      stt = s_norm[5];
         double acc = 0.0;
+#pragma unroll 8
         for (uint32_t i = 0; i < n; i++) {
             const ps_feature_match &p = m[idx[i]];
             double x = ((double)p.qx - cxq) * sq, y = ((double)p.qy - cyq) * sq;
@@ -505,6 +512,7 @@ __device__ uint32_t eval_h_block(const ps_feature_match *m, uint32_t n, const do
     if (tid == 0) {
         uint32_t c = 0;
         double es = 0.0, ms = 0.0;
+#pragma unroll 8
         for (uint32_t i = 0; i < n; i++) {
             double e = ebuf[i];
             if (e >= 0.0) {
