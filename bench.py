@@ -579,6 +579,9 @@ def main():
                                "match against the pre-extracted partner",
                    "features": [n_q, n_t], "matches": n_m, "parallelism": "single GPU"},
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        # one frame = t_extract + t_match (a pair-based definition is 2 * t_extract + t_match)
+        "t_extract_ms": round(sum(v for k, v in stage_ms.items() if k != "match"), 4),
+        "t_match_ms": round(stage_ms["match"], 4),
         "roofline": {"bound": "hbm", "kernel": "scale-space stage: 21 blur launches (tile / sliding-window kernels) + 4 extrema launches, "
                                "octaves 1-3 overlapped on side streams",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
