@@ -289,6 +289,64 @@ ps_status ps_ransac_batch(const ps_feature_match *matches, const float *weights,
                           uint32_t n_problems, size_t total_matches, const ps_ransac_params *params,
                           uint32_t *inliers, ps_ransac_result *results);
 
+/* pipeline.rs:238-246 Homography::estimate_similarity -> filter_matches_ransac(_with_seed)
+ * (:2043-2191): similarity RANSAC with Procrustes refinement. out_inliers (capacity n) gets
+ * the inlier indices in increasing order; out_model = a, b, tx, ty, scale, rotation_deg
+ * (SimilarityTransform2D) when *out_has_model. */
+ps_status ps_similarity_estimate(const ps_feature_match *m, size_t n, const ps_ransac_options *opt,
+                                 uint32_t *out_inliers, size_t *out_n, float out_model[6], int32_t *out_has_model);
+
+/* ---- two-view geometry: fundamental matrix and GRIC model selection (geometry.rs) ------- */
+/* geometry.rs:108-128 FundamentalRansacOptions (default: 1.5, 2000, 0.999, 0xF00DCAFE12345678) */
+typedef struct {
+    double sampson_threshold;
+    uint64_t max_iters;
+    double confidence;
+    uint64_t seed;
+} ps_fundamental_options;
+/* geometry.rs:857-1016 estimate_fundamental_ransac_with_options: normalised 8-point LO-RANSAC
+ * with Sampson scoring (the 7-point solver when n == 7; no model below 7 matches). out_f is
+ * the row-major FundamentalMatrix (f64); out_inliers (capacity n) gets the inlier indices in
+ * increasing order. max_iters above 1,000,000 is rejected (the device runs the bound as given). */
+ps_status ps_fundamental_estimate(const ps_feature_match *m, size_t n, const ps_fundamental_options *opt,
+                                  uint32_t *out_inliers, size_t *out_n, double out_f[9], int32_t *out_has_model);
+/* Batch form (one workgroup per problem); no counterpart in the reference. */
+typedef struct {
+    uint32_t offset, n;   /* matches [offset, offset + n) of the batch array */
+    uint64_t seed;
+} ps_fundamental_problem;
+typedef struct {
+    double f[9];
+    uint32_t has_model, n_inliers, iters, _pad;   /* iters: RANSAC iterations run */
+} ps_fundamental_result;
+ps_status ps_engine_fundamental_batch(ps_engine *e, const ps_feature_match *d_matches,
+                                      const ps_fundamental_problem *d_problems, uint32_t n_problems,
+                                      size_t total_matches, double sampson_threshold, uint32_t max_iters,
+                                      double confidence, uint32_t *d_inliers, ps_fundamental_result *d_results);
+ps_status ps_fundamental_batch(const ps_feature_match *matches, const ps_fundamental_problem *problems,
+                               uint32_t n_problems, size_t total_matches, double sampson_threshold, uint32_t max_iters,
+                               double confidence, uint32_t *inliers, ps_fundamental_result *results);
+/* geometry.rs:1233-1341 select_model_gric (Torr's GRIC; sigma floored at 0.25): h is the
+ * row-major f32 homography or null, f the row-major f64 fundamental matrix or null.
+ * has_comparison = 0 is the reference's None; selected: 0 Homography, 1 Fundamental. */
+typedef struct {
+    int32_t has_comparison, selected;
+    double homography_gric, fundamental_gric;
+} ps_gric_result;
+ps_status ps_select_model_gric(const ps_feature_match *m, size_t n, const float *h, const double *f, double sigma,
+                               ps_gric_result *out);
+/* Batch form: per problem its match range and optional models. */
+typedef struct {
+    uint32_t offset, n;
+    uint8_t has_h, has_f;
+    uint8_t _pad[6];
+    float h[9];
+    uint32_t _pad2;
+    double f[9];
+} ps_gric_problem;
+ps_status ps_engine_gric_batch(ps_engine *e, const ps_feature_match *d_matches, const ps_gric_problem *d_problems,
+                               uint32_t n_problems, double sigma, ps_gric_result *d_results);
+
 /* ---- learned proposal ordering for RANSAC: the match-scoring MLP ----------------------
  * No counterpart in the reference (it samples by 1 / ratio, pipeline.rs:2006-2028). An
  * 8 -> 64 -> 64 -> 1 network scores each putative match; its sigmoid output is the sampling

@@ -196,6 +196,20 @@ size_t psr_catalog_search(const psr_catalog *c, const psr_feature *qf, const flo
                           size_t top_k, size_t shortlist_k, float ratio, float ransac_thresh,
                           psr_catalog_hit *out);
 
+/* ---- two-view geometry (ps_oracle_geometry.c; geometry.rs) ---- */
+int psr_jacobi9(const double a[81], double eigvals[9], double eigvecs[81]);
+double psr_sampson_sq(const double f[9], double x1, double y1, double x2, double y2);
+int psr_fit_fundamental_8point(const psr_match *m, const uint32_t *idx, size_t n, double f[9]);
+int psr_fit_fundamental_7point(const psr_match *m, double f[4][9]);
+size_t psr_adaptive_epipolar_iters(size_t best, size_t total, int sample_size, double confidence, size_t min_iters,
+                                   size_t max_iters);
+size_t psr_fundamental_ransac(const psr_match *m, size_t n, double sampson_threshold, size_t max_iters,
+                              double confidence, uint64_t seed, uint32_t *inl, double f[9], int *has_model,
+                              size_t *iters);
+int psr_invert3_f32(const float h[9], double out[9]);
+int psr_select_model_gric(const psr_match *m, size_t n, const float *h, const double *f, double sigma,
+                          int *selected, double *h_gric, double *f_gric);
+
 #ifdef __cplusplus
 }
 #endif

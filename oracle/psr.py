@@ -168,6 +168,26 @@ def lib():
         L.psr_catalog_search.argtypes = [C.POINTER(CatalogC), C.c_void_p, fp, C.c_size_t, C.c_size_t, C.c_size_t,
                                          C.c_float, C.c_float, C.c_void_p]
         L.psr_catalog_search.restype = C.c_size_t
+        dp = C.POINTER(C.c_double)
+        L.psr_jacobi9.argtypes = [dp, dp, dp]
+        L.psr_jacobi9.restype = C.c_int
+        L.psr_sampson_sq.argtypes = [dp, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.psr_sampson_sq.restype = C.c_double
+        L.psr_fit_fundamental_8point.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, dp]
+        L.psr_fit_fundamental_8point.restype = C.c_int
+        L.psr_fit_fundamental_7point.argtypes = [C.c_void_p, dp]
+        L.psr_fit_fundamental_7point.restype = C.c_int
+        L.psr_adaptive_epipolar_iters.argtypes = [C.c_size_t, C.c_size_t, C.c_int, C.c_double, C.c_size_t,
+                                                  C.c_size_t]
+        L.psr_adaptive_epipolar_iters.restype = C.c_size_t
+        L.psr_fundamental_ransac.argtypes = [C.c_void_p, C.c_size_t, C.c_double, C.c_size_t, C.c_double,
+                                             C.c_uint64, C.c_void_p, dp, C.POINTER(C.c_int), C.POINTER(C.c_size_t)]
+        L.psr_fundamental_ransac.restype = C.c_size_t
+        L.psr_invert3_f32.argtypes = [C.c_void_p, dp]
+        L.psr_invert3_f32.restype = C.c_int
+        L.psr_select_model_gric.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_double,
+                                            C.POINTER(C.c_int), dp, dp]
+        L.psr_select_model_gric.restype = C.c_int
         L.psr_set_threads.argtypes = [C.c_int]
         L.psr_get_threads.restype = C.c_int
         L.psr_free.argtypes = [C.c_void_p]
@@ -651,3 +671,80 @@ def catalog_search(cat: Catalog, qf, qd, top_k=5, shortlist_k=16, ratio=0.75, ra
     n = lib().psr_catalog_search(C.byref(cc), qf.ctypes.data, _fp(qd), len(qf), top_k, shortlist_k,
                                  C.c_float(ratio), C.c_float(ransac_threshold), out.ctypes.data)
     return out[:n].copy()
+
+
+# ---------------------------------------------------------------- two-view geometry (geometry.rs)
+
+FUNDAMENTAL_DEFAULT_SEED = 0xF00DCAFE12345678   # geometry.rs:125
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def jacobi9(a):
+    """geometry.rs:211-295: (eigenvalues ascending, eigenvectors as columns) or None."""
+    a = np.ascontiguousarray(a, np.float64).reshape(9, 9)
+    ev = np.zeros(9)
+    vec = np.zeros((9, 9))
+    ok = lib().psr_jacobi9(_dp(a), _dp(ev), _dp(vec))
+    return (ev, vec) if ok else None
+
+
+def sampson_sq(f, p1, p2) -> float:
+    f = np.ascontiguousarray(f, np.float64).reshape(9)
+    return lib().psr_sampson_sq(_dp(f), float(p1[0]), float(p1[1]), float(p2[0]), float(p2[1]))
+
+
+def fit_fundamental_8point(m, idx=None):
+    m = np.ascontiguousarray(m, MATCH_DTYPE)
+    f = np.zeros(9)
+    if idx is None:
+        ok = lib().psr_fit_fundamental_8point(m.ctypes.data, None, len(m), _dp(f))
+    else:
+        ix = np.ascontiguousarray(idx, np.uint32)
+        ok = lib().psr_fit_fundamental_8point(m.ctypes.data, ix.ctypes.data, len(ix), _dp(f))
+    return f.reshape(3, 3) if ok else None
+
+
+def fit_fundamental_7point(m):
+    m = np.ascontiguousarray(m, MATCH_DTYPE)[:7]
+    out = np.zeros((4, 9))
+    k = lib().psr_fit_fundamental_7point(m.ctypes.data, _dp(out))
+    return [out[i].reshape(3, 3).copy() for i in range(k)]
+
+
+def adaptive_epipolar_iters(best, total, sample_size, conf, min_iters, max_iters) -> int:
+    return lib().psr_adaptive_epipolar_iters(best, total, sample_size, conf, min_iters, max_iters)
+
+
+def fundamental_ransac(m, sampson_threshold=1.5, max_iters=2000, confidence=0.999, seed=FUNDAMENTAL_DEFAULT_SEED):
+    """geometry.rs:868-1016: (inlier indices, 3x3 F or None, iterations)."""
+    m = np.ascontiguousarray(m, MATCH_DTYPE)
+    inl = np.zeros(max(len(m), 1), np.uint32)
+    f = np.zeros(9)
+    has = C.c_int(0)
+    it = C.c_size_t(0)
+    n = lib().psr_fundamental_ransac(m.ctypes.data, len(m), sampson_threshold, max_iters, confidence,
+                                     C.c_uint64(seed), inl.ctypes.data, _dp(f), C.byref(has), C.byref(it))
+    return inl[:n].copy(), (f.reshape(3, 3).copy() if has.value else None), it.value
+
+
+def invert3_f32(h):
+    h = np.ascontiguousarray(h, np.float32).reshape(9)
+    out = np.zeros(9)
+    return out.reshape(3, 3) if lib().psr_invert3_f32(h.ctypes.data, _dp(out)) else None
+
+
+def select_model_gric(m, h=None, f=None, sigma=1.0):
+    """geometry.rs:1233-1341: None or (selected: 0 homography / 1 fundamental, h_gric, f_gric)."""
+    m = np.ascontiguousarray(m, MATCH_DTYPE)
+    hh = None if h is None else np.ascontiguousarray(h, np.float32).reshape(9)
+    ff = None if f is None else np.ascontiguousarray(f, np.float64).reshape(9)
+    sel = C.c_int(0)
+    hg = C.c_double(0)
+    fg = C.c_double(0)
+    ok = lib().psr_select_model_gric(m.ctypes.data, len(m), None if hh is None else hh.ctypes.data,
+                                     None if ff is None else ff.ctypes.data, sigma, C.byref(sel), C.byref(hg),
+                                     C.byref(fg))
+    return (sel.value, hg.value, fg.value) if ok else None

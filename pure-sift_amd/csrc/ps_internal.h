@@ -162,6 +162,14 @@ __host__ __device__ inline const uint32_t *ransac_sim_inliers(const void *work, 
     return wu + (size_t)offset * 2 + n;
 }
 
+// fundamental.hip: fundamental-matrix LO-RANSAC and GRIC, one wave per problem
+size_t fundamental_work_bytes(size_t total_matches);
+hipError_t launch_fundamental(const ps_feature_match *matches, const ps_fundamental_problem *probs, uint32_t n_probs,
+                              double sampson_threshold, uint32_t max_iters, double confidence, size_t total_matches,
+                              void *work, uint32_t *inliers, ps_fundamental_result *outs, hipStream_t st);
+hipError_t launch_gric(const ps_feature_match *matches, const ps_gric_problem *probs, uint32_t n_probs, double sigma,
+                       ps_gric_result *outs, hipStream_t st);
+
 // catalog.hip: GPU-resident feature catalog
 struct TrackJob {
     uint32_t moff, n, tfoff, pad;   // matches at moff (n of them), target features at tfoff

@@ -785,7 +785,11 @@ __global__ __launch_bounds__(kRT) void ransac_kernel(const ps_feature_match *mat
             O.sim[3] = best_sim.ty;
         }
     }
-    if (prm.sim_only) return;
+    if (prm.sim_only) {   // the similarity result alone: its inliers go to the output list
+        for (uint32_t i = tid; i < n_sim; i += kRT) out_inl[i] = sim_inl[i];
+        if (tid == 0) O.n_inliers = n_sim;
+        return;
+    }
 
     // ---------------- homography LO-RANSAC (pipeline.rs:2663-2755)
     const double thresh = (double)fth, tsq = thresh * thresh;

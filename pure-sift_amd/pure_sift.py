@@ -83,6 +83,20 @@ class _RansacParamsC(C.Structure):
                 ("min_iters", C.c_uint32), ("max_iters", C.c_uint32)]
 
 
+class _FundamentalOptionsC(C.Structure):
+    _fields_ = [("sampson_threshold", C.c_double), ("max_iters", C.c_uint64), ("confidence", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+FUNDAMENTAL_PROBLEM_DTYPE = np.dtype([("offset", "<u4"), ("n", "<u4"), ("seed", "<u8")])
+FUNDAMENTAL_RESULT_DTYPE = np.dtype([("f", "<f8", (9,)), ("has_model", "<u4"), ("n_inliers", "<u4"),
+                                     ("iters", "<u4"), ("_pad", "<u4")])
+GRIC_PROBLEM_DTYPE = np.dtype([("offset", "<u4"), ("n", "<u4"), ("has_h", "u1"), ("has_f", "u1"), ("_pad", "u1", (6,)),
+                               ("h", "<f4", (9,)), ("_pad2", "<u4"), ("f", "<f8", (9,))])
+GRIC_RESULT_DTYPE = np.dtype([("has_comparison", "<i4"), ("selected", "<i4"), ("homography_gric", "<f8"),
+                              ("fundamental_gric", "<f8")])
+
+
 class _CatalogInfoC(C.Structure):
     _fields_ = [("n_entries", C.c_uint64), ("n_features", C.c_uint64), ("octaves", C.c_uint32),
                 ("scales", C.c_uint32), ("contrast_threshold", C.c_float), ("descriptor_kind", C.c_uint8),
@@ -173,6 +187,15 @@ def lib() -> C.CDLL:
         L.ps_engine_quantize_u128.argtypes = [vp, vp, sz, vp]
         L.ps_engine_hamming_topk.argtypes = [vp, vp, sz, vp, sz, C.c_uint32, vp, vp]
         L.ps_engine_rerank_l2.argtypes = [vp, vp, sz, vp, sz, vp, C.c_uint32, vp, vp]
+        L.ps_similarity_estimate.argtypes = [vp, sz, C.POINTER(_RansacOptionsC), vp, C.POINTER(sz), vp,
+                                             C.POINTER(C.c_int32)]
+        L.ps_fundamental_estimate.argtypes = [vp, sz, C.POINTER(_FundamentalOptionsC), vp, C.POINTER(sz), vp,
+                                              C.POINTER(C.c_int32)]
+        L.ps_fundamental_batch.argtypes = [vp, vp, C.c_uint32, sz, C.c_double, C.c_uint32, C.c_double, vp, vp]
+        L.ps_engine_fundamental_batch.argtypes = [vp, vp, vp, C.c_uint32, sz, C.c_double, C.c_uint32, C.c_double,
+                                                  vp, vp]
+        L.ps_select_model_gric.argtypes = [vp, sz, vp, vp, C.c_double, vp]
+        L.ps_engine_gric_batch.argtypes = [vp, vp, vp, C.c_uint32, C.c_double, vp]
         L.ps_catalog_create.argtypes = [C.c_uint32, C.c_uint32, C.c_float, C.c_uint8, C.POINTER(vp)]
         L.ps_catalog_load_binary.argtypes = [C.c_char_p, sz, C.POINTER(vp)]
         L.ps_catalog_load_file.argtypes = [C.c_char_p, C.POINTER(vp)]
@@ -196,7 +219,9 @@ def lib() -> C.CDLL:
                      "ps_engine_profile_end", "ps_catalog_create", "ps_catalog_load_binary", "ps_catalog_load_file",
                      "ps_catalog_save_binary", "ps_catalog_destroy", "ps_catalog_get_info", "ps_catalog_entry",
                      "ps_catalog_entry_features", "ps_catalog_upsert", "ps_catalog_remove",
-                     "ps_catalog_coarse_votes", "ps_catalog_search", "ps_engine_catalog_search"):
+                     "ps_catalog_coarse_votes", "ps_catalog_search", "ps_engine_catalog_search",
+                     "ps_similarity_estimate", "ps_fundamental_estimate", "ps_fundamental_batch",
+                     "ps_engine_fundamental_batch", "ps_select_model_gric", "ps_engine_gric_batch"):
             getattr(L, name).restype = i32
         _lib = L
     return _lib
@@ -528,6 +553,80 @@ def mlp_loss_grad(params: np.ndarray, x: np.ndarray, y: np.ndarray):
     check(lib().ps_mlp_loss_grad(p.ctypes.data, xx.ctypes.data, yy.ctypes.data, len(xx), g.ctypes.data,
                                  C.byref(loss)))
     return loss.value, g
+
+
+# ---------------------------------------------------------------- two-view geometry
+
+@dataclasses.dataclass
+class FundamentalRansacOptions:
+    """geometry.rs:108-128"""
+    sampson_threshold: float = 1.5
+    max_iters: int = 2000
+    confidence: float = 0.999
+    seed: int = 0xF00DCAFE12345678
+
+    def _c(self) -> _FundamentalOptionsC:
+        return _FundamentalOptionsC(self.sampson_threshold, self.max_iters, self.confidence, self.seed)
+
+
+def estimate_similarity(matches: np.ndarray, options: RansacOptions = RansacOptions()):
+    """Homography::estimate_similarity (pipeline.rs:238-246): (inlier indices, SimilarityTransform2D
+    as (a, b, tx, ty, scale, rotation_deg) or None)."""
+    m = np.ascontiguousarray(matches, MATCH_DTYPE)
+    inl = np.zeros(max(len(m), 1), np.uint32)
+    model = np.zeros(6, np.float32)
+    n = C.c_size_t(0)
+    has = C.c_int32(0)
+    o = options._c()
+    check(lib().ps_similarity_estimate(m.ctypes.data, len(m), C.byref(o), inl.ctypes.data, C.byref(n),
+                                       model.ctypes.data, C.byref(has)))
+    return inl[:n.value].copy(), (tuple(float(v) for v in model) if has.value else None)
+
+
+def estimate_fundamental_ransac_with_options(matches: np.ndarray, options: FundamentalRansacOptions = None):
+    """geometry.rs:868-1016: (inlier indices, 3x3 f64 F or None)."""
+    options = options or FundamentalRansacOptions()
+    m = np.ascontiguousarray(matches, MATCH_DTYPE)
+    inl = np.zeros(max(len(m), 1), np.uint32)
+    f = np.zeros(9)
+    n = C.c_size_t(0)
+    has = C.c_int32(0)
+    o = options._c()
+    check(lib().ps_fundamental_estimate(m.ctypes.data, len(m), C.byref(o), inl.ctypes.data, C.byref(n), f.ctypes.data,
+                                        C.byref(has)))
+    return inl[:n.value].copy(), (f.reshape(3, 3).copy() if has.value else None)
+
+
+def estimate_fundamental_ransac(matches: np.ndarray, sampson_threshold: float):
+    """geometry.rs:857-866"""
+    return estimate_fundamental_ransac_with_options(matches, FundamentalRansacOptions(sampson_threshold))
+
+
+def fundamental_batch(matches: np.ndarray, problems: np.ndarray, sampson_threshold=1.5, max_iters=2000,
+                      confidence=0.999):
+    """Many fundamental RANSAC problems in one launch: (results [FUNDAMENTAL_RESULT_DTYPE], inliers)."""
+    m = np.ascontiguousarray(matches, MATCH_DTYPE)
+    pr = np.ascontiguousarray(problems, FUNDAMENTAL_PROBLEM_DTYPE)
+    res = np.zeros(len(pr), FUNDAMENTAL_RESULT_DTYPE)
+    inl = np.zeros(max(len(m), 1), np.uint32)
+    check(lib().ps_fundamental_batch(m.ctypes.data, pr.ctypes.data, len(pr), len(m), sampson_threshold, max_iters,
+                                     confidence, inl.ctypes.data, res.ctypes.data))
+    return res, inl
+
+
+def select_model_gric(matches: np.ndarray, homography=None, fundamental=None, sigma: float = 1.0):
+    """geometry.rs:1233-1341: None or (selected: "Homography" / "Fundamental", h_gric, f_gric)."""
+    m = np.ascontiguousarray(matches, MATCH_DTYPE)
+    h = None if homography is None else np.ascontiguousarray(homography, np.float32).reshape(9)
+    f = None if fundamental is None else np.ascontiguousarray(fundamental, np.float64).reshape(9)
+    out = np.zeros(1, GRIC_RESULT_DTYPE)
+    check(lib().ps_select_model_gric(m.ctypes.data, len(m), None if h is None else h.ctypes.data,
+                                     None if f is None else f.ctypes.data, sigma, out.ctypes.data))
+    r = out[0]
+    if not r["has_comparison"]:
+        return None
+    return ("Fundamental" if r["selected"] else "Homography", float(r["homography_gric"]),
+            float(r["fundamental_gric"]))
 
 
 HAMMING_MAX_K = 1024

@@ -190,3 +190,22 @@ def test_catalog_needs_a_device():
     assert ei.value.status == ps.PS_E_NO_DEVICE
     with pytest.raises(ps.DeviceError):
         ps.FeatureCatalog(3, 3, 0.015)
+
+
+def test_geometry_layouts():
+    assert ps.FUNDAMENTAL_PROBLEM_DTYPE.itemsize == 16
+    assert ps.FUNDAMENTAL_RESULT_DTYPE.itemsize == 88
+    assert ps.GRIC_PROBLEM_DTYPE.itemsize == 128 and ps.GRIC_PROBLEM_DTYPE.fields["f"][1] == 56
+    assert ps.GRIC_RESULT_DTYPE.itemsize == 24
+    assert C.sizeof(ps._FundamentalOptionsC) == 32
+
+
+def test_geometry_trivial_inputs_need_no_device():
+    """geometry.rs:872-874 (< 7 matches: no model) and :1239-1241 (no matches: None)."""
+    m = np.zeros(6, ps.MATCH_DTYPE)
+    inl, F = ps.estimate_fundamental_ransac(m, 1.5)
+    assert F is None and len(inl) == 0
+    assert ps.select_model_gric(m[:0], np.eye(3), None) is None
+    assert ps.select_model_gric(m, None, None) is None
+    with pytest.raises(ValueError):
+        ps.estimate_similarity(m, ps.RansacOptions(0.1))
