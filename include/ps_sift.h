@@ -347,6 +347,41 @@ typedef struct {
 ps_status ps_engine_gric_batch(ps_engine *e, const ps_feature_match *d_matches, const ps_gric_problem *d_problems,
                                uint32_t n_problems, double sigma, ps_gric_result *d_results);
 
+/* ---- per-pair match post-processing (cli.rs:290-318 run_match_from_bytes) --------------
+ * For each pair's raw matches: summarize_tracking (pipeline.rs:2210-2272), the similarity
+ * (Homography::estimate_similarity) and the homography (Homography::estimate) at the RANSAC
+ * options, the fundamental matrix (estimate_fundamental_ransac_with_options with the default
+ * options and Sampson threshold min(threshold, 3); only for 8 or more matches), GRIC (sigma
+ * max(threshold, 1)) and the kept inliers (the homography's when they outnumber the
+ * similarity's, else the similarity's). */
+typedef struct {
+    uint32_t offset, n;                    /* raw matches [offset, offset + n) */
+    uint32_t qfeat_offset, tfeat_offset;   /* the pair's features in the query / target arrays */
+} ps_pair_problem;
+typedef struct {
+    uint32_t raw_match_count, track_match_count;   /* TrackingSummary */
+    float inlier_ratio, mean_dx, mean_dy, median_dx, median_dy, mean_scale_ratio, mean_rotation_deg;
+    uint32_t has_transform, n_similarity;
+    float transform[6];                            /* a, b, tx, ty, scale, rotation_deg */
+    uint32_t has_homography, n_homography;
+    float homography[9];
+    uint32_t has_fundamental, n_fundamental;
+    double fundamental[9];
+    int32_t has_gric, gric_selected;               /* gric_selected: 0 Homography, 1 Fundamental */
+    double homography_gric, fundamental_gric;
+    uint32_t best_is_homography, n_best;
+} ps_pair_geometry;
+/* Device-resident batch on the engine's stream: d_best_inliers[offset .. offset + n_best) of
+ * each pair receives the kept inlier indices. opt as for ps_homography_estimate. */
+ps_status ps_engine_pair_geometry_batch(ps_engine *e, const ps_feature_match *d_matches, const ps_feature *d_qfeat,
+                                        const ps_feature *d_tfeat, const ps_pair_problem *d_pairs, uint32_t n_pairs,
+                                        size_t total_matches, const ps_ransac_options *opt, uint32_t *d_best_inliers,
+                                        ps_pair_geometry *d_out);
+/* Host form for one pair (q / t: the pair's features; m: its raw matches). */
+ps_status ps_pair_geometry_estimate(const ps_feature *q, size_t nq, const ps_feature *t, size_t nt,
+                                    const ps_feature_match *m, size_t n, const ps_ransac_options *opt,
+                                    uint32_t *out_best_inliers, ps_pair_geometry *out);
+
 /* ---- learned proposal ordering for RANSAC: the match-scoring MLP ----------------------
  * No counterpart in the reference (it samples by 1 / ratio, pipeline.rs:2006-2028). An
  * 8 -> 64 -> 64 -> 1 network scores each putative match; its sigmoid output is the sampling

@@ -748,3 +748,22 @@ def select_model_gric(m, h=None, f=None, sigma=1.0):
                                      None if ff is None else ff.ctypes.data, sigma, C.byref(sel), C.byref(hg),
                                      C.byref(fg))
     return (sel.value, hg.value, fg.value) if ok else None
+
+
+def pair_geometry(qf, tf, m, reproj_threshold=3.0, seed=None):
+    """cli.rs:290-318 post-processing of one pair's raw matches (ransac enabled), restated
+    with the oracle's functions: a dict of the tracking summary, similarity, homography,
+    fundamental (8+ matches), GRIC and the kept inliers."""
+    m = np.ascontiguousarray(m, MATCH_DTYPE)
+    tr = summarize_tracking(qf, tf, m)
+    sim_seed = 0xD1B54A32D192ED03 if seed is None else seed
+    sim_inl, sim = filter_matches_ransac(m, reproj_threshold, sim_seed)
+    hom_inl, H = homography_ransac_ref(m, reproj_threshold, seed)
+    F, f_inl = None, None
+    if len(m) >= 8:
+        f_inl, F, _ = fundamental_ransac(m, min(float(np.float32(reproj_threshold)), 3.0))
+    gric = select_model_gric(m, H, F, max(float(np.float32(reproj_threshold)), 1.0))
+    best = hom_inl if len(hom_inl) > len(sim_inl) else sim_inl
+    return dict(tracking=tr, sim_inliers=sim_inl, similarity=sim, hom_inliers=hom_inl, homography=H,
+                fundamental=F, n_fundamental=None if F is None else len(f_inl), gric=gric, best=best,
+                best_is_homography=len(hom_inl) > len(sim_inl))

@@ -161,13 +161,13 @@ __global__ __launch_bounds__(256) void tracking_kernel(const ps_feature_match *m
     const bool use_inl = nsim > 0;
     const uint32_t na = use_inl ? nsim : n;
     const ps_feature_match *mm = matches + J.moff;
-    const ps_feature *T = tf + J.tfoff;
+    const ps_feature *T = tf + J.tfoff, *Qf = qf + J.qfoff;
     float sum = 0.0f;   // lane l < 5 owns sum l: dx, dy, scale ratio, sin, cos
     for (uint32_t base = 0; base < na; base += 256) {
         const uint32_t k = base + tid;
         if (k < na) {
             const ps_feature_match M = mm[use_inl ? inl[k] : k];
-            const ps_feature Q = qf[M.query_index], Tt = T[M.target_index];
+            const ps_feature Q = Qf[M.query_index], Tt = T[M.target_index];
             s_v[0][tid] = M.dx;
             s_v[1][tid] = M.dy;
             s_v[2][tid] = Q.scale > 1e-6f ? Tt.scale / Q.scale : 0.0f;   // + 0 leaves a sum >= +0 unchanged

@@ -22,6 +22,8 @@ enum AuxSlot {
     AUX_RS_0, AUX_RS_1, AUX_RS_2, AUX_RS_3, AUX_RS_4, AUX_RS_5, AUX_RS_6, AUX_RS_7,
     AUX_CS_0, AUX_CS_1, AUX_CS_2, AUX_CS_3, AUX_CS_4, AUX_CS_5, AUX_CS_6, AUX_CS_7, AUX_CS_8, AUX_CS_9,
     AUX_CS_10, AUX_CS_11,
+    AUX_PG_0, AUX_PG_1, AUX_PG_2, AUX_PG_3, AUX_PG_4, AUX_PG_5, AUX_PG_6, AUX_PG_7, AUX_PG_8, AUX_PG_9,
+    AUX_PG_10, AUX_PG_11,
     AUX_COUNT
 };
 

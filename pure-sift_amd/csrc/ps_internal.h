@@ -172,7 +172,7 @@ hipError_t launch_gric(const ps_feature_match *matches, const ps_gric_problem *p
 
 // catalog.hip: GPU-resident feature catalog
 struct TrackJob {
-    uint32_t moff, n, tfoff, pad;   // matches at moff (n of them), target features at tfoff
+    uint32_t moff, n, tfoff, qfoff;   // matches at moff (n of them), target / query features at tfoff / qfoff
 };
 struct TrackOut {
     float mean_dx, mean_dy, median_dx, median_dy, mean_scale_ratio, mean_rotation_deg;
@@ -186,6 +186,18 @@ hipError_t launch_catalog_votes(const uint64_t *sig, const uint32_t *off, uint32
 hipError_t launch_tracking(const ps_feature_match *matches, const TrackJob *jobs, uint32_t n_jobs,
                            const ps_ransac_result *sim_res, const void *sim_work, size_t sim_total,
                            const ps_feature *qf, const ps_feature *tf, TrackOut *out, hipStream_t st);
+
+// pair_geometry.hip: the per-pair post-processing of a raw match set
+hipError_t launch_pair_setup(const ps_pair_problem *pairs, uint32_t n_pairs, uint64_t sim_seed, uint64_t hom_seed,
+                             uint64_t fund_seed, ps_ransac_problem *rp, ps_ransac_problem *rp3,
+                             ps_fundamental_problem *fp, TrackJob *jobs, hipStream_t st);
+hipError_t launch_pair_gric_problems(const ps_pair_problem *pairs, uint32_t n_pairs, const ps_ransac_result *rr,
+                                     const ps_fundamental_result *fr, ps_gric_problem *gp, hipStream_t st);
+hipError_t launch_pair_finish(const ps_pair_problem *pairs, uint32_t n_pairs, const ps_ransac_result *rr,
+                              const ps_ransac_result *rr3, const void *work_t, size_t total, const uint32_t *hom_inl,
+                              const ps_fundamental_result *fr, const ps_gric_result *gr, const TrackOut *tr,
+                              uint32_t *best_inl, ps_pair_geometry *out, hipStream_t st);
+
 
 // mlp.hip: match-scoring MLP (features, forward, gradient, Adam)
 size_t mlp_grad_scratch_bytes();

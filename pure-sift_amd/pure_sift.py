@@ -93,6 +93,15 @@ FUNDAMENTAL_RESULT_DTYPE = np.dtype([("f", "<f8", (9,)), ("has_model", "<u4"), (
                                      ("iters", "<u4"), ("_pad", "<u4")])
 GRIC_PROBLEM_DTYPE = np.dtype([("offset", "<u4"), ("n", "<u4"), ("has_h", "u1"), ("has_f", "u1"), ("_pad", "u1", (6,)),
                                ("h", "<f4", (9,)), ("_pad2", "<u4"), ("f", "<f8", (9,))])
+PAIR_PROBLEM_DTYPE = np.dtype([("offset", "<u4"), ("n", "<u4"), ("qfeat_offset", "<u4"), ("tfeat_offset", "<u4")])
+PAIR_GEOMETRY_DTYPE = np.dtype([("raw_match_count", "<u4"), ("track_match_count", "<u4"), ("inlier_ratio", "<f4"),
+                                ("mean_dx", "<f4"), ("mean_dy", "<f4"), ("median_dx", "<f4"), ("median_dy", "<f4"),
+                                ("mean_scale_ratio", "<f4"), ("mean_rotation_deg", "<f4"), ("has_transform", "<u4"),
+                                ("n_similarity", "<u4"), ("transform", "<f4", (6,)), ("has_homography", "<u4"),
+                                ("n_homography", "<u4"), ("homography", "<f4", (9,)), ("has_fundamental", "<u4"),
+                                ("n_fundamental", "<u4"), ("fundamental", "<f8", (9,)), ("has_gric", "<i4"),
+                                ("gric_selected", "<i4"), ("homography_gric", "<f8"), ("fundamental_gric", "<f8"),
+                                ("best_is_homography", "<u4"), ("n_best", "<u4")], align=True)
 GRIC_RESULT_DTYPE = np.dtype([("has_comparison", "<i4"), ("selected", "<i4"), ("homography_gric", "<f8"),
                               ("fundamental_gric", "<f8")])
 
@@ -195,6 +204,9 @@ def lib() -> C.CDLL:
         L.ps_engine_fundamental_batch.argtypes = [vp, vp, vp, C.c_uint32, sz, C.c_double, C.c_uint32, C.c_double,
                                                   vp, vp]
         L.ps_select_model_gric.argtypes = [vp, sz, vp, vp, C.c_double, vp]
+        L.ps_engine_pair_geometry_batch.argtypes = [vp, vp, vp, vp, vp, C.c_uint32, sz, C.POINTER(_RansacOptionsC), vp,
+                                                    vp]
+        L.ps_pair_geometry_estimate.argtypes = [vp, sz, vp, sz, vp, sz, C.POINTER(_RansacOptionsC), vp, vp]
         L.ps_engine_gric_batch.argtypes = [vp, vp, vp, C.c_uint32, C.c_double, vp]
         L.ps_catalog_create.argtypes = [C.c_uint32, C.c_uint32, C.c_float, C.c_uint8, C.POINTER(vp)]
         L.ps_catalog_load_binary.argtypes = [C.c_char_p, sz, C.POINTER(vp)]
@@ -221,7 +233,8 @@ def lib() -> C.CDLL:
                      "ps_catalog_entry_features", "ps_catalog_upsert", "ps_catalog_remove",
                      "ps_catalog_coarse_votes", "ps_catalog_search", "ps_engine_catalog_search",
                      "ps_similarity_estimate", "ps_fundamental_estimate", "ps_fundamental_batch",
-                     "ps_engine_fundamental_batch", "ps_select_model_gric", "ps_engine_gric_batch"):
+                     "ps_engine_fundamental_batch", "ps_select_model_gric", "ps_engine_gric_batch",
+                     "ps_engine_pair_geometry_batch", "ps_pair_geometry_estimate"):
             getattr(L, name).restype = i32
         _lib = L
     return _lib
@@ -629,6 +642,21 @@ def select_model_gric(matches: np.ndarray, homography=None, fundamental=None, si
             float(r["fundamental_gric"]))
 
 
+def pair_geometry(query: Features, target: Features, matches: np.ndarray,
+                  options: RansacOptions = RansacOptions()):
+    """The post-processing of run_match_from_bytes (cli.rs:290-318) on one pair's raw matches:
+    (PAIR_GEOMETRY_DTYPE record, kept inlier indices)."""
+    q = np.ascontiguousarray(query.header, FEAT_DTYPE)
+    t = np.ascontiguousarray(target.header, FEAT_DTYPE)
+    m = np.ascontiguousarray(matches, MATCH_DTYPE)
+    out = np.zeros(1, PAIR_GEOMETRY_DTYPE)
+    best = np.zeros(max(len(m), 1), np.uint32)
+    o = options._c()
+    check(lib().ps_pair_geometry_estimate(q.ctypes.data, len(q), t.ctypes.data, len(t), m.ctypes.data, len(m),
+                                          C.byref(o), best.ctypes.data, out.ctypes.data))
+    return out[0], best[:int(out[0]["n_best"])].copy()
+
+
 HAMMING_MAX_K = 1024
 
 
@@ -739,6 +767,12 @@ class Engine:
                      params, d_inliers: int, d_results: int) -> None:
         check(lib().ps_engine_ransac_batch(self.h, d_matches, d_weights or None, d_problems, n_problems, total_matches,
                                            C.byref(params), d_inliers, d_results))
+
+    def pair_geometry_batch(self, d_matches: int, d_qfeat: int, d_tfeat: int, d_pairs: int, n_pairs: int,
+                            total_matches: int, options: RansacOptions, d_best: int, d_out: int) -> None:
+        o = options._c()
+        check(lib().ps_engine_pair_geometry_batch(self.h, d_matches, d_qfeat, d_tfeat, d_pairs, n_pairs, total_matches,
+                                                  C.byref(o), d_best, d_out))
 
     def mlp_features(self, d_matches: int, d_qfeat: int, d_tfeat: int, d_pairs: int, n_pairs: int, d_x: int) -> None:
         check(lib().ps_engine_mlp_features(self.h, d_matches, d_qfeat, d_tfeat, d_pairs, n_pairs, d_x))
