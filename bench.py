@@ -430,6 +430,95 @@ def bench_catalog(n_entries: int = 1000, steps: int = 20, cpu_searches: int = 1)
     return out
 
 
+def bench_pair(steps: int = 20, n_batch: int = 64, cpu_pairs: int = 2) -> dict:
+    """The per-pair match path of run_match_from_bytes (cli.rs:270-318) on 640x480 blob pairs
+    (200 blobs, warped partner): extract both images (4 octaves x 3 scales, upsampled), ratio-0.8
+    match, then the post-processing -- tracking summary, similarity, homography, fundamental
+    matrix, GRIC, kept inliers -- all on one engine stream with the images resident in HBM.
+    Reported for one pair at a time (latency) and for 64 pairs per batch (throughput)."""
+    import numpy as np
+    import torch
+    import pure_sift as ps
+    import synth
+    dev = torch.device("cuda:0")
+    W, H = 640, 480
+    cap = ps.MAX_FEATURES_PER_IMAGE
+    fsz, msz = ps.FEAT_DTYPE.itemsize, ps.MATCH_DTYPE.itemsize
+    cfg = ps.SiftConfig().octaves(4).scales(3).contrast_threshold(0.015)
+    opt = ps.MatchOptions(ratio=0.8, mutual=False)
+    ro = ps.RansacOptions(3.0)
+    pairs = [synth.blob_pair(W, H, 200, seed=100 + k)[:2] for k in range(n_batch)]
+    imgs = torch.from_numpy(np.stack([np.stack([a, b]) for a, b in pairs])).to(dev)   # [k][2][H][W]
+    qf = torch.empty((n_batch, cap * fsz), dtype=torch.uint8, device=dev)
+    tf = torch.empty((n_batch, cap * fsz), dtype=torch.uint8, device=dev)
+    qd = torch.empty((n_batch, cap, 128), dtype=torch.float32, device=dev)
+    td = torch.empty((n_batch, cap, 128), dtype=torch.float32, device=dev)
+    cnt = torch.zeros((n_batch, 4), dtype=torch.int32, device=dev)
+    mout = torch.empty((n_batch, cap * msz), dtype=torch.uint8, device=dev)
+    probs = np.zeros(n_batch, ps.PAIR_PROBLEM_DTYPE)
+    probs["offset"] = np.arange(n_batch) * cap
+    probs["qfeat_offset"] = np.arange(n_batch) * cap
+    probs["tfeat_offset"] = np.arange(n_batch) * cap
+    d_probs = torch.from_numpy(probs.view(np.uint8).copy()).to(dev)
+    d_best = torch.empty(n_batch * cap, dtype=torch.int32, device=dev)
+    d_out = torch.empty(n_batch * ps.PAIR_GEOMETRY_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    eng = ps.Engine(0)
+    c = cnt.data_ptr()
+    st = torch.cuda.ExternalStream(eng.stream())
+
+    def run(n):
+        """the first n pairs; pair k's matches and features sit in slab k of the buffers"""
+        for k in range(n):
+            base = imgs[k].data_ptr()
+            ck = c + 16 * k
+            eng.extract_u8(base, H, W, W, cfg, qf[k].data_ptr(), qd[k].data_ptr(), cap, ck)
+            eng.extract_u8(base + W * H, H, W, W, cfg, tf[k].data_ptr(), td[k].data_ptr(), cap, ck + 4)
+            eng.match(qf[k].data_ptr(), qd[k].data_ptr(), ck, cap, tf[k].data_ptr(), td[k].data_ptr(), ck + 4, cap, opt,
+                      mout[k].data_ptr(), ck + 8)
+            with torch.cuda.stream(st):   # the raw match count into the pair's record, stream-ordered
+                d_probs.view(-1, 16)[k, 4:8].copy_(cnt[k, 2:3].view(torch.uint8), non_blocking=True)
+        eng.pair_geometry_batch(mout.data_ptr(), qf.data_ptr(), tf.data_ptr(), d_probs.data_ptr(), n, n * cap, ro,
+                                d_best.data_ptr(), d_out.data_ptr())
+
+    def timed(fn, reps):
+        fn()
+        eng.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        eng.sync()
+        return (time.perf_counter() - t0) / reps * 1e3
+    ms_single = timed(lambda: run(1), steps)
+    ms_batch = timed(lambda: run(n_batch), max(2, steps // 10))
+    recs = d_out.cpu().numpy().view(ps.PAIR_GEOMETRY_DTYPE)
+    eng.check()
+    out = {"workload": f"640x480 blob pairs (200 blobs, warped partner): extract both + ratio-0.8 match + tracking, "
+                       f"similarity, homography, fundamental, GRIC (RANSAC 3 px)",
+           "value": n_batch / (ms_batch * 1e-3), "unit": "pairs/s", "ms_per_pair_single": ms_single,
+           "ms_per_batch": ms_batch, "batch": n_batch,
+           "raw_matches_mean": float(recs["raw_match_count"].mean()),
+           "homography_inliers_mean": float(recs["n_homography"].mean()),
+           "fundamental_frac": float(recs["has_fundamental"].mean()),
+           "gric_selects_homography_frac": float((recs["gric_selected"] == 0).mean())}
+    if cpu_pairs > 0:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import psr
+        cores = _host_cores()
+        psr.set_threads(cores)
+        t0 = time.perf_counter()
+        for k in range(cpu_pairs):
+            a, b = pairs[k]
+            qh, qdh = psr.extract_features(psr.luma_to_f32(a), 4, 3, 0.015, True)
+            th, tdh = psr.extract_features(psr.luma_to_f32(b), 4, 3, 0.015, True)
+            mm = psr.match(qh, qdh, th, tdh, 0.8)
+            psr.pair_geometry(qh, th, mm, 3.0)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cpu_pairs / dt, "unit": "pairs/s", "cores": cores, "kind": "port",
+                               "sample": f"the first {cpu_pairs} pairs, {dt:.2f} s wall"}
+    eng.close()
+    return out
+
+
 def cpu_baseline(a, b, frames: int):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import psr
@@ -462,7 +551,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-frames", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["frame", "hamming", "precision", "ransac", "batch", "catalog"],
+    ap.add_argument("--workload", choices=["frame", "hamming", "precision", "ransac", "batch", "catalog", "pair"],
                     default="frame",
                     help="frame: the 1080p extract+match headline (default); hamming: the catalog shortlist")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -475,6 +564,9 @@ def main():
               f"(images shard across GPUs with no exchange)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "pair":
+        print(json.dumps(bench_pair(cpu_pairs=0 if args.no_cpu_baseline else 2), default=float), flush=True)
+        return
     if args.workload == "catalog":
         print(json.dumps(bench_catalog(steps=min(args.steps, 20), cpu_searches=0 if args.no_cpu_baseline else 1),
                          default=float), flush=True)
@@ -597,7 +689,8 @@ def main():
                          ("precision", lambda: bench_precision(cpu_queries=0 if args.no_cpu_baseline else 200)),
                          ("ransac", bench_ransac),
                          ("batch", lambda: bench_batch(cpu_images=0 if args.no_cpu_baseline else 4)),
-                         ("catalog", lambda: bench_catalog(cpu_searches=0 if args.no_cpu_baseline else 1))):
+                         ("catalog", lambda: bench_catalog(cpu_searches=0 if args.no_cpu_baseline else 1)),
+                         ("pair", lambda: bench_pair(cpu_pairs=0 if args.no_cpu_baseline else 2))):
             try:
                 sec[name] = fn()
             except Exception as ex:   # a secondary workload never hides the headline line

@@ -43,9 +43,10 @@ ps_status check_iters(uint64_t v) {
 
 ps_status enqueue_fundamental(ps_engine *e, const ps_feature_match *m, const ps_fundamental_problem *probs,
                               uint32_t np, size_t total, double thr, uint32_t iters, double conf, uint32_t *inl,
-                              ps_fundamental_result *res) {
+                              ps_fundamental_result *res, hipStream_t st = nullptr) {
     PS_TRY(ensure(e->aux[AUX_RS_7], ps::fundamental_work_bytes(std::max<size_t>(total, 1))));
-    PS_HIP(ps::launch_fundamental(m, probs, np, thr, iters, conf, total, e->aux[AUX_RS_7].p, inl, res, e->stream));
+    PS_HIP(ps::launch_fundamental(m, probs, np, thr, iters, conf, total, e->aux[AUX_RS_7].p, inl, res,
+                                  st ? st : e->stream));
     return PS_OK;
 }
 
@@ -81,6 +82,19 @@ ps_status enqueue_pair_geometry(ps_engine *e, const ps_feature_match *dm, const 
     const uint64_t sim_seed = opt.has_seed ? opt.seed : kDefaultSimSeed;
     const uint64_t hom_seed = opt.has_seed ? (opt.seed ^ kSeedMix) : kDefaultHomSeed;
     PS_HIP(ps::launch_pair_setup(dp, np, sim_seed, hom_seed, kFundSeed, rp, rp3, fp, jobs, st));
+    // the fundamental RANSAC is independent of the similarity / homography chain: it runs on a
+    // side stream beside it (not under stream capture, where the engine keeps one stream)
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    PS_HIP(hipStreamIsCapturing(st, &cap));
+    const bool fork = cap == hipStreamCaptureStatusNone;
+    hipStream_t fs = fork ? e->side[0] : st;
+    if (fork) {
+        PS_HIP(hipEventRecord(e->fork_ev[0], st));
+        PS_HIP(hipStreamWaitEvent(fs, e->fork_ev[0], 0));
+    }
+    const double sampson = std::min<double>((double)opt.reproj_threshold, 3.0);   // cli.rs:302
+    PS_TRY(enqueue_fundamental(e, dm, fp, np, total, sampson, 2000, 0.999, as<uint32_t>(e->aux[AUX_PG_10]), fr, fs));
+    if (fork) PS_HIP(hipEventRecord(e->join_ev[0], fs));
     ps_ransac_params p;
     std::memset(&p, 0, sizeof p);
     p.reproj_threshold = opt.reproj_threshold;
@@ -101,13 +115,13 @@ ps_status enqueue_pair_geometry(ps_engine *e, const ps_feature_match *dm, const 
         ps_ransac_params p3 = p;
         p3.reproj_threshold = 3.0f;
         rr3 = as<ps_ransac_result>(e->aux[AUX_PG_5]);
-        PS_HIP(ps::launch_ransac(dm, nullptr, rp3, np, p3, total, e->aux[AUX_PG_7].p, as<uint32_t>(e->aux[AUX_PG_10]),
+        PS_TRY(ensure(e->aux[AUX_CS_11], tm * sizeof(uint32_t)));   // inlier list of this run (unused)
+        PS_HIP(ps::launch_ransac(dm, nullptr, rp3, np, p3, total, e->aux[AUX_PG_7].p, as<uint32_t>(e->aux[AUX_CS_11]),
                                  rr3, st, true));
         work3 = e->aux[AUX_PG_7].p;
     }
     PS_HIP(ps::launch_tracking(dm, jobs, np, rr3, work3, total, dq, dt, tr, st));
-    const double sampson = std::min<double>((double)opt.reproj_threshold, 3.0);   // cli.rs:302
-    PS_TRY(enqueue_fundamental(e, dm, fp, np, total, sampson, 2000, 0.999, as<uint32_t>(e->aux[AUX_PG_10]), fr));
+    if (fork) PS_HIP(hipStreamWaitEvent(st, e->join_ev[0], 0));
     PS_HIP(ps::launch_pair_gric_problems(dp, np, rr, fr, gp, st));
     PS_HIP(ps::launch_gric(dm, gp, np, std::max<double>((double)opt.reproj_threshold, 1.0), gr, st));   // cli.rs:309
     PS_HIP(ps::launch_pair_finish(dp, np, rr, rr3, e->aux[AUX_PG_6].p, total, as<uint32_t>(e->aux[AUX_PG_8]), fr, gr,
