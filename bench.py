@@ -565,7 +565,7 @@ def main():
         sys.exit(2)
 
     if args.workload == "pair":
-        print(json.dumps(bench_pair(cpu_pairs=0 if args.no_cpu_baseline else 2), default=float), flush=True)
+        print(json.dumps(bench_pair(cpu_pairs=0 if args.no_cpu_baseline else 64), default=float), flush=True)
         return
     if args.workload == "catalog":
         print(json.dumps(bench_catalog(steps=min(args.steps, 20), cpu_searches=0 if args.no_cpu_baseline else 1),
@@ -690,7 +690,7 @@ def main():
                          ("ransac", bench_ransac),
                          ("batch", lambda: bench_batch(cpu_images=0 if args.no_cpu_baseline else 4)),
                          ("catalog", lambda: bench_catalog(cpu_searches=0 if args.no_cpu_baseline else 1)),
-                         ("pair", lambda: bench_pair(cpu_pairs=0 if args.no_cpu_baseline else 2))):
+                         ("pair", lambda: bench_pair(cpu_pairs=0 if args.no_cpu_baseline else 64))):
             try:
                 sec[name] = fn()
             except Exception as ex:   # a secondary workload never hides the headline line

@@ -126,3 +126,44 @@ def test_gric_closed_form_terms():
     m, F = two_view(30, 0, rotation(0, 0, 0), [0.0, 0.0, 0.0], seed=9)   # identical views
     sel, hg, fg = psr.select_model_gric(m, np.eye(3, dtype=np.float32), None, 1.0)
     assert sel == 0 and hg == 2.0 * 2.0 * 30 + 2.0 * 8.0 and fg == math.inf
+
+
+def _corrected_hypot(x, y):
+    """The device's hypot (pure-sift_amd/csrc/fundamental.hip glibc_hypot) in numpy f64, each
+    operation rounded on its own: sqrt(ax^2 + ay^2) then one correction step (Borges' corrected
+    algorithm without fused multiply-add)."""
+    ax, ay = np.maximum(np.abs(x), np.abs(y)), np.minimum(np.abs(x), np.abs(y))
+    h = np.sqrt(ax * ax + ay * ay)
+    small = h <= 2.0 * ay
+    d1 = h - ay
+    t1a, t2a = ax * (2.0 * d1 - ax), (d1 - 2.0 * (ax - ay)) * d1
+    d2 = h - ax
+    t1b, t2b = 2.0 * d2 * (ax - 2.0 * ay), (4.0 * d2 - ay) * ay + d2 * d2
+    t1, t2 = np.where(small, t1a, t1b), np.where(small, t2a, t2b)
+    out = h - (t1 + t2) / (2.0 * h)
+    tiny = ay <= ax * 2.0 ** -54
+    return np.where(ay == 0.0, ax, np.where(tiny, ax + ay, out))
+
+
+def test_device_hypot_algorithm_equals_libm():
+    """The oracle's Hartley normalisation calls the C library's hypot, which is what the
+    reference's f64::hypot binds on Linux; the device restates it. Over centred-coordinate
+    magnitudes (1e-3 .. 1e5, both signs, equal and very unequal arguments) the restatement must
+    equal libm's hypot bit for bit, and differ from the correctly rounded value somewhere
+    (otherwise the test would not tell the two apart)."""
+    import ctypes
+    libm = ctypes.CDLL("libm.so.6")
+    libm.hypot.restype = ctypes.c_double
+    libm.hypot.argtypes = [ctypes.c_double, ctypes.c_double]
+    rng = np.random.default_rng(7)
+    n = 40000
+    mag = 10.0 ** rng.uniform(-3, 5, size=(n, 2))
+    xy = mag * rng.choice([-1.0, 1.0], size=(n, 2))
+    xy[:200, 1] = xy[:200, 0]                      # equal magnitudes
+    xy[200:400, 1] = xy[200:400, 0] * 1e-17        # ratio beyond 2^54
+    xy[400:500, 1] = 0.0
+    got = _corrected_hypot(xy[:, 0], xy[:, 1])
+    want = np.array([libm.hypot(float(a), float(b)) for a, b in xy])
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+    naive = np.sqrt(xy[:, 0] * xy[:, 0] + xy[:, 1] * xy[:, 1])
+    assert np.count_nonzero(naive != want) > 100
