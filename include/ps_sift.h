@@ -178,6 +178,27 @@ ps_status ps_engine_match(ps_engine *e, const ps_feature *d_qf, const float *d_q
                           const float *d_td, const uint32_t *d_nt, size_t nt_max,
                           const ps_match_options *opt, ps_feature_match *d_out, uint32_t *d_out_count);
 
+/* Many ratio-match problems in one launch chain. A feature set is a row range of a feature /
+ * descriptor array (rows [offset, offset + count); count is device data, clamped to set_cap);
+ * problem k matches query set problems[k].query_set against target set
+ * problems[k].target_set exactly as match_descriptors does (pipeline.rs:1845-1858: Lowe's
+ * ratio, no mutual check) and writes its accepted matches in query order to
+ * d_out[out_offset ..] (room for the query set's count) and their number to
+ * d_out_counts[k]; the records' indices are relative to the two sets. Query and target
+ * arrays (and set tables) may be the same. No counterpart in the reference, whose panorama
+ * calls match_descriptors once per image pair (stitch.rs:500-503). */
+typedef struct {
+    uint32_t offset, count;
+} ps_feature_set;
+typedef struct {
+    uint32_t query_set, target_set, out_offset, _pad;
+} ps_match_problem;
+ps_status ps_engine_match_sets(ps_engine *e, const ps_feature *d_qf, const float *d_qd, size_t q_rows,
+                               const ps_feature_set *d_qsets, uint32_t n_qsets, const ps_feature *d_tf,
+                               const float *d_td, size_t t_rows, const ps_feature_set *d_tsets, uint32_t n_tsets,
+                               uint32_t set_cap, const ps_match_problem *d_problems, uint32_t n_problems,
+                               float ratio, ps_feature_match *d_out, uint32_t *d_out_counts);
+
 /* Device-side error word of the engine (0 = none; set when an internal capacity
  * bound was exceeded during an async call). Synchronises the engine. */
 ps_status ps_engine_check(ps_engine *e);
@@ -381,6 +402,51 @@ ps_status ps_engine_pair_geometry_batch(ps_engine *e, const ps_feature_match *d_
 ps_status ps_pair_geometry_estimate(const ps_feature *q, size_t nq, const ps_feature *t, size_t nt,
                                     const ps_feature_match *m, size_t n, const ps_ransac_options *opt,
                                     uint32_t *out_best_inliers, ps_pair_geometry *out);
+
+/* ---- panorama pairwise stage (stitch.rs:478-571) ------------------------------------------
+ * stitch.rs:45-69 PanoramaOptions: the fields this stage reads (projection, blending and
+ * canvas limits belong to the rendering that follows it). */
+typedef struct {
+    uint32_t octaves, scales;
+    float threshold, ratio, ransac_threshold;
+    uint32_t _pad;
+} ps_panorama_options;
+/* stitch.rs:104-112 PairEdge (row-major h_from_to) */
+typedef struct {
+    uint64_t from, to;
+    double h_from_to[9];
+    uint64_t inliers;
+    double mean_i_from, mean_i_to;
+} ps_pair_edge;
+/* One record per image pair i < j: has_edge = 0 where the reference makes no edge (fewer
+ * than 6 raw matches or kept inliers, no alignment, a singular alignment); otherwise the
+ * edges i -> j (h_from_to, mean_i_from = image i's mean) and j -> i (h_to_from, the means
+ * swapped). raw_matches is the pair's ratio-match count. */
+typedef struct {
+    uint32_t from, to;
+    uint32_t raw_matches, has_edge;
+    uint64_t inliers;
+    double h_from_to[9];
+    double h_to_from[9];
+    double mean_i_from, mean_i_to;
+} ps_panorama_pair;
+/* stitch.rs:478-571: every image's features (extract_sift_features: SIFT descriptors, first
+ * octave upsampled, at most 4096 per image) from its luma plane (what to_luma8 yields), then
+ * for every pair i < j the ratio matches (match_descriptors), similarity and homography RANSAC
+ * at ransac_threshold with the default seeds, the kept inliers (the homography's when at least
+ * as many), the safe alignment homography (resolve_safe_alignment_homography,
+ * visualize.rs:361-374), its inverse (visualize.rs:202-232) and the mean patch intensities of
+ * the kept inliers (stitch.rs:203-226). out gets the pair edges in the reference's order (per
+ * pair with an edge: i -> j, then j -> i); *out_count their number (PS_E_CAPACITY when above
+ * cap). Fewer than 2 images is the reference's error. */
+ps_status ps_panorama_pair_edges(const uint8_t *const *luma, const uint32_t *heights, const uint32_t *widths,
+                                 const uint32_t *row_strides, uint32_t n_images, const ps_panorama_options *opt,
+                                 ps_pair_edge *out, size_t cap, size_t *out_count);
+/* Device-resident form on the engine's stream: images in d_pixels as described by the host
+ * array images (as for ps_engine_extract_batch_u8); d_pairs gets n (n - 1) / 2 records in pair
+ * order ((0,1), (0,2), ..., (1,2), ...). The engine keeps the features until its next call. */
+ps_status ps_engine_panorama_pairs(ps_engine *e, const uint8_t *d_pixels, const ps_image_desc *images,
+                                   uint32_t n_images, const ps_panorama_options *opt, ps_panorama_pair *d_pairs);
 
 /* ---- learned proposal ordering for RANSAC: the match-scoring MLP ----------------------
  * No counterpart in the reference (it samples by 1 / ratio, pipeline.rs:2006-2028). An

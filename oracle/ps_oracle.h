@@ -207,6 +207,12 @@ size_t psr_fundamental_ransac(const psr_match *m, size_t n, double sampson_thres
                               double confidence, uint64_t seed, uint32_t *inl, double f[9], int *has_model,
                               size_t *iters);
 int psr_invert3_f32(const float h[9], double out[9]);
+
+/* ---- panorama pairwise stage helpers (ps_oracle_panorama.c) ---- */
+int psr_is_sane_homography(const float m[9], size_t src_w, size_t src_h);
+int psr_resolve_safe_alignment(int has_h, const float h[9], size_t hom_inliers, int has_sim, const float sim[4],
+                               size_t src_w, size_t src_h, float out[9]);
+double psr_sample_patch_intensity(const float *gray, size_t h, size_t w, float x, float y);
 int psr_select_model_gric(const psr_match *m, size_t n, const float *h, const double *f, double sigma,
                           int *selected, double *h_gric, double *f_gric);
 

@@ -188,6 +188,13 @@ def lib():
         L.psr_select_model_gric.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_double,
                                             C.POINTER(C.c_int), dp, dp]
         L.psr_select_model_gric.restype = C.c_int
+        L.psr_is_sane_homography.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t]
+        L.psr_is_sane_homography.restype = C.c_int
+        L.psr_resolve_safe_alignment.argtypes = [C.c_int, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_size_t,
+                                                 C.c_size_t, C.c_void_p]
+        L.psr_resolve_safe_alignment.restype = C.c_int
+        L.psr_sample_patch_intensity.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_float, C.c_float]
+        L.psr_sample_patch_intensity.restype = C.c_double
         L.psr_set_threads.argtypes = [C.c_int]
         L.psr_get_threads.restype = C.c_int
         L.psr_free.argtypes = [C.c_void_p]
@@ -767,3 +774,75 @@ def pair_geometry(qf, tf, m, reproj_threshold=3.0, seed=None):
     return dict(tracking=tr, sim_inliers=sim_inl, similarity=sim, hom_inliers=hom_inl, homography=H,
                 fundamental=F, n_fundamental=None if F is None else len(f_inl), gric=gric, best=best,
                 best_is_homography=len(hom_inl) > len(sim_inl))
+
+
+# ---- panorama pairwise stage (stitch.rs:478-571) -------------------------------------------
+
+def is_sane_homography(h, w: int, hgt: int) -> bool:
+    """visualize.rs:303-355 is_sane_homography_for_dims."""
+    h = np.ascontiguousarray(h, np.float32).reshape(9)
+    return bool(lib().psr_is_sane_homography(h.ctypes.data, w, hgt))
+
+
+def resolve_safe_alignment(h, hom_inliers: int, sim, w: int, hgt: int):
+    """visualize.rs:361-374 resolve_safe_alignment_homography: h (3x3 f32 or None), sim
+    (dict with a, b, tx, ty, or None); returns the 3x3 f32 alignment or None."""
+    hh = np.zeros(9, np.float32) if h is None else np.ascontiguousarray(h, np.float32).reshape(9)
+    ss = np.zeros(4, np.float32) if sim is None else np.array([sim["a"], sim["b"], sim["tx"], sim["ty"]], np.float32)
+    out = np.zeros(9, np.float32)
+    ok = lib().psr_resolve_safe_alignment(int(h is not None), hh.ctypes.data, hom_inliers, int(sim is not None),
+                                          ss.ctypes.data, w, hgt, out.ctypes.data)
+    return out.reshape(3, 3) if ok else None
+
+
+def sample_patch_intensity(gray, x: float, y: float) -> float:
+    """stitch.rs:203-226 over an f32 luma plane."""
+    gray = np.ascontiguousarray(gray, np.float32)
+    return lib().psr_sample_patch_intensity(gray.ctypes.data, gray.shape[0], gray.shape[1], C.c_float(x),
+                                            C.c_float(y))
+
+
+def panorama_pair(gray_i, gray_j, m, ransac_threshold=3.0):
+    """stitch.rs:504-566 for one image pair's raw matches m (gray_*: f32 luma planes): None
+    where the reference makes no edge, else a dict with h (3x3 f64 from the f32 alignment),
+    h_inv, inliers, mean_i, mean_j."""
+    m = np.ascontiguousarray(m, MATCH_DTYPE)
+    if len(m) < 6:
+        return None
+    sim_inl, sim = filter_matches_ransac(m, ransac_threshold)
+    hom_inl, H = homography_ransac_ref(m, ransac_threshold)
+    best = hom_inl if len(hom_inl) >= len(sim_inl) else sim_inl
+    if len(best) < 6:
+        return None
+    h_ij = resolve_safe_alignment(H, len(hom_inl), sim, gray_i.shape[1], gray_i.shape[0])
+    if h_ij is None:
+        return None
+    h_ji = invert3_f32(h_ij)
+    if h_ji is None:
+        return None
+    sum_i = 0.0
+    sum_j = 0.0
+    for k in best:   # f64 sums in inlier order
+        sum_i += sample_patch_intensity(gray_i, float(m["qx"][k]), float(m["qy"][k]))
+        sum_j += sample_patch_intensity(gray_j, float(m["tx"][k]), float(m["ty"][k]))
+    inv_c = 1.0 / len(best)
+    return dict(h=h_ij.astype(np.float64), h_inv=h_ji, inliers=len(best), mean_i=max(sum_i * inv_c, 1.0),
+                mean_j=max(sum_j * inv_c, 1.0), best=best)
+
+
+def panorama_pairs(images_u8, octaves=3, scales=3, threshold=0.015, ratio=0.75, ransac_threshold=3.0,
+                   features=None):
+    """stitch.rs:478-571 up to the pair edges: features per image (extract_sift_features:
+    SIFT descriptors, upsampled first octave, cap 4096) unless given as [(feat, desc)], then
+    every pair i < j in order; returns [(i, j, raw_match_count, panorama_pair(...) or None)]."""
+    grays = [luma_to_f32(im) for im in images_u8]
+    if features is None:
+        features = [extract_features(g, octaves, scales, threshold, True, kind=0) for g in grays]
+    out = []
+    n = len(images_u8)
+    for i in range(n):
+        for j in range(i + 1, n):
+            (qf, qd), (tf, td) = features[i], features[j]
+            m = match(qf, qd, tf, td, ratio)
+            out.append((i, j, len(m), panorama_pair(grays[i], grays[j], m, ransac_threshold)))
+    return out

@@ -970,26 +970,6 @@ __global__ __launch_bounds__(kFT) void fundamental_kernel(const ps_feature_match
     }
 }
 
-// visualize.rs:202-232 (the f32 homography promoted to f64)
-__device__ bool invert3_f32(const float *h, double o[9]) {
-    double a[9];
-    for (int i = 0; i < 9; i++) a[i] = (double)h[i];
-    const double c00 = a[4] * a[8] - a[5] * a[7];
-    const double c01 = -(a[3] * a[8] - a[5] * a[6]);
-    const double c02 = a[3] * a[7] - a[4] * a[6];
-    const double c10 = -(a[1] * a[8] - a[2] * a[7]);
-    const double c11 = a[0] * a[8] - a[2] * a[6];
-    const double c12 = -(a[0] * a[7] - a[1] * a[6]);
-    const double c20 = a[1] * a[5] - a[2] * a[4];
-    const double c21 = -(a[0] * a[5] - a[2] * a[3]);
-    const double c22 = a[0] * a[4] - a[1] * a[3];
-    const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
-    if (fabs(det) < 1e-12 || !isfinite(det)) return false;
-    const double id = 1.0 / det;
-    const double t[9] = {c00 * id, c10 * id, c20 * id, c01 * id, c11 * id, c21 * id, c02 * id, c12 * id, c22 * id};
-    for (int i = 0; i < 9; i++) o[i] = t[i];
-    return true;
-}
 
 __device__ __forceinline__ double fmin_rust(double a, double b) { return a != a ? b : (b != b ? a : (a < b ? a : b)); }
 

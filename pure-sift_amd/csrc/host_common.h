@@ -24,6 +24,7 @@ enum AuxSlot {
     AUX_CS_10, AUX_CS_11,
     AUX_PG_0, AUX_PG_1, AUX_PG_2, AUX_PG_3, AUX_PG_4, AUX_PG_5, AUX_PG_6, AUX_PG_7, AUX_PG_8, AUX_PG_9,
     AUX_PG_10, AUX_PG_11,
+    AUX_PN_0, AUX_PN_1, AUX_PN_2, AUX_PN_3, AUX_PN_4, AUX_PN_5, AUX_PN_6, AUX_PN_7, AUX_PN_8, AUX_PN_9,
     AUX_COUNT
 };
 

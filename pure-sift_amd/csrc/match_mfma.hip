@@ -133,14 +133,14 @@ __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast
 // the targets, 64 split target rows per LDS tile; the next tile is loaded into registers while
 // the current one is multiplied. Every lane keeps the 4 smallest (|t|^2 - 2 q.t, index) of
 // the targets it sees; both halves' lists are written (8 candidates per query and split).
-__global__ __launch_bounds__(256) void nn_mfma_kernel(const uint32_t *qs, const uint32_t *nq_p, uint32_t nq_max,
-                                                      const uint32_t *ts, const uint32_t *nt_p, uint32_t nt_max,
-                                                      const float *tn, int nsplit, Cand4 *part) {
+// nn_mfma_scan is one workgroup's share: queries qb .. qb + 127 of the split rows qs (nq valid)
+// against target tiles [tb, te) of ts (nt valid rows, norms tn); rows at or past nt load as
+// zero with norm +inf, so they never rank.
+__device__ __forceinline__ void nn_mfma_scan(const uint32_t *qs, uint32_t nq, uint32_t qb, const uint32_t *ts,
+                                             uint32_t nt, const float *tn, uint32_t tb, uint32_t te,
+                                             float (&kd)[4], uint32_t (&ki)[4]) {
     __shared__ __attribute__((aligned(16))) uint32_t s_t[kMT * kMP];
     __shared__ __attribute__((aligned(16))) float s_tn[kMT];
-    const uint32_t nq = min(*nq_p, nq_max), nt = min(*nt_p, nt_max);
-    const uint32_t qb = blockIdx.x * kMQ;
-    if (qb >= nq) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int col = lane & 31, half = lane >> 5;
     const uint32_t q = qb + wave * 32 + col;
@@ -156,12 +156,11 @@ __global__ __launch_bounds__(256) void nn_mfma_kernel(const uint32_t *qs, const 
             bl[s] = make_uint4(0u, 0u, 0u, 0u);
         }
     }
-    float kd[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
-    uint32_t ki[4] = {kNoIdx, kNoIdx, kNoIdx, kNoIdx};
-
-    const uint32_t ntiles = (nt + kMT - 1) / kMT;
-    const uint32_t per = (ntiles + nsplit - 1) / nsplit;
-    const uint32_t tb = blockIdx.y * per, te = min(tb + per, ntiles);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        kd[j] = INFINITY;
+        ki[j] = kNoIdx;
+    }
     constexpr int kLd = kMT * 32 / 256;   // 16-byte loads per thread per tile
     uint4 v[kLd];
     float vn = 0.0f;
@@ -173,7 +172,7 @@ __global__ __launch_bounds__(256) void nn_mfma_kernel(const uint32_t *qs, const 
             v[j] = t0_ + r < nt ? *reinterpret_cast<const uint4 *>(ts + (size_t)(t0_ + r) * 128 + c)   \
                                 : make_uint4(0u, 0u, 0u, 0u);                                         \
         }                                                                                             \
-        if (tid < kMT) vn = tn[t0_ + tid];                                                            \
+        if (tid < kMT) vn = t0_ + tid < nt ? tn[t0_ + tid] : INFINITY;                                \
     } while (0)
     if (tb < te) NN_LOAD(tb);
     for (uint32_t tile = tb; tile < te; tile++) {
@@ -229,11 +228,34 @@ __global__ __launch_bounds__(256) void nn_mfma_kernel(const uint32_t *qs, const 
         }
     }
 #undef NN_LOAD
+}
+
+__device__ __forceinline__ uint32_t nn_tiles_begin(uint32_t nt, int nsplit, uint32_t split, uint32_t &te) {
+    const uint32_t ntiles = (nt + kMT - 1) / kMT;
+    const uint32_t per = (ntiles + nsplit - 1) / nsplit;
+    const uint32_t tb = split * per;
+    te = min(tb + per, ntiles);
+    return tb;
+}
+
+__global__ __launch_bounds__(256) void nn_mfma_kernel(const uint32_t *qs, const uint32_t *nq_p, uint32_t nq_max,
+                                                      const uint32_t *ts, const uint32_t *nt_p, uint32_t nt_max,
+                                                      const float *tn, int nsplit, Cand4 *part) {
+    const uint32_t nq = min(*nq_p, nq_max), nt = min(*nt_p, nt_max);
+    const uint32_t qb = blockIdx.x * kMQ;
+    if (qb >= nq) return;
+    uint32_t te;
+    const uint32_t tb = nn_tiles_begin(nt, nsplit, blockIdx.y, te);
+    float kd[4];
+    uint32_t ki[4];
+    nn_mfma_scan(qs, nq, qb, ts, nt, tn, tb, te, kd, ki);
+    const int lane = threadIdx.x & 63;
+    const uint32_t q = qb + (threadIdx.x >> 6) * 32 + (lane & 31);
     if (q < nq) {
         Cand4 c;
 #pragma unroll
         for (int j = 0; j < 4; j++) { c.d[j] = kd[j]; c.i[j] = ki[j]; }
-        part[((size_t)blockIdx.y * nq_max + q) * 2 + half] = c;
+        part[((size_t)blockIdx.y * nq_max + q) * 2 + (lane >> 5)] = c;
     }
 }
 
@@ -246,21 +268,21 @@ __global__ __launch_bounds__(256) void nn_mfma_kernel(const uint32_t *qs, const 
 // its own 4th); if that exceeds the exact second-best the result equals the full scan's,
 // otherwise the query is flagged and the workgroup scans all targets for it exactly
 // (pipeline.rs:1700-1740 order rules).
-__global__ __launch_bounds__(256) void nn_rescore_kernel(const Cand4 *part, int nsplit, const float *qd, const float *qn,
-                                                         const uint32_t *nq_p, uint32_t nq_max, const float *td,
-                                                         const uint32_t *nt_p, uint32_t nt_max,
-                                                         const unsigned long long *tmax, uint32_t gen,
-                                                         uint32_t *best_idx, float *best, float *second) {
+// nn_rescore_block is one workgroup's share: queries 32 qblk .. + 31 (nq valid) of lists
+// part[(split * pstride + q) * 2 + half], targets td (nt of them, nt_scan scanned exactly).
+__device__ __forceinline__ void nn_rescore_block(const Cand4 *part, int nsplit, size_t pstride, uint32_t qblk,
+                                                 const float *qd, const float *qn, uint32_t nq, const float *td,
+                                                 uint32_t nt, uint32_t nt_scan, const unsigned long long *tmax,
+                                                 uint32_t gen, uint32_t *best_idx, float *best, float *second) {
     __shared__ uint32_t s_flag[32];
     __shared__ uint32_t s_nflag;
     __shared__ float sb[256], ss[256];
     __shared__ uint32_t si[256];
     if (threadIdx.x == 0) s_nflag = 0;
     __syncthreads();
-    const uint32_t q = blockIdx.x * 32 + (threadIdx.x >> 3);
+    const uint32_t q = qblk * 32 + (threadIdx.x >> 3);
     const int j = threadIdx.x & 7;
-    const bool active = q < min(*nq_p, nq_max);
-    const uint32_t nt = *nt_p;
+    const bool active = q < nq;
     // lane j: entry j % 4 of the half-(j / 4) list of every split
     float md[kMaxNNSplit];
     uint32_t mi[kMaxNNSplit];
@@ -271,7 +293,7 @@ __global__ __launch_bounds__(256) void nn_rescore_kernel(const Cand4 *part, int 
 #pragma unroll
         for (int s = 0; s < kMaxNNSplit; s++) {   // loads first, then the bookkeeping
             const int ss = s < nsplit ? s : 0;
-            const float *e = reinterpret_cast<const float *>(part + ((size_t)ss * nq_max + qs) * 2 + (j >> 2));
+            const float *e = reinterpret_cast<const float *>(part + ((size_t)ss * pstride + qs) * 2 + (j >> 2));
             md[s] = e[u];
             mi[s] = reinterpret_cast<const uint32_t *>(e)[4 + u];
         }
@@ -352,7 +374,6 @@ __global__ __launch_bounds__(256) void nn_rescore_kernel(const Cand4 *part, int 
     // reference's sequential rule over its targets, then a tree reduction with the
     // order-independent rule ((value, index) best; second = smallest of the rest)
     const uint32_t nflag = s_nflag;
-    const uint32_t nt_scan = min(nt, nt_max);
     for (uint32_t f = 0; f < nflag; f++) {
         const uint32_t fq = s_flag[f];
         float b = INFINITY, s2 = INFINITY;
@@ -384,6 +405,16 @@ __global__ __launch_bounds__(256) void nn_rescore_kernel(const Cand4 *part, int 
         }
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(256) void nn_rescore_kernel(const Cand4 *part, int nsplit, const float *qd, const float *qn,
+                                                         const uint32_t *nq_p, uint32_t nq_max, const float *td,
+                                                         const uint32_t *nt_p, uint32_t nt_max,
+                                                         const unsigned long long *tmax, uint32_t gen,
+                                                         uint32_t *best_idx, float *best, float *second) {
+    const uint32_t nt = *nt_p;
+    nn_rescore_block(part, nsplit, nq_max, blockIdx.x, qd, qn, min(*nq_p, nq_max), td, nt, min(nt, nt_max), tmax,
+                     gen, best_idx, best, second);
 }
 
 int nn_mfma_splits(size_t nq_max, size_t nt_max) {
@@ -435,6 +466,250 @@ hipError_t launch_nn_mfma(const float *qd, const uint32_t *nq, size_t nq_max, co
     nn_rescore_kernel<<<(unsigned)((nq_max + 31) / 32), 256, 0, st>>>(part, ns, qd, qn, nq, (uint32_t)nq_max, td, nt,
                                                                       (uint32_t)nt_max, tmax, gen, best_idx, best,
                                                                       second);
+    return hipGetLastError();
+}
+
+// ---- grouped form: many (query set, target set) problems in one chain -------------------
+
+// The sets' tables and bounds: a problem whose set index is out of range, or the part of a set
+// past its array's rows or past cap, counts as empty.
+struct SetsView {
+    const ps_feature_set *qsets, *tsets;
+    uint32_t n_qsets, n_tsets, cap, q_rows, t_rows;
+};
+__device__ __forceinline__ uint32_t set_len(const ps_feature_set &S, uint32_t cap, uint32_t rows) {
+    return S.offset >= rows ? 0u : min(min(S.count, cap), rows - S.offset);
+}
+__device__ __forceinline__ void resolve_sets(const ps_match_problem &P, const SetsView &V, ps_feature_set &Q,
+                                             ps_feature_set &T, uint32_t &nq, uint32_t &nt) {
+    Q = P.query_set < V.n_qsets ? V.qsets[P.query_set] : ps_feature_set{0u, 0u};
+    T = P.target_set < V.n_tsets ? V.tsets[P.target_set] : ps_feature_set{0u, 0u};
+    nq = set_len(Q, V.cap, V.q_rows);
+    nt = set_len(T, V.cap, V.t_rows);
+}
+
+// Prep over feature sets: norms and bf16 splits of rows [offset, offset + min(count, cap)) of
+// every set (rows outside the sets are never read by the later passes). On the target side
+// the largest norm goes to tmax[gen & 1] and workgroup (0, 0) zeroes the other word, as in
+// nn_prep_kernel. Grid (cap / 64 blocks, sets).
+__global__ __launch_bounds__(256) void nn_prep_sets_kernel(const float *x, const ps_feature_set *sets, uint32_t cap,
+                                                           uint32_t rows, float *norm, uint32_t *split,
+                                                           int target_side, unsigned long long *tmax, uint32_t gen) {
+    __shared__ float s_max[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const ps_feature_set S = sets[blockIdx.y];
+    const uint32_t n = set_len(S, cap, rows);
+    const uint32_t lb = blockIdx.x * 64;
+    float wmax = 0.0f;
+    if (lb < n) {
+        float2 v[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const uint32_t l = lb + w * 16 + k, r = S.offset + l;
+            v[k] = l < n && r < rows ? reinterpret_cast<const float2 *>(x + (size_t)r * 128)[lane]
+                                     : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const uint32_t l = lb + w * 16 + k, r = S.offset + l;
+            const bool ok = l < n && r < rows;
+            float s = v[k].x * v[k].x + v[k].y * v[k].y;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+            if (ok) {
+                if (lane == 0) norm[r] = s;
+                wmax = fmaxf(wmax, isfinite(s) ? s : INFINITY);
+                const uint32_t hx = bf16_bits_rne(v[k].x), hy = bf16_bits_rne(v[k].y);
+                const float rx = v[k].x - __uint_as_float(hx << 16), ry = v[k].y - __uint_as_float(hy << 16);
+                split[(size_t)r * 128 + lane] = hx | (hy << 16);
+                split[(size_t)r * 128 + 64 + lane] = bf16_bits_rne(rx) | (bf16_bits_rne(ry) << 16);
+            }
+        }
+    }
+    if (!target_side) return;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) tmax[(gen + 1) & 1] = 0ull;
+    if (lane == 0) s_max[w] = wmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float m = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
+        atomicMax(tmax + (gen & 1), ((unsigned long long)gen << 32) | __float_as_uint(m));
+    }
+}
+
+// Pass 1 per problem: grid (query blocks, problems, target splits); lists of problem k at
+// part[((k * nsplit + split) * cap + q) * 2 + half], indices relative to the target set.
+__global__ __launch_bounds__(256) void nn_mfma_sets_kernel(const uint32_t *qsplit, const uint32_t *tsplit,
+                                                           const float *tn, SetsView V, const ps_match_problem *probs,
+                                                           int nsplit, Cand4 *part) {
+    const uint32_t cap = V.cap;
+    ps_feature_set Q, T;
+    uint32_t nq, nt;
+    resolve_sets(probs[blockIdx.y], V, Q, T, nq, nt);
+    const uint32_t qb = blockIdx.x * kMQ;
+    if (qb >= nq) return;
+    uint32_t te;
+    const uint32_t tb = nn_tiles_begin(nt, nsplit, blockIdx.z, te);
+    float kd[4];
+    uint32_t ki[4];
+    nn_mfma_scan(qsplit + (size_t)Q.offset * 128, nq, qb, tsplit + (size_t)T.offset * 128, nt, tn + T.offset, tb, te,
+                 kd, ki);
+    const int lane = threadIdx.x & 63;
+    const uint32_t q = qb + (threadIdx.x >> 6) * 32 + (lane & 31);
+    if (q < nq) {
+        Cand4 c;
+#pragma unroll
+        for (int j = 0; j < 4; j++) { c.d[j] = kd[j]; c.i[j] = ki[j]; }
+        part[(((size_t)blockIdx.y * nsplit + blockIdx.z) * cap + q) * 2 + (lane >> 5)] = c;
+    }
+}
+
+// Pass 2 per problem: grid (32-query blocks, problems); results at [k * cap + q].
+__global__ __launch_bounds__(256) void nn_rescore_sets_kernel(const Cand4 *part, int nsplit, const float *qd,
+                                                              const float *qn, const float *td, SetsView V,
+                                                              const ps_match_problem *probs,
+                                                              const unsigned long long *tmax, uint32_t gen,
+                                                              uint32_t *best_idx, float *best, float *second) {
+    const uint32_t k = blockIdx.y, cap = V.cap;
+    ps_feature_set Q, T;
+    uint32_t nq, nt;
+    resolve_sets(probs[k], V, Q, T, nq, nt);
+    if (blockIdx.x * 32 >= nq) return;
+    const size_t o = (size_t)k * cap;
+    nn_rescore_block(part + (size_t)k * nsplit * cap * 2, nsplit, cap, blockIdx.x, qd + (size_t)Q.offset * 128,
+                     qn + Q.offset, nq, td + (size_t)T.offset * 128, nt, nt, tmax, gen, best_idx + o, best + o,
+                     second + o);
+}
+
+// Ratio test (pipeline.rs:1788-1805) and ordered compaction per problem: one workgroup walks
+// its queries 256 at a time, the output position a ballot prefix over the accepted ones.
+__global__ __launch_bounds__(256) void match_sets_finish_kernel(const ps_feature *qf, const ps_feature *tf, SetsView V,
+                                                                const ps_match_problem *probs,
+                                                                const uint32_t *best_idx, const float *best,
+                                                                const float *second, float ratio_thr,
+                                                                ps_feature_match *out, uint32_t *out_counts) {
+    __shared__ uint32_t s_wave[4];
+    const uint32_t k = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, cap = V.cap;
+    const ps_match_problem P = probs[k];
+    ps_feature_set Q, T;
+    uint32_t nq, nt;
+    resolve_sets(P, V, Q, T, nq, nt);
+    const size_t o = (size_t)k * cap;
+    const ps_feature *QF = qf + Q.offset, *TF = tf + T.offset;
+    ps_feature_match *dst = out + P.out_offset;
+    uint32_t base = 0;
+    for (uint32_t c = 0; c < nq; c += 256) {
+        const uint32_t q = c + tid;
+        bool acc = false;
+        if (q < nq && nt >= 2) {
+            const float bd = best[o + q], sd = second[o + q];
+            if (sd > 1e-12f && isfinite(bd)) acc = sqrtf(bd) / sqrtf(sd) < ratio_thr;
+        }
+        const uint64_t m = __ballot(acc);
+        if (lane == 0) s_wave[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = base, tot = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < 4; w++) {
+            if (w < wave) before += s_wave[w];
+            tot += s_wave[w];
+        }
+        if (acc) {
+            const uint32_t pos = before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            const float d1 = sqrtf(best[o + q]), d2 = sqrtf(second[o + q]);
+            const uint32_t t = best_idx[o + q];
+            const ps_feature a = QF[q], b = TF[t];
+            ps_feature_match r;
+            r.query_index = q;
+            r.target_index = t;
+            r.qx = a.x; r.qy = a.y; r.tx = b.x; r.ty = b.y;
+            r.distance = d1;
+            r.ratio = d1 / d2;
+            r.dx = b.x - a.x;
+            r.dy = b.y - a.y;
+            dst[pos] = r;
+        }
+        base += tot;
+        __syncthreads();
+    }
+    if (tid == 0) out_counts[k] = base;
+}
+
+namespace {
+
+int sets_splits(const MatchSetsArgs &a) {
+    const size_t qb = (a.cap + kMQ - 1) / kMQ, tt = (a.cap + kMT - 1) / kMT;
+    const size_t blocks = qb * a.n_probs;
+    if (blocks == 0 || tt == 0) return 1;
+    size_t ns = (2048 + blocks - 1) / blocks;
+    if (ns > tt) ns = tt;
+    if (ns > (size_t)kMaxNNSplit) ns = kMaxNNSplit;
+    return (int)(ns < 1 ? 1 : ns);
+}
+
+size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+struct SetsLayout {
+    size_t part, qn, tn, qs, ts, bidx, best, second, total;
+};
+
+SetsLayout sets_layout(const MatchSetsArgs &a) {
+    SetsLayout L;
+    const size_t ns = (size_t)sets_splits(a), pc = (size_t)a.n_probs * a.cap;
+    size_t o = 256;   // the two norm-bound words
+    L.part = o;
+    o += al256(ns * pc * 2 * sizeof(Cand4));
+    L.qn = o;
+    o += al256(a.q_rows * sizeof(float));
+    L.tn = a.shared ? L.qn : o;
+    if (!a.shared) o += al256(a.t_rows * sizeof(float));
+    L.qs = o;
+    o += al256(a.q_rows * 128 * sizeof(uint32_t));
+    L.ts = a.shared ? L.qs : o;
+    if (!a.shared) o += al256(a.t_rows * 128 * sizeof(uint32_t));
+    L.bidx = o;
+    o += al256(pc * sizeof(uint32_t));
+    L.best = o;
+    o += al256(pc * sizeof(float));
+    L.second = o;
+    o += al256(pc * sizeof(float));
+    L.total = o;
+    return L;
+}
+
+}  // namespace
+
+size_t match_sets_scratch_bytes(const MatchSetsArgs &a) { return sets_layout(a).total; }
+
+// Four launches for all problems: prep (one per distinct side), MFMA distances, rescore (+
+// exact scans of flagged queries), ratio test + compaction.
+hipError_t launch_match_sets(const MatchSetsArgs &a, void *scratch, uint32_t gen, ps_feature_match *out,
+                             uint32_t *out_counts, hipStream_t st) {
+    if (a.n_probs == 0) return hipSuccess;
+    const SetsLayout L = sets_layout(a);
+    const int ns = sets_splits(a);
+    char *b = static_cast<char *>(scratch);
+    unsigned long long *tmax = reinterpret_cast<unsigned long long *>(b);
+    Cand4 *part = reinterpret_cast<Cand4 *>(b + L.part);
+    float *qn = reinterpret_cast<float *>(b + L.qn), *tn = reinterpret_cast<float *>(b + L.tn);
+    uint32_t *qs = reinterpret_cast<uint32_t *>(b + L.qs), *ts = reinterpret_cast<uint32_t *>(b + L.ts);
+    uint32_t *bidx = reinterpret_cast<uint32_t *>(b + L.bidx);
+    float *best = reinterpret_cast<float *>(b + L.best), *second = reinterpret_cast<float *>(b + L.second);
+    const uint32_t rb = (a.cap + 63) / 64;
+    const SetsView V{a.qsets, a.tsets, a.n_qsets, a.n_tsets, a.cap, (uint32_t)a.q_rows, (uint32_t)a.t_rows};
+    if (a.cap > 0) {
+        if (!a.shared && a.n_qsets > 0)
+            nn_prep_sets_kernel<<<dim3(rb, a.n_qsets), 256, 0, st>>>(a.qd, a.qsets, a.cap, (uint32_t)a.q_rows, qn, qs,
+                                                                     0, tmax, gen);
+        if (a.n_tsets > 0)
+            nn_prep_sets_kernel<<<dim3(rb, a.n_tsets), 256, 0, st>>>(a.td, a.tsets, a.cap, (uint32_t)a.t_rows, tn, ts,
+                                                                     1, tmax, gen);
+        nn_mfma_sets_kernel<<<dim3((a.cap + kMQ - 1) / kMQ, a.n_probs, ns), 256, 0, st>>>(qs, ts, tn, V, a.probs, ns,
+                                                                                          part);
+        nn_rescore_sets_kernel<<<dim3((a.cap + 31) / 32, a.n_probs), 256, 0, st>>>(part, ns, a.qd, qn, a.td, V, a.probs,
+                                                                                   tmax, gen, bidx, best, second);
+    }
+    match_sets_finish_kernel<<<a.n_probs, 256, 0, st>>>(a.qf, a.tf, V, a.probs, bidx, best, second, a.ratio, out,
+                                                        out_counts);
     return hipGetLastError();
 }
 

@@ -124,6 +124,22 @@ size_t nn_mfma_scratch_bytes(size_t nq_max, size_t nt_max);
 hipError_t launch_nn_mfma(const float *qd, const uint32_t *nq, size_t nq_max, const float *td, const uint32_t *nt,
                           size_t nt_max, void *scratch, uint32_t *best_idx, float *best, float *second, uint32_t gen,
                           hipStream_t st);
+// Grouped form over feature sets (one chain for many problems); shared = the query and target
+// arrays and set tables are the same, so one prep pass serves both sides.
+struct MatchSetsArgs {
+    const ps_feature *qf, *tf;
+    const float *qd, *td;
+    size_t q_rows, t_rows;
+    const ps_feature_set *qsets, *tsets;
+    uint32_t n_qsets, n_tsets, cap;
+    const ps_match_problem *probs;
+    uint32_t n_probs;
+    float ratio;
+    bool shared;
+};
+size_t match_sets_scratch_bytes(const MatchSetsArgs &a);
+hipError_t launch_match_sets(const MatchSetsArgs &a, void *scratch, uint32_t gen, ps_feature_match *out,
+                             uint32_t *out_counts, hipStream_t st);
 
 // Squared L2 distance of two 128-D f32 rows summed sequentially in element order
 // with separate roundings (pipeline.rs:1764-1786 without the early exit, which never
@@ -162,6 +178,27 @@ __host__ __device__ inline const uint32_t *ransac_sim_inliers(const void *work, 
     return wu + (size_t)offset * 2 + n;
 }
 
+// visualize.rs:202-232 (the f32 homography promoted to f64)
+__device__ inline bool invert3_f32(const float *h, double o[9]) {
+    double a[9];
+    for (int i = 0; i < 9; i++) a[i] = (double)h[i];
+    const double c00 = a[4] * a[8] - a[5] * a[7];
+    const double c01 = -(a[3] * a[8] - a[5] * a[6]);
+    const double c02 = a[3] * a[7] - a[4] * a[6];
+    const double c10 = -(a[1] * a[8] - a[2] * a[7]);
+    const double c11 = a[0] * a[8] - a[2] * a[6];
+    const double c12 = -(a[0] * a[7] - a[1] * a[6]);
+    const double c20 = a[1] * a[5] - a[2] * a[4];
+    const double c21 = -(a[0] * a[5] - a[2] * a[3]);
+    const double c22 = a[0] * a[4] - a[1] * a[3];
+    const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    if (fabs(det) < 1e-12 || !isfinite(det)) return false;
+    const double id = 1.0 / det;
+    const double t[9] = {c00 * id, c10 * id, c20 * id, c01 * id, c11 * id, c21 * id, c02 * id, c12 * id, c22 * id};
+    for (int i = 0; i < 9; i++) o[i] = t[i];
+    return true;
+}
+
 // fundamental.hip: fundamental-matrix LO-RANSAC and GRIC, one wave per problem
 size_t fundamental_work_bytes(size_t total_matches);
 hipError_t launch_fundamental(const ps_feature_match *matches, const ps_fundamental_problem *probs, uint32_t n_probs,
@@ -197,6 +234,15 @@ hipError_t launch_pair_finish(const ps_pair_problem *pairs, uint32_t n_pairs, co
                               const ps_ransac_result *rr3, const void *work_t, size_t total, const uint32_t *hom_inl,
                               const ps_fundamental_result *fr, const ps_gric_result *gr, const TrackOut *tr,
                               uint32_t *best_inl, ps_pair_geometry *out, hipStream_t st);
+
+// panorama.hip: the panorama's pairwise stage (feature sets, RANSAC problems, pair edges)
+hipError_t launch_pano_sets(const uint32_t *counts, uint32_t n, uint32_t cap, ps_feature_set *sets, hipStream_t st);
+hipError_t launch_pano_ransac_problems(const ps_match_problem *probs, const uint32_t *mcount, uint32_t np,
+                                       uint64_t sim_seed, uint64_t hom_seed, ps_ransac_problem *rp, hipStream_t st);
+hipError_t launch_pano_edges(const uint8_t *pixels, const ps_image_desc *imgs, const ps_match_problem *probs,
+                             const uint32_t *mcount, uint32_t np, const ps_feature_match *matches,
+                             const ps_ransac_result *rr, const uint32_t *hom_inl, const void *work, size_t total,
+                             ps_panorama_pair *out, hipStream_t st);
 
 
 // mlp.hip: match-scoring MLP (features, forward, gradient, Adam)

@@ -106,6 +106,11 @@ GRIC_RESULT_DTYPE = np.dtype([("has_comparison", "<i4"), ("selected", "<i4"), ("
                               ("fundamental_gric", "<f8")])
 
 
+class _PanoramaOptionsC(C.Structure):
+    _fields_ = [("octaves", C.c_uint32), ("scales", C.c_uint32), ("threshold", C.c_float), ("ratio", C.c_float),
+                ("ransac_threshold", C.c_float), ("_pad", C.c_uint32)]
+
+
 class _CatalogInfoC(C.Structure):
     _fields_ = [("n_entries", C.c_uint64), ("n_features", C.c_uint64), ("octaves", C.c_uint32),
                 ("scales", C.c_uint32), ("contrast_threshold", C.c_float), ("descriptor_kind", C.c_uint8),
@@ -120,6 +125,14 @@ class _CatalogEntryC(C.Structure):
 IMAGE_DESC_DTYPE = np.dtype([("offset", "<u8"), ("height", "<u4"), ("width", "<u4"), ("row_stride", "<u4"),
                              ("_pad", "<u4")])
 BATCH_NO_GRAPH = 1
+FEATURE_SET_DTYPE = np.dtype([("offset", "<u4"), ("count", "<u4")])
+MATCH_PROBLEM_DTYPE = np.dtype([("query_set", "<u4"), ("target_set", "<u4"), ("out_offset", "<u4"), ("_pad", "<u4")])
+PANORAMA_PAIR_DTYPE = np.dtype([("from", "<u4"), ("to", "<u4"), ("raw_matches", "<u4"), ("has_edge", "<u4"),
+                                ("inliers", "<u8"), ("h_from_to", "<f8", (9,)), ("h_to_from", "<f8", (9,)),
+                                ("mean_i_from", "<f8"), ("mean_i_to", "<f8")])
+PAIR_EDGE_DTYPE = np.dtype([("from", "<u8"), ("to", "<u8"), ("h_from_to", "<f8", (9,)), ("inliers", "<u8"),
+                            ("mean_i_from", "<f8"), ("mean_i_to", "<f8")])
+assert PANORAMA_PAIR_DTYPE.itemsize == 184 and PAIR_EDGE_DTYPE.itemsize == 112
 
 RANSAC_PROBLEM_DTYPE = np.dtype([("offset", "<u4"), ("n", "<u4"), ("sim_seed", "<u8"), ("hom_seed", "<u8")])
 RANSAC_RESULT_DTYPE = np.dtype([("h", "<f4", (9,)), ("has_model", "<u4"), ("n_inliers", "<u4"), ("hyps", "<u4"),
@@ -207,6 +220,11 @@ def lib() -> C.CDLL:
         L.ps_engine_pair_geometry_batch.argtypes = [vp, vp, vp, vp, vp, C.c_uint32, sz, C.POINTER(_RansacOptionsC), vp,
                                                     vp]
         L.ps_pair_geometry_estimate.argtypes = [vp, sz, vp, sz, vp, sz, C.POINTER(_RansacOptionsC), vp, vp]
+        L.ps_engine_match_sets.argtypes = [vp, vp, vp, sz, vp, C.c_uint32, vp, vp, sz, vp, C.c_uint32, C.c_uint32, vp,
+                                           C.c_uint32, C.c_float, vp, vp]
+        L.ps_engine_panorama_pairs.argtypes = [vp, vp, vp, C.c_uint32, C.POINTER(_PanoramaOptionsC), vp]
+        L.ps_panorama_pair_edges.argtypes = [vp, vp, vp, vp, C.c_uint32, C.POINTER(_PanoramaOptionsC), vp, sz,
+                                             C.POINTER(sz)]
         L.ps_engine_gric_batch.argtypes = [vp, vp, vp, C.c_uint32, C.c_double, vp]
         L.ps_catalog_create.argtypes = [C.c_uint32, C.c_uint32, C.c_float, C.c_uint8, C.POINTER(vp)]
         L.ps_catalog_load_binary.argtypes = [C.c_char_p, sz, C.POINTER(vp)]
@@ -234,7 +252,8 @@ def lib() -> C.CDLL:
                      "ps_catalog_coarse_votes", "ps_catalog_search", "ps_engine_catalog_search",
                      "ps_similarity_estimate", "ps_fundamental_estimate", "ps_fundamental_batch",
                      "ps_engine_fundamental_batch", "ps_select_model_gric", "ps_engine_gric_batch",
-                     "ps_engine_pair_geometry_batch", "ps_pair_geometry_estimate"):
+                     "ps_engine_pair_geometry_batch", "ps_pair_geometry_estimate", "ps_engine_match_sets",
+                     "ps_engine_panorama_pairs", "ps_panorama_pair_edges"):
             getattr(L, name).restype = i32
         _lib = L
     return _lib
@@ -657,6 +676,39 @@ def pair_geometry(query: Features, target: Features, matches: np.ndarray,
     return out[0], best[:int(out[0]["n_best"])].copy()
 
 
+@dataclasses.dataclass
+class PanoramaOptions:
+    """stitch.rs:45-86 PanoramaOptions: the fields the pairwise stage reads, with the
+    reference's defaults (3 octaves, 3 scales, DEFAULT_CONTRAST_THRESHOLD, DEFAULT_MATCH_RATIO,
+    DEFAULT_RANSAC_REPROJ_THRESHOLD)."""
+    octaves: int = 3
+    scales: int = 3
+    threshold: float = 0.015
+    ratio: float = 0.75
+    ransac_threshold: float = 3.0
+
+    def _c(self) -> _PanoramaOptionsC:
+        return _PanoramaOptionsC(self.octaves, self.scales, self.threshold, self.ratio, self.ransac_threshold, 0)
+
+
+def panorama_pair_edges(images, options: PanoramaOptions = PanoramaOptions()) -> np.ndarray:
+    """stitch.rs:478-571 up to the pair edges: images are u8 luma planes (h x w); returns the
+    PAIR_EDGE_DTYPE records in the reference's order (per pair i < j with an edge: i -> j,
+    then j -> i)."""
+    ims = [np.ascontiguousarray(im, np.uint8) for im in images]
+    n = len(ims)
+    ptrs = (C.c_void_p * max(n, 1))(*[im.ctypes.data for im in ims])
+    hs = np.array([im.shape[0] for im in ims] or [0], np.uint32)
+    ws = np.array([im.shape[1] for im in ims] or [0], np.uint32)
+    cap = max(n * (n - 1), 1)
+    out = np.zeros(cap, PAIR_EDGE_DTYPE)
+    cnt = C.c_size_t(0)
+    o = options._c()
+    check(lib().ps_panorama_pair_edges(ptrs, hs.ctypes.data, ws.ctypes.data, None, n, C.byref(o), out.ctypes.data,
+                                       cap, C.byref(cnt)))
+    return out[:cnt.value].copy()
+
+
 HAMMING_MAX_K = 1024
 
 
@@ -773,6 +825,21 @@ class Engine:
         o = options._c()
         check(lib().ps_engine_pair_geometry_batch(self.h, d_matches, d_qfeat, d_tfeat, d_pairs, n_pairs, total_matches,
                                                   C.byref(o), d_best, d_out))
+
+    def match_sets(self, d_qf: int, d_qd: int, q_rows: int, d_qsets: int, n_qsets: int, d_tf: int, d_td: int,
+                   t_rows: int, d_tsets: int, n_tsets: int, set_cap: int, d_problems: int, n_problems: int,
+                   ratio: float, d_out: int, d_out_counts: int) -> None:
+        """Many set-against-set ratio matches in one launch chain (FEATURE_SET_DTYPE /
+        MATCH_PROBLEM_DTYPE tables on the device)."""
+        check(lib().ps_engine_match_sets(self.h, d_qf, d_qd, q_rows, d_qsets, n_qsets, d_tf, d_td, t_rows, d_tsets,
+                                         n_tsets, set_cap, d_problems, n_problems, ratio, d_out, d_out_counts))
+
+    def panorama_pairs(self, d_pixels: int, images: np.ndarray, options: PanoramaOptions, d_pairs: int) -> None:
+        """The panorama's pairwise stage on device images (IMAGE_DESC_DTYPE host array):
+        n (n - 1) / 2 PANORAMA_PAIR_DTYPE records at d_pairs."""
+        im = np.ascontiguousarray(images, IMAGE_DESC_DTYPE)
+        o = options._c()
+        check(lib().ps_engine_panorama_pairs(self.h, d_pixels, im.ctypes.data, len(im), C.byref(o), d_pairs))
 
     def mlp_features(self, d_matches: int, d_qfeat: int, d_tfeat: int, d_pairs: int, n_pairs: int, d_x: int) -> None:
         check(lib().ps_engine_mlp_features(self.h, d_matches, d_qfeat, d_tfeat, d_pairs, n_pairs, d_x))

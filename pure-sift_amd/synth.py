@@ -125,6 +125,23 @@ def blob_pair(width: int, height: int, n_blobs: int, seed: int = 0, band_noise: 
     return img, warp_homography(img, H), H
 
 
+def panorama_tiles(width: int, height: int, n_tiles: int, n_blobs: int, seed: int = 0, tile_frac: float = 0.42,
+                   exposure=None):
+    """Overlapping horizontal crops of one wide blob image, as the reference's 4-tile panorama
+    test builds them (tests/panorama_tests.rs:29-47): tile width = width * tile_frac, step =
+    (width - tile width) / (n_tiles - 1), each tile's pixels scaled by its exposure factor
+    (rounded, clamped to u8). Returns (tiles, step)."""
+    base = blob_image(width, height, n_blobs, seed, band_noise=True)
+    tw = width * int(round(tile_frac * 100)) // 100
+    step = (width - tw) // max(n_tiles - 1, 1)
+    tiles = []
+    for i in range(n_tiles):
+        crop = base[:, i * step:i * step + tw].astype(np.float32)
+        f = 1.0 if exposure is None else float(exposure[i])
+        tiles.append(np.clip(np.round(crop * np.float32(f)), 0, 255).astype(np.uint8))
+    return tiles, step
+
+
 def unit_descriptors(n_query: int, n_db: int, seed: int = 0, planted_frac: float = 0.2, noise: float = 0.05):
     """ReLU(N(0,1)) L2-normalised 128-D database; queries are random, except a planted
     fraction that are database rows + N(0, noise), renormalised. Returns (q, db, planted_idx)

@@ -209,3 +209,24 @@ def test_geometry_trivial_inputs_need_no_device():
     assert ps.select_model_gric(m, None, None) is None
     with pytest.raises(ValueError):
         ps.estimate_similarity(m, ps.RansacOptions(0.1))
+
+
+def test_panorama_layouts_and_argument_errors():
+    """stitch.rs:474-476: fewer than 2 images is the reference's error; bad options are refused
+    before any device work."""
+    assert C.sizeof(ps._PanoramaOptionsC) == 24
+    assert ps.PANORAMA_PAIR_DTYPE.itemsize == 184 and ps.PAIR_EDGE_DTYPE.itemsize == 112
+    assert ps.FEATURE_SET_DTYPE.itemsize == 8 and ps.MATCH_PROBLEM_DTYPE.itemsize == 16
+    img = np.zeros((40, 40), np.uint8)
+    with pytest.raises(ps.DeviceError) as ei:
+        ps.panorama_pair_edges([img])
+    assert ei.value.status == ps.PS_E_INVALID_ARG
+    assert "at least 2 input images" in str(ei.value)
+    with pytest.raises(ValueError):
+        ps.panorama_pair_edges([img, img], ps.PanoramaOptions(octaves=0))
+    with pytest.raises(ValueError):
+        ps.panorama_pair_edges([img, img], ps.PanoramaOptions(ransac_threshold=float("nan")))
+    if not has_gpu():
+        with pytest.raises(ps.DeviceError) as ei:
+            ps.panorama_pair_edges([img, img])
+        assert ei.value.status == ps.PS_E_NO_DEVICE
