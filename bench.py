@@ -519,6 +519,107 @@ def bench_pair(steps: int = 20, n_batch: int = 64, cpu_pairs: int = 2) -> dict:
     return out
 
 
+def bench_panorama(n_tiles: int = 16, steps: int = 10, cpu_sample: bool = True) -> dict:
+    """The panorama's pairwise stage (stitch.rs:478-571) on 16 overlapping 652x480 tiles of one
+    synthetic strip (1700 blobs + band-limited noise, step 319 px, exposures 0.85-1.18):
+    extract every tile (3 octaves x 3 scales, SIFT descriptors, cap 4096), ratio-0.75 match all
+    120 pairs, similarity + homography RANSAC (3 px), safe alignment, inverse, patch intensities
+    -- one engine call with the tiles resident in HBM. The grouped matcher is also timed alone
+    (HIP events on the engine stream) on the same features, with its split-BF16 MFMA work
+    (3 products of 2 * nq * nt * 128 flops per pair) against the dense BF16 peak."""
+    import numpy as np
+    import torch
+    import pure_sift as ps
+    import synth
+    dev = torch.device("cuda:0")
+    width = 5440
+    rng = np.random.default_rng(5)
+    exposure = rng.uniform(0.85, 1.18, n_tiles)
+    tiles, step = synth.panorama_tiles(width, 480, n_tiles, 1700, seed=9, tile_frac=0.12, exposure=exposure)
+    h, w = tiles[0].shape
+    n = len(tiles)
+    npairs = n * (n - 1) // 2
+    pix = torch.from_numpy(np.concatenate([t.ravel() for t in tiles])).to(dev)
+    descs = np.array([(k * h * w, h, w, w, 0) for k in range(n)], ps.IMAGE_DESC_DTYPE)
+    opt = ps.PanoramaOptions()
+    d_pairs = torch.zeros(npairs * ps.PANORAMA_PAIR_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    eng = ps.Engine(0)
+    run = lambda: eng.panorama_pairs(pix.data_ptr(), descs, opt, d_pairs.data_ptr())
+    run()
+    eng.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    eng.sync()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    recs = d_pairs.cpu().numpy().view(ps.PANORAMA_PAIR_DTYPE)
+    # the grouped matcher alone, on the same features (batched extraction into slabs of 4096)
+    cap = ps.MAX_FEATURES_PER_IMAGE
+    fsz = ps.FEAT_DTYPE.itemsize
+    feat = torch.empty(n * cap * fsz, dtype=torch.uint8, device=dev)
+    desc = torch.empty((n * cap, 128), dtype=torch.float32, device=dev)
+    cnt = torch.zeros(n, dtype=torch.int32, device=dev)
+    cfg = ps.SiftConfig().octaves(3).scales(3).contrast_threshold(0.015).descriptor_kind(ps.DescriptorKind.Sift)
+    eng.extract_batch_u8(pix.data_ptr(), descs, cfg, feat.data_ptr(), desc.data_ptr(), cap, cnt.data_ptr())
+    eng.sync()
+    counts = cnt.cpu().numpy().astype(np.int64)
+    sets = np.array([(k * cap, counts[k]) for k in range(n)], ps.FEATURE_SET_DTYPE)
+    probs = np.array([(i, j, k * cap, 0) for k, (i, j) in enumerate((i, j) for i in range(n) for j in range(i + 1, n))],
+                     ps.MATCH_PROBLEM_DTYPE)
+    d_sets = torch.from_numpy(sets.view(np.uint8).copy()).to(dev)
+    d_probs = torch.from_numpy(probs.view(np.uint8).copy()).to(dev)
+    mout = torch.empty(npairs * cap * ps.MATCH_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    mcnt = torch.zeros(npairs, dtype=torch.int32, device=dev)
+    st = torch.cuda.ExternalStream(eng.stream())
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ms_match = []
+    for r in range(steps + 1):
+        with torch.cuda.stream(st):
+            ev0.record(st)
+        eng.match_sets(feat.data_ptr(), desc.data_ptr(), n * cap, d_sets.data_ptr(), n, feat.data_ptr(),
+                       desc.data_ptr(), n * cap, d_sets.data_ptr(), n, cap, d_probs.data_ptr(), npairs, opt.ratio,
+                       mout.data_ptr(), mcnt.data_ptr())
+        with torch.cuda.stream(st):
+            ev1.record(st)
+        eng.sync()
+        if r:
+            ms_match.append(ev0.elapsed_time(ev1))
+    ms_m = float(np.median(ms_match))
+    pq = sum(int(counts[i]) * int(counts[j]) for i, j in zip(probs["query_set"], probs["target_set"]))
+    flops = 3 * 2.0 * pq * 128
+    assert np.array_equal(mcnt.cpu().numpy(), recs["raw_matches"].astype(np.int32))
+    eng.check()
+    eng.close()
+    out = {"workload": f"{n} overlapping {w}x{h} tiles of a synthetic strip (step {step} px): extract all, "
+                       f"ratio-0.75 match all {npairs} pairs, similarity + homography RANSAC 3 px, pair edges",
+           "value": npairs / (ms * 1e-3), "unit": "image pairs/s", "ms_per_stage": ms, "images": n, "pairs": npairs,
+           "features_mean": float(counts.mean()), "raw_matches_mean": float(recs["raw_matches"].mean()),
+           "edges": int(recs["has_edge"].sum()),
+           "grouped_match": {"ms": ms_m, "pairs_nq_nt": pq, "tflops": flops / (ms_m * 1e-3) / 1e12,
+                             "peak_tflops": 2500.0, "frac": flops / (ms_m * 1e-3) / 2.5e15,
+                             "note": "chain of prep, MFMA distances, rescore, compaction for all pairs; "
+                                     "3 BF16 products (hi*hi, hi*lo, lo*hi) of 2*nq*nt*128 flops per pair"}}
+    if cpu_sample:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import psr
+        cores = _host_cores()
+        psr.set_threads(cores)
+        t0 = time.perf_counter()
+        fs = [psr.extract_features(psr.luma_to_f32(t), 3, 3, 0.015, True, kind=0) for t in tiles[:3]]
+        t_ext = (time.perf_counter() - t0) / 3
+        grays = [psr.luma_to_f32(t) for t in tiles[:3]]
+        t0 = time.perf_counter()
+        for i, j in ((0, 1), (0, 2), (1, 2)):
+            m = psr.match(fs[i][0], fs[i][1], fs[j][0], fs[j][1], opt.ratio)
+            psr.panorama_pair(grays[i], grays[j], m, opt.ransac_threshold)
+        t_pair = (time.perf_counter() - t0) / 3
+        t_stage = n * t_ext + npairs * t_pair
+        out["cpu_baseline"] = {"value": npairs / t_stage, "unit": "image pairs/s", "cores": cores, "kind": "port",
+                               "sample": f"3 tiles extracted ({t_ext:.2f} s each) and their 3 pairs matched + "
+                                         f"RANSAC + edges ({t_pair:.2f} s each), scaled to the {n}-tile stage"}
+    return out
+
+
 def cpu_baseline(a, b, frames: int):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import psr
@@ -551,7 +652,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-frames", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["frame", "hamming", "precision", "ransac", "batch", "catalog", "pair"],
+    ap.add_argument("--workload", choices=["frame", "hamming", "precision", "ransac", "batch", "catalog", "pair",
+                                           "panorama"],
                     default="frame",
                     help="frame: the 1080p extract+match headline (default); hamming: the catalog shortlist")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -564,6 +666,9 @@ def main():
               f"(images shard across GPUs with no exchange)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "panorama":
+        print(json.dumps(bench_panorama(cpu_sample=not args.no_cpu_baseline), default=float), flush=True)
+        return
     if args.workload == "pair":
         print(json.dumps(bench_pair(cpu_pairs=0 if args.no_cpu_baseline else 64), default=float), flush=True)
         return
@@ -690,7 +795,8 @@ def main():
                          ("ransac", bench_ransac),
                          ("batch", lambda: bench_batch(cpu_images=0 if args.no_cpu_baseline else 4)),
                          ("catalog", lambda: bench_catalog(cpu_searches=0 if args.no_cpu_baseline else 1)),
-                         ("pair", lambda: bench_pair(cpu_pairs=0 if args.no_cpu_baseline else 64))):
+                         ("pair", lambda: bench_pair(cpu_pairs=0 if args.no_cpu_baseline else 64)),
+                         ("panorama", lambda: bench_panorama(cpu_sample=not args.no_cpu_baseline))):
             try:
                 sec[name] = fn()
             except Exception as ex:   # a secondary workload never hides the headline line

@@ -27,6 +27,7 @@ namespace ps {
 namespace {
 
 constexpr int kRT = 256;            // threads per problem
+constexpr uint32_t kStage = 512;   // points per staging chunk
 constexpr uint64_t kGamma = 0x9E3779B97F4A7C15ull;
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
@@ -106,52 +107,6 @@ __device__ uint32_t eval_sim(const ps_feature_match *m, uint32_t n, float a, flo
 __device__ __forceinline__ bool sim_inlier(const ps_feature_match &p, const Sim &s, float tsq) {
     float dx = (s.a * p.qx - s.b * p.qy + s.tx) - p.tx, dy = (s.b * p.qx + s.a * p.qy + s.ty) - p.ty;
     return dx * dx + dy * dy <= tsq;
-}
-
-// pipeline.rs:1900-1974 over m[idx[0..n)] (sequential sums)
-__device__ bool fit_procrustes(const ps_feature_match *m, const uint32_t *idx, uint32_t n, Sim *out) {
-    if (n == 0) return false;
-    if (n == 1) {
-        *out = Sim{1.0f, 0.0f, m[idx[0]].dx, m[idx[0]].dy};
-        return true;
-    }
-    float inv_n = 1.0f / (float)n;
-    float mqx = 0, mqy = 0, mtx = 0, mty = 0;
-#pragma unroll 8
-    for (uint32_t i = 0; i < n; i++) {
-        const ps_feature_match &p = m[idx[i]];
-        mqx += p.qx;
-        mqy += p.qy;
-        mtx += p.tx;
-        mty += p.ty;
-    }
-    mqx *= inv_n;
-    mqy *= inv_n;
-    mtx *= inv_n;
-    mty *= inv_n;
-    float sqq = 0, na = 0, nb = 0;
-#pragma unroll 8
-    for (uint32_t i = 0; i < n; i++) {
-        const ps_feature_match &p = m[idx[i]];
-        float qx = p.qx - mqx, qy = p.qy - mqy, tx = p.tx - mtx, ty = p.ty - mty;
-        sqq += qx * qx + qy * qy;
-        na += qx * tx + qy * ty;
-        nb += qx * ty - qy * tx;
-    }
-    float a = 1.0f, b = 0.0f;
-    if (sqq > 1e-5f) {
-        float af = na / sqq, bf = nb / sqq;
-        float s2 = af * af + bf * bf;
-        if (s2 >= 0.04f && s2 <= 25.0f) {
-            a = af;
-            b = bf;
-        }
-    }
-    out->a = a;
-    out->b = b;
-    out->tx = mtx - (a * mqx - b * mqy);
-    out->ty = mty - (b * mqx + a * mqy);
-    return true;
 }
 
 // pipeline.rs:2276-2353: cyclic Jacobi on the symmetric 9x9 AtA; eigenvector of the
@@ -354,6 +309,12 @@ struct Shared {
     uint32_t wave_cnt[kRT / 64];
     uint32_t n_quads, n_draws, draw_pos, total;
     double red[kRT];
+    // chunk staging for the order-dependent sums: all threads load / compute per-point
+    // terms into it, thread 0 adds them in point order (no global loads on its serial path)
+    union {
+        double d[4][kStage];
+        float f[4][2 * kStage];
+    } stage;
 };
 
 // Ordered compaction of the indices i in [0, n) whose predicate holds, into out[];
@@ -380,65 +341,199 @@ __device__ uint32_t ordered_compact(uint32_t n, uint32_t *out, Shared &S, Pred p
     return base;
 }
 
-// Cooperative DLT over m[idx[0..n)] (n >= 4), same arithmetic as the sequential fit:
-// centroid / mean-distance sums by thread 0, the 45 AtA entries each summed by one thread
-// in point order, eigenvector by thread 0. Result broadcast through Hout (shared).
+// pipeline.rs:1900-1974 over m[idx[0..n)]: the sequential f32 sums done by thread 0 from the
+// staging buffer (the points, then the per-point products computed in parallel, in chunks).
+// Every thread calls it; the model is broadcast.
+__device__ bool fit_procrustes_block(const ps_feature_match *m, const uint32_t *idx, uint32_t n, Sim *out,
+                                     Shared &S) {
+    __shared__ float s_r[4];
+    if (n == 0) return false;
+    if (n == 1) {
+        *out = Sim{1.0f, 0.0f, m[idx[0]].dx, m[idx[0]].dy};
+        return true;
+    }
+    const int tid = threadIdx.x;
+    float(*F)[2 * kStage] = S.stage.f;
+    const float inv_n = 1.0f / (float)n;
+    float mqx = 0, mqy = 0, mtx = 0, mty = 0;   // thread 0's running sums
+    for (uint32_t c0 = 0; c0 < n; c0 += kStage) {
+        const uint32_t cn = min(kStage, n - c0);
+        for (uint32_t k = tid; k < cn; k += kRT) {
+            const ps_feature_match &p = m[idx[c0 + k]];
+            F[0][k] = p.qx;
+            F[1][k] = p.qy;
+            F[2][k] = p.tx;
+            F[3][k] = p.ty;
+        }
+        __syncthreads();
+        if (tid == 0) {
+#pragma unroll 8
+            for (uint32_t k = 0; k < cn; k++) {
+                mqx += F[0][k];
+                mqy += F[1][k];
+                mtx += F[2][k];
+                mty += F[3][k];
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        s_r[0] = mqx * inv_n;
+        s_r[1] = mqy * inv_n;
+        s_r[2] = mtx * inv_n;
+        s_r[3] = mty * inv_n;
+    }
+    __syncthreads();
+    const float cqx = s_r[0], cqy = s_r[1], ctx = s_r[2], cty = s_r[3];
+    float sqq = 0, na = 0, nb = 0;   // thread 0
+    for (uint32_t c0 = 0; c0 < n; c0 += kStage) {
+        const uint32_t cn = min(kStage, n - c0);
+        for (uint32_t k = tid; k < cn; k += kRT) {
+            const ps_feature_match &p = m[idx[c0 + k]];
+            const float qx = p.qx - cqx, qy = p.qy - cqy, tx = p.tx - ctx, ty = p.ty - cty;
+            F[0][k] = qx * qx + qy * qy;
+            F[1][k] = qx * tx + qy * ty;
+            F[2][k] = qx * ty - qy * tx;
+        }
+        __syncthreads();
+        if (tid == 0) {
+#pragma unroll 8
+            for (uint32_t k = 0; k < cn; k++) {
+                sqq += F[0][k];
+                na += F[1][k];
+                nb += F[2][k];
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        float a = 1.0f, b = 0.0f;
+        if (sqq > 1e-5f) {
+            const float af = na / sqq, bf = nb / sqq;
+            const float s2 = af * af + bf * bf;
+            if (s2 >= 0.04f && s2 <= 25.0f) {
+                a = af;
+                b = bf;
+            }
+        }
+        s_r[0] = a;
+        s_r[1] = b;
+        s_r[2] = ctx - (a * cqx - b * cqy);
+        s_r[3] = cty - (b * cqx + a * cqy);
+    }
+    __syncthreads();
+    *out = Sim{s_r[0], s_r[1], s_r[2], s_r[3]};
+    __syncthreads();
+    return true;
+}
+
+// Entry j of the two DLT rows of a normalised correspondence (x, y) -> (xp, yp):
+// r1 = (-x, -y, -1, 0, 0, 0, x xp, y xp, xp), r2 = (0, 0, 0, -x, -y, -1, x yp, y yp, yp),
+// selected without a dynamically indexed array (which would live in scratch memory).
+__device__ __forceinline__ double dlt_row1(int j, double x, double y, double xp) {
+    const double v[9] = {-x, -y, -1.0, 0.0, 0.0, 0.0, x * xp, y * xp, xp};
+    double r = v[0];
+#pragma unroll
+    for (int i = 1; i < 9; i++) r = j == i ? v[i] : r;
+    return r;
+}
+__device__ __forceinline__ double dlt_row2(int j, double x, double y, double yp) {
+    const double v[9] = {0.0, 0.0, 0.0, -x, -y, -1.0, x * yp, y * yp, yp};
+    double r = v[0];
+#pragma unroll
+    for (int i = 1; i < 9; i++) r = j == i ? v[i] : r;
+    return r;
+}
+
+// Cooperative DLT over m[idx[0..n)] (n >= 4), same arithmetic as the sequential fit. The points
+// pass through the staging buffer in chunks: thread 0 adds the centroid and mean-distance
+// terms in point order (the distances are computed in parallel), each of the 45 AtA entries is
+// summed by one thread in point order from the staged normalised coordinates, and thread 0
+// takes the eigenvector. Result broadcast through Hout (shared).
 __device__ bool fit_dlt_block(const ps_feature_match *m, const uint32_t *idx, uint32_t n, double *Hout, Shared &S) {
     __shared__ double s_norm[6];
     __shared__ double s_ata[45];
-    __shared__ int s_ok;
+    __shared__ int s_ok, s_bad;
     const int tid = threadIdx.x;
+    if (tid == 0) s_bad = 0;
+    __syncthreads();
+    double cxq = 0, cyq = 0, cxt = 0, cyt = 0;   // thread 0's running sums
+    for (uint32_t c0 = 0; c0 < n; c0 += kStage) {
+        const uint32_t cn = min(kStage, n - c0);
+        for (uint32_t k = tid; k < cn; k += kRT) {
+            const ps_feature_match &p = m[idx[c0 + k]];
+            const double qx = p.qx, qy = p.qy, tx = p.tx, ty = p.ty;
+            if (!isfinite(qx) || !isfinite(qy) || !isfinite(tx) || !isfinite(ty)) s_bad = 1;
+            S.stage.d[0][k] = qx;
+            S.stage.d[1][k] = qy;
+            S.stage.d[2][k] = tx;
+            S.stage.d[3][k] = ty;
+        }
+        __syncthreads();
+        if (tid == 0 && !s_bad) {
+#pragma unroll 8
+            for (uint32_t k = 0; k < cn; k++) {
+                cxq += S.stage.d[0][k];
+                cyq += S.stage.d[1][k];
+                cxt += S.stage.d[2][k];
+                cyt += S.stage.d[3][k];
+            }
+        }
+        __syncthreads();
+    }
+    // a non-finite point fails the fit (the sequential loop stops at the first one)
+    if (s_bad) {
+        __syncthreads();
+        return false;
+    }
+    const double inv_n = 1.0 / (double)n;
     if (tid == 0) {
-        double inv_n = 1.0 / (double)n, cxq = 0, cyq = 0, cxt = 0, cyt = 0;
-        int ok = 1;
-#pragma unroll 8
-        for (uint32_t i = 0; i < n; i++) {
-            const ps_feature_match &p = m[idx[i]];
-            double qx = p.qx, qy = p.qy, tx = p.tx, ty = p.ty;
-            if (!isfinite(qx) || !isfinite(qy) || !isfinite(tx) || !isfinite(ty)) {
-                ok = 0;
-                break;
-            }
-            cxq += qx;
-            cyq += qy;
-            cxt += tx;
-            cyt += ty;
-        }
-        cxq *= inv_n;
-        cyq *= inv_n;
-        cxt *= inv_n;
-        cyt *= inv_n;
-        double mdq = 0, mdt = 0;
-        if (ok) {
-#pragma unroll 8
-            for (uint32_t i = 0; i < n; i++) {
-                const ps_feature_match &p = m[idx[i]];
-                double dxq = (double)p.qx - cxq, dyq = (double)p.qy - cyq, dxt = (double)p.tx - cxt,
-                       dyt = (double)p.ty - cyt;
-                mdq += sqrt(dxq * dxq + dyq * dyq);
-                mdt += sqrt(dxt * dxt + dyt * dyt);
-            }
-            mdq *= inv_n;
-            mdt *= inv_n;
-            if (!isfinite(mdq) || !isfinite(mdt) || mdq < 1e-6 || mdt < 1e-6) ok = 0;
-        }
-        const double SQ2 = 1.41421356237309504880;
-        s_norm[0] = cxq;
-        s_norm[1] = cyq;
-        s_norm[2] = cxt;
-        s_norm[3] = cyt;
-        s_norm[4] = ok ? SQ2 / mdq : 0.0;
-        s_norm[5] = ok ? SQ2 / mdt : 0.0;
-        s_ok = ok;
+        s_norm[0] = cxq * inv_n;
+        s_norm[1] = cyq * inv_n;
+        s_norm[2] = cxt * inv_n;
+        s_norm[3] = cyt * inv_n;
     }
     __syncthreads();
+    {
+        const double mxq = s_norm[0], myq = s_norm[1], mxt = s_norm[2], myt = s_norm[3];
+        double mdq = 0, mdt = 0;
+        for (uint32_t c0 = 0; c0 < n; c0 += kStage) {
+            const uint32_t cn = min(kStage, n - c0);
+            for (uint32_t k = tid; k < cn; k += kRT) {
+                const ps_feature_match &p = m[idx[c0 + k]];
+                const double dxq = (double)p.qx - mxq, dyq = (double)p.qy - myq, dxt = (double)p.tx - mxt,
+                             dyt = (double)p.ty - myt;
+                S.stage.d[0][k] = sqrt(dxq * dxq + dyq * dyq);
+                S.stage.d[1][k] = sqrt(dxt * dxt + dyt * dyt);
+            }
+            __syncthreads();
+            if (tid == 0) {
+#pragma unroll 8
+                for (uint32_t k = 0; k < cn; k++) {
+                    mdq += S.stage.d[0][k];
+                    mdt += S.stage.d[1][k];
+                }
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            mdq *= inv_n;
+            mdt *= inv_n;
+            const bool ok = !(!isfinite(mdq) || !isfinite(mdt) || mdq < 1e-6 || mdt < 1e-6);
+            const double SQ2 = 1.41421356237309504880;
+            s_norm[4] = ok ? SQ2 / mdq : 0.0;
+            s_norm[5] = ok ? SQ2 / mdt : 0.0;
+            s_ok = ok;
+        }
+        __syncthreads();
+    }
     if (!s_ok) {
         __syncthreads();
         return false;
     }
-    if (tid < 45) {
+    {
         // entry (a, b), a <= b, in row-major upper-triangle order
-        int a = 0, r = tid;
+        int a = 0, r = tid < 45 ? tid : 0;
         while (r >= 9 - a) {
             r -= 9 - a;
             a++;
@@ -447,16 +542,27 @@ __device__ bool fit_dlt_block(const ps_feature_match *m, const uint32_t *idx, ui
         const double cxq = s_norm[0], cyq = s_norm[1], cxt = s_norm[2], cyt = s_norm[3], sq = s_norm[4],
                      stt = s_norm[5];
         double acc = 0.0;
+        for (uint32_t c0 = 0; c0 < n; c0 += kStage) {
+            const uint32_t cn = min(kStage, n - c0);
+            for (uint32_t k = tid; k < cn; k += kRT) {
+                const ps_feature_match &p = m[idx[c0 + k]];
+                S.stage.d[0][k] = ((double)p.qx - cxq) * sq;
+                S.stage.d[1][k] = ((double)p.qy - cyq) * sq;
+                S.stage.d[2][k] = ((double)p.tx - cxt) * stt;
+                S.stage.d[3][k] = ((double)p.ty - cyt) * stt;
+            }
+            __syncthreads();
+            if (tid < 45) {
 #pragma unroll 8
-        for (uint32_t i = 0; i < n; i++) {
-            const ps_feature_match &p = m[idx[i]];
-            double x = ((double)p.qx - cxq) * sq, y = ((double)p.qy - cyq) * sq;
-            double xp = ((double)p.tx - cxt) * stt, yp = ((double)p.ty - cyt) * stt;
-            double r1[9] = {-x, -y, -1.0, 0.0, 0.0, 0.0, x * xp, y * xp, xp};
-            double r2[9] = {0.0, 0.0, 0.0, -x, -y, -1.0, x * yp, y * yp, yp};
-            acc += r1[a] * r1[b] + r2[a] * r2[b];
+                for (uint32_t k = 0; k < cn; k++) {
+                    const double x = S.stage.d[0][k], y = S.stage.d[1][k], xp = S.stage.d[2][k], yp = S.stage.d[3][k];
+                    acc += dlt_row1(a, x, y, xp) * dlt_row1(b, x, y, xp) +
+                           dlt_row2(a, x, y, yp) * dlt_row2(b, x, y, yp);
+                }
+            }
+            __syncthreads();
         }
-        s_ata[tid] = acc;
+        if (tid < 45) s_ata[tid] = acc;
     }
     __syncthreads();
     if (tid == 0) {
@@ -497,45 +603,53 @@ __device__ bool fit_dlt_block(const ps_feature_match *m, const uint32_t *idx, ui
     return ok;
 }
 
-// Cooperative sequential-order scoring: per-match errors in parallel, then thread 0 adds
-// them in index order. Returns the count; *err / *mass broadcast through shared memory.
+// Cooperative sequential-order scoring: per-match errors (and weights) computed in parallel
+// into the staging buffer, chunk by chunk, and added by thread 0 in index order. Returns the
+// count; *err / *mass broadcast through shared memory.
 __device__ uint32_t eval_h_block(const ps_feature_match *m, uint32_t n, const double *H, double tsq,
-                                 const float *wts, double *ebuf, double *err, double *mass) {
+                                 const float *wts, double *err, double *mass, Shared &S) {
     __shared__ uint32_t s_c;
     __shared__ double s_e, s_m;
     const int tid = threadIdx.x;
-    for (uint32_t i = tid; i < n; i += kRT) {
-        double e = reproj_err(m[i], H);
-        ebuf[i] = (e >= 0.0 && e <= tsq) ? e : -1.0;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t c = 0;
-        double es = 0.0, ms = 0.0;
+    uint32_t c = 0;
+    double es = 0.0, ms = 0.0;
+    for (uint32_t c0 = 0; c0 < n; c0 += kStage) {
+        const uint32_t cn = min(kStage, n - c0);
+        for (uint32_t k = tid; k < cn; k += kRT) {
+            const double e = reproj_err(m[c0 + k], H);
+            S.stage.d[0][k] = (e >= 0.0 && e <= tsq) ? e : -1.0;
+            if (wts) S.stage.d[1][k] = (double)wts[c0 + k];
+        }
+        __syncthreads();
+        if (tid == 0) {
 #pragma unroll 8
-        for (uint32_t i = 0; i < n; i++) {
-            double e = ebuf[i];
-            if (e >= 0.0) {
-                c++;
-                es += e;
-                if (wts) ms += (double)wts[i];
+            for (uint32_t k = 0; k < cn; k++) {
+                const double e = S.stage.d[0][k];
+                if (e >= 0.0) {
+                    c++;
+                    es += e;
+                    if (wts) ms += S.stage.d[1][k];
+                }
             }
         }
+        __syncthreads();
+    }
+    if (tid == 0) {
         s_c = c;
         s_e = es;
         s_m = ms;
     }
     __syncthreads();
-    const uint32_t c = s_c;
+    const uint32_t cc = s_c;
     *err = s_e;
     if (mass) *mass = s_m;
     __syncthreads();
-    return c;
+    return cc;
 }
 
 // pipeline.rs:2605-2630 (two refits on the current inliers while they improve)
 __device__ void local_opt_block(const ps_feature_match *m, uint32_t n, double *H, uint32_t *bc, double *be,
-                                double tsq, uint32_t *ibuf, double *ebuf, Shared &S) {
+                                double tsq, uint32_t *ibuf, Shared &S) {
     __shared__ double s_R[9];
     for (int it = 0; it < 2; it++) {
         const uint32_t ni = ordered_compact(n, ibuf, S, [&](uint32_t i) {
@@ -545,7 +659,7 @@ __device__ void local_opt_block(const ps_feature_match *m, uint32_t n, double *H
         if (ni < 4) break;
         if (!fit_dlt_block(m, ibuf, ni, s_R, S)) break;
         double re;
-        const uint32_t rc = eval_h_block(m, n, s_R, tsq, nullptr, ebuf, &re, nullptr);
+        const uint32_t rc = eval_h_block(m, n, s_R, tsq, nullptr, &re, nullptr, S);
         if (rc > *bc || (rc == *bc && re < *be)) {
             __syncthreads();
             if (threadIdx.x < 9) H[threadIdx.x] = s_R[threadIdx.x];
@@ -583,8 +697,7 @@ __global__ __launch_bounds__(kRT) void ransac_kernel(const ps_feature_match *mat
     const uint32_t n = P.n;
     const ps_feature_match *m = matches + P.offset;
     const float *wts = prm.sampler == 2 ? weights + P.offset : nullptr;
-    double *cdf = work_d + (size_t)P.offset * 2;           // n doubles
-    double *ebuf = cdf + n;                                 // n doubles
+    double *cdf = work_d + (size_t)P.offset * 2;           // n doubles (then n unused)
     uint32_t *ibuf = work_u + (size_t)P.offset * 2;         // n u32
     uint32_t *sim_inl = ibuf + n;                           // n u32
     uint32_t *out_inl = inliers + P.offset;
@@ -605,10 +718,14 @@ __global__ __launch_bounds__(kRT) void ransac_kernel(const ps_feature_match *mat
     bool have_sim = false;
     uint32_t n_sim = 0;
     // reference weights 1 / clamp(ratio) (pipeline.rs:2006-2028), prefix sums in f64 in order
+    // (weights computed in parallel into the staging buffer, the running sum by thread 0 in
+    // order, the normalisation in parallel)
     auto build_cdf = [&](bool ref_weights) {
-        if (tid == 0) {
-            double acc = 0.0;
-            for (uint32_t i = 0; i < n; i++) {
+        double acc = 0.0;   // thread 0
+        for (uint32_t c0 = 0; c0 < n; c0 += kStage) {
+            const uint32_t cn = min(kStage, n - c0);
+            for (uint32_t k = tid; k < cn; k += kRT) {
+                const uint32_t i = c0 + k;
                 double wv;
                 if (!ref_weights) {
                     wv = (double)wts[i];
@@ -621,15 +738,26 @@ __global__ __launch_bounds__(kRT) void ransac_kernel(const ps_feature_match *mat
                     }
                     wv = 1.0 / r;
                 }
-                acc += wv;
-                cdf[i] = acc;
+                S.stage.d[0][k] = wv;
             }
-            if (acc > 0.0) {
-                double inv = 1.0 / acc;
-                for (uint32_t i = 0; i < n; i++) cdf[i] *= inv;
+            __syncthreads();
+            if (tid == 0) {
+#pragma unroll 8
+                for (uint32_t k = 0; k < cn; k++) {
+                    acc += S.stage.d[0][k];
+                    cdf[c0 + k] = acc;
+                }
+                S.red[0] = acc;
             }
-            cdf[n - 1] = 1.0;
+            __syncthreads();
         }
+        const double total = S.red[0];
+        if (total > 0.0) {
+            const double inv = 1.0 / total;
+            for (uint32_t i = tid; i < n; i += kRT) cdf[i] *= inv;
+        }
+        __syncthreads();
+        if (tid == 0) cdf[n - 1] = 1.0;
         __syncthreads();
     };
 
@@ -740,13 +868,16 @@ __global__ __launch_bounds__(kRT) void ransac_kernel(const ps_feature_match *mat
             uint32_t ni = ordered_compact(n, sim_inl, S, [&](uint32_t i) { return sim_inlier(m[i], best_sim, ftsq); });
             __shared__ float s_ref[4];
             __shared__ int s_has_ref;
-            if (tid == 0) {
+            {
                 Sim ref;
-                s_has_ref = fit_procrustes(m, sim_inl, ni, &ref);
-                s_ref[0] = ref.a;
-                s_ref[1] = ref.b;
-                s_ref[2] = ref.tx;
-                s_ref[3] = ref.ty;
+                const bool hr = fit_procrustes_block(m, sim_inl, ni, &ref, S);
+                if (tid == 0) {
+                    s_has_ref = hr;
+                    s_ref[0] = ref.a;
+                    s_ref[1] = ref.b;
+                    s_ref[2] = ref.tx;
+                    s_ref[3] = ref.ty;
+                }
             }
             __syncthreads();
             const bool has_ref = s_has_ref;
@@ -759,16 +890,18 @@ __global__ __launch_bounds__(kRT) void ransac_kernel(const ps_feature_match *mat
                 }
                 __syncthreads();
             }
-            if (tid == 0) {
+            {
                 Sim fin;
-                int hf = fit_procrustes(m, sim_inl, ni, &fin);
-                if (hf) {
-                    s_ref[0] = fin.a;
-                    s_ref[1] = fin.b;
-                    s_ref[2] = fin.tx;
-                    s_ref[3] = fin.ty;
+                const bool hf = fit_procrustes_block(m, sim_inl, ni, &fin, S);
+                if (tid == 0) {
+                    if (hf) {
+                        s_ref[0] = fin.a;
+                        s_ref[1] = fin.b;
+                        s_ref[2] = fin.tx;
+                        s_ref[3] = fin.ty;
+                    }
+                    s_has_ref = hf || has_ref;
                 }
-                s_has_ref = hf || has_ref;
             }
             __syncthreads();
             have_sim = s_has_ref;
@@ -804,7 +937,7 @@ __global__ __launch_bounds__(kRT) void ransac_kernel(const ps_feature_match *mat
         }
         __syncthreads();
         have = true;
-        bc = eval_h_block(m, n, s_H, tsq, nullptr, ebuf, &be, nullptr);
+        bc = eval_h_block(m, n, s_H, tsq, nullptr, &be, nullptr, S);
     }
     if (n < 4) {
         if (prm.sim_seed_model) {
@@ -827,12 +960,18 @@ __global__ __launch_bounds__(kRT) void ransac_kernel(const ps_feature_match *mat
     }
     double wsum = 0.0;
     if (sampler == 2) {
-        if (tid == 0) {
-            double acc = 0.0;
-            for (uint32_t i = 0; i < n; i++) acc += (double)wts[i];
-            S.red[0] = acc;
+        double acc = 0.0;   // thread 0, in order
+        for (uint32_t c0 = 0; c0 < n; c0 += kStage) {
+            const uint32_t cn = min(kStage, n - c0);
+            for (uint32_t k = tid; k < cn; k += kRT) S.stage.d[0][k] = (double)wts[c0 + k];
+            __syncthreads();
+            if (tid == 0) {
+#pragma unroll 8
+                for (uint32_t k = 0; k < cn; k++) acc += S.stage.d[0][k];
+                S.red[0] = acc;
+            }
+            __syncthreads();
         }
-        __syncthreads();
         wsum = S.red[0];
         __syncthreads();
     }
@@ -849,7 +988,7 @@ __global__ __launch_bounds__(kRT) void ransac_kernel(const ps_feature_match *mat
         double bmass = 0.0;
         if (prm.adaptive && sampler == 2 && have) {
             double e2;
-            eval_h_block(m, n, s_H, tsq, wts, ebuf, &e2, &bmass);
+            eval_h_block(m, n, s_H, tsq, wts, &e2, &bmass, S);
         }
         if (!prm.adaptive) max_it = cap;
         else if (sampler == 2) max_it = have && wsum > 0 ? bound_from(bc, bmass) : cap;
@@ -950,14 +1089,14 @@ __global__ __launch_bounds__(kRT) void ransac_kernel(const ps_feature_match *mat
                 __syncthreads();
                 if (tid < 9) s_H[tid] = h.H[tid];
                 __syncthreads();
-                if (prm.local_opt) local_opt_block(m, n, s_H, &c, &e, tsq, ibuf, ebuf, S);
+                if (prm.local_opt) local_opt_block(m, n, s_H, &c, &e, tsq, ibuf, S);
                 bc = c;
                 be = e;
                 have = true;
                 if (prm.adaptive) {
                     if (sampler == 2) {
                         double e2, bmass = 0.0;
-                        eval_h_block(m, n, s_H, tsq, wts, ebuf, &e2, &bmass);
+                        eval_h_block(m, n, s_H, tsq, wts, &e2, &bmass, S);
                         max_it = wsum > 0 ? bound_from(bc, bmass) : cap;
                     } else {
                         max_it = bound_from(bc, 0.0);
@@ -973,7 +1112,7 @@ __global__ __launch_bounds__(kRT) void ransac_kernel(const ps_feature_match *mat
         ni = n_sim;
         for (uint32_t i = tid; i < ni; i += kRT) out_inl[i] = sim_inl[i];
     } else {
-        if (prm.local_opt) local_opt_block(m, n, s_H, &bc, &be, tsq, ibuf, ebuf, S);
+        if (prm.local_opt) local_opt_block(m, n, s_H, &bc, &be, tsq, ibuf, S);
         ni = ordered_compact(n, out_inl, S, [&](uint32_t i) {
             double e = reproj_err(m[i], s_H);
             return e >= 0.0 && e <= tsq;
