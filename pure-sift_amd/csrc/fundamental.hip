@@ -647,7 +647,9 @@ struct FundShared {
     double herr[kFT];
     uint32_t hcnt[kFT], hok[kFT];
     uint32_t quad[kFT][8], qvalid[kFT];
-    uint32_t draws[1024];
+    uint32_t draws[1024 + 32];   // sampler draws plus one sample's worth of lookahead
+    uint8_t len[1024];           // draws consumed by the sample starting at each position
+    uint32_t start[kFT];         // buffer position of each sample of the batch
     uint32_t n_quads, n_draws, draw_pos, wave_cnt;
     double best[9], be;
     uint32_t bc, have, ok, ni;
@@ -764,6 +766,31 @@ __device__ uint32_t eval_f_block(const ps_feature_match *m, uint32_t n, const do
     return c;
 }
 
+// The sample that starts at draws[0]: distinct match indices in draw order until K of them
+// (valid) or A attempts (invalid); returns the number of draws it consumes. The sampler's
+// sequential grouping is the chain of such samples, each starting where the previous ended.
+template <int K, int A>
+__device__ __forceinline__ uint32_t scan_sample(const uint32_t *draws, uint32_t (&q)[K], bool &valid) {
+    uint32_t ch = 0, at = 0;
+#pragma unroll
+    for (int j = 0; j < K; j++) q[j] = 0;
+    while (true) {
+        const uint32_t c = draws[at++];
+        bool dup = false;
+#pragma unroll
+        for (int j = 0; j < K; j++) dup |= (uint32_t)j < ch && q[j] == c;
+        if (!dup) {
+#pragma unroll
+            for (int j = 0; j < K; j++)
+                if ((uint32_t)j == ch) q[j] = c;
+            ch++;
+        }
+        if (ch == (uint32_t)K || at == (uint32_t)A) break;
+    }
+    valid = ch == (uint32_t)K;
+    return at;
+}
+
 __global__ __launch_bounds__(kFT) void fundamental_kernel(const ps_feature_match *matches,
                                                           const ps_fundamental_problem *probs, double thresh_in,
                                                           uint32_t max_iters, double confidence, double *work_d,
@@ -847,55 +874,53 @@ __global__ __launch_bounds__(kFT) void fundamental_kernel(const ps_feature_match
     __syncthreads();
     const uint32_t cap = max_iters > 64 ? max_iters : 64, min_it = 64 < cap ? 64 : cap;
     uint32_t max_it = cap, it = 0;
-    uint64_t draw = 0;
+    // the SplitMix64 stream: the buffer holds draws [base, base + 1024 + 32) and the next sample
+    // starts at stream position pos
+    uint64_t base = 0, pos = 0;
+    bool filled = false;
     while (it < max_it) {
-        // 1. group draws into the next kFT iterations (geometry.rs:929-940), thread 0
-        if (lane == 0) {
-            S.n_quads = 0;
-            S.n_draws = 0;
-            S.draw_pos = 0;
-        }
-        __syncthreads();
-        {
-            uint32_t chosen = 0, att = 0;
-            uint32_t q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            while (true) {
-                if (S.draw_pos >= S.n_draws) {
-                    __syncthreads();
-                    for (int k = lane; k < 1024; k += kFT) {
-                        const uint64_t z = fsplitmix_at(P.seed, draw + (uint64_t)k);
-                        S.draws[k] = fcdf_pick(cdf, n, (double)(z >> 11) * (1.0 / 9007199254740992.0));
-                    }
-                    __syncthreads();
-                    if (lane == 0) {
-                        S.n_draws = 1024;
-                        S.draw_pos = 0;
-                    }
-                    draw += 1024;
-                    __syncthreads();
-                }
-                if (lane == 0) {
-                    while (S.draw_pos < S.n_draws && S.n_quads < (uint32_t)kFT) {
-                        const uint32_t c = S.draws[S.draw_pos++];
-                        att++;
-                        bool dup = false;
-                        for (uint32_t j = 0; j < chosen; j++) dup |= q[j] == c;
-                        if (!dup) q[chosen++] = c;
-                        if (chosen == 8 || att == 32) {
-                            const uint32_t k = S.n_quads++;
-                            S.qvalid[k] = chosen == 8;
-                            for (int j = 0; j < 8; j++) S.quad[k][j] = q[j];
-                            chosen = 0;
-                            att = 0;
-                        }
-                    }
+        // 1. the next kFT samples (geometry.rs:929-940: 8 distinct matches within 32 draws), the
+        // sequential grouping done in parallel: every buffer position gets the length of a sample
+        // starting there, lane 0 follows the chain of sample starts, each lane rebuilds one sample
+        uint32_t nq = 0;
+        while (nq < (uint32_t)kFT) {
+            if (!filled || pos >= base + 1024) {
+                base = pos;
+                filled = true;
+                for (uint32_t k = lane; k < 1024 + 32; k += kFT) {
+                    const uint64_t z = fsplitmix_at(P.seed, base + k);
+                    S.draws[k] = fcdf_pick(cdf, n, (double)(z >> 11) * (1.0 / 9007199254740992.0));
                 }
                 __syncthreads();
-                if (S.n_quads >= (uint32_t)kFT) break;
+                for (uint32_t k = lane; k < 1024; k += kFT) {
+                    uint32_t qq[8];
+                    bool v;
+                    S.len[k] = (uint8_t)scan_sample<8, 32>(S.draws + k, qq, v);
+                }
                 __syncthreads();
             }
+            if (lane == 0) {
+                uint64_t p = pos;
+                uint32_t k = nq;
+                while (p < base + 1024 && k < (uint32_t)kFT) {
+                    S.start[k++] = (uint32_t)(p - base);
+                    p += S.len[p - base];
+                }
+                S.n_quads = k;
+                S.draw_pos = (uint32_t)(p - base);
+            }
             __syncthreads();
-            draw -= (uint64_t)(S.n_draws - S.draw_pos);   // rewind the unconsumed draws
+            const uint32_t nq2 = S.n_quads;
+            if ((uint32_t)lane >= nq && (uint32_t)lane < nq2) {
+                uint32_t qq[8];
+                bool v;
+                scan_sample<8, 32>(S.draws + S.start[lane], qq, v);
+                S.qvalid[lane] = v;
+#pragma unroll
+                for (int j = 0; j < 8; j++) S.quad[lane][j] = qq[j];
+            }
+            pos = base + S.draw_pos;
+            nq = nq2;
             __syncthreads();
         }
         // 2. fit and score one hypothesis per lane

@@ -28,6 +28,7 @@ namespace {
 
 constexpr int kRT = 256;            // threads per problem
 constexpr uint32_t kStage = 512;   // points per staging chunk
+constexpr uint32_t kDraws = 1024;  // sampler draws per buffer
 constexpr uint64_t kGamma = 0x9E3779B97F4A7C15ull;
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
@@ -295,6 +296,31 @@ __device__ uint32_t eval_h_seq(const ps_feature_match *m, uint32_t n, const doub
     return c;
 }
 
+// The sample that starts at draws[0]: distinct match indices in draw order until K of them
+// (valid) or A attempts (invalid); returns the number of draws it consumes. The sampler's
+// sequential grouping is the chain of such samples, each starting where the previous ended.
+template <int K, int A>
+__device__ __forceinline__ uint32_t scan_sample(const uint32_t *draws, uint32_t (&q)[K], bool &valid) {
+    uint32_t ch = 0, at = 0;
+#pragma unroll
+    for (int j = 0; j < K; j++) q[j] = 0;
+    while (true) {
+        const uint32_t c = draws[at++];
+        bool dup = false;
+#pragma unroll
+        for (int j = 0; j < K; j++) dup |= (uint32_t)j < ch && q[j] == c;
+        if (!dup) {
+#pragma unroll
+            for (int j = 0; j < K; j++)
+                if ((uint32_t)j == ch) q[j] = c;
+            ch++;
+        }
+        if (ch == (uint32_t)K || at == (uint32_t)A) break;
+    }
+    valid = ch == (uint32_t)K;
+    return at;
+}
+
 struct Hyp {
     double H[9];
     double err, mass;
@@ -305,7 +331,9 @@ struct Shared {
     Hyp hyp[kRT];
     uint32_t quad[kRT][4];
     uint32_t qvalid[kRT];
-    uint32_t draws[1024];
+    uint32_t draws[kDraws + 16];   // a buffer of sampler draws plus one sample's worth of lookahead
+    uint8_t len[kDraws];           // draws consumed by the sample starting at each position
+    uint32_t start[kRT];           // buffer position of each sample of the batch
     uint32_t wave_cnt[kRT / 64];
     uint32_t n_quads, n_draws, draw_pos, total;
     double red[kRT];
@@ -995,57 +1023,54 @@ __global__ __launch_bounds__(kRT) void ransac_kernel(const ps_feature_match *mat
         else max_it = bound_from(bc, 0.0);
     }
     uint32_t it = 0;
-    uint64_t draw = 0;   // next position in the homography SplitMix64 stream
+    // the homography SplitMix64 stream: the buffer holds draws [base, base + kDraws + 16) and the
+    // next sample starts at stream position pos (both uniform)
+    uint64_t base = 0, pos = 0;
+    bool filled = false;
     while (it < max_it) {
-        // 1. group draws into the next kRT iterations (thread 0; draws computed in parallel)
-        if (tid == 0) {
-            S.n_quads = 0;
-            S.n_draws = 0;
-            S.draw_pos = 0;
-        }
-        __syncthreads();
-        {
-            uint32_t chosen = 0, att = 0;
-            uint32_t q[4] = {0, 0, 0, 0};
-            while (true) {
-                if (S.draw_pos >= S.n_draws) {
-                    __syncthreads();
-                    // refill 1024 draws
-                    for (int k = tid; k < 1024; k += kRT) {
-                        double u = unit_draw(splitmix_at(P.hom_seed, draw + (uint64_t)k));
-                        S.draws[k] = sampler == 1 ? uniform_pick(n, u) : cdf_pick(cdf, n, u);
-                    }
-                    __syncthreads();
-                    if (tid == 0) {
-                        S.n_draws = 1024;
-                        S.draw_pos = 0;
-                    }
-                    draw += 1024;
-                    __syncthreads();
-                }
-                if (tid == 0) {
-                    while (S.draw_pos < S.n_draws && S.n_quads < kRT) {
-                        const uint32_t c = S.draws[S.draw_pos++];
-                        att++;
-                        bool dup = false;
-                        for (uint32_t j = 0; j < chosen; j++) dup |= q[j] == c;
-                        if (!dup) q[chosen++] = c;
-                        if (chosen == 4 || att == 16) {
-                            const uint32_t k = S.n_quads++;
-                            S.qvalid[k] = chosen == 4;
-                            for (int j = 0; j < 4; j++) S.quad[k][j] = q[j];
-                            chosen = 0;
-                            att = 0;
-                        }
-                    }
+        // 1. the next kRT samples (4 distinct matches within 16 draws each), the sequential
+        // grouping done in parallel: every buffer position gets the length of a sample starting
+        // there, thread 0 follows the chain of sample starts from pos, and each thread rebuilds
+        // one sample; a chain that reaches the end of the buffer refills it from there
+        uint32_t nq = 0;
+        while (nq < (uint32_t)kRT) {
+            if (!filled || pos >= base + kDraws) {
+                base = pos;
+                filled = true;
+                for (uint32_t k = tid; k < kDraws + 16; k += kRT) {
+                    const double u = unit_draw(splitmix_at(P.hom_seed, base + k));
+                    S.draws[k] = sampler == 1 ? uniform_pick(n, u) : cdf_pick(cdf, n, u);
                 }
                 __syncthreads();
-                if (S.n_quads >= (uint32_t)kRT) break;
+                for (uint32_t k = tid; k < kDraws; k += kRT) {
+                    uint32_t qq[4];
+                    bool v;
+                    S.len[k] = (uint8_t)scan_sample<4, 16>(S.draws + k, qq, v);
+                }
                 __syncthreads();
             }
-            // unconsumed draws of the buffer: rewind the stream position to the first one
+            if (tid == 0) {
+                uint64_t p = pos;
+                uint32_t k = nq;
+                while (p < base + kDraws && k < (uint32_t)kRT) {
+                    S.start[k++] = (uint32_t)(p - base);
+                    p += S.len[p - base];
+                }
+                S.n_quads = k;
+                S.total = (uint32_t)(p - base);
+            }
             __syncthreads();
-            draw -= (uint64_t)(S.n_draws - S.draw_pos);
+            const uint32_t nq2 = S.n_quads;
+            if ((uint32_t)tid >= nq && (uint32_t)tid < nq2) {
+                uint32_t qq[4];
+                bool v;
+                scan_sample<4, 16>(S.draws + S.start[tid], qq, v);
+                S.qvalid[tid] = v;
+#pragma unroll
+                for (int j = 0; j < 4; j++) S.quad[tid][j] = qq[j];
+            }
+            pos = base + S.total;
+            nq = nq2;
             __syncthreads();
         }
         // 2. evaluate the kRT hypotheses, one per thread
