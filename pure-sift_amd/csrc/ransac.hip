@@ -334,6 +334,9 @@ struct Shared {
     uint32_t draws[kDraws + 16];   // a buffer of sampler draws plus one sample's worth of lookahead
     uint8_t len[kDraws];           // draws consumed by the sample starting at each position
     uint32_t start[kRT];           // buffer position of each sample of the batch
+    uint32_t scan_c[kRT];          // replay filter: prefix-best count (+ validity bit 31)
+    uint32_t cand[kRT];            // replay filter: hypotheses that may improve the best
+    uint32_t n_cand;
     uint32_t wave_cnt[kRT / 64];
     uint32_t n_quads, n_draws, draw_pos, total;
     double red[kRT];
@@ -1103,32 +1106,80 @@ __global__ __launch_bounds__(kRT) void ransac_kernel(const ps_feature_match *mat
             }
         }
         __syncthreads();
-        // 3. replay in order
-        for (int k = 0; k < kRT && it < max_it; k++) {
-            it++;
-            const Hyp &h = S.hyp[k];
-            if (!h.valid) continue;
-            uint32_t c = h.count;
-            double e = h.err;
-            if (c > bc || (c == bc && e < be)) {
-                __syncthreads();
-                if (tid < 9) s_H[tid] = h.H[tid];
-                __syncthreads();
-                if (prm.local_opt) local_opt_block(m, n, s_H, &c, &e, tsq, ibuf, S);
-                bc = c;
-                be = e;
-                have = true;
-                if (prm.adaptive) {
-                    if (sampler == 2) {
-                        double e2, bmass = 0.0;
-                        eval_h_block(m, n, s_H, tsq, wts, &e2, &bmass, S);
-                        max_it = wsum > 0 ? bound_from(bc, bmass) : cap;
-                    } else {
-                        max_it = bound_from(bc, 0.0);
-                    }
+        // 3. replay in order. Only a hypothesis better than every earlier valid one of the batch
+        // and than the best at the batch start can improve the best (the best never gets worse),
+        // so a parallel prefix-best scan picks those; the others only advance the count, which
+        // is settled at the end exactly as the sequential loop would leave it.
+        {
+            const Hyp &mine = S.hyp[tid];
+            const bool mv = mine.valid;
+            uint32_t pc = mv ? (mine.count | 0x80000000u) : 0u;   // prefix best, inclusive
+            double pe = mv ? mine.err : 0.0;
+            S.scan_c[tid] = pc;
+            S.red[tid] = pe;
+            __syncthreads();
+            for (int o = 1; o < kRT; o <<= 1) {
+                uint32_t oc = 0;
+                double oe = 0.0;
+                if (tid >= o) {
+                    oc = S.scan_c[tid - o];
+                    oe = S.red[tid - o];
                 }
                 __syncthreads();
+                // the better of (oc, oe) and (pc, pe); an invalid one (bit 31 clear) never wins
+                if ((oc >> 31) && (!(pc >> 31) || (oc & 0x7fffffffu) > (pc & 0x7fffffffu) ||
+                                   ((oc & 0x7fffffffu) == (pc & 0x7fffffffu) && oe < pe))) {
+                    pc = oc;
+                    pe = oe;
+                }
+                S.scan_c[tid] = pc;
+                S.red[tid] = pe;
+                __syncthreads();
             }
+            bool cnd = mv && (mine.count > bc || (mine.count == bc && mine.err < be));
+            if (cnd && tid > 0) {
+                const uint32_t qc = S.scan_c[tid - 1];
+                const double qe = S.red[tid - 1];
+                if (qc >> 31) cnd = mine.count > (qc & 0x7fffffffu) || (mine.count == (qc & 0x7fffffffu) && mine.err < qe);
+            }
+            if (tid == 0) S.n_cand = 0;
+            __syncthreads();
+            const uint32_t nc = ordered_compact((uint32_t)kRT, S.cand, S, [&](uint32_t) { return cnd; });
+            const uint32_t it0 = it;   // the sequential loop visits k = 0, 1, ... while it0 + k < max_it
+            int k_last = -1;
+            for (uint32_t ci = 0; ci < nc; ci++) {
+                const uint32_t k = S.cand[ci];
+                if (it0 + k >= max_it) break;
+                k_last = (int)k;
+                const Hyp &h = S.hyp[k];
+                uint32_t c = h.count;
+                double e = h.err;
+                if (c > bc || (c == bc && e < be)) {
+                    __syncthreads();
+                    if (tid < 9) s_H[tid] = h.H[tid];
+                    __syncthreads();
+                    if (prm.local_opt) local_opt_block(m, n, s_H, &c, &e, tsq, ibuf, S);
+                    bc = c;
+                    be = e;
+                    have = true;
+                    if (prm.adaptive) {
+                        if (sampler == 2) {
+                            double e2, bmass = 0.0;
+                            eval_h_block(m, n, s_H, tsq, wts, &e2, &bmass, S);
+                            max_it = wsum > 0 ? bound_from(bc, bmass) : cap;
+                        } else {
+                            max_it = bound_from(bc, 0.0);
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            // the iterations it counts: up to the batch end or the (final) bound, at least past
+            // the last visited hypothesis
+            int64_t jend = (int64_t)max_it - (int64_t)it0;
+            if (jend < (int64_t)(k_last + 1)) jend = k_last + 1;
+            if (jend > kRT) jend = kRT;
+            it = it0 + (uint32_t)jend;
         }
         __syncthreads();
     }
