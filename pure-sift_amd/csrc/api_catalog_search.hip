@@ -32,7 +32,7 @@ struct ps_catalog {
     std::vector<uint32_t> width, height;
     std::vector<uint32_t> off{0};                     // n_entries + 1 feature offsets
     std::unordered_map<std::string, size_t> index;    // id -> entry (last wins, catalog.rs:284-297)
-    DevBuf feat, desc, sig, off_d, cnt_d;
+    DevBuf feat, desc, sig, off_d;
     size_t n_features() const { return off.back(); }
 };
 
@@ -57,7 +57,6 @@ void free_catalog(ps_catalog *c) {
     free_buf(c->desc);
     free_buf(c->sig);
     free_buf(c->off_d);
-    free_buf(c->cnt_d);
 }
 
 // std::str::from_utf8 acceptance: RFC 3629, no overlongs, no surrogates, <= U+10FFFF
@@ -113,12 +112,8 @@ struct EntrySrc {
 
 ps_status upload_offsets(ps_catalog *c, hipStream_t st) {
     const size_t ne = c->ids.size();
-    std::vector<uint32_t> cnt(ne);
-    for (size_t e = 0; e < ne; e++) cnt[e] = c->off[e + 1] - c->off[e];
     PS_TRY(ensure(c->off_d, (ne + 1) * sizeof(uint32_t)));
-    PS_TRY(ensure(c->cnt_d, std::max<size_t>(ne, 1) * sizeof(uint32_t)));
     PS_HIP(hipMemcpyAsync(c->off_d.p, c->off.data(), (ne + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-    if (ne) PS_HIP(hipMemcpyAsync(c->cnt_d.p, cnt.data(), ne * sizeof(uint32_t), hipMemcpyHostToDevice, st));
     return PS_OK;
 }
 
@@ -345,8 +340,7 @@ ps_status search_impl(ps_engine *e, const ps_catalog *c, const ps_feature *d_qf,
     if (ne == 0 || nq == 0 || top_k == 0) return PS_OK;   // catalog.rs:584-586
     if (nq > kMaxQueries) return set_err(PS_E_INVALID_ARG, "at most 262144 query features per catalog search");
     hipStream_t st = e->stream;
-    DevBuf &cat_feat = const_cast<DevBuf &>(c->feat), &cat_desc = const_cast<DevBuf &>(c->desc),
-           &cat_cnt = const_cast<DevBuf &>(c->cnt_d);
+    DevBuf &cat_feat = const_cast<DevBuf &>(c->feat), &cat_desc = const_cast<DevBuf &>(c->desc);
 
     // 1. coarse votes and the shortlist (catalog.rs:594-638)
     std::vector<uint32_t> votes;
@@ -377,18 +371,41 @@ ps_status search_impl(ps_engine *e, const ps_catalog *c, const ps_feature *d_qf,
     PS_TRY(ensure(e->aux[AUX_CS_2], total * sizeof(ps_feature_match)));
     PS_TRY(ensure(e->aux[AUX_CS_3], (ns + 1) * sizeof(uint32_t)));
     ps_feature_match *dm = as<ps_feature_match>(e->aux[AUX_CS_2]);
-    uint32_t *dcnt = as<uint32_t>(e->aux[AUX_CS_3]);   // [0, ns): match counts, [ns]: nq
-    const uint32_t nq32 = (uint32_t)nq;
-    PS_HIP(hipMemcpyAsync(dcnt + ns, &nq32, sizeof nq32, hipMemcpyHostToDevice, st));
-    ps_match_options opt;
-    std::memset(&opt, 0, sizeof opt);
-    opt.ratio = ratio;
+    uint32_t *dcnt = as<uint32_t>(e->aux[AUX_CS_3]);   // [0, ns): match counts
+    // one grouped match: the query set against each shortlisted entry's features
+    std::vector<ps_feature_set> tsets(ns + 1);
+    std::vector<ps_match_problem> mprob(ns);
+    uint32_t set_cap = (uint32_t)nq;
     for (size_t k = 0; k < ns; k++) {
         const uint32_t ent = slots[k].entry, o = c->off[ent], nt = c->off[ent + 1] - o;
-        PS_TRY(enqueue_match(e, d_qf, d_qd, dcnt + ns, nq, as<ps_feature>(cat_feat) + o,
-                             as<float>(cat_desc) + (size_t)o * 128, as<uint32_t>(cat_cnt) + ent, nt, opt, dm + k * nq,
-                             dcnt + k));
+        tsets[k] = ps_feature_set{o, nt};
+        mprob[k] = ps_match_problem{0u, (uint32_t)k, (uint32_t)(k * nq), 0u};
+        set_cap = std::max(set_cap, nt);
     }
+    tsets[ns] = ps_feature_set{0u, (uint32_t)nq};   // the query set
+    PS_TRY(ensure(e->aux[AUX_CS_12], (ns + 1) * sizeof(ps_feature_set) + ns * sizeof(ps_match_problem)));
+    ps_feature_set *dsets = as<ps_feature_set>(e->aux[AUX_CS_12]);
+    ps_match_problem *dmp = reinterpret_cast<ps_match_problem *>(dsets + ns + 1);
+    for (size_t k = 0; k < ns; k++) mprob[k].query_set = (uint32_t)ns;
+    PS_HIP(hipMemcpyAsync(dsets, tsets.data(), (ns + 1) * sizeof(ps_feature_set), hipMemcpyHostToDevice, st));
+    PS_HIP(hipMemcpyAsync(dmp, mprob.data(), ns * sizeof(ps_match_problem), hipMemcpyHostToDevice, st));
+    ps::MatchSetsArgs ma{};
+    ma.qf = d_qf;
+    ma.qd = d_qd;
+    ma.q_rows = nq;
+    ma.qsets = dsets;
+    ma.n_qsets = (uint32_t)ns + 1;
+    ma.tf = as<ps_feature>(cat_feat);
+    ma.td = as<float>(cat_desc);
+    ma.t_rows = c->off[c->ids.size()];
+    ma.tsets = dsets;
+    ma.n_tsets = (uint32_t)ns;
+    ma.cap = set_cap;
+    ma.probs = dmp;
+    ma.n_probs = (uint32_t)ns;
+    ma.ratio = ratio;
+    ma.shared = false;
+    PS_TRY(enqueue_match_sets(e, ma, e->aux[AUX_CS_13], dm, dcnt));
     std::vector<uint32_t> mcnt(ns);
     PS_HIP(hipMemcpyAsync(mcnt.data(), dcnt, ns * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     PS_HIP(hipStreamSynchronize(st));

@@ -10,15 +10,9 @@
 
 using namespace pshost;
 
-namespace {
+namespace pshost {
 
-constexpr uint64_t kDefaultSimSeed = 0xD1B54A32D192ED03ull;   // pipeline.rs:173
-constexpr uint64_t kDefaultHomSeed = 0x517CC1B727220A95ull;   // pipeline.rs:175
-constexpr uint32_t kPanoCap = PS_MAX_FEATURES;                // extract_sift_features' cap
-constexpr uint32_t kMaxProblems = 65535;                      // one grid dimension
-
-size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
-
+// grouped exact ratio match on the engine stream (scratch: the caller's workspace slot)
 ps_status enqueue_match_sets(ps_engine *e, const ps::MatchSetsArgs &a, DevBuf &scratch, ps_feature_match *out,
                              uint32_t *counts) {
     void *const before = scratch.p;
@@ -29,6 +23,17 @@ ps_status enqueue_match_sets(ps_engine *e, const ps::MatchSetsArgs &a, DevBuf &s
     PS_HIP(ps::launch_match_sets(a, scratch.p, e->nn_gen, out, counts, e->stream));
     return PS_OK;
 }
+
+}  // namespace pshost
+
+namespace {
+
+constexpr uint64_t kDefaultSimSeed = 0xD1B54A32D192ED03ull;   // pipeline.rs:173
+constexpr uint64_t kDefaultHomSeed = 0x517CC1B727220A95ull;   // pipeline.rs:175
+constexpr uint32_t kPanoCap = PS_MAX_FEATURES;                // extract_sift_features' cap
+constexpr uint32_t kMaxProblems = 65535;                      // one grid dimension
+
+size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 ps_status check_pano_options(const ps_panorama_options *o) {
     ps_sift_config cfg{};

@@ -21,7 +21,7 @@ enum AuxSlot {
     AUX_PM_0, AUX_PM_1, AUX_PM_2, AUX_PM_3, AUX_PM_4, AUX_PM_5, AUX_PM_6, AUX_PM_7,
     AUX_RS_0, AUX_RS_1, AUX_RS_2, AUX_RS_3, AUX_RS_4, AUX_RS_5, AUX_RS_6, AUX_RS_7,
     AUX_CS_0, AUX_CS_1, AUX_CS_2, AUX_CS_3, AUX_CS_4, AUX_CS_5, AUX_CS_6, AUX_CS_7, AUX_CS_8, AUX_CS_9,
-    AUX_CS_10, AUX_CS_11,
+    AUX_CS_10, AUX_CS_11, AUX_CS_12, AUX_CS_13,
     AUX_PG_0, AUX_PG_1, AUX_PG_2, AUX_PG_3, AUX_PG_4, AUX_PG_5, AUX_PG_6, AUX_PG_7, AUX_PG_8, AUX_PG_9,
     AUX_PG_10, AUX_PG_11,
     AUX_PN_0, AUX_PN_1, AUX_PN_2, AUX_PN_3, AUX_PN_4, AUX_PN_5, AUX_PN_6, AUX_PN_7, AUX_PN_8, AUX_PN_9,
@@ -79,6 +79,9 @@ ps_status prepare_extract(ps_engine *e, size_t h, size_t w, const ps_sift_config
 ps_status enqueue_match(ps_engine *e, const ps_feature *qf, const float *qd, const uint32_t *nq, size_t nq_max,
                         const ps_feature *tf, const float *td, const uint32_t *nt, size_t nt_max,
                         const ps_match_options &opt, ps_feature_match *out, uint32_t *out_count);
+// many set-against-set ratio matches in one launch chain (ps_engine_match_sets)
+ps_status enqueue_match_sets(ps_engine *e, const ps::MatchSetsArgs &a, DevBuf &scratch, ps_feature_match *out,
+                             uint32_t *counts);
 ps_status enqueue_extract(ps_engine *e, const void *src, bool src_u8, size_t h, size_t w, size_t stride,
                           const ps_sift_config &cfg, ps_feature *d_feat, float *d_desc, uint32_t cap,
                           uint32_t *d_count);
