@@ -354,4 +354,50 @@ private:
     std::unique_ptr<ps_catalog, Del> h_;
 };
 
+// stitch.rs:45-86 PanoramaOptions (the pairwise stage's fields, the reference's defaults)
+struct PanoramaOptions {
+    uint32_t octaves = 3, scales = 3;
+    float threshold = 0.015f, ratio = 0.75f, ransac_threshold = 3.0f;
+};
+// stitch.rs:104-112 PairEdge
+struct PairEdge {
+    std::size_t from, to;
+    double h_from_to[3][3];
+    std::size_t inliers;
+    double mean_i_from, mean_i_to;
+};
+struct LumaImage {
+    const uint8_t *pixels;
+    uint32_t height, width, row_stride;
+};
+// stitch.rs:478-571: the pair edges of all image pairs, in the reference's order
+inline std::vector<PairEdge> panorama_pair_edges(const std::vector<LumaImage> &images,
+                                                 const PanoramaOptions &o = PanoramaOptions()) {
+    std::vector<const uint8_t *> px;
+    std::vector<uint32_t> hs, ws, ss;
+    for (const LumaImage &im : images) {
+        px.push_back(im.pixels);
+        hs.push_back(im.height);
+        ws.push_back(im.width);
+        ss.push_back(im.row_stride);
+    }
+    const ps_panorama_options c{o.octaves, o.scales, o.threshold, o.ratio, o.ransac_threshold, 0};
+    const std::size_t n = images.size();
+    std::vector<ps_pair_edge> raw(n > 1 ? n * (n - 1) : 1);
+    std::size_t cnt = 0;
+    check(ps_panorama_pair_edges(px.data(), hs.data(), ws.data(), ss.data(), (uint32_t)n, &c, raw.data(), raw.size(),
+                                 &cnt));
+    std::vector<PairEdge> out(cnt);
+    for (std::size_t k = 0; k < cnt; k++) {
+        out[k].from = raw[k].from;
+        out[k].to = raw[k].to;
+        for (int r = 0; r < 3; r++)
+            for (int col = 0; col < 3; col++) out[k].h_from_to[r][col] = raw[k].h_from_to[r * 3 + col];
+        out[k].inliers = raw[k].inliers;
+        out[k].mean_i_from = raw[k].mean_i_from;
+        out[k].mean_i_to = raw[k].mean_i_to;
+    }
+    return out;
+}
+
 }  // namespace pure_sift

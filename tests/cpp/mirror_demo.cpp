@@ -1,7 +1,8 @@
 // Exercises the C++ host mirror (pure-sift_amd/pure_sift.hpp) end to end: validation
 // errors, detect_keypoints, detect_and_compute (f32 and u8), compute_descriptors, match_ (plain
-// and mutual).
+// and mutual), the feature catalog and the panorama pair edges.
 // Reads a raw u8 image (h, w from argv) and prints a one-line summary the GPU test checks.
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -64,7 +65,24 @@ int main(int argc, char **argv) {
         if (names.size() != 2 || names[1].id != "target" || names[1].feature_count != ft.size()) bad++;
     }
     if (!cat.remove("decoy") || cat.remove("decoy") || cat.size() != 1) bad++;
-    std::printf("keypoints=%zu features=%zu,%zu described=%zu matches=%zu mutual=%zu catalog_hits=%zu bad=%d\n",
-                kps.size(), fq.size(), ft.size(), fk.size(), m.size(), mm.size(), hits.size(), bad);
+    // panorama pairwise stage over (query, target, query): the two identical images align by the identity
+    std::vector<pure_sift::LumaImage> pano{{q8.data(), (uint32_t)h, (uint32_t)w, (uint32_t)w},
+                                           {t8.data(), (uint32_t)h, (uint32_t)w, (uint32_t)w},
+                                           {q8.data(), (uint32_t)h, (uint32_t)w, (uint32_t)w}};
+    auto edges = pure_sift::panorama_pair_edges(pano);
+    bool ident = false;
+    for (const auto &e : edges)
+        if (e.from == 0 && e.to == 2)
+            ident = std::abs(e.h_from_to[0][2]) < 0.5 && std::abs(e.h_from_to[1][2]) < 0.5 &&
+                    e.mean_i_from == e.mean_i_to;
+    if (edges.size() % 2 != 0 || !ident) bad++;
+    try {
+        pure_sift::panorama_pair_edges({pano[0]});
+        bad++;
+    } catch (const pure_sift::DeviceError &) {
+    }
+    std::printf("keypoints=%zu features=%zu,%zu described=%zu matches=%zu mutual=%zu catalog_hits=%zu pano_edges=%zu "
+                "bad=%d\n",
+                kps.size(), fq.size(), ft.size(), fk.size(), m.size(), mm.size(), hits.size(), edges.size(), bad);
     return bad ? 1 : 0;
 }

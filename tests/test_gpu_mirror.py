@@ -1,8 +1,8 @@
 """The C++ host mirror (pure-sift_amd/pure_sift.hpp) end to end: tests/cpp/mirror_demo (built
 with the library) runs validation, detect_keypoints, detect_and_compute (f32 and u8),
-compute_descriptors, match_ (plain and mutual) and the feature catalog (upsert, search, save /
-load, remove) through the C ABI in its own process; its counts must equal the Python mirror's on
-the same images."""
+compute_descriptors, match_ (plain and mutual), the feature catalog (upsert, search, save /
+load, remove) and the panorama pair edges through the C ABI in its own process; its counts must
+equal the Python mirror's on the same images."""
 import os
 import re
 import subprocess
@@ -39,4 +39,5 @@ def test_cpp_mirror_demo(gpu, tmp_path):
     assert int(got["matches"]) == len(gpu.Matcher(gpu.MatchOptions(0.8)).match_(fa, fb))
     assert int(got["mutual"]) == len(gpu.Matcher(gpu.MatchOptions(0.8, True)).match_(fa, fb))
     assert int(got["catalog_hits"]) >= 1
+    assert int(got["pano_edges"]) == len(gpu.panorama_pair_edges([a, b, a])) >= 2
     assert (tmp_path / "demo.psift").exists()
