@@ -189,3 +189,33 @@ def test_panorama_two_images_and_other_options(gpu):
     assert np.array_equal(edges[0]["h_from_to"], e["h"].ravel())
     assert np.array_equal(edges[1]["h_from_to"], e["h_inv"].ravel())
     assert edges[0]["mean_i_from"] == e["mean_i"] and edges[1]["mean_i_from"] == e["mean_j"]
+
+
+def test_panorama_uneven_sizes_and_blank_image(gpu):
+    """Images of different sizes (the alignment's sanity test uses the query image's size), a
+    flat image with no features (its pairs have no matches and no edge), and a 2x-downscaled
+    tile: the records still equal the oracle's."""
+    import torch
+    images, _ = _scene()
+    small = images[1][::2, ::2].copy()
+    flat = np.full((200, 260), 97, np.uint8)
+    imgs = [images[0][:, :600].copy(), images[1], flat, small, images[2][40:, :]]
+    opt = gpu.PanoramaOptions()
+    want = _oracle_pairs(gpu, imgs, opt)
+    assert all(n == 0 and e is None for i, j, n, e in want if 2 in (i, j))
+    assert any(e is not None for _, _, _, e in want)
+    descs, chunks, off = [], [], 0
+    for im in imgs:
+        h, w = im.shape
+        descs.append((off, h, w, w, 0))
+        chunks.append(im.ravel())
+        off += im.size
+    dev = torch.device("cuda:0")
+    pix = torch.from_numpy(np.concatenate(chunks)).to(dev)
+    n = len(imgs)
+    dpairs = torch.zeros(n * (n - 1) // 2 * gpu.PANORAMA_PAIR_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    eng = gpu.Engine(0)
+    eng.panorama_pairs(pix.data_ptr(), np.array(descs, gpu.IMAGE_DESC_DTYPE), opt, dpairs.data_ptr())
+    eng.sync()
+    _check_records(dpairs.cpu().numpy().view(gpu.PANORAMA_PAIR_DTYPE), want)
+    eng.close()
