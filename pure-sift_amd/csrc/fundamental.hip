@@ -926,7 +926,8 @@ __global__ __launch_bounds__(kFT) void fundamental_kernel(const ps_feature_match
         // 2. fit and score one hypothesis per lane
         {
             S.hok[lane] = 0;
-            if (S.qvalid[lane]) {
+            // the adaptive bound only shrinks as the best improves: past it nothing is replayed
+            if (S.qvalid[lane] && it + (uint32_t)lane < max_it) {
                 uint32_t idx[8];
                 for (int j = 0; j < 8; j++) idx[j] = S.quad[lane][j];
                 double f[9];

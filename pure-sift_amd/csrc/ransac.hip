@@ -853,7 +853,8 @@ __global__ __launch_bounds__(kRT) void ransac_kernel(const ps_feature_match *mat
             while (it < max_it) {
                 // hypotheses it+1 .. it+kRT, one per thread
                 const uint32_t t = it + tid;   // 0-based iteration index
-                {
+                s_sv[tid] = 0;
+                if (t < max_it) {   // past the (shrinking) bound a hypothesis is never replayed
                     uint32_t i1 = cdf_pick(cdf, n, unit_draw(splitmix_at(P.sim_seed, 2ull * t)));
                     uint32_t i2 = cdf_pick(cdf, n, unit_draw(splitmix_at(P.sim_seed, 2ull * t + 1)));
                     if (i2 == i1) i2 = (i1 + 1) % n;
@@ -1083,7 +1084,9 @@ __global__ __launch_bounds__(kRT) void ransac_kernel(const ps_feature_match *mat
             h.count = 0;
             h.err = 0.0;
             h.mass = 0.0;
-            if (S.qvalid[tid]) {
+            // the bound only shrinks as the best improves (not so for the weighted sampler's
+            // mass-based bound), so a hypothesis past it is never replayed: skip its fit
+            if (S.qvalid[tid] && (sampler == 2 || it + (uint32_t)tid < max_it)) {
                 const uint32_t idx[4] = {S.quad[tid][0], S.quad[tid][1], S.quad[tid][2], S.quad[tid][3]};
                 double qx[4], qy[4], tx[4], ty[4];
                 for (int i = 0; i < 4; i++) {
