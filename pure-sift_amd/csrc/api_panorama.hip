@@ -87,14 +87,21 @@ ps_status enqueue_panorama(ps_engine *e, const uint8_t *d_pixels, const ps_image
     float *desc = as<float>(e->aux[AUX_PN_2]);
     ps_feature_match *matches = as<ps_feature_match>(e->aux[AUX_PN_5]);
 
-    // host tables: the pair list in the reference's order (stitch.rs:500-505)
-    std::vector<ps_match_problem> hp(np);
-    for (uint32_t i = 0, k = 0; i < n; i++)
-        for (uint32_t j = i + 1; j < n; j++, k++) hp[k] = ps_match_problem{i, j, k * kPanoCap, 0u};
-    PS_HIP(hipMemcpyAsync(d_img, images, n * sizeof(ps_image_desc), hipMemcpyHostToDevice, st));
-    PS_HIP(hipMemcpyAsync(probs, hp.data(), np * sizeof(ps_match_problem), hipMemcpyHostToDevice, st));
-    // the copies read pageable host memory: finish them before the vector goes away
-    PS_HIP(hipStreamSynchronize(st));
+    // host tables: the image descriptors and the pair list in the reference's order
+    // (stitch.rs:500-505); uploaded again only when they differ from the last call's (the
+    // copies read pageable host memory, so an upload waits for them)
+    const bool same = e->pano_tables == small && e->pano_images.size() == n &&
+                      std::memcmp(e->pano_images.data(), images, n * sizeof(ps_image_desc)) == 0;
+    if (!same) {
+        std::vector<ps_match_problem> hp(np);
+        for (uint32_t i = 0, k = 0; i < n; i++)
+            for (uint32_t j = i + 1; j < n; j++, k++) hp[k] = ps_match_problem{i, j, k * kPanoCap, 0u};
+        PS_HIP(hipMemcpyAsync(d_img, images, n * sizeof(ps_image_desc), hipMemcpyHostToDevice, st));
+        PS_HIP(hipMemcpyAsync(probs, hp.data(), np * sizeof(ps_match_problem), hipMemcpyHostToDevice, st));
+        PS_HIP(hipStreamSynchronize(st));
+        e->pano_images.assign(images, images + n);
+        e->pano_tables = small;
+    }
 
     PS_TRY(ps_engine_extract_batch_u8(e, d_pixels, images, n, &cfg, feat, desc, kPanoCap, counts, 0));
     PS_HIP(ps::launch_pano_sets(counts, n, kPanoCap, sets, st));

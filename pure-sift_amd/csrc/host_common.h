@@ -53,6 +53,10 @@ struct ps_engine {
     std::vector<uint64_t> graph_key;
     ps::PyrPlan plan;
     uint32_t host_counts[4] = {};
+    // panorama tables last uploaded (image descriptors, then the pair list follows from their
+    // number): an unchanged call re-uses them without a host round trip
+    std::vector<ps_image_desc> pano_images;
+    const void *pano_tables = nullptr;
     // stage profiling: per recorded call, events [extract: 0..4][match: 5..6]
     std::vector<hipEvent_t> prof_ev;
     uint32_t prof_max = 0, prof_ext = 0, prof_mat = 0;
