@@ -435,7 +435,10 @@ def bench_pair(steps: int = 20, n_batch: int = 64, cpu_pairs: int = 2) -> dict:
     (200 blobs, warped partner): extract both images (4 octaves x 3 scales, upsampled), ratio-0.8
     match, then the post-processing -- tracking summary, similarity, homography, fundamental
     matrix, GRIC, kept inliers -- all on one engine stream with the images resident in HBM.
-    Reported for one pair at a time (latency) and for 64 pairs per batch (throughput)."""
+    Reported for one pair at a time (latency) and for 64 pairs per batch (throughput: one batched
+    extraction of the 128 images, one grouped match of the 64 pairs, one post-processing batch;
+    the same 64 pairs as a loop of per-pair calls beside it, with a check that both give the
+    same records)."""
     import numpy as np
     import torch
     import pure_sift as ps
@@ -480,6 +483,38 @@ def bench_pair(steps: int = 20, n_batch: int = 64, cpu_pairs: int = 2) -> dict:
         eng.pair_geometry_batch(mout.data_ptr(), qf.data_ptr(), tf.data_ptr(), d_probs.data_ptr(), n, n * cap, ro,
                                 d_best.data_ptr(), d_out.data_ptr())
 
+    # the batch path: all 2n images in one batched extraction (captured graph), one grouped match
+    # of every pair, one post-processing batch; features of pair k at rows 2k (query) and 2k+1
+    # (target) of one array
+    descs = np.array([((k * 2 + side) * H * W, H, W, W, 0) for k in range(n_batch) for side in (0, 1)],
+                     ps.IMAGE_DESC_DTYPE)
+    bf = torch.empty((2 * n_batch, cap * fsz), dtype=torch.uint8, device=dev)
+    bd = torch.empty((2 * n_batch, cap, 128), dtype=torch.float32, device=dev)
+    bcnt = torch.zeros(2 * n_batch, dtype=torch.int32, device=dev)
+    sets = torch.zeros((2 * n_batch, 2), dtype=torch.int32, device=dev)
+    sets[:, 0] = torch.arange(2 * n_batch, device=dev, dtype=torch.int32) * cap
+    mprob = np.array([(2 * k, 2 * k + 1, k * cap, 0) for k in range(n_batch)], ps.MATCH_PROBLEM_DTYPE)
+    d_mprob = torch.from_numpy(mprob.view(np.uint8).copy()).to(dev)
+    bprobs = np.zeros(n_batch, ps.PAIR_PROBLEM_DTYPE)
+    bprobs["offset"] = np.arange(n_batch) * cap
+    bprobs["qfeat_offset"] = np.arange(n_batch) * 2 * cap
+    bprobs["tfeat_offset"] = (np.arange(n_batch) * 2 + 1) * cap
+    d_bprobs = torch.from_numpy(bprobs.view(np.int32).copy()).to(dev).view(-1, 4)
+    mcnt = torch.zeros(n_batch, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+
+    def run_batch():
+        eng.extract_batch_u8(imgs.data_ptr(), descs, cfg, bf.data_ptr(), bd.data_ptr(), cap, bcnt.data_ptr())
+        with torch.cuda.stream(st):   # set sizes = the extracted counts, stream-ordered
+            sets[:, 1].copy_(bcnt, non_blocking=True)
+        eng.match_sets(bf.data_ptr(), bd.data_ptr(), 2 * n_batch * cap, sets.data_ptr(), 2 * n_batch, bf.data_ptr(),
+                       bd.data_ptr(), 2 * n_batch * cap, sets.data_ptr(), 2 * n_batch, cap, d_mprob.data_ptr(),
+                       n_batch, opt.ratio, mout.data_ptr(), mcnt.data_ptr())
+        with torch.cuda.stream(st):
+            d_bprobs[:, 1].copy_(mcnt, non_blocking=True)
+        eng.pair_geometry_batch(mout.data_ptr(), bf.data_ptr(), bf.data_ptr(), d_bprobs.data_ptr(), n_batch,
+                                n_batch * cap, ro, d_best.data_ptr(), d_out.data_ptr())
+
     def timed(fn, reps):
         fn()
         eng.sync()
@@ -489,13 +524,18 @@ def bench_pair(steps: int = 20, n_batch: int = 64, cpu_pairs: int = 2) -> dict:
         eng.sync()
         return (time.perf_counter() - t0) / reps * 1e3
     ms_single = timed(lambda: run(1), steps)
-    ms_batch = timed(lambda: run(n_batch), max(2, steps // 10))
+    ms_loop = timed(lambda: run(n_batch), max(2, steps // 10))
+    recs_loop = d_out.cpu().numpy().view(ps.PAIR_GEOMETRY_DTYPE).copy()
+    ms_batch = timed(run_batch, max(2, steps // 10))
+    # the batch path gives every pair the record of the per-pair path
+    same_as_loop = bool(np.array_equal(d_out.cpu().numpy().view(ps.PAIR_GEOMETRY_DTYPE), recs_loop))
     recs = d_out.cpu().numpy().view(ps.PAIR_GEOMETRY_DTYPE)
     eng.check()
     out = {"workload": f"640x480 blob pairs (200 blobs, warped partner): extract both + ratio-0.8 match + tracking, "
                        f"similarity, homography, fundamental, GRIC (RANSAC 3 px)",
            "value": n_batch / (ms_batch * 1e-3), "unit": "pairs/s", "ms_per_pair_single": ms_single,
-           "ms_per_batch": ms_batch, "batch": n_batch,
+           "ms_per_batch": ms_batch, "batch": n_batch, "ms_per_batch_pair_loop": ms_loop,
+           "batch_equals_pair_loop": same_as_loop,
            "raw_matches_mean": float(recs["raw_match_count"].mean()),
            "homography_inliers_mean": float(recs["n_homography"].mean()),
            "fundamental_frac": float(recs["has_fundamental"].mean()),
