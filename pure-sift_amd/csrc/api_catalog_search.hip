@@ -406,27 +406,10 @@ ps_status search_impl(ps_engine *e, const ps_catalog *c, const ps_feature *d_qf,
     ma.ratio = ratio;
     ma.shared = false;
     PS_TRY(enqueue_match_sets(e, ma, e->aux[AUX_CS_13], dm, dcnt));
-    std::vector<uint32_t> mcnt(ns);
-    PS_HIP(hipMemcpyAsync(mcnt.data(), dcnt, ns * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    PS_HIP(hipStreamSynchronize(st));
-
-    // 3. RANSAC problems for the entries with matches (catalog.rs:656-671)
-    std::vector<ps_ransac_problem> probs;
-    std::vector<ps::TrackJob> jobs;
-    std::vector<size_t> slot_of;
-    for (size_t k = 0; k < ns; k++) {
-        if (mcnt[k] == 0) continue;
-        ps_ransac_problem p;
-        p.offset = (uint32_t)(k * nq);
-        p.n = mcnt[k];
-        p.sim_seed = kSimSeed;
-        p.hom_seed = kHomSeed;
-        probs.push_back(p);
-        jobs.push_back({(uint32_t)(k * nq), mcnt[k], c->off[slots[k].entry], 0});
-        slot_of.push_back(k);
-    }
-    const size_t np = probs.size();
-    if (np == 0) return PS_OK;
+    // 3. RANSAC problems and tracking jobs of every slot, built on the device from the match
+    // counts (catalog.rs:656-671; a slot without matches runs an empty problem and is dropped
+    // from the hits below)
+    const size_t np = ns;
     const bool same_thr = thr == kDefaultReproj;
     const size_t wbytes = ps::ransac_work_bytes(total);
     PS_TRY(ensure(e->aux[AUX_CS_4], np * sizeof(ps_ransac_problem)));
@@ -440,8 +423,7 @@ ps_status search_impl(ps_engine *e, const ps_catalog *c, const ps_feature *d_qf,
     ps::TrackJob *djob = as<ps::TrackJob>(e->aux[AUX_CS_5]);
     ps_ransac_result *dres = as<ps_ransac_result>(e->aux[AUX_CS_8]);
     ps::TrackOut *dtrk = as<ps::TrackOut>(e->aux[AUX_CS_9]);
-    PS_HIP(hipMemcpyAsync(dprob, probs.data(), np * sizeof(ps_ransac_problem), hipMemcpyHostToDevice, st));
-    PS_HIP(hipMemcpyAsync(djob, jobs.data(), np * sizeof(ps::TrackJob), hipMemcpyHostToDevice, st));
+    PS_HIP(ps::launch_catalog_problems(dcnt, dsets, (uint32_t)ns, (uint32_t)nq, kSimSeed, kHomSeed, dprob, djob, st));
     ps_ransac_params prm;
     std::memset(&prm, 0, sizeof prm);
     prm.reproj_threshold = thr;
@@ -471,19 +453,24 @@ ps_status search_impl(ps_engine *e, const ps_catalog *c, const ps_feature *d_qf,
                                dtrk, st));
     std::vector<ps_ransac_result> res(np);
     std::vector<ps::TrackOut> trk(np);
+    std::vector<uint32_t> mcnt(np);
     PS_HIP(hipMemcpyAsync(res.data(), dres, np * sizeof(ps_ransac_result), hipMemcpyDeviceToHost, st));
     PS_HIP(hipMemcpyAsync(trk.data(), dtrk, np * sizeof(ps::TrackOut), hipMemcpyDeviceToHost, st));
+    PS_HIP(hipMemcpyAsync(mcnt.data(), dcnt, np * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     PS_HIP(hipStreamSynchronize(st));
 
-    // 4. hits (catalog.rs:664-700)
-    std::vector<ps_catalog_hit> hits(np);
+    // 4. hits of the entries with matches (catalog.rs:664-700)
+    std::vector<ps_catalog_hit> hits;
+    hits.reserve(np);
     for (size_t p = 0; p < np; p++) {
-        const Cand &k = slots[slot_of[p]];
+        if (mcnt[p] == 0) continue;
+        const Cand &k = slots[p];
         const ps_ransac_result &R = res[p];
         const ps::TrackOut &T = trk[p];
-        ps_catalog_hit &h = hits[p];
+        hits.emplace_back();
+        ps_catalog_hit &h = hits.back();
         std::memset(&h, 0, sizeof h);
-        const uint32_t raw = probs[p].n, best = std::max(R.n_inliers, R.n_sim);
+        const uint32_t raw = mcnt[p], best = std::max(R.n_inliers, R.n_sim);
         h.entry = k.entry;
         h.coarse_votes = k.votes;
         h.raw_match_count = raw;

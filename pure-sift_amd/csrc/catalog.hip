@@ -160,6 +160,10 @@ __global__ __launch_bounds__(256) void tracking_kernel(const ps_feature_match *m
     const uint32_t *inl = ransac_sim_inliers(sim_work, sim_total, J.moff, n);
     const bool use_inl = nsim > 0;
     const uint32_t na = use_inl ? nsim : n;
+    if (na == 0) {   // no matches: no summary (the caller drops the entry)
+        if (tid == 0) out[blockIdx.x] = TrackOut{};
+        return;
+    }
     const ps_feature_match *mm = matches + J.moff;
     const ps_feature *T = tf + J.tfoff, *Qf = qf + J.qfoff;
     float sum = 0.0f;   // lane l < 5 owns sum l: dx, dy, scale ratio, sin, cos
@@ -215,7 +219,25 @@ __global__ __launch_bounds__(256) void tracking_kernel(const ps_feature_match *m
     }
 }
 
+// catalog.rs:656-671 on the device: slot k's RANSAC problem and tracking job over its match
+// block (k * nq, count k); a slot without matches gets an empty problem
+__global__ void catalog_problems_kernel(const uint32_t *cnt, const ps_feature_set *tsets, uint32_t ns, uint32_t nq,
+                                        uint64_t sim_seed, uint64_t hom_seed, ps_ransac_problem *rp, TrackJob *jobs) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ns) return;
+    rp[k] = ps_ransac_problem{k * nq, cnt[k], sim_seed, hom_seed};
+    jobs[k] = TrackJob{k * nq, cnt[k], tsets[k].offset, 0u};
+}
+
 }  // namespace
+
+hipError_t launch_catalog_problems(const uint32_t *cnt, const ps_feature_set *tsets, uint32_t ns, uint32_t nq,
+                                   uint64_t sim_seed, uint64_t hom_seed, ps_ransac_problem *rp, TrackJob *jobs,
+                                   hipStream_t st) {
+    if (ns == 0) return hipSuccess;
+    catalog_problems_kernel<<<(ns + 255) / 256, 256, 0, st>>>(cnt, tsets, ns, nq, sim_seed, hom_seed, rp, jobs);
+    return hipGetLastError();
+}
 
 hipError_t launch_dequant_u8(const uint8_t *q, uint32_t n, float *out, hipStream_t st) {
     if (n == 0) return hipSuccess;

@@ -220,6 +220,9 @@ hipError_t launch_dequant_u8(const uint8_t *q, uint32_t n, float *out, hipStream
 hipError_t launch_catalog_votes(const uint64_t *sig, const uint32_t *off, uint32_t n_entries, const uint64_t *q,
                                 uint32_t nq, bool mih, uint32_t *votes, hipStream_t st);
 // sim_res / sim_work: results and workspace of a RANSAC launch over the same matches at 3 px
+hipError_t launch_catalog_problems(const uint32_t *cnt, const ps_feature_set *tsets, uint32_t ns, uint32_t nq,
+                                   uint64_t sim_seed, uint64_t hom_seed, ps_ransac_problem *rp, TrackJob *jobs,
+                                   hipStream_t st);
 hipError_t launch_tracking(const ps_feature_match *matches, const TrackJob *jobs, uint32_t n_jobs,
                            const ps_ransac_result *sim_res, const void *sim_work, size_t sim_total,
                            const ps_feature *qf, const ps_feature *tf, TrackOut *out, hipStream_t st);
