@@ -25,6 +25,7 @@ namespace ps {
 namespace {
 
 constexpr int kFT = 64;   // threads (and hypotheses per chunk) per problem
+constexpr uint32_t kFStage = 256;   // points per staging chunk
 constexpr uint64_t kGammaF = 0x9E3779B97F4A7C15ull;
 
 __device__ __forceinline__ uint64_t fmix64(uint64_t z) {
@@ -654,7 +655,21 @@ struct FundShared {
     double best[9], be;
     uint32_t bc, have, ok, ni;
     double norm[6];
+    // chunk staging for the order-dependent sums: all lanes load / compute per-point terms,
+    // lane 0 adds them in point order
+    double stage[4][kFStage];
+    uint32_t bad;
 };
+
+// entry j of the 8-point row (geometry.rs:586-604) of a normalised correspondence, selected
+// without a dynamically indexed array
+__device__ __forceinline__ double fund_row_at(int j, double x1, double y1, double x2, double y2) {
+    const double v[9] = {x2 * x1, x2 * y1, x2, y2 * x1, y2 * y1, y2, x1, y1, 1.0};
+    double r = v[0];
+#pragma unroll
+    for (int i = 1; i < 9; i++) r = j == i ? v[i] : r;
+    return r;
+}
 
 // ordered compaction (single wave): indices i in [0, n) with pred(i), in increasing order
 template <typename Pred>
@@ -676,46 +691,113 @@ __device__ uint32_t compact_wave(uint32_t n, uint32_t *out, Pred pred) {
 // rank-2 step on lane 0's LDS matrices. Result in S.best-like storage `fout` (shared).
 __device__ bool fit8_block(const ps_feature_match *m, const uint32_t *idx, uint32_t ni, FundShared &S, double *fout) {
     const int lane = threadIdx.x;
+    // Hartley normalisation of both point sets (geometry.rs:547-583): the points and their
+    // glibc-hypot distances staged in chunks, lane 0 adding them in point order; a non-finite
+    // point, or a degenerate mean distance, of either set fails the fit
+    if (lane == 0) S.bad = ni == 0 ? 1u : 0u;
+    __syncthreads();
+    double cq[2] = {0.0, 0.0}, ct[2] = {0.0, 0.0};   // lane 0's running sums
+    for (uint32_t c0 = 0; c0 < ni; c0 += kFStage) {
+        const uint32_t cn = min(kFStage, ni - c0);
+        for (uint32_t k = lane; k < cn; k += kFT) {
+            const ps_feature_match &p = m[idx[c0 + k]];
+            const double qx = p.qx, qy = p.qy, tx = p.tx, ty = p.ty;
+            if (!isfinite(qx) || !isfinite(qy) || !isfinite(tx) || !isfinite(ty)) S.bad = 1u;
+            S.stage[0][k] = qx;
+            S.stage[1][k] = qy;
+            S.stage[2][k] = tx;
+            S.stage[3][k] = ty;
+        }
+        __syncthreads();
+        if (lane == 0 && !S.bad) {
+#pragma unroll 8
+            for (uint32_t k = 0; k < cn; k++) {
+                cq[0] += S.stage[0][k];
+                cq[1] += S.stage[1][k];
+                ct[0] += S.stage[2][k];
+                ct[1] += S.stage[3][k];
+            }
+        }
+        __syncthreads();
+    }
+    if (S.bad) {
+        __syncthreads();
+        return false;
+    }
+    const double inv_n = 1.0 / (double)ni;
     if (lane == 0) {
-        Norm N;
-        bool ok = normalize_params(ni, [&](uint32_t i, double &x, double &y) {
-            x = (double)m[idx[i]].qx;
-            y = (double)m[idx[i]].qy;
-        }, N.qcx, N.qcy, N.qs);
-        ok = ok && normalize_params(ni, [&](uint32_t i, double &x, double &y) {
-            x = (double)m[idx[i]].tx;
-            y = (double)m[idx[i]].ty;
-        }, N.tcx, N.tcy, N.ts);
-        S.norm[0] = N.qcx;
-        S.norm[1] = N.qcy;
-        S.norm[2] = N.qs;
-        S.norm[3] = N.tcx;
-        S.norm[4] = N.tcy;
-        S.norm[5] = N.ts;
-        S.ok = ok;
+        S.norm[0] = cq[0] * inv_n;
+        S.norm[1] = cq[1] * inv_n;
+        S.norm[3] = ct[0] * inv_n;
+        S.norm[4] = ct[1] * inv_n;
     }
     __syncthreads();
+    {
+        const double qcx = S.norm[0], qcy = S.norm[1], tcx = S.norm[3], tcy = S.norm[4];
+        double mdq = 0.0, mdt = 0.0;
+        for (uint32_t c0 = 0; c0 < ni; c0 += kFStage) {
+            const uint32_t cn = min(kFStage, ni - c0);
+            for (uint32_t k = lane; k < cn; k += kFT) {
+                const ps_feature_match &p = m[idx[c0 + k]];
+                S.stage[0][k] = glibc_hypot((double)p.qx - qcx, (double)p.qy - qcy);
+                S.stage[1][k] = glibc_hypot((double)p.tx - tcx, (double)p.ty - tcy);
+            }
+            __syncthreads();
+            if (lane == 0) {
+#pragma unroll 8
+                for (uint32_t k = 0; k < cn; k++) {
+                    mdq += S.stage[0][k];
+                    mdt += S.stage[1][k];
+                }
+            }
+            __syncthreads();
+        }
+        if (lane == 0) {
+            mdq *= inv_n;
+            mdt *= inv_n;
+            const bool ok = isfinite(mdq) && !(mdq < 1e-9) && isfinite(mdt) && !(mdt < 1e-9);
+            S.norm[2] = 1.41421356237309504880 / mdq;
+            S.norm[5] = 1.41421356237309504880 / mdt;
+            S.ok = ok;
+        }
+        __syncthreads();
+    }
     if (!S.ok) {
         __syncthreads();
         return false;
     }
     const Norm N{S.norm[0], S.norm[1], S.norm[2], S.norm[3], S.norm[4], S.norm[5]};
-    if (lane < 45) {
-        int a = 0, r = lane;
+    {
+        int a = 0, r = lane < 45 ? lane : 0;
         while (r >= 9 - a) {
             r -= 9 - a;
             a++;
         }
         const int b = a + r;
         double acc = 0.0;
+        for (uint32_t c0 = 0; c0 < ni; c0 += kFStage) {
+            const uint32_t cn = min(kFStage, ni - c0);
+            for (uint32_t k = lane; k < cn; k += kFT) {
+                const ps_feature_match &p = m[idx[c0 + k]];
+                S.stage[0][k] = ((double)p.qx - N.qcx) * N.qs;
+                S.stage[1][k] = ((double)p.qy - N.qcy) * N.qs;
+                S.stage[2][k] = ((double)p.tx - N.tcx) * N.ts;
+                S.stage[3][k] = ((double)p.ty - N.tcy) * N.ts;
+            }
+            __syncthreads();
+            if (lane < 45) {
 #pragma unroll 4
-        for (uint32_t k = 0; k < ni; k++) {
-            double row[9];
-            fund_row(N, m[idx[k]], row);
-            acc += row[a] * row[b];
+                for (uint32_t k = 0; k < cn; k++) {
+                    const double x1 = S.stage[0][k], y1 = S.stage[1][k], x2 = S.stage[2][k], y2 = S.stage[3][k];
+                    acc += fund_row_at(a, x1, y1, x2, y2) * fund_row_at(b, x1, y1, x2, y2);
+                }
+            }
+            __syncthreads();
         }
-        S.J.a[a * 9 + b][0] = acc;
-        S.J.a[b * 9 + a][0] = acc;
+        if (lane < 45) {
+            S.J.a[a * 9 + b][0] = acc;
+            S.J.a[b * 9 + a][0] = acc;
+        }
     }
     __syncthreads();
     if (lane == 0) {
@@ -732,38 +814,44 @@ __device__ bool fit8_block(const ps_feature_match *m, const uint32_t *idx, uint3
     return ok;
 }
 
-// count / truncated error of the model in fsh over all n matches: per-match values in
-// parallel into ebuf, then lane 0 sums in index order
-__device__ uint32_t eval_f_block(const ps_feature_match *m, uint32_t n, const double *fsh, double tsq, double *ebuf,
-                                 FundShared &S, double *err) {
+// count / truncated error of the model in fsh over all n matches: per-match values computed
+// in parallel into the staging buffer, chunk by chunk, lane 0 adding them in index order
+__device__ uint32_t eval_f_block(const ps_feature_match *m, uint32_t n, const double *fsh, double tsq, FundShared &S,
+                                 double *err) {
     const int lane = threadIdx.x;
     double f[9];
     for (int i = 0; i < 9; i++) f[i] = fsh[i];
-    for (uint32_t i = lane; i < n; i += kFT) ebuf[i] = msampson(f, m[i]);
-    __syncthreads();
     __shared__ double s_e;
     __shared__ uint32_t s_c;
-    if (lane == 0) {
-        uint32_t c = 0;
-        double e = 0.0;
+    uint32_t c = 0;   // lane 0
+    double e = 0.0;
+    for (uint32_t c0 = 0; c0 < n; c0 += kFStage) {
+        const uint32_t cn = min(kFStage, n - c0);
+        for (uint32_t k = lane; k < cn; k += kFT) S.stage[0][k] = msampson(f, m[c0 + k]);
+        __syncthreads();
+        if (lane == 0) {
 #pragma unroll 8
-        for (uint32_t i = 0; i < n; i++) {
-            const double d2 = ebuf[i];
-            if (d2 <= tsq) {
-                c++;
-                e += d2;
-            } else {
-                e += tsq;
+            for (uint32_t k = 0; k < cn; k++) {
+                const double d2 = S.stage[0][k];
+                if (d2 <= tsq) {
+                    c++;
+                    e += d2;
+                } else {
+                    e += tsq;
+                }
             }
         }
+        __syncthreads();
+    }
+    if (lane == 0) {
         s_c = c;
         s_e = e;
     }
     __syncthreads();
     *err = s_e;
-    const uint32_t c = s_c;
+    const uint32_t cc = s_c;
     __syncthreads();
-    return c;
+    return cc;
 }
 
 // The sample that starts at draws[0]: distinct match indices in draw order until K of them
@@ -802,7 +890,6 @@ __global__ __launch_bounds__(kFT) void fundamental_kernel(const ps_feature_match
     const uint32_t n = P.n;
     const ps_feature_match *m = matches + P.offset;
     double *cdf = work_d + (size_t)P.offset * 2;
-    double *ebuf = cdf + n;
     uint32_t *ibuf = work_u + P.offset;
     uint32_t *out_inl = inliers + P.offset;
     ps_fundamental_result &O = outs[blockIdx.x];
@@ -977,7 +1064,7 @@ __global__ __launch_bounds__(kFT) void fundamental_kernel(const ps_feature_match
         __shared__ double s_ref[9];
         if (!fit8_block(m, ibuf, ni, S, s_ref)) break;
         double e;
-        const uint32_t c = eval_f_block(m, n, s_ref, tsq, ebuf, S, &e);
+        const uint32_t c = eval_f_block(m, n, s_ref, tsq, S, &e);
         bool take = c > S.bc || (c == S.bc && e < S.be);
         __syncthreads();
         if (!take) break;
