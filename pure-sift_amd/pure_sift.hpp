@@ -7,7 +7,7 @@
 //
 // Validation happens at construction (Result<_, String> in the reference -> ConfigError);
 // compute calls never fall back to the CPU: a device failure throws DeviceError (the
-// reference's infallible signatures become a panic in the Rust wrapper).
+// reference's infallible signatures become a panic in a Rust wrapper over the same C ABI).
 #pragma once
 
 #include <algorithm>
