@@ -45,11 +45,49 @@ def alg_bytes(h, w, octaves=4, upsample=True):
 
 
 def _host_cores() -> int:
+    """Threads for the CPU baseline: every core this process may run on, within the host CPU
+    share granted per GPU (OMP_NUM_THREADS where the launcher sets it; 16 on the MI355X boxes
+    used here, one GPU each)."""
     try:
         cores = len(os.sched_getaffinity(0))
     except Exception:
         cores = os.cpu_count() or 1
-    return max(1, min(cores, 16))
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        cores = min(cores, int(share))
+    return max(1, cores)
+
+
+def _cpu_info() -> dict:
+    """nproc, the CPU model and the affinity of this process, for the baseline's record."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except Exception:
+        aff = None
+    return {"nproc": os.cpu_count(), "affinity": aff, "cpu_model": model,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def _median_runs(fn, runs: int, warmup: int):
+    """Median wall time (s) of `runs` calls of fn after `warmup` untimed calls."""
+    import statistics
+    for _ in range(warmup):
+        fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return statistics.median(ts), ts
 
 
 def _stream_events(eng):
@@ -124,12 +162,10 @@ def bench_hamming(steps: int = 20, cpu_queries: int = 1000) -> dict:
         import psr
         cores = _host_cores()
         psr.set_threads(cores)
-        t0 = time.perf_counter()
-        psr.hamming_topk(cat, qs[:cpu_queries], k)
-        dt = time.perf_counter() - t0
+        dt, ts = _median_runs(lambda: psr.hamming_topk(cat, qs[:cpu_queries], k), runs=5, warmup=1)
         out["cpu_baseline"] = {"value": cpu_queries / dt, "unit": "queries/s (top-k only)", "cores": cores,
-                               "kind": "port", "sample": f"{cpu_queries} of the 1000 queries over the full 1M "
-                                                         f"catalog, {dt:.1f} s wall"}
+                               "kind": "port", "sample": f"all {cpu_queries} queries over the full 1M catalog, median "
+                                                         f"of 5 runs after 1 warm-up ({sum(ts):.1f} s timed)"}
     return out
 
 
@@ -203,12 +239,11 @@ def bench_precision(steps: int = 5, cpu_queries: int = 200) -> dict:
         import psr
         cores = _host_cores()
         psr.set_threads(cores)
-        t0 = time.perf_counter()
-        psr.nn2(q[:cpu_queries], db)
-        dt = time.perf_counter() - t0
+        dt, ts = _median_runs(lambda: psr.nn2(q[:cpu_queries], db), runs=3, warmup=1)
         out["cpu_baseline"] = {"value": cpu_queries / nq / dt, "unit": "solves/s (exact f32)", "cores": cores,
                                "kind": "port", "sample": f"{cpu_queries} of the 50k queries against all 200k "
-                                                         f"targets, {dt:.1f} s wall, scaled to a full solve"}
+                                                         f"targets, median of 3 runs after 1 warm-up "
+                                                         f"({sum(ts):.1f} s timed), scaled to a full solve"}
     return out
 
 
@@ -417,12 +452,15 @@ def bench_catalog(n_entries: int = 1000, steps: int = 20, cpu_searches: int = 1)
         ref = psr.catalog_parse(data)
         ref.build_index()   # built once at load, as the reference's rebuild_indices (catalog.rs:284-297)
         qfh = qf[:nq * fsz].cpu().numpy().view(psr.FEAT_DTYPE).copy()
-        t0 = time.perf_counter()
-        for _ in range(cpu_searches):
-            rh = psr.catalog_search(ref, qfh, qdh, 5, 16, 0.75, 3.0)
-        dt = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": cpu_searches / dt, "unit": "searches/s", "cores": cores, "kind": "port",
-                               "sample": f"{cpu_searches} search(es) of the same query and catalog, {dt:.1f} s wall"}
+        box = {}
+
+        def search():
+            box["rh"] = psr.catalog_search(ref, qfh, qdh, 5, 16, 0.75, 3.0)
+        dt, ts = _median_runs(search, runs=cpu_searches, warmup=2)
+        rh = box["rh"]
+        out["cpu_baseline"] = {"value": 1.0 / dt, "unit": "searches/s", "cores": cores, "kind": "port",
+                               "sample": f"median of {cpu_searches} searches of the same query and catalog after 2 "
+                                         f"warm-ups ({sum(ts):.1f} s timed)"}
         out["hits_equal_oracle"] = bool(len(rh) == len(hits) and all(
             int(a["entry"]) == int(b["entry"]) and int(a["match_count"]) == int(b["match_count"])
             for a, b in zip(rh, hits)))
@@ -663,29 +701,41 @@ def bench_panorama(n_tiles: int = 16, steps: int = 10, cpu_sample: bool = True) 
     return out
 
 
-def cpu_baseline(a, b, frames: int):
+def cpu_baseline(a, b, runs: int = 10, warmup: int = 2, runs_1: int = 10):
+    """The oracle (C restatement of the reference's CPU path, OpenMP where the reference uses
+    Rayon) on the same 1080p frame: median of `runs` frames after `warmup` warm-ups with all
+    granted cores, and the same with one thread; extract and match timed separately."""
+    import statistics
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import psr
     cores = _host_cores()
-    psr.set_threads(cores)
     fb = psr.extract_features(psr.luma_to_f32(b), 4, 3, 0.015, True)
-    t0 = time.perf_counter()
-    for _ in range(frames):
-        qh, qd = psr.extract_features(psr.luma_to_f32(a), 4, 3, 0.015, True)
-        psr.match(qh, qd, fb[0], fb[1], 0.8)
-    dt = time.perf_counter() - t0
-    psr.set_threads(1)
-    t1 = time.perf_counter()
-    n1 = max(1, frames // 20)
-    for _ in range(n1):
-        qh, qd = psr.extract_features(psr.luma_to_f32(a), 4, 3, 0.015, True)
-        psr.match(qh, qd, fb[0], fb[1], 0.8)
-    dt1 = time.perf_counter() - t1
+
+    def measure(threads, n, w):
+        psr.set_threads(threads)
+        te, tm = [], []
+        for i in range(w + n):
+            t0 = time.perf_counter()
+            qh, qd = psr.extract_features(psr.luma_to_f32(a), 4, 3, 0.015, True)
+            t1 = time.perf_counter()
+            psr.match(qh, qd, fb[0], fb[1], 0.8)
+            t2 = time.perf_counter()
+            if i >= w:
+                te.append(t1 - t0)
+                tm.append(t2 - t1)
+        frame = [x + y for x, y in zip(te, tm)]
+        return statistics.median(frame), statistics.median(te), statistics.median(tm), sum(frame)
+
+    med, med_e, med_m, tot = measure(cores, runs, warmup)
+    med1, med1_e, med1_m, tot1 = measure(1, runs_1, warmup)
     psr.set_threads(cores)
-    return {"value": frames / dt, "unit": "frames/s", "cores": cores, "kind": "port",
-            "sample": f"{frames} frames of the same 1080p workload (oracle extract + ratio match, "
-                      f"partner pre-extracted), {dt:.1f} s wall",
-            "value_1_thread": n1 / dt1}
+    return {"value": 1.0 / med, "unit": "frames/s", "cores": cores, "kind": "port",
+            "sample": f"median of {runs} frames after {warmup} warm-ups of the same 1080p workload (oracle "
+                      f"extract + ratio match, partner pre-extracted), {tot:.1f} s timed; 1-thread: median of "
+                      f"{runs_1} after {warmup} warm-ups, {tot1:.1f} s timed",
+            "t_extract_ms": med_e * 1e3, "t_match_ms": med_m * 1e3,
+            "value_1_thread": 1.0 / med1, "t_extract_ms_1_thread": med1_e * 1e3, "t_match_ms_1_thread": med1_m * 1e3,
+            "host": _cpu_info()}
 
 
 def main():
@@ -693,7 +743,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-frames", type=int, default=40)
+    ap.add_argument("--cpu-runs", type=int, default=10,
+                    help="CPU baseline: timed frames (median) after 2 warm-ups, all granted cores and 1 thread")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=["frame", "hamming", "precision", "ransac", "batch", "catalog", "pair",
                                            "panorama"],
@@ -716,17 +767,17 @@ def main():
         print(json.dumps(bench_pair(cpu_pairs=0 if args.no_cpu_baseline else 64), default=float), flush=True)
         return
     if args.workload == "catalog":
-        print(json.dumps(bench_catalog(steps=min(args.steps, 20), cpu_searches=0 if args.no_cpu_baseline else 1),
+        print(json.dumps(bench_catalog(steps=min(args.steps, 20), cpu_searches=0 if args.no_cpu_baseline else 10),
                          default=float), flush=True)
         return
     if args.workload == "batch":
-        print(json.dumps(bench_batch(cpu_images=0 if args.no_cpu_baseline else 4), default=float), flush=True)
+        print(json.dumps(bench_batch(cpu_images=0 if args.no_cpu_baseline else 16), default=float), flush=True)
         return
     if args.workload == "ransac":
         print(json.dumps(bench_ransac(), default=float), flush=True)
         return
     if args.workload == "precision":
-        print(json.dumps(bench_precision(steps=min(args.steps, 10), cpu_queries=0 if args.no_cpu_baseline else 200)),
+        print(json.dumps(bench_precision(steps=min(args.steps, 10), cpu_queries=0 if args.no_cpu_baseline else 2000)),
               flush=True)
         return
     if args.workload == "hamming":
@@ -829,15 +880,15 @@ def main():
                      "alg_bytes_per_image": ALG_BYTES_1080P},
     }
     eng.close()
-    if not args.no_cpu_baseline and args.cpu_frames > 0:
-        out["cpu_baseline"] = cpu_baseline(a, b, args.cpu_frames)
+    if not args.no_cpu_baseline and args.cpu_runs > 0:
+        out["cpu_baseline"] = cpu_baseline(a, b, runs=args.cpu_runs, runs_1=args.cpu_runs)
     if not args.no_secondary:
         sec = {}
         for name, fn in (("hamming", lambda: bench_hamming(cpu_queries=0 if args.no_cpu_baseline else 1000)),
-                         ("precision", lambda: bench_precision(cpu_queries=0 if args.no_cpu_baseline else 200)),
+                         ("precision", lambda: bench_precision(cpu_queries=0 if args.no_cpu_baseline else 2000)),
                          ("ransac", bench_ransac),
-                         ("batch", lambda: bench_batch(cpu_images=0 if args.no_cpu_baseline else 4)),
-                         ("catalog", lambda: bench_catalog(cpu_searches=0 if args.no_cpu_baseline else 1)),
+                         ("batch", lambda: bench_batch(cpu_images=0 if args.no_cpu_baseline else 16)),
+                         ("catalog", lambda: bench_catalog(cpu_searches=0 if args.no_cpu_baseline else 10)),
                          ("pair", lambda: bench_pair(cpu_pairs=0 if args.no_cpu_baseline else 64)),
                          ("panorama", lambda: bench_panorama(cpu_sample=not args.no_cpu_baseline))):
             try:

@@ -273,7 +273,11 @@ ps_status ps_homography_estimate(const ps_feature_match *m, size_t n, const ps_r
  * sampler and termination rule. The reference configuration is PS_SAMPLER_REFERENCE,
  * confidence 0.999, min 64, max 2000 iterations, local_opt = sim_seed_model = adaptive = 1.
  * PS_SAMPLER_WEIGHTS samples proportionally to caller weights (e.g. a learned inlier score)
- * and, when adaptive, stops when the sampled inlier mass reaches the confidence. */
+ * and, when adaptive, stops when the sampled inlier mass reaches the confidence. A problem
+ * whose weights have no usable mass (all zero, a total that is not finite and positive, or a
+ * weight that is negative or not finite) is sampled uniformly instead, with the uniform
+ * sampler's stop rule: uniform sampling is the guided sampler's fallback. The host-pointer
+ * form ps_ransac_batch rejects negative or non-finite weights with PS_E_INVALID_ARG. */
 #define PS_SAMPLER_REFERENCE 0   /* 1 / clamp(ratio, 0.05, 1), pipeline.rs:2006-2028 */
 #define PS_SAMPLER_UNIFORM 1
 #define PS_SAMPLER_WEIGHTS 2

@@ -1334,9 +1334,19 @@ size_t psr_homography_ransac(const psr_match *m, size_t n, float reproj, uint64_
     uint32_t *sim_inl = (uint32_t *)malloc(sizeof(uint32_t) * n);
     double *wd = NULL;
     double wsum = 0.0;
-    if (prm->sampler == 2) {
+    int sampler = prm->sampler;
+    if (sampler == 2) {
+        /* caller weights with no usable mass (a weight negative or not finite, or a total
+           that is not finite and positive) fall back to uniform sampling: the reference's
+           sampler always has positive mass (pipeline.rs:2006-2028) */
+        int ok = 1;
         wd = (double *)malloc(sizeof(double) * n);
-        for (size_t i = 0; i < n; i++) { wd[i] = (double)prm->weights[i]; wsum += wd[i]; }
+        for (size_t i = 0; i < n; i++) {
+            wd[i] = (double)prm->weights[i];
+            if (!(wd[i] >= 0.0) || !isfinite(wd[i])) ok = 0;
+            wsum += wd[i];
+        }
+        if (!ok || !isfinite(wsum) || !(wsum > 0.0)) { sampler = 1; wsum = 0.0; }
     }
     if (prm->sim_seed_model) {
         psr_similarity s; int hs;
@@ -1357,7 +1367,7 @@ size_t psr_homography_ransac(const psr_match *m, size_t n, float reproj, uint64_
         free(sim_inl); free(wd);
         return 0;
     }
-    double *cdf = prm->sampler == 1 ? NULL : build_cdf(m, n, prm->sampler == 2 ? prm->weights : NULL);
+    double *cdf = sampler == 1 ? NULL : build_cdf(m, n, sampler == 2 ? prm->weights : NULL);
     uint64_t st = hom_seed;
     size_t min_it = (size_t)prm->min_iters, cap = (size_t)prm->max_iters;
     /* termination: reference uses inlier_count/total; the uniform and guided variants use
@@ -1365,7 +1375,7 @@ size_t psr_homography_ransac(const psr_match *m, size_t n, float reproj, uint64_
     double bmass = 0.0;
     size_t max_it;
     if (!prm->adaptive) max_it = cap;
-    else if (prm->sampler == 2) {
+    else if (sampler == 2) {
         if (have) { double e; eval_h(m, n, H, tsq, &e, wd, &bmass); }
         max_it = have && wsum > 0 ? psr_adaptive_ransac_iters((size_t)(bmass / wsum * 1e9), 1000000000, 4, prm->confidence, min_it, cap) : cap;
     } else max_it = psr_adaptive_ransac_iters(bc, n, 4, prm->confidence, min_it, cap);
@@ -1377,7 +1387,7 @@ size_t psr_homography_ransac(const psr_match *m, size_t n, float reproj, uint64_
         size_t chosen = 0, att = 0;
         while (chosen < 4 && att < 16) {
             att++;
-            size_t c = prm->sampler == 1 ? sample_uniform(n, &st) : sample_cdf(cdf, n, &st);
+            size_t c = sampler == 1 ? sample_uniform(n, &st) : sample_cdf(cdf, n, &st);
             int dup = 0;
             for (size_t j = 0; j < chosen; j++) if (si[j] == c) dup = 1;
             if (!dup) si[chosen++] = c;
@@ -1394,7 +1404,7 @@ size_t psr_homography_ransac(const psr_match *m, size_t n, float reproj, uint64_
             memcpy(H, Hc, sizeof(H));
             have = 1;
             if (prm->adaptive) {
-                if (prm->sampler == 2) {
+                if (sampler == 2) {
                     double e2; eval_h(m, n, H, tsq, &e2, wd, &bmass);
                     max_it = wsum > 0 ? psr_adaptive_ransac_iters((size_t)(bmass / wsum * 1e9), 1000000000, 4, prm->confidence, min_it, cap) : cap;
                 } else max_it = psr_adaptive_ransac_iters(bc, n, 4, prm->confidence, min_it, cap);

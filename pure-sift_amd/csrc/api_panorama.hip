@@ -236,6 +236,9 @@ ps_status ps_panorama_pair_edges(const uint8_t *const *luma, const uint32_t *hei
     std::vector<ps_panorama_pair> pairs(np);
     PS_HIP(hipMemcpyAsync(pairs.data(), e->aux[AUX_PN_9].p, np * sizeof(ps_panorama_pair), hipMemcpyDeviceToHost, st));
     PS_HIP(hipStreamSynchronize(st));
+    // an image whose extraction overflowed the candidate capacity has incomplete features:
+    // report it instead of edges built from them (the single-image path fails the same way)
+    PS_TRY(ps_engine_check(e));
     size_t ne = 0;
     for (const ps_panorama_pair &p : pairs) {
         if (!p.has_edge) continue;

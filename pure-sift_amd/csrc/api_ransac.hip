@@ -92,6 +92,10 @@ ps_status ps_ransac_batch(const ps_feature_match *matches, const float *weights,
         return set_err(PS_E_INVALID_ARG, "null argument");
     if (params->sampler == PS_SAMPLER_WEIGHTS && total_matches && !weights)
         return set_err(PS_E_INVALID_ARG, "the weighted sampler needs weights");
+    if (params->sampler == PS_SAMPLER_WEIGHTS)
+        for (size_t i = 0; i < total_matches; i++)
+            if (!(weights[i] >= 0.0f) || !std::isfinite(weights[i]))
+                return set_err(PS_E_INVALID_ARG, "sampling weights must be finite and non-negative");
     if (total_matches >= 0x7fffffffull) return set_err(PS_E_INVALID_ARG, "too many matches");
     for (uint32_t i = 0; i < n_problems; i++)
         if ((size_t)problems[i].offset + problems[i].n > total_matches)

@@ -182,8 +182,6 @@ ps_status engine_new(int device, ps_engine **out) {
     }
     for (int i = 0; i < ps::kMaxOct && err == hipSuccess; i++)
         err = hipEventCreateWithFlags(&e->fork_ev[i], hipEventDisableTiming);
-    const char *mk = std::getenv("PS_MATCH_KERNEL");
-    e->match_valu = mk && std::strcmp(mk, "valu") == 0;
     if (err != hipSuccess) {
         release_streams(e);
         delete e;
@@ -319,7 +317,7 @@ ps::PyrView pyr_view(ps_engine *e) {
 }
 
 // Gaussian + DoG pyramid (pipeline.rs:550-638) into e->pyr (src is device memory) and, with
-// `extrema`, every octave's extrema appended to e->cand (cnt[0] candidates, cnt[1] error bits).
+// `extrema`, every octave's extrema appended to e->cand (cnt[0] candidates, cnt[kErrWord] error bits).
 // Octave o+1 depends only on level S of octave o, so once that level is written octave o+1
 // (the next, four times smaller and launch-latency bound) starts on a side stream while
 // octave o's last two levels and its extrema finish on its own stream; every side stream is
@@ -366,7 +364,7 @@ ps_status enqueue_scale_space(ps_engine *e, const void *src, bool src_u8, size_t
     };
     auto octave_extrema = [&](int o, hipStream_t st) -> hipError_t {
         return ps::launch_extrema(P + p.gauss(o, 0), p.h[o], p.w[o], p.S, cfg.contrast_threshold, p.scale_factor[o],
-                                  p.eff_scale[o], p.log2k, o, as<ps::Cand>(e->cand), cnt + 0, ps::kCandCap, cnt + 1,
+                                  p.eff_scale[o], p.log2k, o, as<ps::Cand>(e->cand), cnt + 0, ps::kCandCap, cnt + kErrWord,
                                   st);
     };
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
@@ -405,14 +403,14 @@ ps_status enqueue_scale_space(ps_engine *e, const void *src, bool src_u8, size_t
 }
 
 // pyramid -> extrema -> ordered (and, for extraction, capped) candidates in e->sorted;
-// counters: cnt[0] candidates, cnt[1] error bits, cnt[2] ordered keypoints, cnt[3] features
+// counters: cnt[0] candidates, cnt[2] ordered keypoints, cnt[3] features, cnt[kErrWord] error bits
 ps_status enqueue_keypoints(ps_engine *e, const void *src, bool src_u8, size_t h, size_t w, size_t stride,
                             const ps_sift_config &cfg, uint32_t limit) {
     PS_TRY(ensure(e->cnt, 64));
     uint32_t *cnt = as<uint32_t>(e->cnt);
     PS_TRY(ensure(e->rank, ps::rank_scratch_bytes(ps::kCandCap)));
     PS_TRY(ensure(e->sorted, (size_t)limit * sizeof(ps::Cand)));
-    PS_HIP(ps::launch_fill_u32(cnt, 0u, 16, e->stream));
+    PS_HIP(ps::launch_fill_u32(cnt, 0u, kErrWord, e->stream));   // every counter but the error word
     prof_mark(e, 0);
     PS_TRY(enqueue_scale_space(e, src, src_u8, h, w, stride, cfg, true));
     prof_mark(e, 2);
@@ -474,13 +472,6 @@ ps_status enqueue_nn(ps_engine *e, const float *qd, const uint32_t *nq, size_t n
                                        e->stream));
         return PS_OK;
     }
-    if (e->match_valu) {
-        int ns = ps::nn_splits(nqm, ntm);
-        PS_TRY(ensure(scratch, (size_t)ns * nqm * sizeof(ps::NNPart)));
-        PS_HIP(ps::launch_nn_partial(qd, nq, nqm, td, nt, ntm, ns, as<ps::NNPart>(scratch), e->stream));
-        PS_HIP(ps::launch_nn_merge(as<ps::NNPart>(scratch), ns, nq, nqm, bidx, bd, sd, e->stream));
-        return PS_OK;
-    }
     void *const before = scratch.p;
     PS_TRY(ensure(scratch, ps::nn_mfma_scratch_bytes(nqm, ntm)));
     if (++e->nn_gen == 0) e->nn_gen = 1;
@@ -519,10 +510,23 @@ ps_status enqueue_match(ps_engine *e, const ps_feature *qf, const float *qd, con
     return PS_OK;
 }
 
+// The error word (counter word kErrWord) accumulates the error bits of every extraction on the
+// engine since the last check; a check reads and clears it.
+ps_status clear_err_word(ps_engine *e) {
+    PS_TRY(ensure(e->cnt, 64));
+    PS_HIP(ps::launch_fill_u32(as<uint32_t>(e->cnt) + kErrWord, 0u, 1, e->stream));
+    return PS_OK;
+}
+
 ps_status check_err_word(ps_engine *e) {
     uint32_t err = 0;
-    PS_HIP(hipMemcpyAsync(&err, as<uint32_t>(e->cnt) + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream));
+    PS_HIP(hipMemcpyAsync(&err, as<uint32_t>(e->cnt) + kErrWord, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          e->stream));
     PS_HIP(hipStreamSynchronize(e->stream));
+    if (err) {
+        PS_TRY(clear_err_word(e));
+        PS_HIP(hipStreamSynchronize(e->stream));
+    }
     if (err & ps::kErrCandOverflow) return set_err(PS_E_CAPACITY, "more than 2^20 extremum candidates in one image");
     return PS_OK;
 }
@@ -548,6 +552,7 @@ ps_status extract_host(const T *img, size_t h, size_t w, size_t row_stride, cons
     PS_TRY(ensure(e->desc, PS_MAX_FEATURES * 128 * sizeof(float)));
     PS_TRY(ensure(e->mcnt, 64));
     uint32_t *dcount = as<uint32_t>(e->mcnt);
+    PS_TRY(clear_err_word(e));
     PS_TRY(enqueue_extract(e, e->img.p, u8, h, w, w, *cfg, as<ps_feature>(e->feat), as<float>(e->desc),
                            PS_MAX_FEATURES, dcount));
     PS_HIP(hipMemcpyAsync(&e->host_counts[0], dcount, sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream));
@@ -627,6 +632,7 @@ ps_status ps_detect_keypoints(const float *img, size_t h, size_t w, size_t row_s
     if (h && w)
         PS_HIP(hipMemcpy2DAsync(e->img.p, w * sizeof(float), img, row_stride * sizeof(float), w * sizeof(float), h,
                                 hipMemcpyHostToDevice, e->stream));
+    PS_TRY(clear_err_word(e));
     PS_TRY(enqueue_keypoints(e, e->img.p, false, h, w, w, *cfg, ps::kCandCap));
     PS_HIP(hipMemcpyAsync(&e->host_counts[0], as<uint32_t>(e->cnt) + 2, sizeof(uint32_t), hipMemcpyDeviceToHost,
                           e->stream));

@@ -42,7 +42,6 @@ struct ps_engine {
     DevBuf qf, qd, tf, td, mcnt, mout;             // host-API match staging
     DevBuf part, part2, bidx, bdist, sdist, ridx, ridx_d, fin;  // matcher scratch
     DevBuf aux[AUX_COUNT];
-    bool match_valu = false;                       // PS_MATCH_KERNEL=valu: direct f32 kernel
     uint32_t nn_gen = 0;                           // per-call tag of the exact matcher's scratch words
     // batched extraction: child engines (own stream + workspace each) run images
     // concurrently; the last batch is kept as an instantiated graph
@@ -64,6 +63,13 @@ struct ps_engine {
 };
 
 namespace pshost {
+
+// Counter word of an engine's `cnt` buffer that accumulates the error bits of its extractions
+// (ps::kErrCandOverflow) until ps_engine_check reads and clears it; words 0..kErrWord-1 are
+// the per-extraction counters.
+constexpr int kErrWord = 15;
+ps_status clear_err_word(ps_engine *e);
+ps_status check_err_word(ps_engine *e);
 
 ps_status set_err(ps_status s, const std::string &msg);
 ps_status ensure(DevBuf &b, size_t bytes);

@@ -1,173 +1,12 @@
-// Brute-force Lowe-ratio matcher, bit-exact with pure-sift 0.2.0's
-// match_descriptors_with_options (src/pipeline.rs:1700-1841).
-//
-// Every (query, target) distance is the reference's sequential f32 sum
-// sum += (q_i - t_i)^2 over i = 0..127 with separate roundings. The early exit of
-// descriptor_dist_sq_early_exit (pipeline.rs:1712) never changes a result
-// (partial sums are monotone and it only fires when the sum already reaches the
-// running second-best), so it is not needed for exactness. The per-query state
-// (best, second, best index) is merged across threads and workgroups with an
-// order-independent rule -- best = min distance, lowest index on ties; second =
-// min(second_a, second_b, max(best_a, best_b)) -- which equals the reference's
-// sequential scan in index order.
+// Ratio test, mutual check and ordered compaction of the Lowe-ratio matcher, bit-exact with
+// pure-sift 0.2.0's match_descriptors_with_options (src/pipeline.rs:1744-1841). The nearest /
+// second-nearest distances come from the split-BF16 MFMA pass with exact f32 rescoring
+// (match_mfma.hip); this file turns them into FeatureMatch records in query order.
 #include "ps_internal.h"
 
 #pragma clang fp contract(off)
 
 namespace ps {
-
-constexpr int kQT = 64;       // queries per workgroup
-constexpr int kTT = 64;       // targets per tile
-constexpr int kDP = 132;      // padded row stride in LDS (floats)
-
-__device__ __forceinline__ void nn_push(float d, uint32_t t, float &b, float &s, uint32_t &bi) {
-    if (d < b) { s = b; b = d; bi = t; }          // pipeline.rs:1779-1785
-    else if (d < s) s = d;
-}
-
-__device__ __forceinline__ void nn_combine(float &b, float &s, uint32_t &bi, float b2, float s2, uint32_t bi2) {
-    float other;
-    if (b2 < b || (b2 == b && bi2 < bi)) { other = b; b = b2; bi = bi2; }
-    else other = b2;
-    s = fminf(fminf(s, s2), other);
-}
-
-__global__ __launch_bounds__(256) void nn_partial_kernel(const float *qd, const uint32_t *nq_p, uint32_t nq_max,
-                                                         const float *td, const uint32_t *nt_p, uint32_t nt_max,
-                                                         int nsplit, NNPart *part) {
-    __shared__ __attribute__((aligned(16))) float sq[kQT * kDP];
-    __shared__ __attribute__((aligned(16))) float st[kTT * kDP];
-    const uint32_t nq = min(*nq_p, nq_max), nt = min(*nt_p, nt_max);
-    const uint32_t q0 = blockIdx.x * kQT;
-    if (q0 >= nq) return;
-    const int tid = threadIdx.x;
-    const uint32_t ntiles = (nt + kTT - 1) / kTT;
-    const uint32_t per = (ntiles + nsplit - 1) / nsplit;
-    const uint32_t tb = blockIdx.y * per, te = min(tb + per, ntiles);
-
-    for (int i = tid; i < kQT * 32; i += 256) {
-        int r = i >> 5, c = (i & 31) * 4;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (q0 + r < nq) v = *reinterpret_cast<const float4 *>(qd + (size_t)(q0 + r) * 128 + c);
-        *reinterpret_cast<float4 *>(sq + r * kDP + c) = v;
-    }
-    const int tx = tid & 15, ty = tid >> 4;
-    float best[4], second[4];
-    uint32_t bidx[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) { best[i] = INFINITY; second[i] = INFINITY; bidx[i] = 0; }
-
-    for (uint32_t tile = tb; tile < te; tile++) {
-        const uint32_t t0 = tile * kTT;
-        float4 v[kTT * 32 / 256];
-#pragma unroll
-        for (int j = 0; j < kTT * 32 / 256; j++) {
-            int i = tid + j * 256, r = i >> 5, c = (i & 31) * 4;
-            v[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (t0 + r < nt) v[j] = *reinterpret_cast<const float4 *>(td + (size_t)(t0 + r) * 128 + c);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < kTT * 32 / 256; j++) {
-            int i = tid + j * 256, r = i >> 5, c = (i & 31) * 4;
-            *reinterpret_cast<float4 *>(st + r * kDP + c) = v[j];
-        }
-        __syncthreads();
-        float acc[4][4];
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) acc[i][j] = 0.0f;
-#pragma unroll 2
-        for (int d = 0; d < 128; d += 4) {
-            float4 a[4], b[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) a[i] = *reinterpret_cast<const float4 *>(sq + (ty * 4 + i) * kDP + d);
-#pragma unroll
-            for (int j = 0; j < 4; j++) b[j] = *reinterpret_cast<const float4 *>(st + (tx * 4 + j) * kDP + d);
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    float e;
-                    e = a[i].x - b[j].x; acc[i][j] += e * e;
-                    e = a[i].y - b[j].y; acc[i][j] += e * e;
-                    e = a[i].z - b[j].z; acc[i][j] += e * e;
-                    e = a[i].w - b[j].w; acc[i][j] += e * e;
-                }
-        }
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            uint32_t t = t0 + tx * 4 + j;
-            if (t < nt) {
-#pragma unroll
-                for (int i = 0; i < 4; i++) nn_push(acc[i][j], t, best[i], second[i], bidx[i]);
-            }
-        }
-    }
-    // merge the 16 partial states of each query (threads with equal ty)
-    __syncthreads();
-    float *sb = sq;                                   // reuse LDS: [64 queries][16] x 3
-    float *ss = sq + kQT * 16;
-    uint32_t *si = reinterpret_cast<uint32_t *>(sq + 2 * kQT * 16);
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        int q = ty * 4 + i;
-        sb[q * 16 + tx] = best[i];
-        ss[q * 16 + tx] = second[i];
-        si[q * 16 + tx] = bidx[i];
-    }
-    __syncthreads();
-    if (tid < kQT && q0 + tid < nq) {
-        float b = sb[tid * 16], s = ss[tid * 16];
-        uint32_t bi = si[tid * 16];
-        for (int k = 1; k < 16; k++) nn_combine(b, s, bi, sb[tid * 16 + k], ss[tid * 16 + k], si[tid * 16 + k]);
-        NNPart p;
-        p.best = b; p.second = s; p.idx = bi; p.pad = 0;
-        part[(size_t)blockIdx.y * nq_max + q0 + tid] = p;
-    }
-}
-
-int nn_splits(size_t nq_max, size_t nt_max) {
-    size_t qtiles = (nq_max + kQT - 1) / kQT, ttiles = (nt_max + kTT - 1) / kTT;
-    if (qtiles == 0 || ttiles == 0) return 1;
-    size_t ns = (2048 + qtiles - 1) / qtiles;
-    if (ns > ttiles) ns = ttiles;
-    if (ns > 256) ns = 256;
-    return (int)(ns < 1 ? 1 : ns);
-}
-
-hipError_t launch_nn_partial(const float *qd, const uint32_t *nq, size_t nq_max, const float *td, const uint32_t *nt,
-                             size_t nt_max, int nsplit, NNPart *part, hipStream_t st) {
-    if (nq_max == 0) return hipSuccess;
-    dim3 grid((unsigned)((nq_max + kQT - 1) / kQT), (unsigned)nsplit);
-    nn_partial_kernel<<<grid, 256, 0, st>>>(qd, nq, (uint32_t)nq_max, td, nt, (uint32_t)nt_max, nsplit, part);
-    return hipGetLastError();
-}
-
-__global__ void nn_merge_kernel(const NNPart *part, int nsplit, const uint32_t *nq_p, uint32_t nq_max,
-                                uint32_t *best_idx, float *best, float *second) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= min(*nq_p, nq_max)) return;
-    NNPart p = part[q];
-    float b = p.best, s = p.second;
-    uint32_t bi = p.idx;
-    for (int k = 1; k < nsplit; k++) {
-        NNPart o = part[(size_t)k * nq_max + q];
-        nn_combine(b, s, bi, o.best, o.second, o.idx);
-    }
-    best_idx[q] = bi;
-    if (best) best[q] = b;
-    if (second) second[q] = s;
-}
-
-hipError_t launch_nn_merge(const NNPart *part, int nsplit, const uint32_t *nq, size_t nq_max, uint32_t *best_idx,
-                           float *best, float *second, hipStream_t st) {
-    if (nq_max == 0) return hipSuccess;
-    nn_merge_kernel<<<(unsigned)((nq_max + 255) / 256), 256, 0, st>>>(part, nsplit, nq, (uint32_t)nq_max, best_idx,
-                                                                      best, second);
-    return hipGetLastError();
-}
 
 // Ratio test (pipeline.rs:1788-1805), optional mutual check (pipeline.rs:1815-1840) and
 // ordered compaction of the accepted matches in query order: match_flag_kernel decides every

@@ -101,15 +101,6 @@ hipError_t launch_cand_to_keypoints(const Cand *c, const uint32_t *n, uint32_t c
                                     hipStream_t st);
 
 // match.hip
-struct NNPart {
-    float best, second;
-    uint32_t idx, pad;
-};
-int nn_splits(size_t nq_max, size_t nt_max);
-hipError_t launch_nn_partial(const float *qd, const uint32_t *nq, size_t nq_max, const float *td,
-                             const uint32_t *nt, size_t nt_max, int nsplit, NNPart *part, hipStream_t st);
-hipError_t launch_nn_merge(const NNPart *part, int nsplit, const uint32_t *nq, size_t nq_max, uint32_t *best_idx,
-                           float *best, float *second, hipStream_t st);
 // nq_max <= 262144 (1024 blocks of 256 queries)
 size_t match_finish_scratch_bytes(size_t nq_max);
 hipError_t launch_match_finish(const ps_feature *qf, const ps_feature *tf, const uint32_t *nq, size_t nq_max,

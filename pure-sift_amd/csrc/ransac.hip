@@ -984,18 +984,22 @@ __global__ __launch_bounds__(kRT) void ransac_kernel(const ps_feature_match *mat
         }
         return;
     }
-    const int sampler = prm.sampler;
-    if (sampler == 0) {
-        if (!prm.sim_seed_model || n == 1) build_cdf(true);   // otherwise the similarity stage built it
-    } else if (sampler == 2) {
-        build_cdf(false);
-    }
+    int sampler = prm.sampler;
     double wsum = 0.0;
     if (sampler == 2) {
+        // the caller's weights need usable mass: every weight finite and non-negative and a
+        // finite positive total; otherwise the problem falls back to uniform sampling (the
+        // reference's 1 / ratio sampler always has positive mass, pipeline.rs:2006-2028)
         double acc = 0.0;   // thread 0, in order
+        if (tid == 0) S.total = 0;
+        __syncthreads();
         for (uint32_t c0 = 0; c0 < n; c0 += kStage) {
             const uint32_t cn = min(kStage, n - c0);
-            for (uint32_t k = tid; k < cn; k += kRT) S.stage.d[0][k] = (double)wts[c0 + k];
+            for (uint32_t k = tid; k < cn; k += kRT) {
+                const double wv = (double)wts[c0 + k];
+                if (!(wv >= 0.0) || !isfinite(wv)) S.total = 1u;
+                S.stage.d[0][k] = wv;
+            }
             __syncthreads();
             if (tid == 0) {
 #pragma unroll 8
@@ -1005,7 +1009,17 @@ __global__ __launch_bounds__(kRT) void ransac_kernel(const ps_feature_match *mat
             __syncthreads();
         }
         wsum = S.red[0];
+        const bool usable = S.total == 0u && isfinite(wsum) && wsum > 0.0;
         __syncthreads();
+        if (!usable) {
+            sampler = 1;
+            wsum = 0.0;
+        }
+    }
+    if (sampler == 0) {
+        if (!prm.sim_seed_model || n == 1) build_cdf(true);   // otherwise the similarity stage built it
+    } else if (sampler == 2) {
+        build_cdf(false);
     }
     const uint32_t min_it = prm.min_iters, cap = prm.max_iters;
     uint32_t max_it;

@@ -317,6 +317,17 @@ def evaluate(eng, ds, y, params, test_ids, confidence=0.99):
     return out
 
 
+def split_by_base(n_pairs, n_base, train_frac):
+    """Train / test pair ids split by base image (pair k uses base k % n_base): the first
+    round(n_base * train_frac) bases train, the others test."""
+    ids = np.arange(n_pairs)
+    n_train_base = int(round(n_base * train_frac))
+    if not 0 < n_train_base < n_base:
+        raise ValueError("the split needs at least one training and one test base image")
+    base = ids % n_base
+    return ids[base < n_train_base], ids[base >= n_train_base]
+
+
 def run(n_pairs=4096, n_base=256, train_frac=0.75, steps=600, seed=0, outlier_attrs="true", log=None):
     import torch
     eng = ps.Engine(0)
@@ -337,13 +348,15 @@ def _run(eng, n_pairs, n_base, train_frac, steps, seed, outlier_attrs, log):
     t_lab = time.perf_counter() - t
     if log:
         log(f"labels: {y.mean():.3f} inlier fraction, {t_lab:.1f} s")
-    ids = np.arange(n_pairs)
-    n_train = int(n_pairs * train_frac)
+    # split by base image: pair k warps base image k % n_base, so the held-out pairs' query
+    # images (and their features) never appear in a training pair
+    train_ids, test_ids = split_by_base(n_pairs, n_base, train_frac)
     t = time.perf_counter()
-    params, curve = train(eng, ds, y, ids[:n_train], steps=steps, seed=seed, log=log)
+    params, curve = train(eng, ds, y, train_ids, steps=steps, seed=seed, log=log)
     t_train = time.perf_counter() - t
-    ev = evaluate(eng, ds, y, params, ids[n_train:])
-    ev.update({"pairs": n_pairs, "train_pairs": n_train, "test_pairs": n_pairs - n_train,
+    ev = evaluate(eng, ds, y, params, test_ids)
+    ev.update({"pairs": n_pairs, "train_pairs": int(len(train_ids)), "test_pairs": int(len(test_ids)),
+               "split": f"by base image: bases [0, {int(round(n_base * train_frac))}) train, the rest test",
                "matches": int(len(ds.matches)), "label_inlier_frac": float(y.mean()), "loss_curve": curve,
                "seconds": {"data": t_data, "labels": t_lab, "train": t_train}})
     return ev, params

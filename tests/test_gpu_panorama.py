@@ -219,3 +219,29 @@ def test_panorama_uneven_sizes_and_blank_image(gpu):
     eng.sync()
     _check_records(dpairs.cpu().numpy().view(gpu.PANORAMA_PAIR_DTYPE), want)
     eng.close()
+
+
+def _dense_lattice(n, period=4, sigma=0.9):
+    """u8 image of identical bright blobs on a `period`-pixel lattice: about one extremum
+    candidate per lattice site, so n = 4608 gives ~1.3M candidates, above the 2^20 capacity."""
+    yy, xx = np.mgrid[0:n, 0:n].astype(np.float64)
+    py = np.exp(-((np.mod(yy, period) - period / 2) ** 2) / (2 * sigma ** 2))
+    px = np.exp(-((np.mod(xx, period) - period / 2) ** 2) / (2 * sigma ** 2))
+    return np.round((0.2 + 0.6 * py * px) * 255).astype(np.uint8)
+
+
+def test_candidate_overflow_is_reported(gpu):
+    """An image with more than 2^20 extremum candidates: the host panorama call reports
+    PS_E_CAPACITY instead of edges built from incomplete features, as the single-image
+    extraction does; the next call on the pooled engines is unaffected."""
+    big = _dense_lattice(4608)
+    small = synth.blob_image(320, 240, 40, seed=1)
+    with pytest.raises(gpu.DeviceError) as ei:
+        gpu.panorama_pair_edges([small, big, small])
+    assert ei.value.status == gpu.PS_E_CAPACITY
+    with pytest.raises(gpu.DeviceError) as ei:
+        gpu.Sift(gpu.SiftConfig().octaves(3).scales(3)).detect_and_compute(big)
+    assert ei.value.status == gpu.PS_E_CAPACITY
+    tiles = synth.panorama_tiles(1600, 480, 3, 300, seed=4)[0]
+    ok = gpu.panorama_pair_edges(tiles)
+    assert len(ok) >= 2

@@ -125,3 +125,78 @@ def test_batch_matches_oracle(gpu, cfg):
             assert _same_h(r["h"].reshape(3, 3), Hr), k
         if n >= 4:
             assert r["hyps"] == hyps, (k, r["hyps"], hyps)
+
+
+def _fallback_batch(gpu, weight_fn, host=True):
+    """Three problems sampled by weights from weight_fn(m), next to the same problems sampled
+    uniformly; returns both result sets."""
+    ms, probs, off = [], [], 0
+    for k, (a, b) in enumerate([(40, 60), (150, 350), (12, 30)]):
+        m = _problem(a, b, seed=700 + k)
+        ms.append(m)
+        probs.append((off, len(m), 11 + k, 1000 + 7 * k))
+        off += len(m)
+    allm = np.concatenate(ms)
+    pr = np.array(probs, gpu.RANSAC_PROBLEM_DTYPE)
+    kw = dict(confidence=0.99, min_iters=1, max_iters=3000, local_opt=True, sim_seed_model=False, adaptive=True)
+    wts = weight_fn(allm).astype(np.float32)
+    if host:
+        res_w, inl_w = gpu.ransac_batch(allm, pr, gpu.ransac_params(reproj_threshold=3.0, sampler=2, **kw),
+                                        weights=wts)
+    else:
+        import torch
+        dev = torch.device("cuda:0")
+        eng = gpu.Engine(0)
+        try:
+            dm = torch.from_numpy(allm.view(np.uint8).copy()).to(dev)
+            dw = torch.from_numpy(wts).to(dev)
+            dp = torch.from_numpy(pr.view(np.uint8).copy()).to(dev)
+            di = torch.zeros(len(allm), dtype=torch.int32, device=dev)
+            dr = torch.zeros(len(pr) * gpu.RANSAC_RESULT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+            prm = gpu.ransac_params(reproj_threshold=3.0, sampler=2, **kw)
+            eng.ransac_batch(dm.data_ptr(), dw.data_ptr(), dp.data_ptr(), len(pr), len(allm), prm, di.data_ptr(),
+                             dr.data_ptr())
+            eng.sync()
+            eng.check()
+            res_w = dr.cpu().numpy().view(gpu.RANSAC_RESULT_DTYPE).copy()
+            inl_w = di.cpu().numpy().view(np.uint32).copy()
+        finally:
+            eng.close()
+    res_u, inl_u = gpu.ransac_batch(allm, pr, gpu.ransac_params(reproj_threshold=3.0, sampler=1, **kw))
+    return ms, probs, wts, (res_w, inl_w), (res_u, inl_u), kw
+
+
+def test_weighted_sampler_without_mass_falls_back_to_uniform(gpu):
+    """All-zero weights (no sampling mass): each problem gets exactly the uniform sampler's result
+    (inliers, model, hypothesis count), which also equals the oracle's run."""
+    ms, probs, wts, (rw, iw), (ru, iu), kw = _fallback_batch(gpu, lambda m: np.zeros(len(m)))
+    assert np.array_equal(rw, ru) and np.array_equal(iw, iu)
+    for k, m in enumerate(ms):
+        o, n = probs[k][0], probs[k][1]
+        rin, Hr, hyps = psr.homography_ransac(m, 3.0, sim_seed=probs[k][2], hom_seed=probs[k][3], sampler=2,
+                                              weights=wts[o:o + n], **kw)
+        assert np.array_equal(iw[o:o + rw[k]["n_inliers"]], rin) and rw[k]["hyps"] == hyps
+        assert rw[k]["has_model"] == 1
+
+
+@pytest.mark.parametrize("bad", [np.nan, -1.0, np.inf])
+def test_weighted_sampler_bad_weights(gpu, bad):
+    """A NaN, negative or infinite weight: the host-pointer entry point rejects the call; the
+    device-resident form samples that problem uniformly (the kernel cannot return an error per
+    problem)."""
+    def fn(m):
+        w = 1.0 - m["ratio"].astype(np.float64) + 0.01
+        w[5] = bad
+        return w
+    with pytest.raises(gpu.DeviceError):
+        _fallback_batch(gpu, fn, host=True)
+    ms, probs, wts, (rw, iw), (ru, iu), kw = _fallback_batch(gpu, fn, host=False)
+    # problem 0 holds the bad weight and falls back; the others keep their weighted sampling
+    assert np.array_equal(rw[0], ru[0])
+    o, n = probs[0][0], probs[0][1]
+    assert np.array_equal(iw[o:o + rw[0]["n_inliers"]], iu[o:o + ru[0]["n_inliers"]])
+    for k in (1, 2):
+        o, n = probs[k][0], probs[k][1]
+        rin, Hr, hyps = psr.homography_ransac(ms[k], 3.0, sim_seed=probs[k][2], hom_seed=probs[k][3], sampler=2,
+                                              weights=wts[o:o + n], **kw)
+        assert np.array_equal(iw[o:o + rw[k]["n_inliers"]], rin) and rw[k]["hyps"] == hyps

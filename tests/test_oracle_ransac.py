@@ -148,6 +148,24 @@ def test_variant_samplers_recover_model(sampler):
         assert hyps * 3 <= hu
 
 
+@pytest.mark.parametrize("bad", [0.0, np.nan, -1.0, np.inf])
+def test_weighted_sampler_fallback_is_uniform(bad):
+    """Weights without usable mass (all zero, or one NaN / negative / infinite weight) sample
+    uniformly with the uniform stop rule: same inliers, model and hypothesis count as sampler 1."""
+    H, pts, m = _eighty_percent_outliers()
+    w = np.zeros(len(m), np.float32) if bad == 0.0 else np.full(len(m), 0.5, np.float32)
+    w[3] = bad
+    kw = dict(confidence=0.99, min_iters=1, max_iters=100000, adaptive=True)
+    got = psr.homography_ransac(m, 3.0, 0, 0x5EED, sampler=2, weights=w, **kw)
+    uni = psr.homography_ransac(m, 3.0, 0, 0x5EED, sampler=1, **kw)
+    assert np.array_equal(got[0], uni[0]) and got[2] == uni[2]
+    assert np.array_equal(got[1], uni[1])
+    # a usable weight vector does change the sampling
+    w2 = np.full(len(m), 0.5, np.float32)
+    w2[:12] = 5.0
+    assert psr.homography_ransac(m, 3.0, 0, 0x5EED, sampler=2, weights=w2, **kw)[2] < uni[2]
+
+
 def test_mlp_gradient_finite_differences():
     """The oracle's MLP gradient against central differences of its own loss (float64
     arithmetic on f32 parameters; small steps stay off the ReLU kinks)."""
