@@ -133,6 +133,15 @@ ps_status ensure(DevBuf &b, size_t bytes) {
     return PS_OK;
 }
 
+// The engine's counter words; a fresh buffer is cleared once so that the error word, which
+// only ps_engine_check / clear_err_word reset, starts at zero.
+ps_status ensure_cnt(ps_engine *e) {
+    void *const before = e->cnt.p;
+    PS_TRY(ensure(e->cnt, 64));
+    if (e->cnt.p != before) PS_HIP(ps::launch_fill_u32(as<uint32_t>(e->cnt), 0u, 16, e->stream));
+    return PS_OK;
+}
+
 std::mutex g_mu;
 bool g_inited = false;
 int g_device = 0;
@@ -406,7 +415,7 @@ ps_status enqueue_scale_space(ps_engine *e, const void *src, bool src_u8, size_t
 // counters: cnt[0] candidates, cnt[2] ordered keypoints, cnt[3] features, cnt[kErrWord] error bits
 ps_status enqueue_keypoints(ps_engine *e, const void *src, bool src_u8, size_t h, size_t w, size_t stride,
                             const ps_sift_config &cfg, uint32_t limit) {
-    PS_TRY(ensure(e->cnt, 64));
+    PS_TRY(ensure_cnt(e));
     uint32_t *cnt = as<uint32_t>(e->cnt);
     PS_TRY(ensure(e->rank, ps::rank_scratch_bytes(ps::kCandCap)));
     PS_TRY(ensure(e->sorted, (size_t)limit * sizeof(ps::Cand)));
@@ -452,7 +461,7 @@ ps_status prepare_extract(ps_engine *e, size_t h, size_t w, const ps_sift_config
     if (!ps::make_plan((int)h, (int)w, cfg, p)) return set_err(PS_E_CONFIG, "unsupported blur radius");
     const uint32_t kMax = PS_MAX_FEATURES;
     PS_TRY(ensure(e->pyr, p.total_floats * sizeof(float)));
-    PS_TRY(ensure(e->cnt, 64));
+    PS_TRY(ensure_cnt(e));
     PS_TRY(ensure(e->cand, (size_t)ps::kCandCap * sizeof(ps::Cand)));
     PS_TRY(ensure(e->rank, ps::rank_scratch_bytes(ps::kCandCap)));
     PS_TRY(ensure(e->sorted, (size_t)kMax * sizeof(ps::Cand)));
@@ -513,7 +522,7 @@ ps_status enqueue_match(ps_engine *e, const ps_feature *qf, const float *qd, con
 // The error word (counter word kErrWord) accumulates the error bits of every extraction on the
 // engine since the last check; a check reads and clears it.
 ps_status clear_err_word(ps_engine *e) {
-    PS_TRY(ensure(e->cnt, 64));
+    PS_TRY(ensure_cnt(e));
     PS_HIP(ps::launch_fill_u32(as<uint32_t>(e->cnt) + kErrWord, 0u, 1, e->stream));
     return PS_OK;
 }
@@ -707,7 +716,7 @@ ps_status ps_compute_descriptors(const float *img, size_t h, size_t w, size_t ro
     PS_TRY(ensure(e->img, h * w * sizeof(float)));
     PS_HIP(hipMemcpy2DAsync(e->img.p, w * sizeof(float), img, row_stride * sizeof(float), w * sizeof(float), h,
                             hipMemcpyHostToDevice, e->stream));
-    PS_TRY(ensure(e->cnt, 64));
+    PS_TRY(ensure_cnt(e));
     PS_TRY(enqueue_scale_space(e, e->img.p, false, h, w, w, cfg, false));
     const ps::PyrPlan &p = e->plan;
     if (p.n_oct == 0) return PS_OK;
@@ -789,7 +798,7 @@ ps_status ps_build_pyramid(const float *img, size_t h, size_t w, size_t row_stri
     if (h && w)
         PS_HIP(hipMemcpy2DAsync(e->img.p, w * sizeof(float), img, row_stride * sizeof(float), w * sizeof(float), h,
                                 hipMemcpyHostToDevice, e->stream));
-    PS_TRY(ensure(e->cnt, 64));
+    PS_TRY(ensure_cnt(e));
     PS_TRY(enqueue_scale_space(e, e->img.p, false, h, w, w, *cfg, false));
     if (p.total_floats)
         PS_HIP(hipMemcpyAsync(out, e->pyr.p, p.total_floats * sizeof(float), hipMemcpyDeviceToHost, e->stream));

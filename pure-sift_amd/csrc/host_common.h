@@ -68,6 +68,7 @@ namespace pshost {
 // (ps::kErrCandOverflow) until ps_engine_check reads and clears it; words 0..kErrWord-1 are
 // the per-extraction counters.
 constexpr int kErrWord = 15;
+ps_status ensure_cnt(ps_engine *e);
 ps_status clear_err_word(ps_engine *e);
 ps_status check_err_word(ps_engine *e);
 
