@@ -178,6 +178,21 @@ ps_status ps_engine_match(ps_engine *e, const ps_feature *d_qf, const float *d_q
                           const float *d_td, const uint32_t *d_nt, size_t nt_max,
                           const ps_match_options *opt, ps_feature_match *d_out, uint32_t *d_out_count);
 
+/* No counterpart in the reference: ps_engine_match for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, results and validation as ps_engine_match with the exact FP32
+ * path (opt->precision must be PS_PRECISION_FP32); every scratch buffer comes from a static
+ * 17 MiB device array that each GPU holds until the process ends, so the call allocates
+ * nothing (the array is set up by ps_init / ps_engine_create, else by the first call on a
+ * device). Limits: nq_max <= 4096 and nt_max <= 4096 (PS_E_INVALID_ARG beyond). All calls on
+ * one GPU share the array: calls that overlap on one GPU (several engines or threads) race
+ * and must be serialised by the caller. */
+ps_status ps_engine_match_static(ps_engine *e, const ps_feature *d_qf, const float *d_qd,
+                                 const uint32_t *d_nq, size_t nq_max, const ps_feature *d_tf,
+                                 const float *d_td, const uint32_t *d_nt, size_t nt_max,
+                                 const ps_match_options *opt, ps_feature_match *d_out,
+                                 uint32_t *d_out_count);
+
 /* Many ratio-match problems in one launch chain. A feature set is a row range of a feature /
  * descriptor array (rows [offset, offset + count); count is device data, clamped to set_cap);
  * problem k matches query set problems[k].query_set against target set

@@ -165,6 +165,8 @@ ps_status init_locked(int device) {
     PS_HIP(hipGetDeviceProperties(&prop, device));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return set_err(PS_E_NO_DEVICE, std::string("libps_sift.so is built for gfx950, device is ") + prop.gcnArchName);
+    PS_HIP(hipSetDevice(device));
+    PS_TRY(static_scratch_setup(device, nullptr));
     g_device = device;
     g_inited = true;
     return PS_OK;
@@ -195,6 +197,12 @@ ps_status engine_new(int device, ps_engine **out) {
         release_streams(e);
         delete e;
         return set_err(PS_E_DEVICE, std::string("engine stream: ") + hipGetErrorString(err));
+    }
+    ps_status st = static_scratch_setup(device, nullptr);
+    if (st != PS_OK) {
+        release_streams(e);
+        delete e;
+        return st;
     }
     *out = e;
     return PS_OK;

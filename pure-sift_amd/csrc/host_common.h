@@ -85,6 +85,10 @@ ps_status engine_new(int device, ps_engine **out);
 void engine_free(ps_engine *e);
 void release_batch(ps_engine *e);
 ps_status validate_cfg(const ps_sift_config *c, char *msg, size_t cap);
+ps_status validate_opt(const ps_match_options *o, char *msg, size_t cap);
+// device address of the no-allocation matcher's static scratch on `device` (set up once per
+// device, with that device current)
+ps_status static_scratch_setup(int device, unsigned char **out);
 ps_status prepare_extract(ps_engine *e, size_t h, size_t w, const ps_sift_config &cfg);
 // exact ratio match (pipeline.rs:1744-1841) of device features, counts as device u32s
 ps_status enqueue_match(ps_engine *e, const ps_feature *qf, const float *qd, const uint32_t *nq, size_t nq_max,

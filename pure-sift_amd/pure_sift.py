@@ -180,6 +180,8 @@ def lib() -> C.CDLL:
         L.ps_engine_check.argtypes = [vp]
         L.ps_engine_extract_u8.argtypes = [vp, vp, sz, sz, sz, C.POINTER(_SiftConfigC), vp, vp, sz, u32p]
         L.ps_engine_match.argtypes = [vp, vp, vp, u32p, sz, vp, vp, u32p, sz, C.POINTER(_MatchOptionsC), vp, u32p]
+        L.ps_engine_match_static.argtypes = [vp, vp, vp, u32p, sz, vp, vp, u32p, sz, C.POINTER(_MatchOptionsC), vp,
+                                             u32p]
         L.ps_debug_fail_next.argtypes = []
         L.ps_engine_profile_begin.argtypes = [vp, C.c_uint32]
         L.ps_engine_profile_end.argtypes = [vp, C.POINTER(C.c_float), u32p]
@@ -245,7 +247,7 @@ def lib() -> C.CDLL:
                      "ps_engine_quantize_u128", "ps_engine_hamming_topk", "ps_engine_rerank_l2","ps_init", "ps_validate_sift_config", "ps_validate_match_options", "ps_detect_and_compute",
                      "ps_detect_and_compute_u8", "ps_detect_keypoints", "ps_match", "ps_build_pyramid", "ps_nn2",
                      "ps_engine_create", "ps_engine_destroy", "ps_engine_sync", "ps_engine_check",
-                     "ps_engine_extract_u8", "ps_engine_match", "ps_engine_profile_begin",
+                     "ps_engine_extract_u8", "ps_engine_match", "ps_engine_match_static", "ps_engine_profile_begin",
                      "ps_engine_profile_end", "ps_catalog_create", "ps_catalog_load_binary", "ps_catalog_load_file",
                      "ps_catalog_save_binary", "ps_catalog_destroy", "ps_catalog_get_info", "ps_catalog_entry",
                      "ps_catalog_entry_features", "ps_catalog_upsert", "ps_catalog_remove",
@@ -789,6 +791,14 @@ class Engine:
         u = lambda p: C.cast(p, C.POINTER(C.c_uint32))
         check(lib().ps_engine_match(self.h, d_qf, d_qd, u(d_nq), nq_max, d_tf, d_td, u(d_nt), nt_max, C.byref(c),
                                     d_out, u(d_out_count)))
+
+    def match_static(self, d_qf, d_qd, d_nq, nq_max, d_tf, d_td, d_nt, nt_max, opt: MatchOptions, d_out,
+                     d_out_count):
+        """ps_engine_match without device allocation (static scratch; nq_max, nt_max <= 4096)."""
+        c = opt._c()
+        u = lambda p: C.cast(p, C.POINTER(C.c_uint32))
+        check(lib().ps_engine_match_static(self.h, d_qf, d_qd, u(d_nq), nq_max, d_tf, d_td, u(d_nt), nt_max,
+                                           C.byref(c), d_out, u(d_out_count)))
 
     def sync(self) -> None:
         check(lib().ps_engine_sync(self.h))
