@@ -701,6 +701,32 @@ def bench_panorama(n_tiles: int = 16, steps: int = 10, cpu_sample: bool = True) 
     return out
 
 
+def fp64_statement(a, b) -> dict:
+    """Errors of the GPU results on the frame pair against the float64 evaluation of the same
+    algorithm (fp64_errors.py): keypoints (all, uncapped), a sample of 256 descriptors, the
+    ratio matches and the homography of the matches."""
+    import numpy as np
+    import fp64_errors as fe
+    import pure_sift as ps
+    t0 = time.perf_counter()
+    img = a.astype(np.float32) * np.float32(1.0 / 255.0)      # luma_to_array2, pipeline.rs:957-966
+    sift = ps.Sift(ps.SiftConfig().octaves(4).scales(3).contrast_threshold(0.015))
+    pyr32 = sift.build_pyramid(img)
+    p64 = fe.pyramid64(img, 4, 3, True)
+    kps = sift.detect_keypoints(img)
+    fq, ft = sift.detect_and_compute(a), sift.detect_and_compute(b)
+    m = ps.Matcher(ps.MatchOptions(0.8)).match_(fq, ft)
+    inl, H = ps.homography_estimate(m, ps.RansacOptions(3.0))
+    out = {"gaussian_planes_abs_max": max(float(np.abs(g - np.asarray(g32, np.float64)).max())
+                                          for (g, _), (g32, _) in zip(p64, pyr32)),
+           "keypoints": fe.keypoint_errors(kps, pyr32, p64, 3),
+           "descriptors": fe.descriptor_errors(fq.header, fq.descriptors, kps, p64, 4, sample=256),
+           "matches": fe.match_errors(fq.descriptors, ft.descriptors, m),
+           "homography": fe.homography_errors(m, inl, H) if H is not None else None}
+    out["seconds"] = time.perf_counter() - t0
+    return out
+
+
 def cpu_baseline(a, b, runs: int = 10, warmup: int = 2, runs_1: int = 10):
     """The oracle (C restatement of the reference's CPU path, OpenMP where the reference uses
     Rayon) on the same 1080p frame: median of `runs` frames after `warmup` warm-ups with all
@@ -752,6 +778,8 @@ def main():
                     help="frame: the 1080p extract+match headline (default); hamming: the catalog shortlist")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device-to-device copy first (a known-size reference for HBM counters)")
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="skip the errors of the frame's results against the float64 evaluation")
     ap.add_argument("--no-secondary", action="store_true",
                     help="frame workload only, without the secondary workloads in the same JSON line")
     args = ap.parse_args()
@@ -880,6 +908,8 @@ def main():
                      "alg_bytes_per_image": ALG_BYTES_1080P},
     }
     eng.close()
+    if not args.no_fp64:
+        out["fp64_errors"] = fp64_statement(a, b)
     if not args.no_cpu_baseline and args.cpu_runs > 0:
         out["cpu_baseline"] = cpu_baseline(a, b, runs=args.cpu_runs, runs_1=args.cpu_runs)
     if not args.no_secondary:

@@ -1,0 +1,45 @@
+"""Errors of the GPU results against the float64 evaluation of the same algorithm
+(fp64_errors.py), on the 640x480 pair and the 1080p frame pair: keypoint position and scale
+(the f32 Taylor solve alone, and the whole f32 chain), descriptors (at the feature's own
+orientation; orientation error to the float64 peak), match distances and ratios, and the
+homography's reprojection against a float64 DLT of its inliers. The bounds are the north-star
+tolerances (1e-3 px, 1e-4 octave, 1e-3 descriptor L2) at the 99th percentile, where the rare
+branch flips of a float64 decision (the Taylor step's |offset| <= 0.5 test, an orientation
+bin) are counted and reported instead."""
+import numpy as np
+import pytest
+
+import fp64_errors as fe
+import psr
+import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("size", [(640, 480, 200, False, 0), (1920, 1080, 2000, True, 1)])
+def test_gpu_results_against_fp64(gpu, size):
+    w, h, n, band, seed = size
+    a, b, _ = synth.blob_pair(w, h, n, seed=seed, band_noise=band)
+    img = psr.luma_to_f32(a)
+    cfg = gpu.SiftConfig().octaves(4).scales(3)
+    sift = gpu.Sift(cfg)
+    pyr32 = sift.build_pyramid(img)
+    p64 = fe.pyramid64(img, 4, 3, True)
+    kps = sift.detect_keypoints(img)
+    ke = fe.keypoint_errors(kps, pyr32, p64, 3)
+    fq, ft = sift.detect_and_compute(a), sift.detect_and_compute(b)
+    de = fe.descriptor_errors(fq.header, fq.descriptors, kps, p64, 4, sample=128)
+    m = gpu.Matcher(gpu.MatchOptions(0.8)).match_(fq, ft)
+    me = fe.match_errors(fq.descriptors, ft.descriptors, m)
+    inl, H = gpu.homography_estimate(m, gpu.RansacOptions(3.0))
+    he = fe.homography_errors(m, inl, H)
+    print(f"{w}x{h} vs FP64: keypoints {ke}; descriptors {de}; matches {me}; homography {he}")
+    assert ke["unlocated"] == 0
+    assert ke["solve"]["pos_px_max"] < 1e-4 and ke["solve"]["scale_oct_max"] < 1e-6
+    assert ke["total"]["pos_px_p99"] < 1e-3 and ke["total"]["scale_oct_p99"] < 1e-4
+    assert ke["total"]["over_1e-3_px"] <= max(2, 0.002 * len(kps))
+    assert de["l2_max"] < 1e-3 and de["orientation_rad_median"] < 1e-5
+    assert me["distance_abs_max"] < 1e-6 and me["ratio_abs_max"] < 1e-6
+    # the model is the reference's last local-optimisation refit, whose inlier set can differ
+    # slightly from the final inliers it is compared on here: a fraction of a pixel
+    assert he["reproj_px_max"] < 2.0
