@@ -266,6 +266,11 @@ ps_status ps_engine_autotune_blur(ps_engine *e, const uint32_t *hw, uint32_t n_s
                                   uint32_t *n_tuned);
 void ps_autotune_reset(void);
 uint32_t ps_autotune_entries(void);
+/* Entry i of the tuned table: radius, source mode, height class, width class (floor(2 log2 of
+ * the level's height / width)) and the launch shape chosen (tile rows, or PS_BLUR_SLIDE | chunk
+ * rows for the sliding-window kernel), e.g. to persist a table or inspect it. */
+#define PS_BLUR_SLIDE 65536
+ps_status ps_autotune_entry(uint32_t i, int32_t out[5]);
 
 /* ---- homography RANSAC ---------------------------------------------------------------
  * pipeline.rs:179-213 RansacOptions: reproj_threshold in [0.5, 100]; seed used when

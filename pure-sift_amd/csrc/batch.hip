@@ -299,4 +299,18 @@ uint32_t ps_autotune_entries(void) {
     return (uint32_t)g_tiles.size();
 }
 
+ps_status ps_autotune_entry(uint32_t i, int32_t out[5]) {
+    if (!out) return set_err(PS_E_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    if (i >= g_tiles.size()) return set_err(PS_E_INVALID_ARG, "autotune entry index out of range");
+    auto it = g_tiles.begin();
+    std::advance(it, i);
+    out[0] = std::get<0>(it->first);
+    out[1] = std::get<1>(it->first);
+    out[2] = std::get<2>(it->first);
+    out[3] = std::get<3>(it->first);
+    out[4] = it->second;
+    return PS_OK;
+}
+
 }  // extern "C"

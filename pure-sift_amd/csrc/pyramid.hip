@@ -166,7 +166,10 @@ __global__ __launch_bounds__(kBlurThreads) void blur_kernel(BlurArgs a, Taps tap
 // kernels are bit-identical.
 constexpr int kSlideTH = kBlurSlideStep;
 
-template <int R, int SRC>
+// CENTER: the level also writes its DoG plane or its source values (the pyramid dump, and the
+// first level of octaves >= 1, which materialises G0): only then are the ring rows' centre
+// values kept, so the common case needs a third less LDS and fits more workgroups per CU.
+template <int R, int SRC, bool CENTER>
 __global__ __launch_bounds__(kBlurThreads) void blur_slide_kernel(BlurArgs a, Taps taps, int chunk) {
     constexpr int NT = 2 * R + 1;
     constexpr int IW = kTW + 2 * R;
@@ -180,7 +183,7 @@ __global__ __launch_bounds__(kBlurThreads) void blur_slide_kernel(BlurArgs a, Ta
     static_assert(kBlurThreads == 4 * 64 && TH == 4 * VR && TH * (kTW / 4) == kBlurThreads, "step shape");
     __shared__ __attribute__((aligned(16))) float s_in[TH * IWP];
     __shared__ __attribute__((aligned(16))) float s_h[2 * RING * kTW];
-    __shared__ __attribute__((aligned(16))) float s_c[RING * kTW];
+    __shared__ __attribute__((aligned(16))) float s_c[CENTER ? RING * kTW : 4];
     // the wave index is uniform: readfirstlane lets row arithmetic stay scalar
     const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int h = a.h, w = a.w;
@@ -230,7 +233,8 @@ __global__ __launch_bounds__(kBlurThreads) void blur_slide_kernel(BlurArgs a, Ta
             const float4 o = make_float4(a0, a1, a2, a3);
             *reinterpret_cast<float4 *>(s_h + slot * kTW + hc) = o;
             *reinterpret_cast<float4 *>(s_h + (slot + RING) * kTW + hc) = o;
-            *reinterpret_cast<float4 *>(s_c + slot * kTW + hc) = make_float4(v[R], v[R + 1], v[R + 2], v[R + 3]);
+            if constexpr (CENTER)
+                *reinterpret_cast<float4 *>(s_c + slot * kTW + hc) = make_float4(v[R], v[R + 1], v[R + 2], v[R + 3]);
         }
         if (k + 1 < n_steps) fetch(k + 1);
         __syncthreads();
@@ -254,13 +258,15 @@ __global__ __launch_bounds__(kBlurThreads) void blur_slide_kernel(BlurArgs a, Ta
             for (int j = 0; j < VR; j++) {
                 const int y = y_start + r0 + j;
                 if (y < y_end && x < w) {
-                    int cs = base + R + j;
-                    if (cs >= RING) cs -= RING;
-                    const float center = s_c[cs * kTW + lane];
                     const size_t o = (size_t)y * w + x;
                     if (a.dst_g) a.dst_g[o] = acc[j];
-                    if (a.dst_d) a.dst_d[o] = acc[j] - center;   // pipeline.rs:611
-                    if (a.dst_src) a.dst_src[o] = center;
+                    if constexpr (CENTER) {
+                        int cs = base + R + j;
+                        if (cs >= RING) cs -= RING;
+                        const float center = s_c[cs * kTW + lane];
+                        if (a.dst_d) a.dst_d[o] = acc[j] - center;   // pipeline.rs:611
+                        if (a.dst_src) a.dst_src[o] = center;
+                    }
                 }
             }
         }
@@ -271,22 +277,27 @@ template <int R>
 static hipError_t blur_slide_dispatch(int src_mode, const BlurArgs &a, const Taps &t, int chunk, hipStream_t st) {
     if (chunk <= 0 || chunk % kSlideTH) return hipErrorInvalidValue;
     dim3 grid((a.w + kTW - 1) / kTW, (a.h + chunk - 1) / chunk);
+    const bool center = a.dst_d || a.dst_src;
+#define PS_SLIDE(M)                                                                                  \
+    (center ? (blur_slide_kernel<R, M, true><<<grid, kBlurThreads, 0, st>>>(a, t, chunk), 0)         \
+            : (blur_slide_kernel<R, M, false><<<grid, kBlurThreads, 0, st>>>(a, t, chunk), 0))
     switch (src_mode) {
-    case SRC_F32: blur_slide_kernel<R, SRC_F32><<<grid, kBlurThreads, 0, st>>>(a, t, chunk); break;
-    case SRC_DOWN2: blur_slide_kernel<R, SRC_DOWN2><<<grid, kBlurThreads, 0, st>>>(a, t, chunk); break;
+    case SRC_F32: PS_SLIDE(SRC_F32); break;
+    case SRC_DOWN2: PS_SLIDE(SRC_DOWN2); break;
     case SRC_U8:
     case SRC_F32_UP:
     case SRC_U8_UP:
         if constexpr (R == 4 || R == 5) {
-            if (src_mode == SRC_U8) blur_slide_kernel<R, SRC_U8><<<grid, kBlurThreads, 0, st>>>(a, t, chunk);
-            else if (src_mode == SRC_F32_UP) blur_slide_kernel<R, SRC_F32_UP><<<grid, kBlurThreads, 0, st>>>(a, t, chunk);
-            else blur_slide_kernel<R, SRC_U8_UP><<<grid, kBlurThreads, 0, st>>>(a, t, chunk);
+            if (src_mode == SRC_U8) PS_SLIDE(SRC_U8);
+            else if (src_mode == SRC_F32_UP) PS_SLIDE(SRC_F32_UP);
+            else PS_SLIDE(SRC_U8_UP);
             break;
         } else {
             return hipErrorInvalidValue;
         }
     default: return hipErrorInvalidValue;
     }
+#undef PS_SLIDE
     return hipGetLastError();
 }
 
@@ -453,7 +464,10 @@ __global__ __launch_bounds__(256) void extrema_kernel(const float *gauss, int h,
     static_assert(S <= 16 && kExRows <= 32 && kExW - 2 <= 64, "queue entry fields");
     __shared__ uint16_t s_q[64 * kExRows * S / 2];
     __shared__ uint32_t s_qn;
-    const int x0 = 1 + blockIdx.x * 64, y0 = 1 + blockIdx.y * kExRows;
+    // XCD-contiguous tile order (as the blur kernels): horizontally neighbouring tiles share
+    // the cache lines of their halo columns, and land on the same XCD's L2
+    const uint32_t tile = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int x0 = 1 + (int)(tile % gridDim.x) * 64, y0 = 1 + (int)(tile / gridDim.x) * kExRows;
     const size_t P = (size_t)h * w;
     if (threadIdx.x == 0) s_qn = 0;
     {

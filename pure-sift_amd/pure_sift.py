@@ -39,6 +39,7 @@ SIFT_DESCRIPTOR_LEN = 128
 DEFAULT_MATCH_RATIO = 0.75
 DEFAULT_RANSAC_REPROJ_THRESHOLD = 3.0
 MAX_FEATURES_PER_IMAGE = 4096
+BLUR_SLIDE = 65536  # PS_BLUR_SLIDE: flag bit of a sliding-window launch shape in the tuned table
 
 PS_OK = 0
 PS_E_INVALID_ARG = -1
@@ -202,6 +203,8 @@ def lib() -> C.CDLL:
         L.ps_autotune_reset.restype = None
         L.ps_autotune_entries.argtypes = []
         L.ps_autotune_entries.restype = C.c_uint32
+        L.ps_autotune_entry.argtypes = [C.c_uint32, C.POINTER(C.c_int32)]
+        L.ps_autotune_entry.restype = i32
         L.ps_nn2_precision.argtypes = [vp, sz, vp, sz, C.c_int, vp, vp, vp, vp]
         L.ps_engine_nn2.argtypes = [vp, vp, u32p, sz, vp, u32p, sz, C.c_int, vp, vp, vp, vp]
         L.ps_quantize_u128.argtypes = [vp, sz, vp]

@@ -90,6 +90,17 @@ def test_empty_match_needs_no_device():
     assert len(m.match_(g, g)) == 0
 
 
+def test_autotune_table_inspection_needs_no_device():
+    txt = open(ps.HEADER_PATH).read()
+    assert int(re.search(r"#define PS_BLUR_SLIDE (\d+)", txt).group(1)) == ps.BLUR_SLIDE
+    L = ps.lib()
+    L.ps_autotune_reset()
+    assert L.ps_autotune_entries() == 0
+    out = (C.c_int32 * 5)()
+    assert L.ps_autotune_entry(0, out) == ps.PS_E_INVALID_ARG
+    assert L.ps_autotune_entry(0, None) == ps.PS_E_INVALID_ARG
+
+
 def test_status_strings():
     L = ps.lib()
     for s in (ps.PS_OK, ps.PS_E_INVALID_ARG, ps.PS_E_CONFIG, ps.PS_E_DEVICE, ps.PS_E_CAPACITY, ps.PS_E_NO_DEVICE,

@@ -4,6 +4,8 @@ autotuner, through the C ABI.
 Bars: every image of a batch gets exactly the features, descriptors and count of a
 single-image extraction of the same pixels (graph capture, graph replay and eager launch
 alike); after autotuning, the pyramid is still bit-exact with the oracle."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -77,7 +79,18 @@ def test_autotuned_pyramid_still_bit_exact(gpu):
     eng = gpu.Engine(0)
     try:
         n = eng.autotune_blur([(120, 160), (97, 131), (480, 640)], cfg)
-        assert n > 0 and gpu.lib().ps_autotune_entries() > 0
+        L = gpu.lib()
+        assert n > 0 and L.ps_autotune_entries() > 0
+        out = (ctypes.c_int32 * 5)()
+        for i in range(L.ps_autotune_entries()):
+            assert L.ps_autotune_entry(i, out) == 0
+            r, mode, hc, wc, shape = list(out)
+            assert r >= 1 and 0 <= mode <= 4 and hc > 0 and wc > 0
+            if shape & gpu.BLUR_SLIDE:
+                assert (shape & 0xffff) in (64, 128, 192)
+            else:
+                assert shape in (16, 32, 64)
+        assert L.ps_autotune_entry(L.ps_autotune_entries(), out) != 0
         for (h, w) in [(120, 160), (97, 131)]:
             img = psr.luma_to_f32(synth.blob_image(w, h, 20, seed=h))
             dev = gpu.Sift(cfg).build_pyramid(img)
