@@ -26,13 +26,14 @@ sys.path.insert(0, os.path.join(REPO, "pure-sift_amd"))
 HBM_PEAK_GBS = 8000.0
 # 32-bit integer VALU issue rate: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 VALU_INT_PEAK_TOPS = 39.32
-ALG_BYTES_1080P = 663_033_600   # W*H + 4*P*(6 + 5 + 4), P = 11,016,000 (4 octaves, upsampled)
-# HBM bytes per 1080p frame moved by the scale-space kernels (blur + extrema), from rocprofv3
-# FETCH_SIZE and WRITE_SIZE passes scaled by a 256 MiB copy (profiles/pyramid_extrema_1080p_hbm_traffic.md)
-TRAFFIC_BYTES_1080P = 819_377_923
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "pyramid_extrema_1080p_hbm_traffic.md")
 
 
 def alg_bytes(h, w, octaves=4, upsample=True):
+    """Algorithmic HBM bytes of the scale-space stage (S = 3) per image: the u8 input, then per
+    octave of P pixels 5 Gaussian planes written (levels 0..S+1; level S+2 is never stored on
+    the extraction path), the 4 source planes the blur levels 1..S+1 read, and the 4 planes
+    (levels 1..S+1) the extrema pass reads: W*H + 4*P*(5 + 4 + 4)."""
     hh, ww = (2 * h, 2 * w) if upsample else (h, w)
     P = 0
     for _ in range(octaves):
@@ -41,7 +42,24 @@ def alg_bytes(h, w, octaves=4, upsample=True):
         P += hh * ww
         hh //= 2
         ww //= 2
-    return h * w + 4 * P * (6 + 5 + 4)
+    return h * w + 4 * P * (5 + 4 + 4)
+
+
+ALG_BYTES_1080P = alg_bytes(1080, 1920)
+
+
+def traffic_bytes_1080p():
+    """HBM bytes per 1080p frame moved by the scale-space kernels (blur + extrema), as measured
+    by rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by a 256 MiB copy; read from the
+    committed profile summary (its `traffic_bytes_per_frame:` line), None when absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_frame:"):
+                    return int(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return None
 
 
 def _host_cores() -> int:
@@ -901,10 +919,11 @@ def main():
         # one frame = t_extract + t_match (a pair-based definition is 2 * t_extract + t_match)
         "t_extract_ms": round(sum(v for k, v in stage_ms.items() if k != "match"), 4),
         "t_match_ms": round(stage_ms["match"], 4),
-        "roofline": {"bound": "hbm", "kernel": "scale-space stage: 21 blur launches (tile / sliding-window kernels) + 4 extrema launches, "
-                               "octaves 1-3 overlapped on side streams",
+        "roofline": {"bound": "hbm", "kernel": "scale-space stage: 17 blur launches (tile / sliding-window kernels) + 4 extrema launches "
+                               "(which form level S+2 only around slice-S candidates), octaves 1-3 overlapped on "
+                               "side streams",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES_1080P,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_bytes_1080p(),
                      "alg_bytes_per_image": ALG_BYTES_1080P},
     }
     eng.close()

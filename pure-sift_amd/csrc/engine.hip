@@ -382,7 +382,7 @@ ps_status enqueue_scale_space(ps_engine *e, const void *src, bool src_u8, size_t
     auto octave_extrema = [&](int o, hipStream_t st) -> hipError_t {
         return ps::launch_extrema(P + p.gauss(o, 0), p.h[o], p.w[o], p.S, cfg.contrast_threshold, p.scale_factor[o],
                                   p.eff_scale[o], p.log2k, o, as<ps::Cand>(e->cand), cnt + 0, ps::kCandCap, cnt + kErrWord,
-                                  st);
+                                  p.lvl_taps[p.S + 2], p.lvl_r[p.S + 2], st);
     };
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     PS_HIP(hipStreamIsCapturing(e->stream, &cap));
@@ -407,7 +407,9 @@ ps_status enqueue_scale_space(ps_engine *e, const void *src, bool src_u8, size_t
             PS_HIP(hipEventRecord(e->fork_ev[o], st));
             PS_HIP(hipStreamWaitEvent(lane(o + 1), e->fork_ev[o], 0));
         }
-        for (int s = p.S + 1; s <= p.S + 2; s++) PS_HIP(level(o, s, st));
+        // level S+2 feeds only the upper DoG plane of slice-S candidates: the extraction path
+        // leaves it to the extrema kernel, which forms its few needed values from level S+1
+        for (int s = p.S + 1; s <= (extrema ? p.S + 1 : p.S + 2); s++) PS_HIP(level(o, s, st));
         if (o == 0) prof_mark(e, 1);
         if (extrema) PS_HIP(octave_extrema(o, st));
     }

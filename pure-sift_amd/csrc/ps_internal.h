@@ -80,9 +80,11 @@ constexpr int kBlurSlide = 1 << 16;
 constexpr int kBlurSlideStep = 16;
 hipError_t launch_blur(int radius, int src_mode, const BlurArgs &a, const Taps &t, int shape, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st);
+// Extrema of one octave from Gaussian levels 0..S+1; level S+2 is not read: where a slice-S
+// candidate needs it, it is formed from level S+1 with top_taps (radius top_r).
 hipError_t launch_extrema(const float *gauss_octave, int h, int w, int S, float thr, float mult,
                           const float *eff_scale, double log2k, int octave, Cand *out, uint32_t *count,
-                          uint32_t cap, uint32_t *err, hipStream_t st);
+                          uint32_t cap, uint32_t *err, const Taps &top_taps, int top_r, hipStream_t st);
 // Orders candidates: scan order if n <= limit, else (|response| desc, scan order); writes the
 // first min(n, limit) into out and that count into out_count.
 size_t rank_scratch_bytes(uint32_t cap_in);

@@ -7,6 +7,8 @@
 // (no FMA: -ffp-contract=off plus the pragma below), Gaussian weights come from
 // the host (glibc expf, exactly pipeline.rs:275-296), and the extremum tests /
 // refinement repeat the reference's f32 expressions operation by operation.
+#include <type_traits>
+
 #include "ps_internal.h"
 
 #pragma clang fp contract(off)
@@ -433,10 +435,10 @@ __device__ __forceinline__ void refine3d(const float *ps, const float *cs, const
     odx = 0.0f; ody = 0.0f; ods = 0.0f; oi = center;
 }
 
-// 64 x ROWS interior pixels per 256-thread workgroup (ROWS = 32 for large octaves, 16 for
-// small ones, whose launches are latency bound): the S searchable DoG planes of the tile
-// plus a one-pixel halo are staged in LDS (every load of a thread in flight together). The
-// DoG planes are not stored in HBM on the extraction path: DoG s = G(s+1) - G(s) is formed
+// 64 x ROWS interior pixels per tile (ROWS = 32 for large octaves, 16 for small ones, whose
+// launches are latency bound), 256 threads per workgroup: the S searchable DoG planes of the
+// tile plus a one-pixel halo are staged in LDS (every load of a thread in flight together).
+// The DoG planes are not stored in HBM on the extraction path: DoG s = G(s+1) - G(s) is formed
 // here from the Gaussian planes (the same single f32 subtraction the blur kernel performs,
 // pipeline.rs:611), so the blur writes one plane per level instead of two.
 // Phase 1: each thread tests its ROWS/4 pixels x S slices against the contrast threshold and the 8
@@ -447,10 +449,16 @@ __device__ __forceinline__ void refine3d(const float *ps, const float *cs, const
 // one atomic per workgroup.
 constexpr int kExW = 66;
 constexpr long kExBigLevel = 1L << 20;   // levels of at least this many pixels use 32-row tiles
+// Radius of the blur producing G(S+2) from G(S+1) (the level's sigma is fixed by S: base sigma
+// 1.6, k = 2^(1/S)); launch_extrema checks it against the plan's radius.
+constexpr int top_radius(int S) {
+    return S == 3 ? 10 : S == 4 ? 8 : S == 5 ? 7 : S == 6 ? 6 : S <= 9 ? 5 : 4;
+}
 template <int PL, int kExRows>
 __global__ __launch_bounds__(256) void extrema_kernel(const float *gauss, int h, int w, float thr, float mult,
                                                       EffScale eff, double log2k, int octave, Cand *out,
-                                                      uint32_t *count, uint32_t cap, uint32_t *err) {
+                                                      uint32_t *count, uint32_t cap, uint32_t *err, Taps top,
+                                                      int top_r) {
     constexpr int S = PL - 2;
     constexpr int kExH = kExRows + 2;
     constexpr int per_plane = kExH * kExW;
@@ -464,12 +472,20 @@ __global__ __launch_bounds__(256) void extrema_kernel(const float *gauss, int h,
     static_assert(S <= 16 && kExRows <= 32 && kExW - 2 <= 64, "queue entry fields");
     __shared__ uint16_t s_q[64 * kExRows * S / 2];
     __shared__ uint32_t s_qn;
+    __shared__ uint32_t s_wcnt[4], s_base, s_tcnt[4];
+    // G(S+2) (see phase 2): the round's slice-S entries, then per batch of TB of them the
+    // horizontal sums (TB x HR rows x 3 columns) and the 3 x 3 vertical sums
+    constexpr int TR = top_radius(S), TNT = 2 * TR + 1, HR = TNT + 2, TB = 256 / HR;
+    __shared__ uint16_t s_top[256];
+    __shared__ float s_hrow[TB * HR * 3];
+    __shared__ float s_up[TB * 9];
     // XCD-contiguous tile order (as the blur kernels): horizontally neighbouring tiles share
     // the cache lines of their halo columns, and land on the same XCD's L2
     const uint32_t tile = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
     const int x0 = 1 + (int)(tile % gridDim.x) * 64, y0 = 1 + (int)(tile / gridDim.x) * kExRows;
     const size_t P = (size_t)h * w;
     if (threadIdx.x == 0) s_qn = 0;
+    int nan_seen = 0;
     {
         // Gaussian planes 1..S+1 give DoG planes 1..S
         float v[S + 1][NL];
@@ -486,21 +502,40 @@ __global__ __launch_bounds__(256) void extrema_kernel(const float *gauss, int h,
             int i = threadIdx.x + j * 256;
 #pragma unroll
             for (int pl = 0; pl < S; pl++)
-                if (i < per_plane) s_dog[pl * per_plane + i] = v[pl + 1][j] - v[pl][j];
+                if (i < per_plane) {
+                    const float d = v[pl + 1][j] - v[pl][j];
+                    s_dog[pl * per_plane + i] = d;
+                    nan_seen |= d != d;
+                }
         }
     }
-    __syncthreads();
+    // a NaN in the staged planes (only from NaN or infinite pixels of an f32 input) switches
+    // the tile to the reference's literal comparisons, where a NaN neighbour fails them all
+    const bool tile_nan = __syncthreads_or(nan_seen);
     // phase 1 (pipeline.rs:800-850, the threshold and in-plane part): each thread owns RT
     // consecutive rows of one column; per slice it reads the (RT+2) x 3 window around them
-    // once and evaluates all 8 comparisons of each pixel without branches, collecting
-    // survivors in a bit mask (bit (s - s0) * RT + j for up to 64 / RT slices from s0); the
-    // wave then queues them with one LDS atomic.
+    // once and decides each pixel without branches, collecting survivors in a bit mask (bit
+    // (s - s0) * RT + j for up to 64 / RT slices from s0); the wave then queues them with one
+    // LDS atomic. "val above all 8 neighbours" is evaluated as val > (maximum of the 8): the
+    // maximum of finite floats is one of its operands, so the decision is the same as the
+    // reference's 8 strict comparisons, for ties as well. The row maxima (and minima) of the
+    // 3-wide window are shared by the three output rows that use them.
     {
         constexpr int RT = kExRows / 4;
         constexpr int SC = 64 / RT;   // slices per mask
         const int tx = threadIdx.x & 63, lx = tx + 1;
         const int ly0 = 1 + (threadIdx.x >> 6) * RT;
-        const bool col_ok = x0 + tx < w - 1;
+        // rows j of this thread that lie in the level's interior (all of them off the last tile
+        // row), repeated for each slice position of a mask; empty outside the interior columns
+        uint64_t rows_ok = 0;
+        if (x0 + tx < w - 1) {
+            uint64_t rm = 0;
+#pragma unroll
+            for (int j = 0; j < RT; j++)
+                if (y0 + ly0 + j - 1 < h - 1) rm |= 1ull << j;
+#pragma unroll
+            for (int k = 0; k < SC; k++) rows_ok |= rm << (k * RT);
+        }
         auto flush = [&](uint64_t found, int s0) {
             const uint32_t nf = (uint32_t)__popcll(found);
             uint32_t incl = nf;
@@ -520,49 +555,128 @@ __global__ __launch_bounds__(256) void extrema_kernel(const float *gauss, int h,
                 s_q[pos++] = (uint16_t)(((sl - 1) << 11) | ((ly0 + j - 1) << 6) | (lx - 1));
             }
         };
-        uint64_t found = 0;
+        auto scan = [&](auto literal) {
+            uint64_t found = 0;
 #pragma unroll
-        for (int s = 1; s <= S; s++) {
-            const float *cs = s_dog + (s - 1) * per_plane + (ly0 - 1) * kExW + lx - 1;
-            float r[RT + 2][3];
+            for (int s = 1; s <= S; s++) {
+                const float *cs = s_dog + (s - 1) * per_plane + (ly0 - 1) * kExW + lx - 1;
+                float r[RT + 2][3];
 #pragma unroll
-            for (int i = 0; i < RT + 2; i++)
+                for (int i = 0; i < RT + 2; i++)
 #pragma unroll
-                for (int c = 0; c < 3; c++) r[i][c] = cs[i * kExW + c];
+                    for (int c = 0; c < 3; c++) r[i][c] = cs[i * kExW + c];
+                if constexpr (decltype(literal)::value) {
 #pragma unroll
-            for (int j = 0; j < RT; j++) {
-                const float val = r[j + 1][1];
-                bool gt = true, lt = true;
+                    for (int j = 0; j < RT; j++) {
+                        const float val = r[j + 1][1];
+                        bool gt = true, lt = true;
 #pragma unroll
-                for (int dy = 0; dy < 3; dy++)
+                        for (int dy = 0; dy < 3; dy++)
 #pragma unroll
-                    for (int dx = 0; dx < 3; dx++)
-                        if (dy != 1 || dx != 1) {
-                            gt &= val > r[j + dy][dx];
-                            lt &= val < r[j + dy][dx];
-                        }
-                const bool ok = col_ok && (y0 + ly0 + j - 1 < h - 1) && fabsf(val) > thr && (val > 0.0f ? gt : lt);
-                found |= (uint64_t)ok << (((s - 1) % SC) * RT + j);
+                            for (int dx = 0; dx < 3; dx++)
+                                if (dy != 1 || dx != 1) {
+                                    gt &= val > r[j + dy][dx];
+                                    lt &= val < r[j + dy][dx];
+                                }
+                        const bool ok = fabsf(val) > thr && (val > 0.0f ? gt : lt);
+                        found |= (uint64_t)ok << (((s - 1) % SC) * RT + j);
+                    }
+                } else {
+                    float hx[RT + 2], hn[RT + 2];
+#pragma unroll
+                    for (int i = 0; i < RT + 2; i++) {
+                        hx[i] = fmaxf(fmaxf(r[i][0], r[i][1]), r[i][2]);
+                        hn[i] = fminf(fminf(r[i][0], r[i][1]), r[i][2]);
+                    }
+#pragma unroll
+                    for (int j = 0; j < RT; j++) {
+                        const float val = r[j + 1][1];
+                        const float m8 = fmaxf(fmaxf(hx[j], hx[j + 2]), fmaxf(r[j + 1][0], r[j + 1][2]));
+                        const float n8 = fminf(fminf(hn[j], hn[j + 2]), fminf(r[j + 1][0], r[j + 1][2]));
+                        // |val| > thr and val > m8 (val > 0), or |val| > thr and -val > -n8
+                        const float u = val > 0.0f ? m8 : -n8;
+                        const bool ok = fabsf(val) > fmaxf(thr, u);
+                        found |= (uint64_t)ok << (((s - 1) % SC) * RT + j);
+                    }
+                }
+                if (s % SC == 0 || s == S) {
+                    flush(found & rows_ok, s - (s - 1) % SC);
+                    found = 0;
+                }
             }
-            if (s % SC == 0 || s == S) {
-                flush(found, s - (s - 1) % SC);
-                found = 0;
-            }
-        }
+        };
+        if (tile_nan) scan(std::true_type{});
+        else scan(std::false_type{});
     }
     __syncthreads();
     // phase 2 (pipeline.rs:852-893); accepted candidates of a round reserve their output
     // slots with one global atomic per workgroup (ballot prefix inside the waves)
-    __shared__ uint32_t s_wcnt[4], s_base;
     const uint32_t nq = s_qn;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (uint32_t q0 = 0; q0 < nq; q0 += 256) {
         // queue entry q0 + 4*lane + wave: a short queue is spread over all four waves
         const uint32_t qi = q0 + 4 * (uint32_t)lane + (uint32_t)wv;
+        const uint32_t e = qi < nq ? s_q[qi] : 0u;
+        const bool is_top = qi < nq && (int)(e >> 11) + 1 == S;
+        // The upper neighbour plane of slice-S entries is DoG S+1 = G(S+2) - G(S+1). The
+        // extraction path does not store G(S+2) (no other stage reads it): its 3 x 3 values
+        // around each such entry are computed here from G(S+1) with that level's taps, in the
+        // blur kernels' order (horizontal sums over clamped columns, then the vertical sum over
+        // clamped rows, taps in index order from 0.0f), so they are the values the level's
+        // blur would have written. The round's slice-S entries are compacted; per batch of TB
+        // entries one thread forms the three horizontal sums of one window row and one thread
+        // one vertical sum.
+        float up[9];
+        {
+            const uint64_t tm = __ballot(is_top);
+            if (lane == 0) s_tcnt[wv] = (uint32_t)__popcll(tm);
+            __syncthreads();
+            uint32_t my_top = (uint32_t)__popcll(tm & ((1ull << lane) - 1ull)), n_top = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (k < wv) my_top += s_tcnt[k];
+                n_top += s_tcnt[k];
+            }
+            if (is_top) s_top[my_top] = (uint16_t)e;
+            __syncthreads();
+            for (uint32_t t0 = 0; t0 < n_top; t0 += TB) {
+                const int nb_t = (int)min((uint32_t)TB, n_top - t0);
+                if ((int)threadIdx.x < nb_t * HR) {
+                    const int k = (int)threadIdx.x / HR, rr = (int)threadIdx.x - k * HR;
+                    const uint32_t te = s_top[t0 + k];
+                    const int ty = y0 + (int)((te >> 6) & 31u), tx = x0 + (int)(te & 63u);
+                    const float *src = gauss + (size_t)(S + 1) * P + (size_t)min(max(ty - 1 - TR + rr, 0), h - 1) * w;
+                    float v[TNT + 2];
+#pragma unroll
+                    for (int q = 0; q < TNT + 2; q++) v[q] = src[min(max(tx - 1 - TR + q, 0), w - 1)];
+#pragma unroll
+                    for (int cc = 0; cc < 3; cc++) {
+                        float acc = 0.0f;
+#pragma unroll
+                        for (int q = 0; q < TNT; q++) acc += v[cc + q] * top.w[q];
+                        s_hrow[(k * HR + rr) * 3 + cc] = acc;
+                    }
+                }
+                __syncthreads();
+                if ((int)threadIdx.x < nb_t * 9) {
+                    const int k = (int)threadIdx.x / 9, o = (int)threadIdx.x - k * 9;
+                    const float *hr = s_hrow + (k * HR + o / 3) * 3 + o % 3;
+                    float acc = 0.0f;
+#pragma unroll
+                    for (int q = 0; q < TNT; q++) acc += hr[q * 3] * top.w[q];
+                    s_up[threadIdx.x] = acc;
+                }
+                __syncthreads();
+                if (is_top && my_top - t0 < (uint32_t)nb_t) {
+#pragma unroll
+                    for (int o = 0; o < 9; o++) up[o] = s_up[(my_top - t0) * 9 + o];
+                }
+                __syncthreads();
+            }
+        }
         bool accept = false;
         Cand c;
         if (qi < nq) {
-            const uint32_t e = s_q[qi];
             const int s = (int)(e >> 11) + 1, ly = (int)((e >> 6) & 31u) + 1, lx = (int)(e & 63u) + 1;
             const int x = x0 + lx - 1, y = y0 + ly - 1;
             // 3 x 3 x 3 neighbourhood (slices s-1, s, s+1) from LDS, or for the outer planes
@@ -585,7 +699,7 @@ __global__ __launch_bounds__(256) void extrema_kernel(const float *gauss, int h,
                     for (int dy = 0; dy < 3; dy++)
 #pragma unroll
                         for (int dx = 0; dx < 3; dx++) {
-                            a[dy * 3 + dx] = g1[(size_t)dy * w + dx];
+                            a[dy * 3 + dx] = pl == S + 1 ? up[dy * 3 + dx] : g1[(size_t)dy * w + dx];
                             b[dy * 3 + dx] = g0[(size_t)dy * w + dx];
                         }
 #pragma unroll
@@ -643,8 +757,9 @@ __global__ __launch_bounds__(256) void extrema_kernel(const float *gauss, int h,
 
 hipError_t launch_extrema(const float *gauss_octave, int h, int w, int S, float thr, float mult, const float *eff_scale,
                           double log2k, int octave, Cand *out, uint32_t *count, uint32_t cap, uint32_t *err,
-                          hipStream_t st) {
+                          const Taps &top_taps, int top_r, hipStream_t st) {
     if (h < 3 || w < 3 || S < 1) return hipSuccess;
+    if (S < 3 || S > 10 || top_r != top_radius(S)) return hipErrorInvalidValue;
     EffScale e;
     for (int i = 0; i < kMaxLevels; i++) e.v[i] = eff_scale[i];
     const bool big = (long)h * w >= kExBigLevel;
@@ -654,10 +769,10 @@ hipError_t launch_extrema(const float *gauss_octave, int h, int w, int S, float 
     case SS:                                                                                                  \
         if (big)                                                                                              \
             extrema_kernel<SS + 2, 32><<<grid, 256, 0, st>>>(gauss_octave, h, w, thr, mult, e, log2k, octave,  \
-                                                             out, count, cap, err);                           \
+                                                             out, count, cap, err, top_taps, top_r);          \
         else                                                                                                  \
             extrema_kernel<SS + 2, 16><<<grid, 256, 0, st>>>(gauss_octave, h, w, thr, mult, e, log2k, octave,  \
-                                                             out, count, cap, err);                           \
+                                                             out, count, cap, err, top_taps, top_r);          \
         break;
     switch (S) {
         PS_EXTREMA_CASE(3) PS_EXTREMA_CASE(4) PS_EXTREMA_CASE(5) PS_EXTREMA_CASE(6)

@@ -100,3 +100,78 @@ def test_autotuned_pyramid_still_bit_exact(gpu):
     finally:
         eng.close()
         gpu.lib().ps_autotune_reset()
+
+
+def test_full_256_image_batch(gpu):
+    """The 256-image batching workload at full size: seeded blob/noise u8 images with widths
+    U[320, 2048] and heights U[240, 1536] (1000 + i seeds, about 200 blobs per 640x480 of
+    area), blur shapes autotuned over all 256 sizes, one captured graph then its replay.
+    Every image must equal a single-image extraction of the same device pixels bit for bit
+    (count, feature records, descriptors); a sample of four images (the two smallest, a
+    mid-size and the largest under 1 Mpx) is also checked against the oracle."""
+    import torch
+    import learned_ransac as lr
+    from test_gpu_parity import _check_features
+    dev = torch.device("cuda:0")
+    sizes = synth.batch_sizes(256, seed=3)          # (w, h)
+    descs, off = [], 0
+    for (w, h) in sizes:
+        descs.append((off, h, w, w, 0))
+        off += w * h
+    pix = torch.empty(off, dtype=torch.uint8, device=dev)
+    for i, (w, h) in enumerate(sizes):
+        nb = max(20, int(200 * w * h / (640 * 480)))
+        pix[descs[i][0]:descs[i][0] + w * h] = lr.render_blobs(1, 1000 + i, dev, width=w, height=h,
+                                                                 n_blobs=nb)[0].reshape(-1)
+    torch.cuda.synchronize()
+    n, cap, fsz = len(sizes), gpu.MAX_FEATURES_PER_IMAGE, gpu.FEAT_DTYPE.itemsize
+    im = np.array(descs, gpu.IMAGE_DESC_DTYPE)
+    cfg = gpu.SiftConfig().octaves(4).scales(3).contrast_threshold(0.015)
+    eng = gpu.Engine(0)
+    try:
+        assert eng.autotune_blur([(h, w) for (w, h) in sizes], cfg) > 0
+        ref_f = torch.zeros((n, cap * fsz), dtype=torch.uint8, device=dev)
+        ref_d = torch.zeros((n, cap, 128), dtype=torch.float32, device=dev)
+        ref_c = torch.zeros(n, dtype=torch.int32, device=dev)
+        for i, (o, h, w, s, _) in enumerate(descs):
+            eng.extract_u8(pix.data_ptr() + o, h, w, s, cfg, ref_f[i].data_ptr(), ref_d[i].data_ptr(), cap,
+                           ref_c.data_ptr() + 4 * i)
+        eng.sync()
+        eng.check()
+        rc = ref_c.cpu().numpy()
+        RF, RD = ref_f.cpu().numpy(), ref_d.cpu().numpy()
+        del ref_f, ref_d
+        assert rc.min() > 50 and rc.max() > 1000
+        bf = torch.zeros((n, cap * fsz), dtype=torch.uint8, device=dev)
+        bd = torch.zeros((n, cap, 128), dtype=torch.float32, device=dev)
+        bc = torch.zeros(n, dtype=torch.int32, device=dev)
+        for _ in range(2):                         # capture, then replay
+            bc.fill_(-1)
+            bd.zero_()
+            torch.cuda.synchronize()
+            eng.extract_batch_u8(pix.data_ptr(), im, cfg, bf.data_ptr(), bd.data_ptr(), cap, bc.data_ptr(), 0)
+            eng.sync()
+            eng.check()
+            assert np.array_equal(bc.cpu().numpy(), rc)
+            F, D = bf.cpu().numpy(), bd.cpu().numpy()
+            for i in range(n):
+                k = rc[i]
+                assert np.array_equal(F[i, :k * fsz], RF[i, :k * fsz]), i
+                assert np.array_equal(D[i, :k], RD[i, :k]), i
+    finally:
+        eng.close()
+        gpu.lib().ps_autotune_reset()
+    area = np.array([w * h for (w, h) in sizes])
+    order = np.argsort(area, kind="stable")
+    small = [int(order[0]), int(order[1])]
+    mid = int(order[len(order) // 4])
+    big = int(order[np.searchsorted(area[order], 1 << 20) - 1])
+    host = pix.cpu().numpy()
+    for i in small + [mid, big]:
+        o, h, w, s, _ = descs[i]
+        img = psr.luma_to_f32(host[o:o + w * h].reshape(h, w))
+        rh, rd = psr.extract_features(img, 4, 3, 0.015, True)
+        k = rc[i]
+        got = gpu.Features.from_arrays(np.frombuffer(F[i, :k * fsz].tobytes(), gpu.FEAT_DTYPE), D[i, :k])
+        assert len(got) == len(rh), (i, w, h)
+        _check_features(got, rh, rd)

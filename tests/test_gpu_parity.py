@@ -93,11 +93,41 @@ def test_keypoints_exact_640x480(gpu, upsample):
     _cmp_keypoints(dev, ref)
 
 
-@pytest.mark.parametrize("scales", [3, 5, 8, 9, 10])
+@pytest.mark.parametrize("shape", SMALL_SHAPES + [(24, 300), (300, 24)])
+def test_keypoints_exact_small(gpu, shape):
+    """Small and thin levels, where the window of the level S+2 values formed around slice-S
+    candidates reaches past the borders on every side and is clamped like the blur's."""
+    img = _rand_img(*shape, seed=shape[0] * 11 + shape[1])
+    cfg = gpu.SiftConfig().octaves(4).scales(3).contrast_threshold(0.005)
+    dev = gpu.Sift(cfg).detect_keypoints(img)
+    ref = psr.detect_keypoints(img, 4, 3, 0.005, True)
+    _cmp_keypoints(dev, ref)
+
+
+def test_keypoints_exact_with_nan_pixels(gpu):
+    """An f32 image with NaN pixels: the blur spreads them over a patch of every level, and
+    the extremum test must reject exactly what the reference's strict comparisons reject
+    (a NaN neighbour fails all of them), so tiles holding a NaN take the literal comparison
+    path; the keypoints elsewhere, and their number, equal the oracle's."""
+    _, img = _blob_f32(640, 480, 200, seed=4)
+    img = img.copy()
+    img[100, 100] = np.nan
+    img[300:303, 500] = np.nan
+    img[479, 0] = np.nan
+    cfg = gpu.SiftConfig().octaves(4).scales(3)
+    dev = gpu.Sift(cfg).detect_keypoints(img)
+    ref = psr.detect_keypoints(img, 4, 3, 0.015, True)
+    assert len(ref) > 50 and np.isfinite(ref["x"]).all()
+    _cmp_keypoints(dev, ref)
+
+
+@pytest.mark.parametrize("scales", [3, 4, 5, 6, 7, 8, 9, 10])
 def test_keypoints_exact_scales(gpu, scales):
-    """Slice counts whose survivor masks fill one or two 64-bit words in the extremum
-    kernel's in-plane pass: the upsampled first octave (1200 x 900) takes 32-row tiles (8 rows
-    per thread: S = 8 fills one word, S > 8 flushes twice), the later octaves 16-row tiles."""
+    """Every supported slice count: the survivor masks of the extremum kernel's in-plane pass
+    fill one or two 64-bit words (the upsampled first octave, 1200 x 900, takes 32-row tiles,
+    8 rows per thread: S = 8 fills one word, S > 8 flushes twice; the later octaves 16-row
+    tiles), and each S has its own radius (10, 8, 7, 6, 5, 5, 5, 4) for the level S+2 values
+    that the extremum kernel forms around slice-S candidates instead of reading a stored plane."""
     u8, img = _blob_f32(600, 450, 200, seed=scales)
     cfg = gpu.SiftConfig().octaves(3).scales(scales)
     dev = gpu.Sift(cfg).detect_keypoints(img)
