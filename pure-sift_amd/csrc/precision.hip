@@ -36,6 +36,10 @@ constexpr int kPQB = 256;        // queries per workgroup (4 waves x 64)
 constexpr int kPTT = 64;         // targets per LDS tile
 constexpr int kPCand = 4;        // candidates kept per lane
 constexpr int kPRescore = 8;     // candidates rescored per query
+// Ranking keys are |t|^2 + kKeyBias - 2 q.t: positive for unit descriptors and far from the
+// subnormal range, so replacing their low mantissa bits (see the top-k epilogue) keeps them
+// ordinary floats near the value
+constexpr float kKeyBias = 8.0f;
 
 template <bool FP8>
 struct Prec {
@@ -127,6 +131,7 @@ __global__ __launch_bounds__(256, 2) void prec_topk_kernel(const uint8_t *qlow, 
     __shared__ __attribute__((aligned(32))) uint8_t s_t[2][kPTT * P::kLdsStride];
     __shared__ __attribute__((aligned(16))) float s_tn[2][kPTT];
     __shared__ uint32_t s_te[2][kPTT];
+    __shared__ int s_pad[2];   // the tile holds padding rows (a suffix of the rows, so: its last row does)
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, h = l >> 5, c = l & 31;
     const uint32_t tile0 = blockIdx.y * tiles_per_split;
     const uint32_t tile1 = min(ntiles, tile0 + tiles_per_split);
@@ -190,8 +195,9 @@ __global__ __launch_bounds__(256, 2) void prec_topk_kernel(const uint8_t *qlow, 
             PREC_PUT(3, pre3);                                                                      \
         }                                                                                           \
         if (tid < kPTT) {                                                                           \
-            s_tn[buf_st][tid] = pre_tn;                                                             \
+            s_tn[buf_st][tid] = pre_tn + kKeyBias;                                                  \
             s_te[buf_st][tid] = pre_te;                                                             \
+            if (tid == kPTT - 1) s_pad[buf_st] = isinf(pre_tn) ? 1 : 0;                             \
         }                                                                                           \
     } while (0)
 
@@ -224,11 +230,23 @@ __global__ __launch_bounds__(256, 2) void prec_topk_kernel(const uint8_t *qlow, 
                 }
             }
         }
-        // epilogue: D lane l, register r -> query column c, target row 8(r/4) + 4h + r%4
+        // epilogue: D lane l, register r -> query column c, target row 8(r/4) + 4h + r%4.
+        // Each of the lane's 32 values per query set becomes a key: the ranking value offset by
+        // kKeyBias, (|t|^2 + kKeyBias) - 2 q.t, with its 5 low mantissa bits replaced by the
+        // value's position p = 16u + r (a change below 2^-18 relative, far inside the
+        // approximation the FP32 rescore removes). A lane's keys are then distinct, so a tree of
+        // fminf (which returns one of its operands) yields the smallest key and with it its
+        // target; the same tournament (of 3-input min / median operations) carries the second
+        // smallest. When the second smallest is not below the lane's 4th-best key, at most the
+        // smallest enters the list (one insertion, no per-value branch: almost every (tile,
+        // lane) after a split's first tiles); when exactly two keys are below it, those two
+        // enter. Otherwise, and in the tile holding padding rows (+inf norms), every value is
+        // tried in turn.
         const uint32_t tbase = tile * kPTT;
+        const bool pad_tile = s_pad[buf] != 0;
 #pragma unroll
         for (int s = 0; s < 2; s++) {
-            float vmin = INFINITY;
+            float key[32];
 #pragma unroll
             for (int u = 0; u < 2; u++)
 #pragma unroll
@@ -237,20 +255,68 @@ __global__ __launch_bounds__(256, 2) void prec_topk_kernel(const uint8_t *qlow, 
                     const float tv[4] = {t4.x, t4.y, t4.z, t4.w};
 #pragma unroll
                     for (int q4 = 0; q4 < 4; q4++) {
-                        float v = __builtin_fmaf(-2.0f, acc[s][u][rr * 4 + q4], tv[q4]);
-                        acc[s][u][rr * 4 + q4] = v;
-                        vmin = fminf(vmin, v);
+                        const int p = u * 16 + rr * 4 + q4;
+                        const float v = __builtin_fmaf(-2.0f, acc[s][u][rr * 4 + q4], tv[q4]);
+                        key[p] = __uint_as_float((__float_as_uint(v) & ~31u) | (uint32_t)p);
                     }
                 }
-            if (vmin < bv[s][kPCand - 1]) {
+            const float thr = bv[s][kPCand - 1];
+            bool slow = pad_tile;
+            if (!pad_tile) {
+                // 3-way tournament: triples give (min3, med3), merging sorted (first, second)
+                // groups a, b, c gives min3(a1, b1, c1) and min(med3(a1, b1, c1), min3(a2, b2, c2))
+                float f[11], g[11];
 #pragma unroll
-                for (int u = 0; u < 2; u++)
+                for (int i = 0; i < 10; i++) {
+                    f[i] = fminf(fminf(key[3 * i], key[3 * i + 1]), key[3 * i + 2]);
+                    g[i] = __builtin_amdgcn_fmed3f(key[3 * i], key[3 * i + 1], key[3 * i + 2]);
+                }
+                f[10] = fminf(key[30], key[31]);
+                g[10] = fmaxf(key[30], key[31]);
+                auto merge3 = [](float a1, float a2, float b1, float b2, float c1, float c2, float &n1, float &n2) {
+                    n1 = fminf(fminf(a1, b1), c1);
+                    n2 = fminf(__builtin_amdgcn_fmed3f(a1, b1, c1), fminf(fminf(a2, b2), c2));
+                };
+                auto merge2 = [](float a1, float a2, float b1, float b2, float &n1, float &n2) {
+                    n1 = fminf(a1, b1);
+                    n2 = __builtin_amdgcn_fmed3f(a1, b1, fminf(a2, b2));
+                };
+                float f1[4], g1[4];
 #pragma unroll
-                    for (int r = 0; r < 16; r++) {
-                        const float v = acc[s][u][r];
-                        if (v < bv[s][kPCand - 1])
-                            cand_insert(v, tbase + u * 32 + 8 * (r >> 2) + 4 * h + (r & 3), bv[s], bi[s]);
+                for (int i = 0; i < 3; i++)
+                    merge3(f[3 * i], g[3 * i], f[3 * i + 1], g[3 * i + 1], f[3 * i + 2], g[3 * i + 2], f1[i], g1[i]);
+                merge2(f[9], g[9], f[10], g[10], f1[3], g1[3]);
+                float f2, g2, lo0, hi0;
+                merge3(f1[0], g1[0], f1[1], g1[1], f1[2], g1[2], f2, g2);
+                merge2(f2, g2, f1[3], g1[3], lo0, hi0);
+                auto target = [&](float k) {
+                    const uint32_t p = __float_as_uint(k) & 31u;
+                    return tbase + (p >> 4) * 32 + 8 * ((p & 15) >> 2) + 4 * h + (p & 3);
+                };
+                if (hi0 < thr) {
+                    // two or more new candidates (mostly in a split's first tiles): exactly two
+                    // are the two smallest keys; more take the per-value loop
+                    int below = 0;
+#pragma unroll
+                    for (int p = 0; p < 32; p++) below += key[p] < thr ? 1 : 0;
+                    if (below > 2) {
+                        slow = true;
+                    } else {
+                        cand_insert(lo0, target(lo0), bv[s], bi[s]);
+                        cand_insert(hi0, target(hi0), bv[s], bi[s]);
                     }
+                } else if (lo0 < thr) {
+                    cand_insert(lo0, target(lo0), bv[s], bi[s]);
+                }
+            }
+            if (slow) {
+#pragma unroll
+                for (int p = 0; p < 32; p++) {
+                    // padding rows: +inf norms give +inf (or NaN once marked) keys, which never rank
+                    const float k = key[p];
+                    if (k < bv[s][kPCand - 1])
+                        cand_insert(k, tbase + (p >> 4) * 32 + 8 * ((p & 15) >> 2) + 4 * h + (p & 3), bv[s], bi[s]);
+                }
             }
         }
         if (more) {
@@ -346,7 +412,7 @@ __global__ __launch_bounds__(256) void prec_rescore_kernel(const PCand *part, ui
         best_idx[gq] = bidx == 0xffffffffu ? 0u : bidx;
         best[gq] = bd;
         second[gq] = sd;
-        if (approx_best) approx_best[gq] = bv + qn[gq];
+        if (approx_best) approx_best[gq] = (bv - kKeyBias) + qn[gq];
     }
 }
 
