@@ -20,6 +20,7 @@
 // A and B lanes read the same k positions (lane half h takes k in [64h, 64h + 64) of the
 // row), so the k order inside an instruction does not affect the sum.
 #include <algorithm>
+#include <atomic>
 #include <type_traits>
 
 #include "ps_internal.h"
@@ -43,7 +44,9 @@ constexpr float kKeyBias = 8.0f;
 
 template <bool FP8>
 struct Prec {
-    static constexpr int kRowBytes = FP8 ? 128 : 256;
+    // BF16 rows carry 144 values: the 128 components and the bias-shifted norm term (see
+    // prep_bf16_kernel); FP8 rows the 128 components
+    static constexpr int kRowBytes = FP8 ? 128 : 288;
     static constexpr int kLdsStride = kRowBytes + (FP8 ? 32 : 16);   // pad: rows start in different banks
 };
 
@@ -64,17 +67,46 @@ __device__ __forceinline__ uint32_t bf16_rne(float x) {
     return (b + 0x7fffu + ((b >> 16) & 1u)) >> 16;
 }
 
-// One wave per row; rows >= *n (padding) become zero with norm +inf.
+// One wave per row of 144 BF16 values (round to nearest even); rows >= *n (padding) become
+// zero with norm +inf. The MFMA's k = 144 folds the ranking value's norm term in: a query row
+// is (q, 1, 1, 1, 0...), a target row (-2 t, n1, n2, n3, 0...) with n1 + n2 + n3 the
+// target's |t|^2 + kKeyBias split into three BF16 terms (each the rounded remainder of the
+// previous, so their sum is within 2^-24 relative of it), so the FP32 accumulator holds
+// (|t|^2 + kKeyBias) - 2 q.t directly; scaling by -2 is exact in BF16. Padding targets get
+// (0, +inf, 0, 0, ...), a +inf value.
 __global__ __launch_bounds__(256) void prep_bf16_kernel(const float *x, const uint32_t *n_p, uint32_t rows,
-                                                        uint32_t *out, float *norm) {
+                                                        bool target, uint32_t *out, float *norm) {
     const uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (r >= rows) return;
     const bool valid = r < *n_p;
     float2 v = valid ? reinterpret_cast<const float2 *>(x + (size_t)r * 128)[lane] : make_float2(0.f, 0.f);
-    out[(size_t)r * 64 + lane] = bf16_rne(v.x) | (bf16_rne(v.y) << 16);
-    float s = wave_sum(v.x * v.x + v.y * v.y);
+    const float sc = target ? -2.0f : 1.0f;
+    uint32_t *row = out + (size_t)r * 72;
+    row[lane] = bf16_rne(sc * v.x) | (bf16_rne(sc * v.y) << 16);
+    const float s = wave_sum(v.x * v.x + v.y * v.y);
     if (lane == 0) norm[r] = valid ? s : INFINITY;
+    if (lane < 8) {
+        uint32_t e[2] = {0u, 0u};   // values 128 + 2 lane, 128 + 2 lane + 1
+        if (lane < 2) {
+            uint32_t n[3];
+            if (!target) {
+                n[0] = n[1] = n[2] = 0x3f80u;   // 1.0
+            } else if (!valid) {
+                n[0] = 0x7f80u;                 // +inf
+                n[1] = n[2] = 0u;
+            } else {
+                const float t = s + kKeyBias;
+                n[0] = bf16_rne(t);
+                const float r1 = t - __uint_as_float(n[0] << 16);
+                n[1] = bf16_rne(r1);
+                n[2] = bf16_rne(r1 - __uint_as_float(n[1] << 16));
+            }
+            e[0] = n[2 * lane];
+            e[1] = lane == 0 ? n[1] : 0u;
+        }
+        row[64 + lane] = e[0] | (e[1] << 16);
+    }
 }
 
 __global__ __launch_bounds__(256) void prep_fp8_kernel(const float *x, const uint32_t *n_p, uint32_t rows,
@@ -139,8 +171,8 @@ __global__ __launch_bounds__(256, 2) void prec_topk_kernel(const uint8_t *qlow, 
     const uint32_t qbase = blockIdx.x * kPQB + w * 64;
 
     // query operands (B), both 32-query sets of this wave
-    constexpr int KS = FP8 ? 2 : 8;   // MFMA k-steps over 128
     using BT = typename std::conditional<FP8, i32x8, bf16x8>::type;
+    constexpr int KS = P::kRowBytes / 2 / (int)sizeof(BT);   // MFMA k-steps per lane half (9 BF16, 2 FP8)
     BT b[2][KS];
     int qe[2];
 #pragma unroll
@@ -161,22 +193,29 @@ __global__ __launch_bounds__(256, 2) void prec_topk_kernel(const uint8_t *qlow, 
         }
 
     // cooperative tile load: kPTT rows x kRowBytes, 16 B per load, staged through VGPRs
-    // (named registers: 4 loads per thread for BF16 rows, 2 for FP8)
-    constexpr int kLoads = kPTT * P::kRowBytes / 16 / 256;
+    // (kLoads per thread: 5 for BF16 rows, the last by half the threads, 2 for FP8)
+    constexpr int kChunks = kPTT * P::kRowBytes / 16;
+    constexpr int kLoads = (kChunks + 255) / 256;
     constexpr int kSegs = P::kRowBytes / 16;
-    static_assert(kLoads == 2 || kLoads == 4, "tile load split");
-    uint4 pre0, pre1, pre2 = {}, pre3 = {};
+    static_assert(kLoads <= 5, "tile load split");
+    uint4 pre0 = {}, pre1 = {}, pre2 = {}, pre3 = {}, pre4 = {};
     float pre_tn = 0.0f;
     uint32_t pre_te = 127u;
+// chunk I (16 bytes) of the tile for this thread, in the named register preI
+#define PREC_LD1(I)                                                                                 \
+    do {                                                                                            \
+        if constexpr ((I) < kLoads)                                                                 \
+            if ((I) * 256 + tid < kChunks) pre##I = src_[(I) * 256 + tid];                          \
+    } while (0)
+#define PREC_PUT1(I)                                                                                \
+    do {                                                                                            \
+        if constexpr ((I) < kLoads)                                                                 \
+            if ((I) * 256 + tid < kChunks) PREC_PUT((I), pre##I);                                   \
+    } while (0)
 #define PREC_LOAD_TILE(TILE)                                                                        \
     do {                                                                                            \
         const uint4 *src_ = reinterpret_cast<const uint4 *>(tlow + (size_t)(TILE) * kPTT * P::kRowBytes); \
-        pre0 = src_[tid];                                                                           \
-        pre1 = src_[tid + 256];                                                                     \
-        if constexpr (kLoads == 4) {                                                                \
-            pre2 = src_[tid + 512];                                                                 \
-            pre3 = src_[tid + 768];                                                                 \
-        }                                                                                           \
+        PREC_LD1(0); PREC_LD1(1); PREC_LD1(2); PREC_LD1(3); PREC_LD1(4);                           \
         if (tid < kPTT) {                                                                           \
             pre_tn = tn[(size_t)(TILE) * kPTT + tid];                                               \
             pre_te = FP8 ? (uint32_t)texp[(size_t)(TILE) * kPTT + tid] : 127u;                      \
@@ -188,12 +227,7 @@ __global__ __launch_bounds__(256, 2) void prec_topk_kernel(const uint8_t *qlow, 
 #define PREC_STORE_TILE(BUF)                                                                        \
     do {                                                                                            \
         const int buf_st = (BUF);                                                                   \
-        PREC_PUT(0, pre0);                                                                          \
-        PREC_PUT(1, pre1);                                                                          \
-        if constexpr (kLoads == 4) {                                                                \
-            PREC_PUT(2, pre2);                                                                      \
-            PREC_PUT(3, pre3);                                                                      \
-        }                                                                                           \
+        PREC_PUT1(0); PREC_PUT1(1); PREC_PUT1(2); PREC_PUT1(3); PREC_PUT1(4);                      \
         if (tid < kPTT) {                                                                           \
             s_tn[buf_st][tid] = pre_tn + kKeyBias;                                                  \
             s_te[buf_st][tid] = pre_te;                                                             \
@@ -211,15 +245,23 @@ __global__ __launch_bounds__(256, 2) void prec_topk_kernel(const uint8_t *qlow, 
         const bool more = tile + 1 < tile1;
         if (more) PREC_LOAD_TILE(tile + 1);
         f32x16 acc[2][2];
+        BT a_op[2][KS];
+        int te_op[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const uint8_t *arow = &s_t[buf][(u * 32 + c) * P::kLdsStride + h * (P::kRowBytes / 2)];
+            te_op[u] = (int)s_te[buf][u * 32 + c];
+#pragma unroll
+            for (int t = 0; t < KS; t++) a_op[u][t] = *reinterpret_cast<const BT *>(arow + t * sizeof(BT));
+        }
 #pragma unroll
         for (int u = 0; u < 2; u++) {
 #pragma unroll
             for (int s = 0; s < 2; s++) acc[s][u] = f32x16{};
-            const uint8_t *arow = &s_t[buf][(u * 32 + c) * P::kLdsStride + h * (P::kRowBytes / 2)];
-            const int te = (int)s_te[buf][u * 32 + c];
+            const int te = te_op[u];
 #pragma unroll
             for (int t = 0; t < KS; t++) {
-                BT a = *reinterpret_cast<const BT *>(arow + t * sizeof(BT));
+                const BT a = a_op[u][t];
 #pragma unroll
                 for (int s = 0; s < 2; s++) {
                     if constexpr (FP8)
@@ -256,7 +298,9 @@ __global__ __launch_bounds__(256, 2) void prec_topk_kernel(const uint8_t *qlow, 
 #pragma unroll
                     for (int q4 = 0; q4 < 4; q4++) {
                         const int p = u * 16 + rr * 4 + q4;
-                        const float v = __builtin_fmaf(-2.0f, acc[s][u][rr * 4 + q4], tv[q4]);
+                        // BF16: the MFMA already formed (|t|^2 + kKeyBias) - 2 q.t
+                        const float v = FP8 ? __builtin_fmaf(-2.0f, acc[s][u][rr * 4 + q4], tv[q4])
+                                            : acc[s][u][rr * 4 + q4];
                         key[p] = __uint_as_float((__float_as_uint(v) & ~31u) | (uint32_t)p);
                     }
                 }
@@ -326,6 +370,8 @@ __global__ __launch_bounds__(256, 2) void prec_topk_kernel(const uint8_t *qlow, 
             buf ^= 1;
         }
     }
+#undef PREC_LD1
+#undef PREC_PUT1
 #undef PREC_LOAD_TILE
 #undef PREC_PUT
 #undef PREC_STORE_TILE
@@ -422,17 +468,47 @@ struct PrecLayout {
     uint32_t nsplit, tiles_per_split, ntiles;
 };
 
+// compute units of the current device (256 on MI355X), cached per device
+int device_cus() {
+    static std::atomic<int> cached[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int n = cached[dev].load(std::memory_order_relaxed);
+    if (n <= 0) {
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cached[dev].store(n, std::memory_order_relaxed);
+    }
+    return n;
+}
+
 PrecLayout prec_layout(int prec, size_t nqm, size_t ntm) {
     auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
     PrecLayout L{};
     L.nq_pad = (nqm + kPQB - 1) / kPQB * kPQB;
     L.nt_pad = (ntm + kPTT - 1) / kPTT * kPTT;
-    L.row_bytes = prec == 2 ? 128 : 256;
+    L.row_bytes = prec == 2 ? 128 : 288;
     L.ntiles = (uint32_t)(L.nt_pad / kPTT);
     const uint32_t qblocks = (uint32_t)(L.nq_pad / kPQB);
-    uint32_t ns = std::max<uint32_t>(1, (1024 + qblocks - 1) / qblocks);
-    ns = std::min<uint32_t>(ns, 16);
-    ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, L.ntiles / 4));
+    // target splits: the smallest count whose workgroups fill their last round of resident
+    // slots (two per CU) to at least 90%, else the best-filled count (at most 16 splits, at
+    // least 4 tiles each)
+    const uint32_t slots = 2u * (uint32_t)device_cus();
+    const uint32_t ns_max = std::min<uint32_t>(16, std::max<uint32_t>(1, L.ntiles / 4));
+    uint32_t ns = 1;
+    double best_fill = -1.0;
+    for (uint32_t k = 1; k <= ns_max; k++) {
+        const uint64_t wgs = (uint64_t)qblocks * k;
+        const uint64_t rounds = (wgs + slots - 1) / slots;
+        const double fill = (double)wgs / (double)(rounds * slots);
+        if (fill >= 0.9) {
+            ns = k;
+            break;
+        }
+        if (fill > best_fill) {
+            best_fill = fill;
+            ns = k;
+        }
+    }
     L.tiles_per_split = (L.ntiles + ns - 1) / ns;
     L.nsplit = (L.ntiles + L.tiles_per_split - 1) / L.tiles_per_split;
     size_t o = 0;
@@ -469,8 +545,10 @@ hipError_t launch_nn_precision(int prec, const float *qd, const uint32_t *nq, si
     PCand *part = reinterpret_cast<PCand *>(base + L.off_part);
     const uint32_t qrows = (uint32_t)L.nq_pad, trows = (uint32_t)L.nt_pad;
     if (prec == 1) {
-        prep_bf16_kernel<<<(qrows + 3) / 4, 256, 0, st>>>(qd, nq, qrows, reinterpret_cast<uint32_t *>(qlow), qn);
-        prep_bf16_kernel<<<(trows + 3) / 4, 256, 0, st>>>(td, nt, trows, reinterpret_cast<uint32_t *>(tlow), tn);
+        prep_bf16_kernel<<<(qrows + 3) / 4, 256, 0, st>>>(qd, nq, qrows, false, reinterpret_cast<uint32_t *>(qlow),
+                                                          qn);
+        prep_bf16_kernel<<<(trows + 3) / 4, 256, 0, st>>>(td, nt, trows, true, reinterpret_cast<uint32_t *>(tlow),
+                                                          tn);
     } else {
         prep_fp8_kernel<<<(qrows + 3) / 4, 256, 0, st>>>(qd, nq, qrows, reinterpret_cast<uint32_t *>(qlow), qn, qe);
         prep_fp8_kernel<<<(trows + 3) / 4, 256, 0, st>>>(td, nt, trows, reinterpret_cast<uint32_t *>(tlow), tn, te);
