@@ -1,14 +1,15 @@
-// Orientation assignment and 128-D descriptors, one wavefront per keypoint /
+// Orientation assignment and 128-D descriptors, one workgroup per keypoint /
 // feature (pure-sift 0.2.0 src/pipeline.rs:1206-1577).
 //
 // Gradients are evaluated on the fly from the octave's Gaussian level instead of
 // materialising magnitude/angle planes (pipeline.rs:1168-1203): same expressions,
-// and the windows never touch the zeroed border of the reference's planes. Device
-// expf/sinf/cosf, a polynomial atan2 and the parallel histogram accumulation make these
-// results tolerance-level (not bit-level) equal to the reference.
+// and the windows never touch the zeroed border of the reference's planes. The hardware
+// exponential (v_exp_f32, within 2 ulp), sinf/cosf, a polynomial atan2, fused multiply-adds
+// and the parallel histogram accumulation make these results tolerance-level (not bit-level)
+// equal to the reference.
 #include "ps_internal.h"
 
-#pragma clang fp contract(off)
+#pragma clang fp contract(fast)
 
 namespace ps {
 
@@ -102,6 +103,7 @@ __global__ __launch_bounds__(kDescThreads) void orient_kernel(PyrView pv, const 
 
     const float sigma_ori = fmaxf(1.5f * l.scale, 1.0f);
     const float two_sigma_sq = 2.0f * sigma_ori * sigma_ori;
+    const float neg_inv_2s2 = -1.0f / two_sigma_sq;
     const int radius = max((int)ceilf(3.0f * sigma_ori), 2);
     const int cx = (int)roundf(l.x), cy = (int)roundf(l.y);
     const float bin_scale = 36.0f / kTwoPi;
@@ -136,7 +138,7 @@ __global__ __launch_bounds__(kDescThreads) void orient_kernel(PyrView pv, const 
             float m = sqrtf(dx * dx + dy * dy);
             if (m <= 1e-7f) continue;
             float a = rem_euclid(grad_atan2(dy, dx), kTwoPi);
-            float weight = expf(-rsq[u] / two_sigma_sq);
+            float weight = __expf(rsq[u] * neg_inv_2s2);
             int bin = ((int)floorf(a * bin_scale)) % 36;
             atomicAdd(&wh[bin], m * weight);
         }
@@ -322,7 +324,7 @@ __global__ __launch_bounds__(kDescThreads) void descriptor_kernel(PyrView pv, co
             const float a = rem_euclid(grad_atan2(gdy, gdx), kTwoPi);
             float r_rot, c_rot, r_bin, c_bin;
             bins(qy, qx, r_rot, c_rot, r_bin, c_bin);
-            const float wm = m * expf((r_rot * r_rot + c_rot * c_rot) * exp_scale);
+            const float wm = m * __expf((r_rot * r_rot + c_rot * c_rot) * exp_scale);
             const float o_bin = rem_euclid(a - theta, kTwoPi) * bin_per_rad;
             const int o0 = (int)floorf(o_bin);
             const float dr = r_bin - (float)r0, dc = c_bin - (float)c0, dof = o_bin - (float)o0;
