@@ -184,6 +184,104 @@ __device__ bool smallest_eigvec9(double (&a)[9][9], double (&h)[9]) {
     return ok;
 }
 
+// The same cyclic Jacobi, run by one wave for a matrix held in LDS (row-major 9x9, symmetric):
+// lane r < 9 keeps row r of the matrix and row r of the eigenvector matrix. Each rotation's
+// angle is computed identically by every lane from the broadcast d[p], d[q], a[p][q]; lane r
+// updates its own pair (a[r][p], a[r][q]) and (v[r][p], v[r][q]) with the reference's
+// expressions, and lanes p and q then take their rows from the updated columns (symmetry), so
+// every element goes through exactly the operations of smallest_eigvec9. Writes the
+// eigenvector to h (LDS) from lanes 0..8 and returns the wave-uniform success flag.
+// lane `src` (uniform) of x, through the scalar registers
+__device__ __forceinline__ double readlane_f64(double x, int src) {
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), src);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+__device__ bool smallest_eigvec9_wave(const double *A, double *h) {
+    const int lane = threadIdx.x & 63;
+    const int r = lane < 9 ? lane : 8;
+    const bool mine = lane < 9;
+    double row[9], vr[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        row[k] = A[r * 9 + k];
+        vr[k] = k == r ? 1.0 : 0.0;
+    }
+    for (int sweep = 0; sweep < 20; sweep++) {
+        double mo = 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const double av = fabs(row[k]);
+            if (mine && k > r && av > mo) mo = av;
+        }
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) {
+            const double om = __shfl_xor(mo, o);
+            if (om > mo) mo = om;
+        }
+        mo = readlane_f64(mo, 0);
+        if (mo < 1e-13) break;
+#pragma unroll
+        for (int p = 0; p < 8; p++)
+#pragma unroll
+            for (int q = p + 1; q < 9; q++) {
+                const double apq = readlane_f64(row[p], q);   // a[q][p] = a[p][q], kept current by lane q
+                if (fabs(apq) < 1e-15) continue;
+                const double app = readlane_f64(row[p], p), aqq = readlane_f64(row[q], q);
+                const double tau = (aqq - app) / (2.0 * apq);
+                const double t = tau >= 0.0 ? 1.0 / (tau + sqrt(1.0 + tau * tau))
+                                            : -1.0 / (-tau + sqrt(1.0 + tau * tau));
+                const double c = 1.0 / sqrt(1.0 + t * t), sn = t * c;
+                const double dp = app - t * apq, dq = aqq + t * apq;
+                if (r != p && r != q) {
+                    const double arp = row[p], arq = row[q];
+                    row[p] = c * arp - sn * arq;
+                    row[q] = sn * arp + c * arq;
+                }
+                {
+                    const double vrp = vr[p], vrq = vr[q];
+                    vr[p] = c * vrp - sn * vrq;
+                    vr[q] = sn * vrp + c * vrq;
+                }
+                // rows p and q from the updated columns p and q
+#pragma unroll
+                for (int k = 0; k < 9; k++) {
+                    if (k == p || k == q) continue;
+                    const double kp = readlane_f64(row[p], k), kq = readlane_f64(row[q], k);
+                    if (r == p) row[k] = kp;
+                    if (r == q) row[k] = kq;
+                }
+                if (r == p) {
+                    row[p] = dp;
+                    row[q] = 0.0;
+                }
+                if (r == q) {
+                    row[q] = dq;
+                    row[p] = 0.0;
+                }
+            }
+    }
+    // smallest eigenvalue, first index on ties
+    int mi = 0;
+    double me = readlane_f64(row[0], 0);
+#pragma unroll
+    for (int i = 1; i < 9; i++) {
+        const double di = readlane_f64(row[i], i);
+        if (di < me) {
+            me = di;
+            mi = i;
+        }
+    }
+    double x = vr[0];
+#pragma unroll
+    for (int i = 1; i < 9; i++) x = mi == i ? vr[i] : x;
+    const bool fin = !mine || isfinite(x);
+    if (mine) h[r] = x;
+    return __all(fin);
+}
+
 // pipeline.rs:2357-2404
 __device__ bool degenerate_tri(const double (&px)[4], const double (&py)[4]) {
     const int T[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
@@ -596,15 +694,23 @@ __device__ bool fit_dlt_block(const ps_feature_match *m, const uint32_t *idx, ui
         if (tid < 45) s_ata[tid] = acc;
     }
     __syncthreads();
+    // the symmetric AtA, then its smallest eigenvector by wave 0
+    __shared__ double s_full[81], s_hv[9];
+    __shared__ int s_eig_ok;
+    if (tid < 81) {
+        const int a = tid / 9, b = tid % 9, lo = a < b ? a : b, hi = a < b ? b : a;
+        s_full[tid] = s_ata[lo * 9 - lo * (lo - 1) / 2 + (hi - lo)];
+    }
+    __syncthreads();
+    if (tid < 64) {
+        const bool eok = smallest_eigvec9_wave(s_full, s_hv);
+        if (tid == 0) s_eig_ok = eok;
+    }
+    __syncthreads();
     if (tid == 0) {
-        double ata[9][9];
-        int k = 0;
-        for (int a = 0; a < 9; a++)
-            for (int b = a; b < 9; b++) ata[a][b] = s_ata[k++];
-        for (int a = 0; a < 9; a++)
-            for (int b = 0; b < a; b++) ata[a][b] = ata[b][a];
         double hv[9];
-        int ok = smallest_eigvec9(ata, hv);
+        for (int i = 0; i < 9; i++) hv[i] = s_hv[i];
+        int ok = s_eig_ok;
         if (ok) {
             const double cxq = s_norm[0], cyq = s_norm[1], cxt = s_norm[2], cyt = s_norm[3], sq = s_norm[4],
                          stt = s_norm[5];
