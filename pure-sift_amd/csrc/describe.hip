@@ -4,7 +4,7 @@
 // Gradients are evaluated on the fly from the octave's Gaussian level instead of
 // materialising magnitude/angle planes (pipeline.rs:1168-1203): same expressions,
 // and the windows never touch the zeroed border of the reference's planes. The hardware
-// exponential (v_exp_f32, within 2 ulp), sinf/cosf, a polynomial atan2, fused multiply-adds
+// exponential (v_exp_f32, within 2 ulp) and square root (v_sqrt_f32, within 1 ulp) per pixel, sinf/cosf, a polynomial atan2, fused multiply-adds
 // and the parallel histogram accumulation make these results tolerance-level (not bit-level)
 // equal to the reference.
 #include "ps_internal.h"
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(kDescThreads) void orient_kernel(PyrView pv, const 
         for (int u = 0; u < kBatch; u++) {
             if (!ok[u]) continue;
             float dx = gxa[u] - gxb[u], dy = gyb[u] - gya[u];    // pipeline.rs:1186-1189
-            float m = sqrtf(dx * dx + dy * dy);
+            float m = __builtin_amdgcn_sqrtf(dx * dx + dy * dy);
             if (m <= 1e-7f) continue;
             float a = rem_euclid(grad_atan2(dy, dx), kTwoPi);
             float weight = __expf(rsq[u] * neg_inv_2s2);
@@ -319,7 +319,7 @@ __global__ __launch_bounds__(kDescThreads) void descriptor_kernel(PyrView pv, co
         // into the lane's 4 corners x 8 orientations (pipeline.rs:1186-1189, :1394-1440)
         auto add_pixel = [&](float gxa, float gxb, float gya, float gyb, int qy, int qx) {
             const float gdx = gxa - gxb, gdy = gyb - gya;
-            const float m = sqrtf(gdx * gdx + gdy * gdy);
+            const float m = __builtin_amdgcn_sqrtf(gdx * gdx + gdy * gdy);
             if (m <= 1e-7f) return;
             const float a = rem_euclid(grad_atan2(gdy, gdx), kTwoPi);
             float r_rot, c_rot, r_bin, c_bin;
