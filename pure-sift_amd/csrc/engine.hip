@@ -335,13 +335,14 @@ ps::PyrView pyr_view(ps_engine *e) {
 
 // Gaussian + DoG pyramid (pipeline.rs:550-638) into e->pyr (src is device memory) and, with
 // `extrema`, every octave's extrema appended to e->cand (cnt[0] candidates, cnt[kErrWord] error bits).
-// Octave o+1 depends only on level S of octave o, so once that level is written octave o+1
-// (the next, four times smaller and launch-latency bound) starts on a side stream while
-// octave o's last two levels and its extrema finish on its own stream; every side stream is
-// joined back into the engine stream before this returns. Candidate order is fixed later by
-// the ordering stage, so it does not matter which stream appends first. No fork happens while
-// the engine stream is being captured: a capturing caller (the batched path) already overlaps
-// whole images on separate streams.
+// Octave o+1 depends only on level S of octave o, so the levels 1..S of all octaves form one
+// dependent chain: it stays on the engine stream (a cross-stream event costs ~10 us of
+// latency), and each octave's leaf work -- the levels above S and the extrema pass -- is forked
+// to a side stream once level S is written; every side stream is joined back into the engine
+// stream before this returns. Candidate order is fixed later by the ordering stage, so it does
+// not matter which stream appends first. No fork happens while the engine stream is being
+// captured: a capturing caller (the batched path) already overlaps whole images on separate
+// streams.
 ps_status enqueue_scale_space(ps_engine *e, const void *src, bool src_u8, size_t h, size_t w, size_t stride,
                               const ps_sift_config &cfg, bool extrema) {
     ps::PyrPlan &p = e->plan;
@@ -387,7 +388,6 @@ ps_status enqueue_scale_space(ps_engine *e, const void *src, bool src_u8, size_t
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     PS_HIP(hipStreamIsCapturing(e->stream, &cap));
     const bool fork = cap == hipStreamCaptureStatusNone;
-    auto lane = [&](int o) { return fork && o > 0 ? e->side[(o - 1) % ps_engine::kSides] : e->stream; };
     ps::BlurArgs a;
     a.src = src;
     a.src_stride = stride;
@@ -401,16 +401,21 @@ ps_status enqueue_scale_space(ps_engine *e, const void *src, bool src_u8, size_t
     int mode = src_u8 ? (p.upsample ? ps::SRC_U8_UP : ps::SRC_U8) : (p.upsample ? ps::SRC_F32_UP : ps::SRC_F32);
     PS_HIP(ps::launch_blur(p.init_r, mode, a, p.init_taps, blur_tile(p.init_r, mode, a.h, a.w), e->stream));
     for (int o = 0; o < p.n_oct; o++) {
-        hipStream_t st = lane(o);
-        for (int s = 1; s <= p.S; s++) PS_HIP(level(o, s, st));
-        if (fork && o + 1 < p.n_oct) {
-            PS_HIP(hipEventRecord(e->fork_ev[o], st));
-            PS_HIP(hipStreamWaitEvent(lane(o + 1), e->fork_ev[o], 0));
+        // the chain: levels 1..S of every octave on the engine stream (octave o+1 starts from
+        // level S of octave o), so it crosses no stream
+        for (int s = 1; s <= p.S; s++) PS_HIP(level(o, s, e->stream));
+        if (o == 0) prof_mark(e, 1);
+        // octave o's leaf work -- its remaining levels and its extrema -- on a side stream beside
+        // the chain (the last octave's on the engine stream: nothing follows it there)
+        const bool side = fork && o + 1 < p.n_oct;
+        hipStream_t st = side ? e->side[o % ps_engine::kSides] : e->stream;
+        if (side) {
+            PS_HIP(hipEventRecord(e->fork_ev[o], e->stream));
+            PS_HIP(hipStreamWaitEvent(st, e->fork_ev[o], 0));
         }
         // level S+2 feeds only the upper DoG plane of slice-S candidates: the extraction path
         // leaves it to the extrema kernel, which forms its few needed values from level S+1
         for (int s = p.S + 1; s <= (extrema ? p.S + 1 : p.S + 2); s++) PS_HIP(level(o, s, st));
-        if (o == 0) prof_mark(e, 1);
         if (extrema) PS_HIP(octave_extrema(o, st));
     }
     if (fork)

@@ -31,8 +31,9 @@ enum AuxSlot {
 struct ps_engine {
     int device = 0;
     hipStream_t stream = nullptr;
-    // octave o >= 1 of an extraction runs on side[(o - 1) % kSides], forked once level S of
-    // octave o - 1 is written and joined back into `stream` at the end of the scale space
+    // the leaf work of octave o of an extraction (levels above S, extrema) runs on
+    // side[o % kSides], forked once level S of octave o is written and joined back into
+    // `stream` at the end of the scale space
     static constexpr int kSides = 3;
     hipStream_t side[kSides] = {};
     hipEvent_t fork_ev[ps::kMaxOct] = {};
