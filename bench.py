@@ -296,8 +296,9 @@ def bench_ransac(n_pairs: int = 4096, n_base: int = 256, steps: int = 600, cpu_p
 
 def bench_batch(n_images: int = 256, reps: int = 5, cpu_images: int = 4) -> dict:
     """Batched extraction of 256 seeded blob/noise u8 images with widths U[320, 2048] and
-    heights U[240, 1536]: one captured HIP graph over 8 concurrent streams, blur tiles picked
-    by the autotuner; compared with the per-image loop and the eager batch."""
+    heights U[240, 1536]: grouped persistent launches (one per pyramid level / stage for all
+    images) captured in one HIP graph, launch shapes picked by the batch autotuner; compared
+    with the per-image loop and the eager batch."""
     import numpy as np
     import torch
     import pure_sift as ps
@@ -347,7 +348,7 @@ def bench_batch(n_images: int = 256, reps: int = 5, cpu_images: int = 4) -> dict
     ms_loop = timed(loop, reps)
     ms_eager = timed(batch(ps.BATCH_NO_GRAPH), reps)
     ms_graph = timed(batch(0), reps)
-    n_tuned = eng.autotune_blur([(h, w) for (w, h) in sizes], cfg)
+    n_tuned = eng.autotune_batch(pix.data_ptr(), im, cfg)
     ms_tuned = timed(batch(0), reps)
     counts = cnt.cpu().numpy()
     eng.check()
@@ -360,7 +361,7 @@ def bench_batch(n_images: int = 256, reps: int = 5, cpu_images: int = 4) -> dict
            "value": n_images / (best * 1e-3), "unit": "images/s",
            "ms_per_batch": {"per_image_loop": ms_loop, "batch_eager": ms_eager, "batch_graph": ms_graph,
                             "batch_graph_autotuned": ms_tuned},
-           "autotuned_level_classes": n_tuned, "features_mean": float(counts.mean()),
+           "autotuned_launch_classes": n_tuned, "features_mean": float(counts.mean()),
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "alg_bytes_per_batch": alg,
                         "note": "whole extraction (pyramid + extrema + describe) timed; bytes are the "

@@ -244,11 +244,14 @@ ps_status ps_engine_nn2(ps_engine *e, const float *d_q_desc, const uint32_t *d_n
 /* ---- batched extraction ----------------------------------------------------------------
  * Many uneven-size u8 images in one call: image i is d_pixels[offset ..] (height x width,
  * row_stride bytes); its features / descriptors go to d_feat[i * cap ..] / d_desc[i * cap * 128 ..]
- * and its count to d_counts[i]. Same results as one ps_engine_extract_u8 per image. The
- * images run concurrently on up to 8 internal streams; the sequence is captured into a HIP
- * graph on first use and replayed while images, buffers and config are unchanged
- * (PS_BATCH_NO_GRAPH launches eagerly instead). No counterpart in the reference, whose
- * callers loop over images (cli.rs:2093-2113, stitch.rs:493-499). */
+ * and its count to d_counts[i]. Same results as one ps_engine_extract_u8 per image. Every
+ * stage runs once for the whole batch as a grouped persistent launch (one pyramid level, one
+ * octave's extrema, the ordering, orientations and descriptors of all images per launch); the
+ * sequence is captured into a HIP graph on first use and replayed while images, buffers, config
+ * and tuned launch shapes are unchanged (PS_BATCH_NO_GRAPH launches eagerly instead). The
+ * workspace grows with the batch: about 56 MB per image for extremum candidates and their
+ * ordering, plus 20 bytes per pyramid pixel. No counterpart in the reference, whose callers loop
+ * over images (cli.rs:2093-2113, stitch.rs:493-499). */
 typedef struct {
     uint64_t offset;
     uint32_t height, width, row_stride, _pad;
@@ -264,6 +267,15 @@ ps_status ps_engine_extract_batch_u8(ps_engine *e, const uint8_t *d_pixels, cons
  * the process. *n_tuned (may be null) gets the number of level classes tuned. */
 ps_status ps_engine_autotune_blur(ps_engine *e, const uint32_t *hw, uint32_t n_sizes, const ps_sift_config *cfg,
                                   uint32_t *n_tuned);
+/* Grouped-launch autotuner for a batch (images as for ps_engine_extract_batch_u8): times
+ * each launch shape of every grouped blur launch of the batch (tile heights 16 / 32 / 64,
+ * sliding-window chunks of 64 / 128 / 192 rows) and both extrema tile heights (16, 32 rows),
+ * in dependency order on the real images, and keeps the fastest per grouped-launch class
+ * (radius, source kind, mean level size class, image count class) for all later batches in
+ * the process. *n_tuned (may be null) gets the number of launch classes tuned. */
+ps_status ps_engine_autotune_batch(ps_engine *e, const uint8_t *d_pixels, const ps_image_desc *images,
+                                   uint32_t n_images, const ps_sift_config *cfg, uint32_t *n_tuned);
+/* Forgets both tuned tables (single-level and grouped). */
 void ps_autotune_reset(void);
 uint32_t ps_autotune_entries(void);
 /* Entry i of the tuned table: radius, source mode, height class, width class (floor(2 log2 of

@@ -7,6 +7,8 @@
 // exponential (v_exp_f32, within 2 ulp) and square root (v_sqrt_f32, within 1 ulp) per pixel, sinf/cosf, a polynomial atan2, fused multiply-adds
 // and the parallel histogram accumulation make these results tolerance-level (not bit-level)
 // equal to the reference.
+#include <algorithm>
+
 #include "ps_internal.h"
 
 #pragma clang fp contract(fast)
@@ -91,13 +93,11 @@ __device__ __forceinline__ void grad(const LocalKp &l, int x, int y, float &m, f
 constexpr int kDescThreads = 256;
 constexpr int kCopies = 32;   // 4 waves x 8 lane groups
 constexpr int kBatch = 4;
-__global__ __launch_bounds__(kDescThreads) void orient_kernel(PyrView pv, const Cand *kps, const uint32_t *nkp,
-                                                              uint32_t max_kp, float *ori, uint32_t *nori) {
+__device__ __forceinline__ void orient_body(const PyrView &pv, const Cand *kps, uint32_t k, float *ori,
+                                            uint32_t *nori) {
     __shared__ float s_whist[kCopies][37];   // +1 pad: copies of a bin sit in different LDS banks
     __shared__ float s_hist[36];
-    const uint32_t k = blockIdx.x;
     const int lane = threadIdx.x;
-    if (k >= min(*nkp, max_kp)) return;
     const Cand kp = kps[k];
     const LocalKp l = local_kp(pv, kp);
 
@@ -202,6 +202,13 @@ __global__ __launch_bounds__(kDescThreads) void orient_kernel(PyrView pv, const 
     if (lane == 0) nori[k] = (uint32_t)min(npeaks, 4);
 }
 
+__global__ __launch_bounds__(kDescThreads) void orient_kernel(PyrView pv, const Cand *kps, const uint32_t *nkp,
+                                                              uint32_t max_kp, float *ori, uint32_t *nori) {
+    const uint32_t k = blockIdx.x;
+    if (k >= min(*nkp, max_kp)) return;
+    orient_body(pv, kps, k, ori, nori);
+}
+
 hipError_t launch_orientations(const PyrView &pv, const Cand *kps, const uint32_t *nkp, uint32_t max_kp, float *ori,
                                uint32_t *nori, hipStream_t st) {
     if (max_kp == 0) return hipSuccess;
@@ -212,9 +219,8 @@ hipError_t launch_orientations(const PyrView &pv, const Cand *kps, const uint32_
 // Flattens (keypoint, orientation) pairs in keypoint order, each keypoint's orientations in
 // peak order (flat_map_iter, pipeline.rs:1566-1571), truncated to cap (pipeline.rs:1654-1656).
 constexpr int kScanThreads = 1024;
-__global__ __launch_bounds__(kScanThreads) void feature_scan_kernel(const uint32_t *nkp, const uint32_t *nori,
-                                                                    uint32_t max_kp, uint32_t cap, uint32_t *feat_kp,
-                                                                    uint32_t *feat_j, uint32_t *nfeat) {
+__device__ __forceinline__ void feature_scan_body(const uint32_t *nkp, const uint32_t *nori, uint32_t max_kp,
+                                                  uint32_t cap, uint32_t *feat_kp, uint32_t *feat_j, uint32_t *nfeat) {
     __shared__ uint32_t s[kScanThreads];
     const uint32_t n = min(*nkp, max_kp);
     const uint32_t per = (max_kp + kScanThreads - 1) / kScanThreads;
@@ -240,6 +246,12 @@ __global__ __launch_bounds__(kScanThreads) void feature_scan_kernel(const uint32
     if (threadIdx.x == kScanThreads - 1) *nfeat = min(s[kScanThreads - 1], cap);
 }
 
+__global__ __launch_bounds__(kScanThreads) void feature_scan_kernel(const uint32_t *nkp, const uint32_t *nori,
+                                                                    uint32_t max_kp, uint32_t cap, uint32_t *feat_kp,
+                                                                    uint32_t *feat_j, uint32_t *nfeat) {
+    feature_scan_body(nkp, nori, max_kp, cap, feat_kp, feat_j, nfeat);
+}
+
 hipError_t launch_feature_scan(const uint32_t *nkp, const uint32_t *nori, uint32_t max_kp, uint32_t cap,
                                uint32_t *feat_kp, uint32_t *feat_j, uint32_t *nfeat, hipStream_t st) {
     feature_scan_kernel<<<1, kScanThreads, 0, st>>>(nkp, nori, max_kp, cap, feat_kp, feat_j, nfeat);
@@ -257,16 +269,13 @@ hipError_t launch_feature_scan(const uint32_t *nkp, const uint32_t *nori, uint32
 constexpr int kLanesPerCell = 10;
 constexpr int kPrivStride = 33;   // 32 accumulators + 1 pad
 constexpr int kListCap = 12;      // box pixels tested per lane before the recorded ones are processed
-__global__ __launch_bounds__(kDescThreads) void descriptor_kernel(PyrView pv, const Cand *kps, const float *ori,
-                                                                  const uint32_t *feat_kp, const uint32_t *feat_j,
-                                                                  const uint32_t *nfeat, uint32_t cap, int kind,
-                                                                  ps_feature *out_f, float *out_d) {
+__device__ __forceinline__ void descriptor_body(const PyrView &pv, const Cand *kps, const float *ori,
+                                                const uint32_t *feat_kp, const uint32_t *feat_j, uint32_t f, int kind,
+                                                ps_feature *out_f, float *out_d) {
     __shared__ float s_priv[kDescThreads * kPrivStride];
     __shared__ uint16_t s_list[kDescThreads * kListCap];
     __shared__ float s_desc[128];
-    const uint32_t f = blockIdx.x;
     const int lane = threadIdx.x;
-    if (f >= min(*nfeat, cap)) return;
     const uint32_t k = feat_kp[f];
     const Cand kp = kps[k];
     const float theta = ori[(size_t)k * 4 + feat_j[f]];
@@ -448,6 +457,15 @@ __global__ __launch_bounds__(kDescThreads) void descriptor_kernel(PyrView pv, co
     }
 }
 
+__global__ __launch_bounds__(kDescThreads) void descriptor_kernel(PyrView pv, const Cand *kps, const float *ori,
+                                                                  const uint32_t *feat_kp, const uint32_t *feat_j,
+                                                                  const uint32_t *nfeat, uint32_t cap, int kind,
+                                                                  ps_feature *out_f, float *out_d) {
+    const uint32_t f = blockIdx.x;
+    if (f >= min(*nfeat, cap)) return;
+    descriptor_body(pv, kps, ori, feat_kp, feat_j, f, kind, out_f, out_d);
+}
+
 hipError_t launch_descriptors(const PyrView &pv, const Cand *kps, const float *ori, const uint32_t *feat_kp,
                               const uint32_t *feat_j, const uint32_t *nfeat, uint32_t cap, int kind, ps_feature *out_f,
                               float *out_d, hipStream_t st) {
@@ -468,6 +486,109 @@ __global__ void cand_to_kp_kernel(const Cand *c, const uint32_t *n, uint32_t cap
 hipError_t launch_cand_to_keypoints(const Cand *c, const uint32_t *n, uint32_t cap, ps_keypoint *out, hipStream_t st) {
     if (cap == 0) return hipSuccess;
     cand_to_kp_kernel<<<(cap + 255) / 256, 256, 0, st>>>(c, n, cap, out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ grouped persistent launches
+// The orientation and descriptor stages of many images in one launch each: the keypoints (or
+// features) of all images are numbered image by image through device-side offsets, and a grid of
+// resident workgroups walks them -- one keypoint or feature at a time per workgroup, as in the
+// single-image kernels, whose bodies these launches run.
+
+// off[i] = sum over j < i of min(counts[j * stride], cap); off[n] = the total. One workgroup.
+__global__ __launch_bounds__(kScanThreads) void group_offsets_kernel(const uint32_t *counts, uint32_t stride,
+                                                                     uint32_t cap, uint32_t n, uint32_t *off) {
+    __shared__ uint32_t s[kScanThreads];
+    const uint32_t per = (n + kScanThreads - 1) / kScanThreads;
+    const uint32_t b = threadIdx.x * per, e = min(b + per, n);
+    uint32_t local = 0;
+    for (uint32_t i = b; i < e; i++) local += min(counts[(size_t)i * stride], cap);
+    s[threadIdx.x] = local;
+    __syncthreads();
+    for (int d = 1; d < kScanThreads; d <<= 1) {
+        uint32_t v = threadIdx.x >= (unsigned)d ? s[threadIdx.x - d] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t o = s[threadIdx.x] - local;
+    for (uint32_t i = b; i < e; i++) {
+        off[i] = o;
+        o += min(counts[(size_t)i * stride], cap);
+    }
+    if (threadIdx.x == kScanThreads - 1) off[n] = s[kScanThreads - 1];
+}
+
+hipError_t launch_group_offsets(const uint32_t *counts, uint32_t stride, uint32_t cap, uint32_t n, uint32_t *off,
+                                hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    group_offsets_kernel<<<1, kScanThreads, 0, st>>>(counts, stride, cap, n, off);
+    return hipGetLastError();
+}
+
+// image of global item g: the last i with off[i] <= g (off non-decreasing, off[0] = 0)
+__device__ __forceinline__ uint32_t find_off(const uint32_t *off, uint32_t n, uint32_t g) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kDescThreads) void orient_group_kernel(const DescJob *jobs, uint32_t n_jobs,
+                                                                    const uint32_t *kp_off) {
+    const uint32_t total = kp_off[n_jobs];
+    uint32_t jb = blockIdx.x < total ? find_off(kp_off, n_jobs, blockIdx.x) : 0;
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+        while (jb + 1 < n_jobs && kp_off[jb + 1] <= g) jb++;
+        const DescJob &J = jobs[jb];
+        orient_body(J.pv, J.kps, g - kp_off[jb], J.ori, J.nori);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kScanThreads) void feature_scan_group_kernel(const DescJob *jobs, uint32_t max_kp,
+                                                                          uint32_t cap) {
+    const DescJob &J = jobs[blockIdx.x];
+    feature_scan_body(J.nkp, J.nori, max_kp, cap, J.feat_kp, J.feat_j, J.nfeat);
+}
+
+__global__ __launch_bounds__(kDescThreads) void descriptor_group_kernel(const DescJob *jobs, uint32_t n_jobs,
+                                                                        const uint32_t *feat_off, int kind) {
+    const uint32_t total = feat_off[n_jobs];
+    uint32_t jb = blockIdx.x < total ? find_off(feat_off, n_jobs, blockIdx.x) : 0;
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+        while (jb + 1 < n_jobs && feat_off[jb + 1] <= g) jb++;
+        const DescJob &J = jobs[jb];
+        descriptor_body(J.pv, J.kps, J.ori, J.feat_kp, J.feat_j, g - feat_off[jb], kind, J.out_f, J.out_d);
+        __syncthreads();
+    }
+}
+
+static uint32_t resident_grid(const void *fn, int threads) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, 0) != hipSuccess)
+        return 0;
+    return (uint32_t)std::max(per, 1) * (uint32_t)std::max(cus, 1);
+}
+
+hipError_t launch_describe_group(const DescJob *jobs, uint32_t n_jobs, uint32_t max_kp, uint32_t cap, int kind,
+                                 uint32_t *kp_off, uint32_t *feat_off, const uint32_t *kp_counts,
+                                 uint32_t kp_stride, const uint32_t *feat_counts, hipStream_t st) {
+    if (n_jobs == 0) return hipSuccess;
+    static const uint32_t g_ori = resident_grid((const void *)orient_group_kernel, kDescThreads);
+    static const uint32_t g_desc = resident_grid((const void *)descriptor_group_kernel, kDescThreads);
+    if (!g_ori || !g_desc) return hipErrorInvalidValue;
+    hipError_t e;
+    if ((e = launch_group_offsets(kp_counts, kp_stride, max_kp, n_jobs, kp_off, st)) != hipSuccess) return e;
+    orient_group_kernel<<<g_ori, kDescThreads, 0, st>>>(jobs, n_jobs, kp_off);
+    feature_scan_group_kernel<<<n_jobs, kScanThreads, 0, st>>>(jobs, max_kp, cap);
+    if ((e = launch_group_offsets(feat_counts, 1, cap, n_jobs, feat_off, st)) != hipSuccess) return e;
+    descriptor_group_kernel<<<g_desc, kDescThreads, 0, st>>>(jobs, n_jobs, feat_off, kind);
     return hipGetLastError();
 }
 

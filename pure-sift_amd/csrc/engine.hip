@@ -1030,8 +1030,7 @@ ps_status ps_engine_profile_end(ps_engine *e, float stage_ms[PS_NUM_STAGES], uin
 ps_status ps_engine_check(ps_engine *e) {
     if (!e) return set_err(PS_E_INVALID_ARG, "null engine");
     PS_HIP(hipSetDevice(e->device));
-    for (ps_engine *c : e->children)
-        if (c->cnt.p) PS_TRY(check_err_word(c));
+    PS_TRY(check_batch_err(e));
     if (!e->cnt.p) {
         PS_HIP(hipStreamSynchronize(e->stream));
         return PS_OK;

@@ -28,6 +28,8 @@ enum AuxSlot {
     AUX_COUNT
 };
 
+struct BatchState;   // batch.hip
+
 struct ps_engine {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -44,10 +46,9 @@ struct ps_engine {
     DevBuf part, part2, bidx, bdist, sdist, ridx, ridx_d, fin;  // matcher scratch
     DevBuf aux[AUX_COUNT];
     uint32_t nn_gen = 0;                           // per-call tag of the exact matcher's scratch words
-    // batched extraction: child engines (own stream + workspace each) run images
-    // concurrently; the last batch is kept as an instantiated graph
-    std::vector<ps_engine *> children;
-    std::vector<hipEvent_t> batch_events;
+    // batched extraction: the grouped launches of the last batch (their job tables and
+    // workspace), kept with an instantiated graph of the sequence
+    BatchState *batch = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     std::vector<uint64_t> graph_key;
@@ -85,6 +86,8 @@ T *as(DevBuf &b) {
 ps_status engine_new(int device, ps_engine **out);
 void engine_free(ps_engine *e);
 void release_batch(ps_engine *e);
+// error words of the last batch's images (checked and cleared like check_err_word)
+ps_status check_batch_err(ps_engine *e);
 ps_status validate_cfg(const ps_sift_config *c, char *msg, size_t cap);
 ps_status validate_opt(const ps_match_options *o, char *msg, size_t cap);
 // device address of the no-allocation matcher's static scratch on `device` (set up once per

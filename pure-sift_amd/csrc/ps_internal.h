@@ -102,6 +102,56 @@ hipError_t launch_descriptors(const PyrView &pv, const Cand *kps, const float *o
 hipError_t launch_cand_to_keypoints(const Cand *c, const uint32_t *n, uint32_t cap, ps_keypoint *out,
                                     hipStream_t st);
 
+// ---- grouped persistent launches (batched extraction): one stage of many images per launch.
+// Units (tiles / strips) of a launch are numbered job by job: job j owns units
+// [unit0_j, unit0_{j+1}); jobs are sorted by unit0 and job 0 starts at unit 0.
+struct BlurJob {
+    BlurArgs a;
+    uint32_t units_x;   // units per row: ceil(w / 64)
+    uint32_t unit0;
+};
+// units of one level under a launch shape (see launch_blur)
+uint32_t group_units(int h, int w, int shape);
+// center: every job writes a DoG or source plane (dst_d / dst_src)
+hipError_t launch_blur_group(int radius, int src_mode, const BlurJob *jobs, uint32_t n_jobs, uint32_t n_units,
+                             const Taps &t, int shape, bool center, hipStream_t st);
+struct ExtremaJob {
+    const float *gauss;   // Gaussian level 0 of the octave (levels 0..S+1 follow, h x w each)
+    Cand *out;
+    uint32_t *count, *err;
+    int h, w;
+    uint32_t units_x, unit0;
+};
+uint32_t extrema_group_units(int h, int w, int rows);
+// one octave index of many images (same config: same thresholds, scales and taps); rows 16 or 32
+hipError_t launch_extrema_group(const ExtremaJob *jobs, uint32_t n_jobs, uint32_t n_units, int S, int rows, float thr,
+                                float mult, const float *eff_scale, double log2k, int octave, uint32_t cap,
+                                const Taps &top_taps, int top_r, hipStream_t st);
+struct RankJob {
+    const Cand *cand;
+    const uint32_t *count;
+    Cand *out;
+    uint32_t *out_count;
+    void *scratch;        // rank_scratch_bytes(cap_in)
+};
+hipError_t launch_rank_group(const RankJob *jobs, uint32_t n_jobs, uint32_t cap_in, uint32_t limit, hipStream_t st);
+struct DescJob {
+    PyrView pv;
+    const Cand *kps;          // ordered keypoints
+    const uint32_t *nkp;      // their count
+    float *ori;               // 4 per keypoint
+    uint32_t *nori;
+    uint32_t *feat_kp, *feat_j;
+    uint32_t *nfeat;          // the image's feature count (written)
+    ps_feature *out_f;
+    float *out_d;
+};
+// orientations, feature flattening and descriptors of many images: kp_off / feat_off get the
+// n_jobs + 1 exclusive offsets of min(kp_counts[i * kp_stride], max_kp) and of the features
+hipError_t launch_describe_group(const DescJob *jobs, uint32_t n_jobs, uint32_t max_kp, uint32_t cap, int kind,
+                                 uint32_t *kp_off, uint32_t *feat_off, const uint32_t *kp_counts,
+                                 uint32_t kp_stride, const uint32_t *feat_counts, hipStream_t st);
+
 // match.hip
 // nq_max <= 262144 (1024 blocks of 256 queries)
 size_t match_finish_scratch_bytes(size_t nq_max);

@@ -7,6 +7,9 @@
 // (no FMA: -ffp-contract=off plus the pragma below), Gaussian weights come from
 // the host (glibc expf, exactly pipeline.rs:275-296), and the extremum tests /
 // refinement repeat the reference's f32 expressions operation by operation.
+#include <algorithm>
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 #include "ps_internal.h"
@@ -65,8 +68,10 @@ __device__ __forceinline__ float load_src(const BlurArgs &a, int y, int x) {
     }
 }
 
+// One output tile at (x0, y0): the body shared by the single-level launch and the grouped
+// (many image levels per launch) persistent launch.
 template <int R, int SRC, int kTH>
-__global__ __launch_bounds__(kBlurThreads) void blur_kernel(BlurArgs a, Taps taps) {
+__device__ __forceinline__ void blur_tile(const BlurArgs &a, const Taps &taps, int x0, int y0) {
     constexpr int IW = kTW + 2 * R;
     constexpr int IWP = (IW + 3) & ~3;
     constexpr int IH = kTH + 2 * R;
@@ -74,8 +79,6 @@ __global__ __launch_bounds__(kBlurThreads) void blur_kernel(BlurArgs a, Taps tap
     __shared__ __attribute__((aligned(16))) float s_in[IH * IWP];
     __shared__ __attribute__((aligned(16))) float s_tmp[IH * kTW];
     const int tid = threadIdx.x;
-    const uint32_t tile = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-    const int x0 = (int)(tile % gridDim.x) * kTW, y0 = (int)(tile / gridDim.x) * kTH;
     const int h = a.h, w = a.w;
 
     // stage the clamped window: all loads of a thread are issued before its LDS stores so
@@ -155,6 +158,12 @@ __global__ __launch_bounds__(kBlurThreads) void blur_kernel(BlurArgs a, Taps tap
     }
 }
 
+template <int R, int SRC, int kTH>
+__global__ __launch_bounds__(kBlurThreads) void blur_kernel(BlurArgs a, Taps taps) {
+    const uint32_t tile = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    blur_tile<R, SRC, kTH>(a, taps, (int)(tile % gridDim.x) * kTW, (int)(tile / gridDim.x) * kTH);
+}
+
 // Sliding-window form of the same blur for large levels. A workgroup owns a 64-column strip
 // and `chunk` output rows of it and walks down in steps of kSlideTH rows: each step stages
 // kSlideTH source rows (with the 2R-column halo), runs the horizontal pass once for them into
@@ -172,7 +181,7 @@ constexpr int kSlideTH = kBlurSlideStep;
 // first level of octaves >= 1, which materialises G0): only then are the ring rows' centre
 // values kept, so the common case needs a third less LDS and fits more workgroups per CU.
 template <int R, int SRC, bool CENTER>
-__global__ __launch_bounds__(kBlurThreads) void blur_slide_kernel(BlurArgs a, Taps taps, int chunk) {
+__device__ __forceinline__ void blur_strip(const BlurArgs &a, const Taps &taps, int x0, int y_start, int chunk) {
     constexpr int NT = 2 * R + 1;
     constexpr int IW = kTW + 2 * R;
     constexpr int IWP = (IW + 3) & ~3;
@@ -189,9 +198,6 @@ __global__ __launch_bounds__(kBlurThreads) void blur_slide_kernel(BlurArgs a, Ta
     // the wave index is uniform: readfirstlane lets row arithmetic stay scalar
     const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int h = a.h, w = a.w;
-    const uint32_t tile = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-    const int x0 = (int)(tile % gridDim.x) * kTW;
-    const int y_start = (int)(tile / gridDim.x) * chunk;
     const int y_end = min(y_start + chunk, h);
     const int n_steps = (y_end - y_start + TH - 1) / TH + P;
     const int hs = y_start - R;   // source row of horizontal-pass row 0
@@ -273,6 +279,12 @@ __global__ __launch_bounds__(kBlurThreads) void blur_slide_kernel(BlurArgs a, Ta
             }
         }
     }
+}
+
+template <int R, int SRC, bool CENTER>
+__global__ __launch_bounds__(kBlurThreads) void blur_slide_kernel(BlurArgs a, Taps taps, int chunk) {
+    const uint32_t tile = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    blur_strip<R, SRC, CENTER>(a, taps, (int)(tile % gridDim.x) * kTW, (int)(tile / gridDim.x) * chunk, chunk);
 }
 
 template <int R>
@@ -454,11 +466,13 @@ constexpr long kExBigLevel = 1L << 20;   // levels of at least this many pixels 
 constexpr int top_radius(int S) {
     return S == 3 ? 10 : S == 4 ? 8 : S == 5 ? 7 : S == 6 ? 6 : S <= 9 ? 5 : 4;
 }
+// One tile whose first interior pixel is (x0, y0): the body shared by the per-octave launch and
+// the grouped persistent launch.
 template <int PL, int kExRows>
-__global__ __launch_bounds__(256) void extrema_kernel(const float *gauss, int h, int w, float thr, float mult,
-                                                      EffScale eff, double log2k, int octave, Cand *out,
-                                                      uint32_t *count, uint32_t cap, uint32_t *err, Taps top,
-                                                      int top_r) {
+__device__ __forceinline__ void extrema_tile(const float *gauss, int h, int w, float thr, float mult,
+                                             const EffScale &eff, double log2k, int octave, Cand *out,
+                                             uint32_t *count, uint32_t cap, uint32_t *err, const Taps &top,
+                                             int x0, int y0) {
     constexpr int S = PL - 2;
     constexpr int kExH = kExRows + 2;
     constexpr int per_plane = kExH * kExW;
@@ -479,10 +493,6 @@ __global__ __launch_bounds__(256) void extrema_kernel(const float *gauss, int h,
     __shared__ uint16_t s_top[256];
     __shared__ float s_hrow[TB * HR * 3];
     __shared__ float s_up[TB * 9];
-    // XCD-contiguous tile order (as the blur kernels): horizontally neighbouring tiles share
-    // the cache lines of their halo columns, and land on the same XCD's L2
-    const uint32_t tile = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-    const int x0 = 1 + (int)(tile % gridDim.x) * 64, y0 = 1 + (int)(tile / gridDim.x) * kExRows;
     const size_t P = (size_t)h * w;
     if (threadIdx.x == 0) s_qn = 0;
     int nan_seen = 0;
@@ -755,6 +765,17 @@ __global__ __launch_bounds__(256) void extrema_kernel(const float *gauss, int h,
     }
 }
 
+template <int PL, int kExRows>
+__global__ __launch_bounds__(256) void extrema_kernel(const float *gauss, int h, int w, float thr, float mult,
+                                                      EffScale eff, double log2k, int octave, Cand *out,
+                                                      uint32_t *count, uint32_t cap, uint32_t *err, Taps top) {
+    // XCD-contiguous tile order (as the blur kernels): horizontally neighbouring tiles share
+    // the cache lines of their halo columns, and land on the same XCD's L2
+    const uint32_t tile = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    extrema_tile<PL, kExRows>(gauss, h, w, thr, mult, eff, log2k, octave, out, count, cap, err, top,
+                              1 + (int)(tile % gridDim.x) * 64, 1 + (int)(tile / gridDim.x) * kExRows);
+}
+
 hipError_t launch_extrema(const float *gauss_octave, int h, int w, int S, float thr, float mult, const float *eff_scale,
                           double log2k, int octave, Cand *out, uint32_t *count, uint32_t cap, uint32_t *err,
                           const Taps &top_taps, int top_r, hipStream_t st) {
@@ -769,10 +790,10 @@ hipError_t launch_extrema(const float *gauss_octave, int h, int w, int S, float 
     case SS:                                                                                                  \
         if (big)                                                                                              \
             extrema_kernel<SS + 2, 32><<<grid, 256, 0, st>>>(gauss_octave, h, w, thr, mult, e, log2k, octave,  \
-                                                             out, count, cap, err, top_taps, top_r);          \
+                                                             out, count, cap, err, top_taps);                 \
         else                                                                                                  \
             extrema_kernel<SS + 2, 16><<<grid, 256, 0, st>>>(gauss_octave, h, w, thr, mult, e, log2k, octave,  \
-                                                             out, count, cap, err, top_taps, top_r);          \
+                                                             out, count, cap, err, top_taps);                 \
         break;
     switch (S) {
         PS_EXTREMA_CASE(3) PS_EXTREMA_CASE(4) PS_EXTREMA_CASE(5) PS_EXTREMA_CASE(6)
@@ -797,8 +818,8 @@ constexpr int kRankGrid = 24;
 
 // scratch words: [0] subset size, [1] threshold bits T, [2] by-response flag
 constexpr uint32_t kSelLds = 12288;   // candidates whose |response| bits fit in LDS for the 4 passes
-__global__ __launch_bounds__(1024) void rank_select_kernel(const Cand *c, const uint32_t *count, uint32_t cap_in,
-                                                           uint32_t limit, uint32_t *words) {
+__device__ __forceinline__ void rank_select_body(const Cand *c, const uint32_t *count, uint32_t cap_in,
+                                                 uint32_t limit, uint32_t *words) {
     __shared__ uint32_t hist[256];
     __shared__ uint32_t s_prefix, s_need;
     __shared__ uint32_t s_bits[kSelLds];
@@ -890,6 +911,11 @@ __global__ __launch_bounds__(1024) void rank_select_kernel(const Cand *c, const 
     if (threadIdx.x == 0) { words[1] = s_prefix; words[2] = 1; }
 }
 
+__global__ __launch_bounds__(1024) void rank_select_kernel(const Cand *c, const uint32_t *count, uint32_t cap_in,
+                                                           uint32_t limit, uint32_t *words) {
+    rank_select_body(c, count, cap_in, limit, words);
+}
+
 // compact record of a subset member for the rank count
 struct RankRec {
     float r;
@@ -899,13 +925,11 @@ struct RankRec {
 
 // one workgroup reservation per 256 candidates (ballot prefix), subset in an arbitrary but
 // complete order; the rank count does not depend on it
-__global__ __launch_bounds__(256) void rank_gather_kernel(const Cand *c, const uint32_t *count, uint32_t cap_in,
-                                                          uint32_t limit, uint32_t *subset, RankRec *rec,
-                                                          uint32_t *rank, uint32_t *words) {
+// candidates [i0, i0 + 256) of n
+__device__ __forceinline__ void rank_gather_block(const Cand *c, uint32_t n, uint32_t limit, uint32_t *subset,
+                                                  RankRec *rec, uint32_t *rank, uint32_t *words, uint32_t i0) {
     __shared__ uint32_t s_w[4], s_base;
-    const uint32_t n = min(*count, cap_in);
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x * blockDim.x >= n) return;
+    const uint32_t i = i0 + threadIdx.x;
     bool take = false;
     Cand ci;
     if (i < n) {
@@ -933,23 +957,32 @@ __global__ __launch_bounds__(256) void rank_gather_kernel(const Cand *c, const u
     rec[pos] = r;
 }
 
+__global__ __launch_bounds__(256) void rank_gather_kernel(const Cand *c, const uint32_t *count, uint32_t cap_in,
+                                                          uint32_t limit, uint32_t *subset, RankRec *rec,
+                                                          uint32_t *rank, uint32_t *words) {
+    const uint32_t n = min(*count, cap_in);
+    if (blockIdx.x * blockDim.x >= n) return;
+    rank_gather_block(c, n, limit, subset, rec, rank, words, blockIdx.x * blockDim.x);
+}
+
 // i-tiles of 256 (one per thread) against j-chunks of kRankJ staged in LDS; a fixed grid
 // walks the (i-tile, j-chunk) pairs
 constexpr int kRankJ = 64;
-__global__ __launch_bounds__(kRankTile) void rank_count_kernel(const RankRec *rec, const uint32_t *words,
-                                                               uint32_t *rank) {
+// this workgroup's share (bx of gx i-tiles, by of gy j-chunks) of one image's rank counts
+__device__ __forceinline__ void rank_count_body(const RankRec *rec, const uint32_t *words, uint32_t *rank,
+                                                uint32_t bx, uint32_t gx, uint32_t by, uint32_t gy) {
     __shared__ float s_r[kRankJ];
     __shared__ uint64_t s_k[kRankJ];
     const uint32_t m = words[0];
     const bool by_resp = words[2] != 0;
     const uint32_t ntile = (m + kRankTile - 1) / kRankTile, nj = (m + kRankJ - 1) / kRankJ;
-    for (uint32_t ti = blockIdx.x; ti < ntile; ti += gridDim.x) {
+    for (uint32_t ti = bx; ti < ntile; ti += gx) {
         const uint32_t i = ti * kRankTile + threadIdx.x;
         float ri = 0.0f;
         uint64_t ki = 0;
         if (i < m) { RankRec a = rec[i]; ri = a.r; ki = a.key; }
         uint32_t acc = 0;
-        for (uint32_t tj = blockIdx.y; tj < nj; tj += gridDim.y) {
+        for (uint32_t tj = by; tj < nj; tj += gy) {
             __syncthreads();
             if (threadIdx.x < kRankJ) {
                 const uint32_t j = tj * kRankJ + threadIdx.x;
@@ -974,12 +1007,25 @@ __global__ __launch_bounds__(kRankTile) void rank_count_kernel(const RankRec *re
     }
 }
 
-__global__ void rank_scatter_kernel(const Cand *c, const uint32_t *subset, const uint32_t *words, uint32_t limit,
-                                    const uint32_t *rank, Cand *out, uint32_t *out_count) {
-    const uint32_t m = words[0];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kRankTile) void rank_count_kernel(const RankRec *rec, const uint32_t *words,
+                                                               uint32_t *rank) {
+    rank_count_body(rec, words, rank, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
+}
+
+__device__ __forceinline__ void rank_scatter_one(const Cand *c, const uint32_t *subset, uint32_t m, uint32_t limit,
+                                                 const uint32_t *rank, Cand *out, uint32_t *out_count, uint32_t i) {
     if (i == 0) *out_count = min(m, limit);
     if (i < m && rank[i] < limit) out[rank[i]] = c[subset[i]];
+}
+
+__global__ void rank_scatter_kernel(const Cand *c, const uint32_t *subset, const uint32_t *words, uint32_t limit,
+                                    const uint32_t *rank, Cand *out, uint32_t *out_count) {
+    rank_scatter_one(c, subset, words[0], limit, rank, out, out_count, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// scratch of one image's ordering: rank[cap_in], subset[cap_in], 64 words, RankRec[cap_in]
+__device__ __forceinline__ uint32_t *rank_words(void *scratch, uint32_t cap_in) {
+    return static_cast<uint32_t *>(scratch) + 2 * (size_t)cap_in;
 }
 
 size_t rank_scratch_bytes(uint32_t cap_in) {
@@ -997,6 +1043,266 @@ hipError_t launch_rank(const Cand *cand, const uint32_t *count, uint32_t cap_in,
     rank_gather_kernel<<<blocks, 256, 0, st>>>(cand, count, cap_in, limit, subset, rec, rank, words);
     rank_count_kernel<<<dim3(kRankGrid, 4 * kRankGrid), kRankTile, 0, st>>>(rec, words, rank);
     rank_scatter_kernel<<<blocks, 256, 0, st>>>(cand, subset, words, limit, rank, out, out_count);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ grouped persistent launches
+// A grouped launch runs one stage for many image levels at once (a level of every image of a
+// batch). Its units -- output tiles, sliding-window strips, extrema tiles -- are numbered job by
+// job (job j owns units [unit0_j, unit0_{j+1})), and a grid of at most as many workgroups as the
+// device holds at once, a multiple of the 8 XCDs, walks them: XCD k owns one contiguous range of
+// units and its workgroups stride through it, so units whose halos overlap run at about the same
+// time under the same L2. The per-unit bodies are the single-level kernels' own, so every image
+// gets exactly the values of its own launches.
+__device__ __forceinline__ void unit_walk(uint32_t total, uint32_t &u, uint32_t &end, uint32_t &step) {
+    const uint32_t k = blockIdx.x % kXcds, q = total / kXcds, r = total % kXcds;
+    const uint32_t lo = k * q + min(k, r);
+    end = lo + q + (k < r ? 1u : 0u);
+    u = lo + blockIdx.x / kXcds;
+    step = gridDim.x / kXcds;
+}
+
+// the job owning unit u: the last one with unit0 <= u (jobs sorted by unit0, job 0 at 0)
+template <typename J>
+__device__ __forceinline__ uint32_t find_job(const J *jobs, uint32_t n, uint32_t u) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].unit0 <= u) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// a workgroup's units only increase, so after the first search the job only moves forward
+template <typename J>
+__device__ __forceinline__ uint32_t next_job(const J *jobs, uint32_t n, uint32_t jb, uint32_t u) {
+    while (jb + 1 < n && jobs[jb + 1].unit0 <= u) jb++;
+    return jb;
+}
+
+template <int R, int SRC, int kTH>
+__global__ __launch_bounds__(kBlurThreads) void blur_group_kernel(const BlurJob *jobs, uint32_t n_jobs,
+                                                                  uint32_t total, Taps taps) {
+    uint32_t u, end, step;
+    unit_walk(total, u, end, step);
+    if (u >= end) return;
+    uint32_t jb = find_job(jobs, n_jobs, u);
+    for (; u < end; u += step) {
+        jb = next_job(jobs, n_jobs, jb, u);
+        const BlurArgs a = jobs[jb].a;
+        const uint32_t t = u - jobs[jb].unit0, nx = jobs[jb].units_x;
+        blur_tile<R, SRC, kTH>(a, taps, (int)(t % nx) * kTW, (int)(t / nx) * kTH);
+        __syncthreads();   // the next unit re-stages the LDS tiles
+    }
+}
+
+template <int R, int SRC, bool CENTER>
+__global__ __launch_bounds__(kBlurThreads) void blur_strip_group_kernel(const BlurJob *jobs, uint32_t n_jobs,
+                                                                        uint32_t total, Taps taps, int chunk) {
+    uint32_t u, end, step;
+    unit_walk(total, u, end, step);
+    if (u >= end) return;
+    uint32_t jb = find_job(jobs, n_jobs, u);
+    for (; u < end; u += step) {
+        jb = next_job(jobs, n_jobs, jb, u);
+        const BlurArgs a = jobs[jb].a;
+        const uint32_t t = u - jobs[jb].unit0, nx = jobs[jb].units_x;
+        blur_strip<R, SRC, CENTER>(a, taps, (int)(t % nx) * kTW, (int)(t / nx) * chunk, chunk);
+        __syncthreads();
+    }
+}
+
+// Workgroups a grouped launch of `fn` uses for n_units units: all that fit on the device at
+// once (occupancy x CUs, queried once per kernel), at most one per unit, rounded to the XCDs.
+static uint32_t group_grid(const void *fn, int threads, uint32_t n_units) {
+    static std::mutex mu;
+    static std::map<const void *, uint32_t> resident;
+    uint32_t cap = 0;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = resident.find(fn);
+        if (it != resident.end()) {
+            cap = it->second;
+        } else {
+            int dev = 0, cus = 0, per = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, 0) != hipSuccess)
+                return 0;
+            cap = (uint32_t)std::max(per, 1) * (uint32_t)std::max(cus, 1);
+            resident[fn] = cap;
+        }
+    }
+    uint32_t g = std::min(cap, (n_units + kXcds - 1) / kXcds * kXcds);
+    return std::max(g / kXcds * kXcds, kXcds);
+}
+
+template <int R, int SRC>
+static hipError_t blur_group_src(const BlurJob *jobs, uint32_t n_jobs, uint32_t n_units, const Taps &t, int shape,
+                                 bool center, hipStream_t st) {
+    if (shape & kBlurSlide) {
+        const int chunk = shape & (kBlurSlide - 1);
+        if (chunk <= 0 || chunk % kSlideTH) return hipErrorInvalidValue;
+        const void *fn = center ? (const void *)blur_strip_group_kernel<R, SRC, true>
+                                : (const void *)blur_strip_group_kernel<R, SRC, false>;
+        const uint32_t g = group_grid(fn, kBlurThreads, n_units);
+        if (!g) return hipErrorInvalidValue;
+        if (center) blur_strip_group_kernel<R, SRC, true><<<g, kBlurThreads, 0, st>>>(jobs, n_jobs, n_units, t, chunk);
+        else blur_strip_group_kernel<R, SRC, false><<<g, kBlurThreads, 0, st>>>(jobs, n_jobs, n_units, t, chunk);
+        return hipGetLastError();
+    }
+#define PS_GROUP_TH(TH)                                                                                     \
+    case TH: {                                                                                              \
+        const uint32_t g = group_grid((const void *)blur_group_kernel<R, SRC, TH>, kBlurThreads, n_units);  \
+        if (!g) return hipErrorInvalidValue;                                                                \
+        blur_group_kernel<R, SRC, TH><<<g, kBlurThreads, 0, st>>>(jobs, n_jobs, n_units, t);                \
+        return hipGetLastError();                                                                           \
+    }
+    switch (shape) {
+        PS_GROUP_TH(16) PS_GROUP_TH(32) PS_GROUP_TH(64)
+    default: return hipErrorInvalidValue;
+    }
+#undef PS_GROUP_TH
+}
+
+template <int R>
+static hipError_t blur_group_dispatch(int src_mode, const BlurJob *jobs, uint32_t n_jobs, uint32_t n_units,
+                                      const Taps &t, int shape, bool center, hipStream_t st) {
+    switch (src_mode) {
+    case SRC_F32: return blur_group_src<R, SRC_F32>(jobs, n_jobs, n_units, t, shape, center, st);
+    case SRC_DOWN2: return blur_group_src<R, SRC_DOWN2>(jobs, n_jobs, n_units, t, shape, center, st);
+    case SRC_U8:
+    case SRC_U8_UP:
+        if constexpr (R == 4 || R == 5) {
+            if (src_mode == SRC_U8) return blur_group_src<R, SRC_U8>(jobs, n_jobs, n_units, t, shape, center, st);
+            return blur_group_src<R, SRC_U8_UP>(jobs, n_jobs, n_units, t, shape, center, st);
+        } else {
+            return hipErrorInvalidValue;
+        }
+    default: return hipErrorInvalidValue;
+    }
+}
+
+uint32_t group_units(int h, int w, int shape) {
+    if (h <= 0 || w <= 0) return 0;
+    const int rows = (shape & kBlurSlide) ? (shape & (kBlurSlide - 1)) : shape;
+    if (rows <= 0) return 0;
+    return (uint32_t)((w + kTW - 1) / kTW) * (uint32_t)((h + rows - 1) / rows);
+}
+
+hipError_t launch_blur_group(int radius, int src_mode, const BlurJob *jobs, uint32_t n_jobs, uint32_t n_units,
+                             const Taps &t, int shape, bool center, hipStream_t st) {
+    if (n_jobs == 0 || n_units == 0) return hipSuccess;
+    switch (radius) {
+    case 1: return blur_group_dispatch<1>(src_mode, jobs, n_jobs, n_units, t, shape, center, st);
+    case 2: return blur_group_dispatch<2>(src_mode, jobs, n_jobs, n_units, t, shape, center, st);
+    case 3: return blur_group_dispatch<3>(src_mode, jobs, n_jobs, n_units, t, shape, center, st);
+    case 4: return blur_group_dispatch<4>(src_mode, jobs, n_jobs, n_units, t, shape, center, st);
+    case 5: return blur_group_dispatch<5>(src_mode, jobs, n_jobs, n_units, t, shape, center, st);
+    case 6: return blur_group_dispatch<6>(src_mode, jobs, n_jobs, n_units, t, shape, center, st);
+    case 7: return blur_group_dispatch<7>(src_mode, jobs, n_jobs, n_units, t, shape, center, st);
+    case 8: return blur_group_dispatch<8>(src_mode, jobs, n_jobs, n_units, t, shape, center, st);
+    case 9: return blur_group_dispatch<9>(src_mode, jobs, n_jobs, n_units, t, shape, center, st);
+    case 10: return blur_group_dispatch<10>(src_mode, jobs, n_jobs, n_units, t, shape, center, st);
+    case 11: return blur_group_dispatch<11>(src_mode, jobs, n_jobs, n_units, t, shape, center, st);
+    case 12: return blur_group_dispatch<12>(src_mode, jobs, n_jobs, n_units, t, shape, center, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int PL, int kExRows>
+__global__ __launch_bounds__(256) void extrema_group_kernel(const ExtremaJob *jobs, uint32_t n_jobs, uint32_t total,
+                                                            float thr, float mult, EffScale eff, double log2k,
+                                                            int octave, uint32_t cap, Taps top) {
+    uint32_t u, end, step;
+    unit_walk(total, u, end, step);
+    if (u >= end) return;
+    uint32_t jb = find_job(jobs, n_jobs, u);
+    for (; u < end; u += step) {
+        jb = next_job(jobs, n_jobs, jb, u);
+        const ExtremaJob J = jobs[jb];
+        const uint32_t t = u - J.unit0;
+        extrema_tile<PL, kExRows>(J.gauss, J.h, J.w, thr, mult, eff, log2k, octave, J.out, J.count, cap, J.err, top,
+                                  1 + (int)(t % J.units_x) * 64, 1 + (int)(t / J.units_x) * kExRows);
+        __syncthreads();
+    }
+}
+
+uint32_t extrema_group_units(int h, int w, int rows) {
+    if (h < 3 || w < 3 || rows <= 0) return 0;
+    return (uint32_t)((w - 2 + 63) / 64) * (uint32_t)((h - 2 + rows - 1) / rows);
+}
+
+hipError_t launch_extrema_group(const ExtremaJob *jobs, uint32_t n_jobs, uint32_t n_units, int S, int rows, float thr,
+                                float mult, const float *eff_scale, double log2k, int octave, uint32_t cap,
+                                const Taps &top_taps, int top_r, hipStream_t st) {
+    if (n_jobs == 0 || n_units == 0) return hipSuccess;
+    if (S < 3 || S > 10 || top_r != top_radius(S) || (rows != 16 && rows != 32)) return hipErrorInvalidValue;
+    EffScale e;
+    for (int i = 0; i < kMaxLevels; i++) e.v[i] = eff_scale[i];
+#define PS_EXG(SS, RR)                                                                                          \
+    {                                                                                                           \
+        const uint32_t g = group_grid((const void *)extrema_group_kernel<SS + 2, RR>, 256, n_units);           \
+        if (!g) return hipErrorInvalidValue;                                                                    \
+        extrema_group_kernel<SS + 2, RR><<<g, 256, 0, st>>>(jobs, n_jobs, n_units, thr, mult, e, log2k, octave, \
+                                                            cap, top_taps);                                     \
+        return hipGetLastError();                                                                               \
+    }
+#define PS_EXG_CASE(SS)          \
+    case SS:                     \
+        if (rows == 32) PS_EXG(SS, 32) \
+        else PS_EXG(SS, 16)
+    switch (S) {
+        PS_EXG_CASE(3) PS_EXG_CASE(4) PS_EXG_CASE(5) PS_EXG_CASE(6)
+        PS_EXG_CASE(7) PS_EXG_CASE(8) PS_EXG_CASE(9) PS_EXG_CASE(10)
+    default: return hipErrorInvalidValue;
+    }
+#undef PS_EXG_CASE
+#undef PS_EXG
+}
+
+// ordering of many images: one select workgroup per image, then fixed grids per image
+constexpr uint32_t kGatherGroup = 64, kScatterGroup = 16, kCountGroup = 8;
+
+__global__ __launch_bounds__(1024) void rank_select_group_kernel(const RankJob *jobs, uint32_t cap_in,
+                                                                 uint32_t limit) {
+    const RankJob J = jobs[blockIdx.x];
+    rank_select_body(J.cand, J.count, cap_in, limit, rank_words(J.scratch, cap_in));
+}
+
+__global__ __launch_bounds__(256) void rank_gather_group_kernel(const RankJob *jobs, uint32_t cap_in, uint32_t limit) {
+    const RankJob J = jobs[blockIdx.y];
+    const uint32_t n = min(*J.count, cap_in);
+    uint32_t *rank = static_cast<uint32_t *>(J.scratch), *subset = rank + cap_in, *words = subset + cap_in;
+    RankRec *rec = reinterpret_cast<RankRec *>(words + 64);
+    for (uint32_t i0 = blockIdx.x * 256; i0 < n; i0 += gridDim.x * 256) {
+        rank_gather_block(J.cand, n, limit, subset, rec, rank, words, i0);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kRankTile) void rank_count_group_kernel(const RankJob *jobs, uint32_t cap_in) {
+    const RankJob J = jobs[blockIdx.z];
+    uint32_t *rank = static_cast<uint32_t *>(J.scratch), *words = rank + 2 * cap_in;
+    rank_count_body(reinterpret_cast<const RankRec *>(words + 64), words, rank, blockIdx.x, gridDim.x, blockIdx.y,
+                    gridDim.y);
+}
+
+__global__ void rank_scatter_group_kernel(const RankJob *jobs, uint32_t cap_in, uint32_t limit) {
+    const RankJob J = jobs[blockIdx.y];
+    const uint32_t *rank = static_cast<const uint32_t *>(J.scratch), *subset = rank + cap_in, *words = subset + cap_in;
+    const uint32_t m = words[0];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < max(m, 1u); i += gridDim.x * blockDim.x)
+        rank_scatter_one(J.cand, subset, m, limit, rank, J.out, J.out_count, i);
+}
+
+hipError_t launch_rank_group(const RankJob *jobs, uint32_t n_jobs, uint32_t cap_in, uint32_t limit, hipStream_t st) {
+    if (n_jobs == 0) return hipSuccess;
+    rank_select_group_kernel<<<n_jobs, 1024, 0, st>>>(jobs, cap_in, limit);
+    rank_gather_group_kernel<<<dim3(kGatherGroup, n_jobs), 256, 0, st>>>(jobs, cap_in, limit);
+    rank_count_group_kernel<<<dim3(kCountGroup, kCountGroup, n_jobs), kRankTile, 0, st>>>(jobs, cap_in);
+    rank_scatter_group_kernel<<<dim3(kScatterGroup, n_jobs), 256, 0, st>>>(jobs, cap_in, limit);
     return hipGetLastError();
 }
 

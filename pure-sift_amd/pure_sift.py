@@ -199,6 +199,7 @@ def lib() -> C.CDLL:
         L.ps_engine_extract_batch_u8.argtypes = [vp, vp, vp, C.c_uint32, C.POINTER(_SiftConfigC), vp, vp, sz, vp,
                                                  C.c_uint32]
         L.ps_engine_autotune_blur.argtypes = [vp, vp, C.c_uint32, C.POINTER(_SiftConfigC), u32p]
+        L.ps_engine_autotune_batch.argtypes = [vp, vp, vp, C.c_uint32, C.POINTER(_SiftConfigC), u32p]
         L.ps_autotune_reset.argtypes = []
         L.ps_autotune_reset.restype = None
         L.ps_autotune_entries.argtypes = []
@@ -245,7 +246,7 @@ def lib() -> C.CDLL:
         L.ps_catalog_search.argtypes = [vp, vp, vp, sz, sz, sz, C.c_float, C.c_float, vp, sz, C.POINTER(sz)]
         L.ps_engine_catalog_search.argtypes = [vp, vp, vp, vp, sz, sz, sz, C.c_float, C.c_float, vp, sz,
                                                C.POINTER(sz)]
-        for name in ("ps_engine_extract_batch_u8", "ps_engine_autotune_blur", "ps_engine_mlp_features", "ps_engine_mlp_forward", "ps_engine_mlp_grad", "ps_engine_mlp_adam",
+        for name in ("ps_engine_extract_batch_u8", "ps_engine_autotune_blur", "ps_engine_autotune_batch", "ps_engine_mlp_features", "ps_engine_mlp_forward", "ps_engine_mlp_grad", "ps_engine_mlp_adam",
                      "ps_mlp_forward", "ps_mlp_loss_grad", "ps_homography_estimate", "ps_ransac_batch", "ps_engine_ransac_batch", "ps_nn2_precision", "ps_engine_nn2", "ps_quantize_u128", "ps_count_coarse_hamming_votes", "ps_hamming_topk", "ps_rerank_l2",
                      "ps_engine_quantize_u128", "ps_engine_hamming_topk", "ps_engine_rerank_l2","ps_init", "ps_validate_sift_config", "ps_validate_match_options", "ps_detect_and_compute",
                      "ps_detect_and_compute_u8", "ps_detect_keypoints", "ps_match", "ps_build_pyramid", "ps_nn2",
@@ -820,6 +821,14 @@ class Engine:
         c = cfg._c()
         n = C.c_uint32(0)
         check(lib().ps_engine_autotune_blur(self.h, hw.ctypes.data, len(hw), C.byref(c), C.byref(n)))
+        return n.value
+
+    def autotune_batch(self, d_pixels: int, images: np.ndarray, cfg: SiftConfig) -> int:
+        """Times the grouped launch shapes of this batch (images as for extract_batch_u8)."""
+        im = np.ascontiguousarray(images, IMAGE_DESC_DTYPE)
+        c = cfg._c()
+        n = C.c_uint32(0)
+        check(lib().ps_engine_autotune_batch(self.h, d_pixels, im.ctypes.data, len(im), C.byref(c), C.byref(n)))
         return n.value
 
     def nn2(self, d_q: int, d_nq: int, nq_max: int, d_t: int, d_nt: int, nt_max: int, precision: int,

@@ -1,9 +1,10 @@
-"""Batched extraction (concurrent child streams, captured HIP graph) and the blur-tile
-autotuner, through the C ABI.
+"""Batched extraction (grouped persistent launches, captured HIP graph) and the blur-shape
+autotuners, through the C ABI.
 
 Bars: every image of a batch gets exactly the features, descriptors and count of a
 single-image extraction of the same pixels (graph capture, graph replay and eager launch
-alike); after autotuning, the pyramid is still bit-exact with the oracle."""
+alike, with default and with autotuned grouped launch shapes); after single-level
+autotuning, the pyramid is still bit-exact with the oracle."""
 import ctypes
 
 import numpy as np
@@ -14,8 +15,9 @@ import synth
 
 pytestmark = pytest.mark.gpu
 
+# (1, 40) has no octave at all (its upsampled height is 2 < 3): it must come out empty
 SIZES = [(240, 320), (241, 333), (480, 640), (77, 1500), (1000, 300), (768, 1024), (240, 320), (513, 515),
-         (900, 1700), (333, 777), (16, 16), (1536, 2048)]
+         (900, 1700), (333, 777), (16, 16), (1, 40), (1536, 2048)]
 
 
 def _pack(images, pad):
@@ -51,11 +53,15 @@ def test_batch_equals_single(gpu):
                        ref_c.data_ptr() + 4 * i)
     eng.sync()
     rc = ref_c.cpu().numpy()
-    assert rc.min() >= 0 and rc.max() > 100
+    assert rc.min() >= 0 and rc.max() > 100 and rc[SIZES.index((1, 40))] == 0
     bf = torch.zeros_like(ref_f)
     bd = torch.zeros_like(ref_d)
     bc = torch.zeros_like(ref_c)
-    for flags in (0, 0, gpu.BATCH_NO_GRAPH):     # capture, replay, eager
+    runs = [0, 0, gpu.BATCH_NO_GRAPH, "tune", 0, 0, gpu.BATCH_NO_GRAPH]   # capture, replay, eager; tuned
+    for flags in runs:
+        if flags == "tune":
+            assert eng.autotune_batch(tpix.data_ptr(), im, cfg) > 0
+            continue
         bf.zero_()
         bd.zero_()
         bc.fill_(-1)
@@ -72,6 +78,7 @@ def test_batch_equals_single(gpu):
             assert np.array_equal(F[i, :k * fsz], RF[i, :k * fsz]), (flags, i)
             assert np.array_equal(D[i, :k], RD[i, :k]), (flags, i)
     eng.close()
+    gpu.lib().ps_autotune_reset()
 
 
 def test_autotuned_pyramid_still_bit_exact(gpu):
@@ -129,7 +136,7 @@ def test_full_256_image_batch(gpu):
     cfg = gpu.SiftConfig().octaves(4).scales(3).contrast_threshold(0.015)
     eng = gpu.Engine(0)
     try:
-        assert eng.autotune_blur([(h, w) for (w, h) in sizes], cfg) > 0
+        assert eng.autotune_batch(pix.data_ptr(), np.array(descs, gpu.IMAGE_DESC_DTYPE), cfg) > 0
         ref_f = torch.zeros((n, cap * fsz), dtype=torch.uint8, device=dev)
         ref_d = torch.zeros((n, cap, 128), dtype=torch.float32, device=dev)
         ref_c = torch.zeros(n, dtype=torch.int32, device=dev)
