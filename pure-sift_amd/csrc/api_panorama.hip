@@ -15,11 +15,11 @@ namespace pshost {
 // grouped exact ratio match on the engine stream (scratch: the caller's workspace slot)
 ps_status enqueue_match_sets(ps_engine *e, const ps::MatchSetsArgs &a, DevBuf &scratch, ps_feature_match *out,
                              uint32_t *counts) {
-    void *const before = scratch.p;
-    PS_TRY(ensure(scratch, ps::match_sets_scratch_bytes(a)));
+    bool fresh = false;
+    PS_TRY(ensure(scratch, ps::match_sets_scratch_bytes(a), &fresh));
     if (++e->nn_gen == 0) e->nn_gen = 1;
     // fresh memory: clear the two norm-bound words at its head (later calls clear them in turn)
-    if (scratch.p != before) PS_HIP(ps::launch_fill_u32(static_cast<uint32_t *>(scratch.p), 0u, 4, e->stream));
+    if (fresh) PS_HIP(ps::launch_fill_u32(static_cast<uint32_t *>(scratch.p), 0u, 4, e->stream));
     PS_HIP(ps::launch_match_sets(a, scratch.p, e->nn_gen, out, counts, e->stream));
     return PS_OK;
 }

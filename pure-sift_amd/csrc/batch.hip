@@ -23,6 +23,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -304,14 +305,14 @@ ps_status build_batch(ps_engine *e, BatchState &B, const uint8_t *d_pixels, cons
     }
     if (!B.full) return PS_OK;
     // ordering and description
-    PS_TRY(ensure(B.rank, (size_t)n * ps::rank_scratch_bytes(ps::kCandCap)));
+    const size_t rank_bytes = ps::rank_scratch_bytes(ps::kCandCap);
+    PS_TRY(ensure(B.rank, (size_t)n * rank_bytes));
+    PS_TRY(ensure(B.off, (size_t)2 * (n + 1) * sizeof(uint32_t)));
     PS_TRY(ensure(B.sorted, (size_t)n * kMax * sizeof(ps::Cand)));
     PS_TRY(ensure(B.ori, (size_t)n * kMax * 4 * sizeof(float)));
     PS_TRY(ensure(B.nori, (size_t)n * kMax * sizeof(uint32_t)));
     PS_TRY(ensure(B.fkp, (size_t)n * kMax * sizeof(uint32_t)));
     PS_TRY(ensure(B.fj, (size_t)n * kMax * sizeof(uint32_t)));
-    PS_TRY(ensure(B.off, (size_t)2 * (n + 1) * sizeof(uint32_t)));
-    const size_t rank_bytes = ps::rank_scratch_bytes(ps::kCandCap);
     std::vector<ps::RankJob> rj(n);
     std::vector<ps::DescJob> dj(n);
     for (uint32_t i = 0; i < n; i++) {
@@ -346,7 +347,6 @@ ps_status build_batch(ps_engine *e, BatchState &B, const uint8_t *d_pixels, cons
         d.out_f = d_feat + (size_t)i * cap;
         d.out_d = d_desc + (size_t)i * cap * 128;
     }
-    B.fixed.clear();
     B.t_rank = put_table(B.fixed, rj);
     B.t_desc = put_table(B.fixed, dj);
     return PS_OK;
@@ -469,9 +469,10 @@ ps_status prepare_batch(ps_engine *e, const uint8_t *d_pixels, const ps_image_de
     if (B.key != key) {
         B.key.clear();
         PS_HIP(hipStreamSynchronize(e->stream));   // an earlier batch may still use the workspace
-        void *const cnt_before = B.cnt.p;
+        // every counter word of the new images starts at zero, the error words included: the
+        // words of an earlier batch's images (or fresh memory) mean nothing for these images
         PS_TRY(ensure(B.cnt, (size_t)std::max(n, 1u) * kCntWords * sizeof(uint32_t)));
-        if (B.cnt.p != cnt_before) PS_HIP(launch_clear_counters(as<uint32_t>(B.cnt), std::max(n, 1u), kCntWords, e->stream));
+        PS_HIP(launch_clear_counters(as<uint32_t>(B.cnt), std::max(n, 1u), kCntWords, e->stream));
         PS_TRY(build_batch(e, B, d_pixels, im, n, cfg, d_feat, d_desc, cap, d_counts));
         B.key = key;
     }
@@ -513,7 +514,10 @@ ps_status check_batch_err(ps_engine *e) {
         PS_HIP(launch_clear_counters(as<uint32_t>(B->cnt), B->n, kCntWords, e->stream));
         PS_HIP(hipStreamSynchronize(e->stream));
     }
-    if (err & ps::kErrCandOverflow) return set_err(PS_E_CAPACITY, "more than 2^20 extremum candidates in one image");
+    for (uint32_t i = 0; i < B->n; i++) {
+        const std::string where = " (image " + std::to_string(i) + " of the batch)";
+        PS_TRY(err_status(&words[(size_t)i * kCntWords], where.c_str()));
+    }
     return PS_OK;
 }
 

@@ -68,7 +68,7 @@ __device__ __forceinline__ LocalKp local_kp(const PyrView &pv, const Cand &kp) {
     int layer = min((int)kp.layer, pv.S + 2);
     l.W = pv.w[oct];
     l.H = pv.h[oct];
-    l.g = pv.base + pv.off[oct] + (size_t)layer * l.W * l.H;
+    l.g = global_ptr(pv.base) + pv.off[oct] + (size_t)layer * l.W * l.H;
     float inv = 1.0f / pv.scale_factor[oct];
     l.x = kp.x * inv;
     l.y = kp.y * inv;
@@ -94,10 +94,10 @@ constexpr int kDescThreads = 256;
 constexpr int kCopies = 32;   // 4 waves x 8 lane groups
 constexpr int kBatch = 4;
 __device__ __forceinline__ void orient_body(const PyrView &pv, const Cand *kps, uint32_t k, float *ori,
-                                            uint32_t *nori) {
+                                            uint32_t *nori, int tid) {
     __shared__ float s_whist[kCopies][37];   // +1 pad: copies of a bin sit in different LDS banks
     __shared__ float s_hist[36];
-    const int lane = threadIdx.x;
+    const int lane = tid;
     const Cand kp = kps[k];
     const LocalKp l = local_kp(pv, kp);
 
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(kDescThreads) void orient_kernel(PyrView pv, const 
                                                               uint32_t max_kp, float *ori, uint32_t *nori) {
     const uint32_t k = blockIdx.x;
     if (k >= min(*nkp, max_kp)) return;
-    orient_body(pv, kps, k, ori, nori);
+    orient_body(pv, kps, k, ori, nori, threadIdx.x);
 }
 
 hipError_t launch_orientations(const PyrView &pv, const Cand *kps, const uint32_t *nkp, uint32_t max_kp, float *ori,
@@ -271,11 +271,11 @@ constexpr int kPrivStride = 33;   // 32 accumulators + 1 pad
 constexpr int kListCap = 12;      // box pixels tested per lane before the recorded ones are processed
 __device__ __forceinline__ void descriptor_body(const PyrView &pv, const Cand *kps, const float *ori,
                                                 const uint32_t *feat_kp, const uint32_t *feat_j, uint32_t f, int kind,
-                                                ps_feature *out_f, float *out_d) {
+                                                ps_feature *out_f, float *out_d, int tid) {
     __shared__ float s_priv[kDescThreads * kPrivStride];
     __shared__ uint16_t s_list[kDescThreads * kListCap];
     __shared__ float s_desc[128];
-    const int lane = threadIdx.x;
+    const int lane = tid;
     const uint32_t k = feat_kp[f];
     const Cand kp = kps[k];
     const float theta = ori[(size_t)k * 4 + feat_j[f]];
@@ -463,7 +463,7 @@ __global__ __launch_bounds__(kDescThreads) void descriptor_kernel(PyrView pv, co
                                                                   ps_feature *out_f, float *out_d) {
     const uint32_t f = blockIdx.x;
     if (f >= min(*nfeat, cap)) return;
-    descriptor_body(pv, kps, ori, feat_kp, feat_j, f, kind, out_f, out_d);
+    descriptor_body(pv, kps, ori, feat_kp, feat_j, f, kind, out_f, out_d, threadIdx.x);
 }
 
 hipError_t launch_descriptors(const PyrView &pv, const Cand *kps, const float *ori, const uint32_t *feat_kp,
@@ -495,14 +495,17 @@ hipError_t launch_cand_to_keypoints(const Cand *c, const uint32_t *n, uint32_t c
 // resident workgroups walks them -- one keypoint or feature at a time per workgroup, as in the
 // single-image kernels, whose bodies these launches run.
 
-// off[i] = sum over j < i of min(counts[j * stride], cap); off[n] = the total. One workgroup.
+// off[i] = sum over j < i of ceil(min(counts[j * stride], cap) / div); off[n] = the total. One
+// workgroup.
 __global__ __launch_bounds__(kScanThreads) void group_offsets_kernel(const uint32_t *counts, uint32_t stride,
-                                                                     uint32_t cap, uint32_t n, uint32_t *off) {
+                                                                     uint32_t cap, uint32_t div, uint32_t n,
+                                                                     uint32_t *off) {
     __shared__ uint32_t s[kScanThreads];
     const uint32_t per = (n + kScanThreads - 1) / kScanThreads;
     const uint32_t b = threadIdx.x * per, e = min(b + per, n);
+    auto item = [&](uint32_t i) { return (min(counts[(size_t)i * stride], cap) + div - 1) / div; };
     uint32_t local = 0;
-    for (uint32_t i = b; i < e; i++) local += min(counts[(size_t)i * stride], cap);
+    for (uint32_t i = b; i < e; i++) local += item(i);
     s[threadIdx.x] = local;
     __syncthreads();
     for (int d = 1; d < kScanThreads; d <<= 1) {
@@ -514,55 +517,48 @@ __global__ __launch_bounds__(kScanThreads) void group_offsets_kernel(const uint3
     uint32_t o = s[threadIdx.x] - local;
     for (uint32_t i = b; i < e; i++) {
         off[i] = o;
-        o += min(counts[(size_t)i * stride], cap);
+        o += item(i);
     }
     if (threadIdx.x == kScanThreads - 1) off[n] = s[kScanThreads - 1];
 }
 
-hipError_t launch_group_offsets(const uint32_t *counts, uint32_t stride, uint32_t cap, uint32_t n, uint32_t *off,
-                                hipStream_t st) {
-    if (n == 0) return hipSuccess;
-    group_offsets_kernel<<<1, kScanThreads, 0, st>>>(counts, stride, cap, n, off);
+hipError_t launch_group_offsets(const uint32_t *counts, uint32_t stride, uint32_t cap, uint32_t div, uint32_t n,
+                                uint32_t *off, hipStream_t st) {
+    if (n == 0 || div == 0) return n ? hipErrorInvalidValue : hipSuccess;
+    group_offsets_kernel<<<1, kScanThreads, 0, st>>>(counts, stride, cap, div, n, off);
     return hipGetLastError();
 }
 
-// image of global item g: the last i with off[i] <= g (off non-decreasing, off[0] = 0)
-__device__ __forceinline__ uint32_t find_off(const uint32_t *off, uint32_t n, uint32_t g) {
-    uint32_t lo = 0, hi = n;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (off[mid] <= g) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
-__global__ __launch_bounds__(kDescThreads) void orient_group_kernel(const DescJob *jobs, uint32_t n_jobs,
-                                                                    const uint32_t *kp_off) {
+__global__ __launch_bounds__(kDescThreads) void orient_group_kernel(const DescJob *__restrict__ jobs, uint32_t n_jobs,
+                                                                    const uint32_t *__restrict__ kp_off) {
     const uint32_t total = kp_off[n_jobs];
     uint32_t jb = blockIdx.x < total ? find_off(kp_off, n_jobs, blockIdx.x) : 0;
+#pragma clang loop unroll(disable)
     for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
         while (jb + 1 < n_jobs && kp_off[jb + 1] <= g) jb++;
         const DescJob &J = jobs[jb];
-        orient_body(J.pv, J.kps, g - kp_off[jb], J.ori, J.nori);
+        orient_body(J.pv, global_ptr(J.kps), g - kp_off[jb], global_ptr(J.ori), global_ptr(J.nori), opaque_tid());
         __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(kScanThreads) void feature_scan_group_kernel(const DescJob *jobs, uint32_t max_kp,
+__global__ __launch_bounds__(kScanThreads) void feature_scan_group_kernel(const DescJob *__restrict__ jobs, uint32_t max_kp,
                                                                           uint32_t cap) {
     const DescJob &J = jobs[blockIdx.x];
-    feature_scan_body(J.nkp, J.nori, max_kp, cap, J.feat_kp, J.feat_j, J.nfeat);
+    feature_scan_body(global_ptr(J.nkp), global_ptr(J.nori), max_kp, cap, global_ptr(J.feat_kp), global_ptr(J.feat_j),
+                      global_ptr(J.nfeat));
 }
 
-__global__ __launch_bounds__(kDescThreads) void descriptor_group_kernel(const DescJob *jobs, uint32_t n_jobs,
-                                                                        const uint32_t *feat_off, int kind) {
+__global__ __launch_bounds__(kDescThreads) void descriptor_group_kernel(const DescJob *__restrict__ jobs, uint32_t n_jobs,
+                                                                        const uint32_t *__restrict__ feat_off, int kind) {
     const uint32_t total = feat_off[n_jobs];
     uint32_t jb = blockIdx.x < total ? find_off(feat_off, n_jobs, blockIdx.x) : 0;
+#pragma clang loop unroll(disable)
     for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
         while (jb + 1 < n_jobs && feat_off[jb + 1] <= g) jb++;
         const DescJob &J = jobs[jb];
-        descriptor_body(J.pv, J.kps, J.ori, J.feat_kp, J.feat_j, g - feat_off[jb], kind, J.out_f, J.out_d);
+        descriptor_body(J.pv, global_ptr(J.kps), global_ptr(J.ori), global_ptr(J.feat_kp), global_ptr(J.feat_j),
+                        g - feat_off[jb], kind, global_ptr(J.out_f), global_ptr(J.out_d), opaque_tid());
         __syncthreads();
     }
 }
@@ -584,10 +580,10 @@ hipError_t launch_describe_group(const DescJob *jobs, uint32_t n_jobs, uint32_t 
     static const uint32_t g_desc = resident_grid((const void *)descriptor_group_kernel, kDescThreads);
     if (!g_ori || !g_desc) return hipErrorInvalidValue;
     hipError_t e;
-    if ((e = launch_group_offsets(kp_counts, kp_stride, max_kp, n_jobs, kp_off, st)) != hipSuccess) return e;
+    if ((e = launch_group_offsets(kp_counts, kp_stride, max_kp, 1, n_jobs, kp_off, st)) != hipSuccess) return e;
     orient_group_kernel<<<g_ori, kDescThreads, 0, st>>>(jobs, n_jobs, kp_off);
     feature_scan_group_kernel<<<n_jobs, kScanThreads, 0, st>>>(jobs, max_kp, cap);
-    if ((e = launch_group_offsets(feat_counts, 1, cap, n_jobs, feat_off, st)) != hipSuccess) return e;
+    if ((e = launch_group_offsets(feat_counts, 1, cap, 1, n_jobs, feat_off, st)) != hipSuccess) return e;
     descriptor_group_kernel<<<g_desc, kDescThreads, 0, st>>>(jobs, n_jobs, feat_off, kind);
     return hipGetLastError();
 }

@@ -121,7 +121,8 @@ bool make_plan(int h, int w, const ps_sift_config &cfg, PyrPlan &p) {
 
 namespace pshost {
 
-ps_status ensure(DevBuf &b, size_t bytes) {
+ps_status ensure(DevBuf &b, size_t bytes, bool *fresh) {
+    if (fresh) *fresh = false;
     if (bytes == 0) bytes = 16;
     if (b.n >= bytes) return PS_OK;
     if (b.p) PS_HIP(hipFree(b.p));
@@ -130,15 +131,18 @@ ps_status ensure(DevBuf &b, size_t bytes) {
     bytes = (bytes + 255) & ~(size_t)255;
     PS_HIP(hipMalloc(&b.p, bytes));
     b.n = bytes;
+    // (a new allocation can have the address of the one just freed: callers that must clear
+    // fresh memory ask for this flag instead of comparing pointers)
+    if (fresh) *fresh = true;
     return PS_OK;
 }
 
 // The engine's counter words; a fresh buffer is cleared once so that the error word, which
 // only ps_engine_check / clear_err_word reset, starts at zero.
 ps_status ensure_cnt(ps_engine *e) {
-    void *const before = e->cnt.p;
-    PS_TRY(ensure(e->cnt, 64));
-    if (e->cnt.p != before) PS_HIP(ps::launch_fill_u32(as<uint32_t>(e->cnt), 0u, 16, e->stream));
+    bool fresh = false;
+    PS_TRY(ensure(e->cnt, 64, &fresh));
+    if (fresh) PS_HIP(ps::launch_fill_u32(as<uint32_t>(e->cnt), 0u, 16, e->stream));
     return PS_OK;
 }
 
@@ -496,11 +500,11 @@ ps_status enqueue_nn(ps_engine *e, const float *qd, const uint32_t *nq, size_t n
                                        e->stream));
         return PS_OK;
     }
-    void *const before = scratch.p;
-    PS_TRY(ensure(scratch, ps::nn_mfma_scratch_bytes(nqm, ntm)));
+    bool fresh = false;
+    PS_TRY(ensure(scratch, ps::nn_mfma_scratch_bytes(nqm, ntm), &fresh));
     if (++e->nn_gen == 0) e->nn_gen = 1;
     // fresh memory: clear the two norm-bound words at its head (later calls clear them in turn)
-    if (scratch.p != before) PS_HIP(ps::launch_fill_u32(static_cast<uint32_t *>(scratch.p), 0u, 4, e->stream));
+    if (fresh) PS_HIP(ps::launch_fill_u32(static_cast<uint32_t *>(scratch.p), 0u, 4, e->stream));
     PS_HIP(ps::launch_nn_mfma(qd, nq, nqm, td, nt, ntm, scratch.p, bidx, bd, sd, e->nn_gen, e->stream));
     return PS_OK;
 }
@@ -542,17 +546,22 @@ ps_status clear_err_word(ps_engine *e) {
     return PS_OK;
 }
 
+ps_status err_status(const uint32_t *words, const char *where) {
+    if (!(words[kErrWord] & ps::kErrCandOverflow)) return PS_OK;
+    char buf[160];
+    snprintf(buf, sizeof buf, "more than 2^20 extremum candidates in one image%s (%u found)", where, words[0]);
+    return set_err(PS_E_CAPACITY, buf);
+}
+
 ps_status check_err_word(ps_engine *e) {
-    uint32_t err = 0;
-    PS_HIP(hipMemcpyAsync(&err, as<uint32_t>(e->cnt) + kErrWord, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                          e->stream));
+    uint32_t words[16] = {};
+    PS_HIP(hipMemcpyAsync(words, as<uint32_t>(e->cnt), sizeof words, hipMemcpyDeviceToHost, e->stream));
     PS_HIP(hipStreamSynchronize(e->stream));
-    if (err) {
+    if (words[kErrWord]) {
         PS_TRY(clear_err_word(e));
         PS_HIP(hipStreamSynchronize(e->stream));
     }
-    if (err & ps::kErrCandOverflow) return set_err(PS_E_CAPACITY, "more than 2^20 extremum candidates in one image");
-    return PS_OK;
+    return err_status(words, "");
 }
 
 template <typename T>

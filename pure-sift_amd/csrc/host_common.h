@@ -73,9 +73,13 @@ constexpr int kErrWord = 15;
 ps_status ensure_cnt(ps_engine *e);
 ps_status clear_err_word(ps_engine *e);
 ps_status check_err_word(ps_engine *e);
+// the error of a 16-word counter block (PS_OK without overflow bits); `where` names the image
+ps_status err_status(const uint32_t *words, const char *where);
 
 ps_status set_err(ps_status s, const std::string &msg);
-ps_status ensure(DevBuf &b, size_t bytes);
+// grows b to at least `bytes` (contents are not kept); *fresh (optional) tells whether memory
+// was allocated by this call
+ps_status ensure(DevBuf &b, size_t bytes, bool *fresh = nullptr);
 ps_status check_fail_injection();
 
 template <typename T>

@@ -146,6 +146,43 @@ struct DescJob {
     ps_feature *out_f;
     float *out_d;
 };
+// A pointer read from a job table is generic to the compiler, so its accesses would be flat
+// loads and stores with 64-bit per-lane addresses. The round trip through the global address
+// space lets address-space inference make them global accesses (scalar base + lane offset), as
+// for kernel-argument pointers.
+template <typename T>
+__device__ __forceinline__ T *global_ptr(T *p) {
+    return (T *)(__attribute__((address_space(1))) T *)(uintptr_t)p;
+}
+__device__ __forceinline__ BlurArgs global_args(BlurArgs a) {
+    a.src = global_ptr(a.src);
+    a.dst_g = global_ptr(a.dst_g);
+    a.dst_d = global_ptr(a.dst_d);
+    a.dst_src = global_ptr(a.dst_src);
+    return a;
+}
+// threadIdx.x through an opaque move: in a persistent loop it keeps the compiler from hoisting
+// every thread-index-derived address of the loop body out of the loop, where they would all
+// stay live (more than doubling the registers of a tile body) for the whole walk
+__device__ __forceinline__ int opaque_tid() {
+    int v;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "v"((int)threadIdx.x));
+    return v;
+}
+// off[i] = sum over j < i of ceil(min(counts[j * stride], cap) / div), off[n] the total
+hipError_t launch_group_offsets(const uint32_t *counts, uint32_t stride, uint32_t cap, uint32_t div, uint32_t n,
+                                uint32_t *off, hipStream_t st);
+// item g of a grouped walk belongs to image find_off(off, n, g): the last i with off[i] <= g
+// (off non-decreasing, off[0] = 0)
+__device__ __forceinline__ uint32_t find_off(const uint32_t *off, uint32_t n, uint32_t g) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
 // orientations, feature flattening and descriptors of many images: kp_off / feat_off get the
 // n_jobs + 1 exclusive offsets of min(kp_counts[i * kp_stride], max_kp) and of the features
 hipError_t launch_describe_group(const DescJob *jobs, uint32_t n_jobs, uint32_t max_kp, uint32_t cap, int kind,

@@ -71,14 +71,13 @@ __device__ __forceinline__ float load_src(const BlurArgs &a, int y, int x) {
 // One output tile at (x0, y0): the body shared by the single-level launch and the grouped
 // (many image levels per launch) persistent launch.
 template <int R, int SRC, int kTH>
-__device__ __forceinline__ void blur_tile(const BlurArgs &a, const Taps &taps, int x0, int y0) {
+__device__ __forceinline__ void blur_tile(const BlurArgs &a, const Taps &taps, int x0, int y0, int tid) {
     constexpr int IW = kTW + 2 * R;
     constexpr int IWP = (IW + 3) & ~3;
     constexpr int IH = kTH + 2 * R;
     constexpr int NT = 2 * R + 1;
     __shared__ __attribute__((aligned(16))) float s_in[IH * IWP];
     __shared__ __attribute__((aligned(16))) float s_tmp[IH * kTW];
-    const int tid = threadIdx.x;
     const int h = a.h, w = a.w;
 
     // stage the clamped window: all loads of a thread are issued before its LDS stores so
@@ -161,7 +160,7 @@ __device__ __forceinline__ void blur_tile(const BlurArgs &a, const Taps &taps, i
 template <int R, int SRC, int kTH>
 __global__ __launch_bounds__(kBlurThreads) void blur_kernel(BlurArgs a, Taps taps) {
     const uint32_t tile = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-    blur_tile<R, SRC, kTH>(a, taps, (int)(tile % gridDim.x) * kTW, (int)(tile / gridDim.x) * kTH);
+    blur_tile<R, SRC, kTH>(a, taps, (int)(tile % gridDim.x) * kTW, (int)(tile / gridDim.x) * kTH, threadIdx.x);
 }
 
 // Sliding-window form of the same blur for large levels. A workgroup owns a 64-column strip
@@ -181,7 +180,8 @@ constexpr int kSlideTH = kBlurSlideStep;
 // first level of octaves >= 1, which materialises G0): only then are the ring rows' centre
 // values kept, so the common case needs a third less LDS and fits more workgroups per CU.
 template <int R, int SRC, bool CENTER>
-__device__ __forceinline__ void blur_strip(const BlurArgs &a, const Taps &taps, int x0, int y_start, int chunk) {
+__device__ __forceinline__ void blur_strip(const BlurArgs &a, const Taps &taps, int x0, int y_start, int chunk,
+                                           int tid) {
     constexpr int NT = 2 * R + 1;
     constexpr int IW = kTW + 2 * R;
     constexpr int IWP = (IW + 3) & ~3;
@@ -196,7 +196,7 @@ __device__ __forceinline__ void blur_strip(const BlurArgs &a, const Taps &taps, 
     __shared__ __attribute__((aligned(16))) float s_h[2 * RING * kTW];
     __shared__ __attribute__((aligned(16))) float s_c[CENTER ? RING * kTW : 4];
     // the wave index is uniform: readfirstlane lets row arithmetic stay scalar
-    const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int h = a.h, w = a.w;
     const int y_end = min(y_start + chunk, h);
     const int n_steps = (y_end - y_start + TH - 1) / TH + P;
@@ -284,7 +284,8 @@ __device__ __forceinline__ void blur_strip(const BlurArgs &a, const Taps &taps, 
 template <int R, int SRC, bool CENTER>
 __global__ __launch_bounds__(kBlurThreads) void blur_slide_kernel(BlurArgs a, Taps taps, int chunk) {
     const uint32_t tile = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-    blur_strip<R, SRC, CENTER>(a, taps, (int)(tile % gridDim.x) * kTW, (int)(tile / gridDim.x) * chunk, chunk);
+    blur_strip<R, SRC, CENTER>(a, taps, (int)(tile % gridDim.x) * kTW, (int)(tile / gridDim.x) * chunk, chunk,
+                               threadIdx.x);
 }
 
 template <int R>
@@ -472,7 +473,7 @@ template <int PL, int kExRows>
 __device__ __forceinline__ void extrema_tile(const float *gauss, int h, int w, float thr, float mult,
                                              const EffScale &eff, double log2k, int octave, Cand *out,
                                              uint32_t *count, uint32_t cap, uint32_t *err, const Taps &top,
-                                             int x0, int y0) {
+                                             int x0, int y0, int tid) {
     constexpr int S = PL - 2;
     constexpr int kExH = kExRows + 2;
     constexpr int per_plane = kExH * kExW;
@@ -494,14 +495,14 @@ __device__ __forceinline__ void extrema_tile(const float *gauss, int h, int w, f
     __shared__ float s_hrow[TB * HR * 3];
     __shared__ float s_up[TB * 9];
     const size_t P = (size_t)h * w;
-    if (threadIdx.x == 0) s_qn = 0;
+    if (tid == 0) s_qn = 0;
     int nan_seen = 0;
     {
         // Gaussian planes 1..S+1 give DoG planes 1..S
         float v[S + 1][NL];
 #pragma unroll
         for (int j = 0; j < NL; j++) {
-            int i = threadIdx.x + j * 256;
+            int i = tid + j * 256;
             int r = i / kExW, c = i - r * kExW;
             size_t o = (size_t)min(y0 - 1 + r, h - 1) * w + min(x0 - 1 + c, w - 1);
 #pragma unroll
@@ -509,7 +510,7 @@ __device__ __forceinline__ void extrema_tile(const float *gauss, int h, int w, f
         }
 #pragma unroll
         for (int j = 0; j < NL; j++) {
-            int i = threadIdx.x + j * 256;
+            int i = tid + j * 256;
 #pragma unroll
             for (int pl = 0; pl < S; pl++)
                 if (i < per_plane) {
@@ -533,8 +534,8 @@ __device__ __forceinline__ void extrema_tile(const float *gauss, int h, int w, f
     {
         constexpr int RT = kExRows / 4;
         constexpr int SC = 64 / RT;   // slices per mask
-        const int tx = threadIdx.x & 63, lx = tx + 1;
-        const int ly0 = 1 + (threadIdx.x >> 6) * RT;
+        const int tx = tid & 63, lx = tx + 1;
+        const int ly0 = 1 + (tid >> 6) * RT;
         // rows j of this thread that lie in the level's interior (all of them off the last tile
         // row), repeated for each slice position of a mask; empty outside the interior columns
         uint64_t rows_ok = 0;
@@ -552,11 +553,11 @@ __device__ __forceinline__ void extrema_tile(const float *gauss, int h, int w, f
 #pragma unroll
             for (int d = 1; d < 64; d <<= 1) {
                 const uint32_t o = __shfl_up(incl, d);
-                if ((threadIdx.x & 63) >= d) incl += o;
+                if ((tid & 63) >= d) incl += o;
             }
             const uint32_t wtot = __shfl(incl, 63);
             uint32_t pos = 0;
-            if ((threadIdx.x & 63) == 63 && wtot) pos = atomicAdd(&s_qn, wtot);
+            if ((tid & 63) == 63 && wtot) pos = atomicAdd(&s_qn, wtot);
             pos = __shfl(pos, 63) + incl - nf;
             while (found) {
                 const int b = __ffsll((unsigned long long)found) - 1;
@@ -622,7 +623,7 @@ __device__ __forceinline__ void extrema_tile(const float *gauss, int h, int w, f
     // phase 2 (pipeline.rs:852-893); accepted candidates of a round reserve their output
     // slots with one global atomic per workgroup (ballot prefix inside the waves)
     const uint32_t nq = s_qn;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = tid & 63, wv = tid >> 6;
     for (uint32_t q0 = 0; q0 < nq; q0 += 256) {
         // queue entry q0 + 4*lane + wave: a short queue is spread over all four waves
         const uint32_t qi = q0 + 4 * (uint32_t)lane + (uint32_t)wv;
@@ -651,8 +652,8 @@ __device__ __forceinline__ void extrema_tile(const float *gauss, int h, int w, f
             __syncthreads();
             for (uint32_t t0 = 0; t0 < n_top; t0 += TB) {
                 const int nb_t = (int)min((uint32_t)TB, n_top - t0);
-                if ((int)threadIdx.x < nb_t * HR) {
-                    const int k = (int)threadIdx.x / HR, rr = (int)threadIdx.x - k * HR;
+                if (tid < nb_t * HR) {
+                    const int k = tid / HR, rr = tid - k * HR;
                     const uint32_t te = s_top[t0 + k];
                     const int ty = y0 + (int)((te >> 6) & 31u), tx = x0 + (int)(te & 63u);
                     const float *src = gauss + (size_t)(S + 1) * P + (size_t)min(max(ty - 1 - TR + rr, 0), h - 1) * w;
@@ -668,13 +669,13 @@ __device__ __forceinline__ void extrema_tile(const float *gauss, int h, int w, f
                     }
                 }
                 __syncthreads();
-                if ((int)threadIdx.x < nb_t * 9) {
-                    const int k = (int)threadIdx.x / 9, o = (int)threadIdx.x - k * 9;
+                if (tid < nb_t * 9) {
+                    const int k = tid / 9, o = tid - k * 9;
                     const float *hr = s_hrow + (k * HR + o / 3) * 3 + o % 3;
                     float acc = 0.0f;
 #pragma unroll
                     for (int q = 0; q < TNT; q++) acc += hr[q * 3] * top.w[q];
-                    s_up[threadIdx.x] = acc;
+                    s_up[tid] = acc;
                 }
                 __syncthreads();
                 if (is_top && my_top - t0 < (uint32_t)nb_t) {
@@ -750,7 +751,7 @@ __device__ __forceinline__ void extrema_tile(const float *gauss, int h, int w, f
         const uint64_t m = __ballot(accept);
         if (lane == 0) s_wcnt[wv] = (uint32_t)__popcll(m);
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (tid == 0) {
             const uint32_t tot = s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
             s_base = tot ? atomicAdd(count, tot) : 0u;
         }
@@ -773,7 +774,7 @@ __global__ __launch_bounds__(256) void extrema_kernel(const float *gauss, int h,
     // the cache lines of their halo columns, and land on the same XCD's L2
     const uint32_t tile = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
     extrema_tile<PL, kExRows>(gauss, h, w, thr, mult, eff, log2k, octave, out, count, cap, err, top,
-                              1 + (int)(tile % gridDim.x) * 64, 1 + (int)(tile / gridDim.x) * kExRows);
+                              1 + (int)(tile % gridDim.x) * 64, 1 + (int)(tile / gridDim.x) * kExRows, threadIdx.x);
 }
 
 hipError_t launch_extrema(const float *gauss_octave, int h, int w, int S, float thr, float mult, const float *eff_scale,
@@ -1082,7 +1083,7 @@ __device__ __forceinline__ uint32_t next_job(const J *jobs, uint32_t n, uint32_t
 }
 
 template <int R, int SRC, int kTH>
-__global__ __launch_bounds__(kBlurThreads) void blur_group_kernel(const BlurJob *jobs, uint32_t n_jobs,
+__global__ __launch_bounds__(kBlurThreads) void blur_group_kernel(const BlurJob *__restrict__ jobs, uint32_t n_jobs,
                                                                   uint32_t total, Taps taps) {
     uint32_t u, end, step;
     unit_walk(total, u, end, step);
@@ -1090,15 +1091,15 @@ __global__ __launch_bounds__(kBlurThreads) void blur_group_kernel(const BlurJob 
     uint32_t jb = find_job(jobs, n_jobs, u);
     for (; u < end; u += step) {
         jb = next_job(jobs, n_jobs, jb, u);
-        const BlurArgs a = jobs[jb].a;
+        const BlurArgs a = global_args(jobs[jb].a);
         const uint32_t t = u - jobs[jb].unit0, nx = jobs[jb].units_x;
-        blur_tile<R, SRC, kTH>(a, taps, (int)(t % nx) * kTW, (int)(t / nx) * kTH);
+        blur_tile<R, SRC, kTH>(a, taps, (int)(t % nx) * kTW, (int)(t / nx) * kTH, opaque_tid());
         __syncthreads();   // the next unit re-stages the LDS tiles
     }
 }
 
 template <int R, int SRC, bool CENTER>
-__global__ __launch_bounds__(kBlurThreads) void blur_strip_group_kernel(const BlurJob *jobs, uint32_t n_jobs,
+__global__ __launch_bounds__(kBlurThreads) void blur_strip_group_kernel(const BlurJob *__restrict__ jobs, uint32_t n_jobs,
                                                                         uint32_t total, Taps taps, int chunk) {
     uint32_t u, end, step;
     unit_walk(total, u, end, step);
@@ -1106,9 +1107,9 @@ __global__ __launch_bounds__(kBlurThreads) void blur_strip_group_kernel(const Bl
     uint32_t jb = find_job(jobs, n_jobs, u);
     for (; u < end; u += step) {
         jb = next_job(jobs, n_jobs, jb, u);
-        const BlurArgs a = jobs[jb].a;
+        const BlurArgs a = global_args(jobs[jb].a);
         const uint32_t t = u - jobs[jb].unit0, nx = jobs[jb].units_x;
-        blur_strip<R, SRC, CENTER>(a, taps, (int)(t % nx) * kTW, (int)(t / nx) * chunk, chunk);
+        blur_strip<R, SRC, CENTER>(a, taps, (int)(t % nx) * kTW, (int)(t / nx) * chunk, chunk, opaque_tid());
         __syncthreads();
     }
 }
@@ -1212,7 +1213,7 @@ hipError_t launch_blur_group(int radius, int src_mode, const BlurJob *jobs, uint
 }
 
 template <int PL, int kExRows>
-__global__ __launch_bounds__(256) void extrema_group_kernel(const ExtremaJob *jobs, uint32_t n_jobs, uint32_t total,
+__global__ __launch_bounds__(256) void extrema_group_kernel(const ExtremaJob *__restrict__ jobs, uint32_t n_jobs, uint32_t total,
                                                             float thr, float mult, EffScale eff, double log2k,
                                                             int octave, uint32_t cap, Taps top) {
     uint32_t u, end, step;
@@ -1223,8 +1224,9 @@ __global__ __launch_bounds__(256) void extrema_group_kernel(const ExtremaJob *jo
         jb = next_job(jobs, n_jobs, jb, u);
         const ExtremaJob J = jobs[jb];
         const uint32_t t = u - J.unit0;
-        extrema_tile<PL, kExRows>(J.gauss, J.h, J.w, thr, mult, eff, log2k, octave, J.out, J.count, cap, J.err, top,
-                                  1 + (int)(t % J.units_x) * 64, 1 + (int)(t / J.units_x) * kExRows);
+        extrema_tile<PL, kExRows>(global_ptr(J.gauss), J.h, J.w, thr, mult, eff, log2k, octave, global_ptr(J.out),
+                                  global_ptr(J.count), cap, global_ptr(J.err), top, 1 + (int)(t % J.units_x) * 64,
+                                  1 + (int)(t / J.units_x) * kExRows, opaque_tid());
         __syncthreads();
     }
 }
@@ -1265,36 +1267,38 @@ hipError_t launch_extrema_group(const ExtremaJob *jobs, uint32_t n_jobs, uint32_
 // ordering of many images: one select workgroup per image, then fixed grids per image
 constexpr uint32_t kGatherGroup = 64, kScatterGroup = 16, kCountGroup = 8;
 
-__global__ __launch_bounds__(1024) void rank_select_group_kernel(const RankJob *jobs, uint32_t cap_in,
+__global__ __launch_bounds__(1024) void rank_select_group_kernel(const RankJob *__restrict__ jobs, uint32_t cap_in,
                                                                  uint32_t limit) {
     const RankJob J = jobs[blockIdx.x];
-    rank_select_body(J.cand, J.count, cap_in, limit, rank_words(J.scratch, cap_in));
+    rank_select_body(global_ptr(J.cand), global_ptr(J.count), cap_in, limit,
+                     rank_words(global_ptr(J.scratch), cap_in));
 }
 
-__global__ __launch_bounds__(256) void rank_gather_group_kernel(const RankJob *jobs, uint32_t cap_in, uint32_t limit) {
+__global__ __launch_bounds__(256) void rank_gather_group_kernel(const RankJob *__restrict__ jobs, uint32_t cap_in, uint32_t limit) {
     const RankJob J = jobs[blockIdx.y];
-    const uint32_t n = min(*J.count, cap_in);
-    uint32_t *rank = static_cast<uint32_t *>(J.scratch), *subset = rank + cap_in, *words = subset + cap_in;
+    const uint32_t n = min(*global_ptr(J.count), cap_in);
+    uint32_t *rank = static_cast<uint32_t *>(global_ptr(J.scratch)), *subset = rank + cap_in, *words = subset + cap_in;
     RankRec *rec = reinterpret_cast<RankRec *>(words + 64);
     for (uint32_t i0 = blockIdx.x * 256; i0 < n; i0 += gridDim.x * 256) {
-        rank_gather_block(J.cand, n, limit, subset, rec, rank, words, i0);
+        rank_gather_block(global_ptr(J.cand), n, limit, subset, rec, rank, words, i0);
         __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(kRankTile) void rank_count_group_kernel(const RankJob *jobs, uint32_t cap_in) {
+__global__ __launch_bounds__(kRankTile) void rank_count_group_kernel(const RankJob *__restrict__ jobs, uint32_t cap_in) {
     const RankJob J = jobs[blockIdx.z];
-    uint32_t *rank = static_cast<uint32_t *>(J.scratch), *words = rank + 2 * cap_in;
+    uint32_t *rank = static_cast<uint32_t *>(global_ptr(J.scratch)), *words = rank + 2 * cap_in;
     rank_count_body(reinterpret_cast<const RankRec *>(words + 64), words, rank, blockIdx.x, gridDim.x, blockIdx.y,
                     gridDim.y);
 }
 
-__global__ void rank_scatter_group_kernel(const RankJob *jobs, uint32_t cap_in, uint32_t limit) {
+__global__ void rank_scatter_group_kernel(const RankJob *__restrict__ jobs, uint32_t cap_in, uint32_t limit) {
     const RankJob J = jobs[blockIdx.y];
-    const uint32_t *rank = static_cast<const uint32_t *>(J.scratch), *subset = rank + cap_in, *words = subset + cap_in;
+    const uint32_t *rank = static_cast<const uint32_t *>(global_ptr(J.scratch)), *subset = rank + cap_in,
+                   *words = subset + cap_in;
     const uint32_t m = words[0];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < max(m, 1u); i += gridDim.x * blockDim.x)
-        rank_scatter_one(J.cand, subset, m, limit, rank, J.out, J.out_count, i);
+        rank_scatter_one(global_ptr(J.cand), subset, m, limit, rank, global_ptr(J.out), global_ptr(J.out_count), i);
 }
 
 hipError_t launch_rank_group(const RankJob *jobs, uint32_t n_jobs, uint32_t cap_in, uint32_t limit, hipStream_t st) {

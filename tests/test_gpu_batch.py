@@ -182,3 +182,34 @@ def test_full_256_image_batch(gpu):
         got = gpu.Features.from_arrays(np.frombuffer(F[i, :k * fsz].tobytes(), gpu.FEAT_DTYPE), D[i, :k])
         assert len(got) == len(rh), (i, w, h)
         _check_features(got, rh, rd)
+
+
+def test_counters_clean_when_the_batch_grows(gpu):
+    """A batch of more images than the last one on the same engine: the counter block grows, and
+    its new memory can have the old block's address. Every counter word of the new batch,
+    error words included, must start at zero: check() reports nothing and the counts equal a
+    fresh engine's, ten times over."""
+    import torch
+    dev = torch.device("cuda:0")
+    sizes = [(240, 320), (300, 280), (200, 360), (256, 256), (180, 400)]
+    imgs = [synth.blob_image(w, h, 40, seed=50 + i) for i, (h, w) in enumerate(sizes)]
+    pix, descs = _pack(imgs, [0] * len(imgs))
+    tpix = torch.from_numpy(pix).to(dev)
+    cap, fsz = gpu.MAX_FEATURES_PER_IMAGE, gpu.FEAT_DTYPE.itemsize
+    cfg = gpu.SiftConfig().octaves(3).scales(3).contrast_threshold(0.015)
+    f = torch.zeros((5, cap * fsz), dtype=torch.uint8, device=dev)
+    d = torch.zeros((5, cap, 128), dtype=torch.float32, device=dev)
+    c = torch.zeros(5, dtype=torch.int32, device=dev)
+    want = None
+    for _ in range(10):
+        eng = gpu.Engine(0)
+        for n in (3, 5):
+            im = np.array(descs[:n], gpu.IMAGE_DESC_DTYPE)
+            eng.extract_batch_u8(tpix.data_ptr(), im, cfg, f.data_ptr(), d.data_ptr(), cap, c.data_ptr())
+            eng.sync()
+            eng.check()
+        got = c.cpu().numpy().copy()
+        if want is None:
+            want = got
+        assert np.array_equal(got, want) and got.min() > 20
+        eng.close()
