@@ -514,6 +514,20 @@ ps_status ps_engine_mlp_adam(ps_engine *e, float *d_params, float *d_m, float *d
 /* host-pointer forms for checks */
 ps_status ps_mlp_forward(const float *params, const float *x, size_t n, float *logit);
 ps_status ps_mlp_loss_grad(const float *params, const float *x, const float *y, size_t n, float *grad, float *loss);
+/* Checkpoint / resume of the network (host buffers of PS_MLP_PARAMS floats). No counterpart in
+ * the reference, whose only persisted state is the catalog. The file: magic "PSMLPCKP",
+ * version 1, the layer sizes 8-64-64-1, the parameter count, the Adam step, flags (bit 0: the
+ * Adam moments m, v follow the parameters), the f32 values and an FNV-1a 64 checksum, all
+ * little-endian; written to `path`.tmp and renamed over `path`, as the catalog's files are
+ * (catalog.rs:203-245). m and v are both null (parameters only) or both given. Loading
+ * checks every header field, the size and the checksum (PS_E_FORMAT, PS_E_IO for file
+ * errors); moments present in the file are copied only when m and v are given, *flags (may be
+ * null) tells whether the file has them. */
+#define PS_MLP_CKPT_MOMENTS 1u
+ps_status ps_mlp_checkpoint_save(const char *path, const float *params, const float *m, const float *v,
+                                 uint32_t adam_step);
+ps_status ps_mlp_checkpoint_load(const char *path, float *params, float *m, float *v, uint32_t *adam_step,
+                                 uint32_t *flags);
 
 /* ---- 128-bit descriptor codes and the catalog shortlist ------------------------------
  * A code is two little-endian u64 words per descriptor: bit i (0..127) of the code is bit

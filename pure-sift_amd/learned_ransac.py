@@ -234,32 +234,49 @@ def pair_rows(ds, pair_ids):
                            for k in pair_ids])
 
 
-def train(eng, ds, y, train_ids, steps=600, batch=65536, lr=3e-3, seed=0, log=None):
+def train(eng, ds, y, train_ids, steps=600, batch=65536, lr=3e-3, seed=0, log=None, resume=None, checkpoint=None):
+    """Adam training of the match-scoring MLP on the GPU kernels. Each step's minibatch is drawn
+    from a generator seeded by (seed, step), so a run resumed from a checkpoint (`resume`: the
+    dict of pure_sift.load_mlp_checkpoint, with the Adam moments) continues exactly as the
+    uninterrupted run would. `checkpoint`: a path that gets the final masters, moments and step.
+    Returns (params, loss curve)."""
     import torch
     dev = torch.device("cuda:0")
     rows = torch.from_numpy(pair_rows(ds, train_ids)).to(dev)
     ty = torch.from_numpy(y).to(dev)
-    tp = torch.from_numpy(ps.mlp_init(seed)).to(dev)
-    tm, tv, tg = torch.zeros_like(tp), torch.zeros_like(tp), torch.zeros_like(tp)
+    first = 1
+    if resume is not None:
+        if resume["m"] is None:
+            raise ValueError("resuming needs a checkpoint with the Adam moments")
+        tp = torch.from_numpy(resume["params"]).to(dev)
+        tm, tv = torch.from_numpy(resume["m"]).to(dev), torch.from_numpy(resume["v"]).to(dev)
+        first = resume["step"] + 1
+    else:
+        tp = torch.from_numpy(ps.mlp_init(seed)).to(dev)
+        tm, tv = torch.zeros_like(tp), torch.zeros_like(tp)
+    tg = torch.zeros_like(tp)
     loss = torch.zeros(1, dtype=torch.float32, device=dev)
     g = torch.Generator(device=dev)
-    g.manual_seed(seed)
     xb = torch.empty((batch, 8), dtype=torch.float32, device=dev)
     yb = torch.empty(batch, dtype=torch.float32, device=dev)
     curve = []
-    for step in range(1, steps + 1):
+    for step in range(first, first + steps):
+        g.manual_seed(seed * 1000003 + step)
         sel = rows[torch.randint(0, len(rows), (batch,), device=dev, generator=g)]
         xb.copy_(ds.x[sel])
         yb.copy_(ty[sel])
         eng.mlp_grad(tp.data_ptr(), xb.data_ptr(), yb.data_ptr(), batch, tg.data_ptr(), loss.data_ptr())
         eng.mlp_adam(tp.data_ptr(), tm.data_ptr(), tv.data_ptr(), tg.data_ptr(), step, lr)
-        if step % 100 == 0 or step == steps:
+        if step % 100 == 0 or step == first + steps - 1:
             eng.sync()
             curve.append((step, float(loss.item())))
             if log:
                 log(f"step {step}: loss {curve[-1][1]:.4f}")
     eng.sync()
-    return tp.cpu().numpy(), curve
+    params = tp.cpu().numpy()
+    if checkpoint is not None:
+        ps.save_mlp_checkpoint(checkpoint, params, tm.cpu().numpy(), tv.cpu().numpy(), first + steps - 1)
+    return params, curve
 
 
 def reproj_error(Hest, Htrue):

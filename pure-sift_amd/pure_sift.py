@@ -196,6 +196,8 @@ def lib() -> C.CDLL:
         L.ps_engine_mlp_adam.argtypes = [vp, vp, vp, vp, vp, C.c_uint32, C.c_float]
         L.ps_mlp_forward.argtypes = [vp, vp, sz, vp]
         L.ps_mlp_loss_grad.argtypes = [vp, vp, vp, sz, vp, vp]
+        L.ps_mlp_checkpoint_save.argtypes = [C.c_char_p, vp, vp, vp, C.c_uint32]
+        L.ps_mlp_checkpoint_load.argtypes = [C.c_char_p, vp, vp, vp, u32p, u32p]
         L.ps_engine_extract_batch_u8.argtypes = [vp, vp, vp, C.c_uint32, C.POINTER(_SiftConfigC), vp, vp, sz, vp,
                                                  C.c_uint32]
         L.ps_engine_autotune_blur.argtypes = [vp, vp, C.c_uint32, C.POINTER(_SiftConfigC), u32p]
@@ -247,7 +249,7 @@ def lib() -> C.CDLL:
         L.ps_engine_catalog_search.argtypes = [vp, vp, vp, vp, sz, sz, sz, C.c_float, C.c_float, vp, sz,
                                                C.POINTER(sz)]
         for name in ("ps_engine_extract_batch_u8", "ps_engine_autotune_blur", "ps_engine_autotune_batch", "ps_engine_mlp_features", "ps_engine_mlp_forward", "ps_engine_mlp_grad", "ps_engine_mlp_adam",
-                     "ps_mlp_forward", "ps_mlp_loss_grad", "ps_homography_estimate", "ps_ransac_batch", "ps_engine_ransac_batch", "ps_nn2_precision", "ps_engine_nn2", "ps_quantize_u128", "ps_count_coarse_hamming_votes", "ps_hamming_topk", "ps_rerank_l2",
+                     "ps_mlp_forward", "ps_mlp_loss_grad", "ps_mlp_checkpoint_save", "ps_mlp_checkpoint_load", "ps_homography_estimate", "ps_ransac_batch", "ps_engine_ransac_batch", "ps_nn2_precision", "ps_engine_nn2", "ps_quantize_u128", "ps_count_coarse_hamming_votes", "ps_hamming_topk", "ps_rerank_l2",
                      "ps_engine_quantize_u128", "ps_engine_hamming_topk", "ps_engine_rerank_l2","ps_init", "ps_validate_sift_config", "ps_validate_match_options", "ps_detect_and_compute",
                      "ps_detect_and_compute_u8", "ps_detect_keypoints", "ps_match", "ps_build_pyramid", "ps_nn2",
                      "ps_engine_create", "ps_engine_destroy", "ps_engine_sync", "ps_engine_check",
@@ -579,6 +581,31 @@ def mlp_forward(params: np.ndarray, x: np.ndarray) -> np.ndarray:
     out = np.zeros(len(xx), np.float32)
     check(lib().ps_mlp_forward(p.ctypes.data, xx.ctypes.data, len(xx), out.ctypes.data))
     return out
+
+
+def save_mlp_checkpoint(path: str, params: np.ndarray, m: np.ndarray = None, v: np.ndarray = None,
+                        step: int = 0) -> None:
+    """Writes the MLP masters (and, for resuming training, the Adam moments and step) to `path`
+    atomically (ps_mlp_checkpoint_save)."""
+    p = np.ascontiguousarray(params, np.float32).reshape(MLP_PARAMS)
+    if (m is None) != (v is None):
+        raise ValueError("give both Adam moments or neither")
+    mm = None if m is None else np.ascontiguousarray(m, np.float32).reshape(MLP_PARAMS)
+    vv = None if v is None else np.ascontiguousarray(v, np.float32).reshape(MLP_PARAMS)
+    check(lib().ps_mlp_checkpoint_save(os.fsencode(path), p.ctypes.data, None if mm is None else mm.ctypes.data,
+                                       None if vv is None else vv.ctypes.data, int(step)))
+
+
+def load_mlp_checkpoint(path: str) -> dict:
+    """{"params", "m", "v" (None without moments), "step"} from a checkpoint file."""
+    p = np.zeros(MLP_PARAMS, np.float32)
+    m = np.zeros(MLP_PARAMS, np.float32)
+    v = np.zeros(MLP_PARAMS, np.float32)
+    step, flags = C.c_uint32(0), C.c_uint32(0)
+    check(lib().ps_mlp_checkpoint_load(os.fsencode(path), p.ctypes.data, m.ctypes.data, v.ctypes.data,
+                                       C.byref(step), C.byref(flags)))
+    has = bool(flags.value & 1)
+    return {"params": p, "m": m if has else None, "v": v if has else None, "step": int(step.value)}
 
 
 def mlp_loss_grad(params: np.ndarray, x: np.ndarray, y: np.ndarray):

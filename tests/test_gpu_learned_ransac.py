@@ -42,3 +42,29 @@ def test_guided_beats_uniform_4096_pairs(gpu):
           f"{ev['equal_inlier_count_frac']:.3f}")
     _check(ev)
     assert np.isfinite(ev["guided"]["reproj_err_px_median"])
+
+
+def test_training_resumes_from_checkpoint(gpu, tmp_path):
+    """Checkpoint / resume of the MLP training: 40 Adam steps in one run equal 25 steps, a
+    checkpoint (masters, moments, step), a load and 15 more steps, bit for bit (the minibatch of
+    a step depends only on the seed and the step; the gradient reduction has a fixed order)."""
+    import torch
+    import learned_ransac as lr
+    import pure_sift as ps
+    eng = ps.Engine(0)
+    try:
+        with torch.cuda.stream(torch.cuda.ExternalStream(eng.stream())):
+            ds = lr.build_dataset(eng, 48, 8, seed=5)
+            y, _ = lr.labels_from_ransac(eng, ds, iters=2000)
+            ids = np.arange(48)
+            whole, _ = lr.train(eng, ds, y, ids, steps=40, batch=4096, seed=2)
+            path = str(tmp_path / "mlp.ckpt")
+            lr.train(eng, ds, y, ids, steps=25, batch=4096, seed=2, checkpoint=path)
+            ck = ps.load_mlp_checkpoint(path)
+            assert ck["step"] == 25 and ck["m"] is not None
+            resumed, curve = lr.train(eng, ds, y, ids, steps=15, batch=4096, seed=2, resume=ck)
+            assert curve[-1][0] == 40
+            assert np.array_equal(resumed.view(np.uint32), whole.view(np.uint32))
+            assert not np.array_equal(whole, ck["params"])
+    finally:
+        eng.close()
