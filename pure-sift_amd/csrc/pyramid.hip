@@ -625,8 +625,10 @@ __device__ __forceinline__ void extrema_tile(const float *gauss, int h, int w, f
     const uint32_t nq = s_qn;
     const int lane = tid & 63, wv = tid >> 6;
     for (uint32_t q0 = 0; q0 < nq; q0 += 256) {
-        // queue entry q0 + 4*lane + wave: a short queue is spread over all four waves
-        const uint32_t qi = q0 + 4 * (uint32_t)lane + (uint32_t)wv;
+        // queue entry q0 + 64*wave + lane: a short queue (a few entries per tile is typical)
+        // occupies as few waves as possible, and a wave without entries skips the per-entry
+        // tests and refinement entirely instead of executing them for one or two live lanes
+        const uint32_t qi = q0 + 64 * (uint32_t)wv + (uint32_t)lane;
         const uint32_t e = qi < nq ? s_q[qi] : 0u;
         const bool is_top = qi < nq && (int)(e >> 11) + 1 == S;
         // The upper neighbour plane of slice-S entries is DoG S+1 = G(S+2) - G(S+1). The
