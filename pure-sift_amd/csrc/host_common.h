@@ -34,8 +34,8 @@ struct ps_engine {
     int device = 0;
     hipStream_t stream = nullptr;
     // the leaf work of octave o of an extraction (levels above S, extrema) runs on
-    // side[o % kSides], forked once level S of octave o is written and joined back into
-    // `stream` at the end of the scale space
+    // side[o % kSides], forked once level S of octave o (of octave 1 for o = 0) is written and
+    // joined back into `stream` at the end of the scale space
     static constexpr int kSides = 3;
     hipStream_t side[kSides] = {};
     hipEvent_t fork_ev[ps::kMaxOct] = {};
