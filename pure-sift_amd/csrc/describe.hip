@@ -85,13 +85,13 @@ __device__ __forceinline__ void grad(const LocalKp &l, int x, int y, float &m, f
     a = rem_euclid(grad_atan2(dy, dx), kTwoPi);
 }
 
-// pipeline.rs:1233-1347. One 256-thread workgroup per keypoint. Histogram updates go to one
-// of 32 private LDS copies (wave, lane % 8), so lanes that share a copy are 8 pixels apart and
+// pipeline.rs:1233-1347. One workgroup of kOriThreads per keypoint. Histogram updates go to one
+// of kCopies private LDS copies (wave, lane % 8), so lanes that share a copy are 8 pixels apart and
 // rarely hit the same bin; the copies are summed in a fixed order (deterministic results).
 // Each thread gathers the gradient samples of 4 pixels before any histogram update, so
 // their loads overlap.
-constexpr int kDescThreads = 256;
-constexpr int kCopies = 32;   // 4 waves x 8 lane groups
+constexpr int kOriThreads = 128;
+constexpr int kCopies = kOriThreads / 64 * 8;   // waves x 8 lane groups
 constexpr int kBatch = 4;
 __device__ __forceinline__ void orient_body(const PyrView &pv, const Cand *kps, uint32_t k, float *ori,
                                             uint32_t *nori, int tid) {
@@ -110,18 +110,18 @@ __device__ __forceinline__ void orient_body(const PyrView &pv, const Cand *kps, 
     const int y_min = max(cy - radius, 1), y_max = max(min(cy + radius, l.H - 2), 1);
     const int x_min = max(cx - radius, 1), x_max = max(min(cx + radius, l.W - 2), 1);
     const float r_sq_max = (float)(radius * radius);
-    for (int i = lane; i < kCopies * 37; i += kDescThreads) (&s_whist[0][0])[i] = 0.0f;
+    for (int i = lane; i < kCopies * 37; i += kOriThreads) (&s_whist[0][0])[i] = 0.0f;
     __syncthreads();
     float *wh = s_whist[(lane >> 6) * 8 + (lane & 7)];
     const int ww = x_max - x_min + 1, hh = y_max - y_min + 1;
     const int total = (ww > 0 && hh > 0) ? ww * hh : 0;
     const float inv_ww = 1.0f / (float)max(ww, 1);
-    for (int p0 = lane; p0 < total; p0 += kBatch * kDescThreads) {
+    for (int p0 = lane; p0 < total; p0 += kBatch * kOriThreads) {
         float gxa[kBatch], gxb[kBatch], gya[kBatch], gyb[kBatch], rsq[kBatch];
         bool ok[kBatch];
 #pragma unroll
         for (int u = 0; u < kBatch; u++) {
-            int p = p0 + u * kDescThreads;
+            int p = p0 + u * kOriThreads;
             int qy, qx;
             divmod_small(p, ww, inv_ww, qy, qx);
             int yy = y_min + qy, xx = x_min + qx;
@@ -202,7 +202,7 @@ __device__ __forceinline__ void orient_body(const PyrView &pv, const Cand *kps, 
     if (lane == 0) nori[k] = (uint32_t)min(npeaks, 4);
 }
 
-__global__ __launch_bounds__(kDescThreads) void orient_kernel(PyrView pv, const Cand *kps, const uint32_t *nkp,
+__global__ __launch_bounds__(kOriThreads) void orient_kernel(PyrView pv, const Cand *kps, const uint32_t *nkp,
                                                               uint32_t max_kp, float *ori, uint32_t *nori) {
     const uint32_t k = blockIdx.x;
     if (k >= min(*nkp, max_kp)) return;
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(kDescThreads) void orient_kernel(PyrView pv, const 
 hipError_t launch_orientations(const PyrView &pv, const Cand *kps, const uint32_t *nkp, uint32_t max_kp, float *ori,
                                uint32_t *nori, hipStream_t st) {
     if (max_kp == 0) return hipSuccess;
-    orient_kernel<<<max_kp, kDescThreads, 0, st>>>(pv, kps, nkp, max_kp, ori, nori);
+    orient_kernel<<<max_kp, kOriThreads, 0, st>>>(pv, kps, nkp, max_kp, ori, nori);
     return hipGetLastError();
 }
 
@@ -259,21 +259,58 @@ hipError_t launch_feature_scan(const uint32_t *nkp, const uint32_t *nori, uint32
 }
 
 // pipeline.rs:1352-1480 (+ RootSIFT, :1012-1019). One 256-thread workgroup per feature and no
-// atomics: a pixel that passes the window test has r_bin, c_bin in (-1, 4), so it belongs to
-// exactly one of the 25 cells (floor(r_bin), floor(c_bin)) in {-1..3}^2 and only feeds the 4
-// spatial bins around that cell. Ten lanes own each cell; they scan the cell's padded
-// bounding box in image space, keep the pixels whose floor(r_bin), floor(c_bin) (the
-// reference's expressions) name their cell, and accumulate 4 corners x 8 orientations into a
-// lane-private LDS region with plain read-modify-writes. A fixed-order sum over the owners of
-// the 4 neighbouring cells then forms each of the 128 bins.
-constexpr int kLanesPerCell = 10;
-constexpr int kPrivStride = 33;   // 32 accumulators + 1 pad
-constexpr int kListCap = 12;      // box pixels tested per lane before the recorded ones are processed
+// shared accumulators: a pixel that passes the window test has r_bin, c_bin in (-1, 4), so it
+// belongs to exactly one of the 25 cells (floor(r_bin), floor(c_bin)) in {-1..3}^2 and only
+// feeds the 4 spatial bins around that cell. Ten lanes own each cell and accumulate 4 corners
+// x 8 orientations into lane-private LDS regions; a fixed-order sum over the owners of the 4
+// neighbouring cells then forms each of the 128 bins.
+//
+// The cell's pixels are found row by row, not by testing every pixel of its bounding box: along
+// an image row, r_bin and c_bin (the reference's expressions, rounded operation by operation)
+// are monotone in x, so the row's pixels with floor(r_bin) = r0 and floor(c_bin) = c0 inside
+// the window form one interval, bounded by where four monotone tests change value. Each
+// transition is located from its closed-form estimate and confirmed by evaluating the test on
+// both sides (a binary search when the estimate misses, e.g. for a nearly axis-parallel
+// boundary), so the intervals hold exactly the pixels the per-pixel test would keep. The ten
+// lanes then take every tenth pixel of the cell's row-major sequence.
+constexpr int kFeatThreads = 128;   // descriptor workgroup: 5 lanes per cell, lanes 125..127 idle
+constexpr int kLanesPerCell = 5;
+constexpr int kPrivStride = 33;     // 32 accumulators + 1 pad
+constexpr int kCellRows = 32;       // bounding-box rows of a cell kept as intervals in LDS (hist width <= ~20 px)
+constexpr int kFeatBatch = 4;       // pixels whose gradient loads are issued together
+
+// (r_rot, c_rot, r_bin, c_bin) of offset (dx, dy) from the keypoint (pipeline.rs:1380-1387), every
+// operation rounded on its own (no contraction): each evaluation for one pixel gives the same
+// bits, and along a row (fixed dy) every result is monotone in dx
+__device__ __forceinline__ void bin_coords(float dx, float dy, float cos_t, float sin_t, float inv_hw, float &r_rot,
+                                           float &c_rot, float &r_bin, float &c_bin) {
+#pragma clang fp contract(off)
+    r_rot = (dx * cos_t + dy * sin_t) * inv_hw;
+    c_rot = (-dx * sin_t + dy * cos_t) * inv_hw;
+    r_bin = r_rot + 1.5f;
+    c_bin = c_rot + 1.5f;
+}
+
+// first q in [0, n] with test(q) true (test(n) taken as true), for a test that is false then
+// true along q; `guess` is tried first
+template <class F>
+__device__ __forceinline__ int first_true(F test, int n, int guess) {
+    guess = min(max(guess, 0), n);
+    if ((guess == n || test(guess)) && (guess == 0 || !test(guess - 1))) return guess;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (test(mid)) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ void descriptor_body(const PyrView &pv, const Cand *kps, const float *ori,
                                                 const uint32_t *feat_kp, const uint32_t *feat_j, uint32_t f, int kind,
                                                 ps_feature *out_f, float *out_d, int tid) {
-    __shared__ float s_priv[kDescThreads * kPrivStride];
-    __shared__ uint16_t s_list[kDescThreads * kListCap];
+    __shared__ float s_priv[kFeatThreads * kPrivStride];
+    __shared__ uint16_t s_rows[25 * kCellRows];   // per cell and box row: first column << 8 | length
     __shared__ float s_desc[128];
     const int lane = tid;
     const uint32_t k = feat_kp[f];
@@ -284,46 +321,90 @@ __device__ __forceinline__ void descriptor_body(const PyrView &pv, const Cand *k
 #pragma unroll
     for (int i = 0; i < 32; i++) priv[i] = 0.0f;
     const int cell = lane / kLanesPerCell, sub = lane - cell * kLanesPerCell;
-    const int r0 = cell / 5 - 1, c0 = cell % 5 - 1;   // cell in {-1..3}^2 (lanes >= 250: none)
-    if (l.W >= 3 && l.H >= 3 && cell < 25) {
-        const float cos_t = cosf(theta), sin_t = sinf(theta);
-        const float hist_width = fmaxf(3.0f * l.scale, 1.5f);
-        const float inv_hw = 1.0f / hist_width;
-        const float exp_scale = -1.0f / (2.0f * 2.0f * 2.0f);
-        const int max_radius = max((int)ceilf(hist_width * 1.41421356237309504880f * 5.0f * 0.5f), 4);
-        const int cx = (int)roundf(l.x), cy = (int)roundf(l.y);
-        const int y_min = max(cy - max_radius, 1), y_max = max(min(cy + max_radius, l.H - 2), 1);
-        const int x_min = max(cx - max_radius, 1), x_max = max(min(cx + max_radius, l.W - 2), 1);
-        const float bin_per_rad = 8.0f / kTwoPi;
-        // bounding box of the cell: r_rot in [r0 - 1.5, r0 - 0.5), c_rot likewise; image offset
-        // (dx, dy) = hw (r cos - c sin, r sin + c cos); padded by 1 px against rounding
-        float bx0 = 1e30f, bx1 = -1e30f, by0 = 1e30f, by1 = -1e30f;
+    const int r0 = cell / 5 - 1, c0 = cell % 5 - 1;   // cell in {-1..3}^2 (lanes >= 125: none)
+    const bool active = l.W >= 3 && l.H >= 3 && cell < 25;
+    const float cos_t = cosf(theta), sin_t = sinf(theta);
+    const float hist_width = fmaxf(3.0f * l.scale, 1.5f);
+    const float inv_hw = 1.0f / hist_width;
+    const float exp_scale = -1.0f / (2.0f * 2.0f * 2.0f);
+    const int max_radius = max((int)ceilf(hist_width * 1.41421356237309504880f * 5.0f * 0.5f), 4);
+    const int cx = (int)roundf(l.x), cy = (int)roundf(l.y);
+    const int y_min = max(cy - max_radius, 1), y_max = max(min(cy + max_radius, l.H - 2), 1);
+    const int x_min = max(cx - max_radius, 1), x_max = max(min(cx + max_radius, l.W - 2), 1);
+    const float bin_per_rad = 8.0f / kTwoPi;
+    // bounding box of the cell: r_rot in [r0 - 1.5, r0 - 0.5), c_rot likewise; image offset
+    // (dx, dy) = hw (r cos - c sin, r sin + c cos); padded by 1 px against rounding
+    float bx0 = 1e30f, bx1 = -1e30f, by0 = 1e30f, by1 = -1e30f;
 #pragma unroll
-        for (int cr = 0; cr < 2; cr++)
+    for (int cr = 0; cr < 2; cr++)
 #pragma unroll
-            for (int cc = 0; cc < 2; cc++) {
-                float rr = ((float)r0 - 1.5f + (float)cr) * hist_width, c2 = ((float)c0 - 1.5f + (float)cc) * hist_width;
-                float ddx = rr * cos_t - c2 * sin_t, ddy = rr * sin_t + c2 * cos_t;
-                bx0 = fminf(bx0, ddx); bx1 = fmaxf(bx1, ddx);
-                by0 = fminf(by0, ddy); by1 = fmaxf(by1, ddy);
+        for (int cc = 0; cc < 2; cc++) {
+            float rr = ((float)r0 - 1.5f + (float)cr) * hist_width, c2 = ((float)c0 - 1.5f + (float)cc) * hist_width;
+            float ddx = rr * cos_t - c2 * sin_t, ddy = rr * sin_t + c2 * cos_t;
+            bx0 = fminf(bx0, ddx); bx1 = fmaxf(bx1, ddx);
+            by0 = fminf(by0, ddy); by1 = fmaxf(by1, ddy);
+        }
+    const int xa = max(x_min, (int)floorf(l.x + bx0) - 1), xb = min(x_max, (int)ceilf(l.x + bx1) + 1);
+    const int ya = max(y_min, (int)floorf(l.y + by0) - 1), yb = min(y_max, (int)ceilf(l.y + by1) + 1);
+    const int ww = xb - xa + 1, hh = yb - ya + 1;
+    const bool any = active && ww > 0 && hh > 0;
+    const bool by_rows = any && ww < 256 && hh <= kCellRows;
+    // the window and cell test of one pixel (pipeline.rs:1389-1396 with floor(r_bin) = r0,
+    // floor(c_bin) = c0), as four tests that are monotone along a row
+    auto r_lo = [&](float r_bin) { return r0 >= 0 ? r_bin >= (float)r0 : r_bin > -1.0f; };
+    auto c_lo = [&](float c_bin) { return c0 >= 0 ? c_bin >= (float)c0 : c_bin > -1.0f; };
+    auto bins_at = [&](int qy, int qx, float &r_rot, float &c_rot, float &r_bin, float &c_bin) {
+        bin_coords((float)(xa + qx) - l.x, (float)(ya + qy) - l.y, cos_t, sin_t, inv_hw, r_rot, c_rot, r_bin, c_bin);
+    };
+    if (by_rows) {
+        for (int qy = sub; qy < hh; qy += kLanesPerCell) {
+            const float dy = (float)(ya + qy) - l.y;
+            auto rb = [&](int qx) {
+                float a, b, r, c;
+                bin_coords((float)(xa + qx) - l.x, dy, cos_t, sin_t, inv_hw, a, b, r, c);
+                return r;
+            };
+            auto cbv = [&](int qx) {
+                float a, b, r, c;
+                bin_coords((float)(xa + qx) - l.x, dy, cos_t, sin_t, inv_hw, a, b, r, c);
+                return c;
+            };
+            // column q where the bin coordinate crosses value t (closed form), rounded up
+            auto cross_r = [&](float t) {
+                const float q = ((t - 1.5f) * hist_width - dy * sin_t) / cos_t + l.x - (float)xa;
+                return (int)fminf(fmaxf(ceilf(q), -1.0f), (float)(ww + 1));
+            };
+            auto cross_c = [&](float t) {
+                const float q = (dy * cos_t - (t - 1.5f) * hist_width) / sin_t + l.x - (float)xa;
+                return (int)fminf(fmaxf(ceilf(q), -1.0f), (float)(ww + 1));
+            };
+            int lo = 0, hi = ww;   // the row's pixels: [lo, hi)
+            const float tr = r0 >= 0 ? (float)r0 : -1.0f, tc = c0 >= 0 ? (float)c0 : -1.0f;
+            if (cos_t > 0.0f) {        // r_bin non-decreasing along the row
+                lo = max(lo, first_true([&](int q) { return r_lo(rb(q)); }, ww, cross_r(tr)));
+                hi = min(hi, first_true([&](int q) { return rb(q) >= (float)(r0 + 1); }, ww, cross_r((float)(r0 + 1))));
+            } else if (cos_t < 0.0f) { // non-increasing
+                hi = min(hi, first_true([&](int q) { return !r_lo(rb(q)); }, ww, cross_r(tr)));
+                lo = max(lo, first_true([&](int q) { return rb(q) < (float)(r0 + 1); }, ww, cross_r((float)(r0 + 1))));
+            } else {
+                const float r = rb(0);
+                if (!(r_lo(r) && r < (float)(r0 + 1))) hi = 0;
             }
-        const int xa = max(x_min, (int)floorf(l.x + bx0) - 1), xb = min(x_max, (int)ceilf(l.x + bx1) + 1);
-        const int ya = max(y_min, (int)floorf(l.y + by0) - 1), yb = min(y_max, (int)ceilf(l.y + by1) + 1);
-        const int ww = xb - xa + 1, hh = yb - ya + 1;
-        const int total = (ww > 0 && hh > 0) ? ww * hh : 0;
-        const float inv_ww = 1.0f / (float)max(ww, 1);
-        // window test and bin coordinates of box pixel (qy, qx) (pipeline.rs:1370-1393)
-        auto bins = [&](int qy, int qx, float &r_rot, float &c_rot, float &r_bin, float &c_bin) {
-            const float dy = (float)(ya + qy) - l.y, dx = (float)(xa + qx) - l.x;
-            r_rot = (dx * cos_t + dy * sin_t) * inv_hw;
-            c_rot = (-dx * sin_t + dy * cos_t) * inv_hw;
-            r_bin = r_rot + 4.0f * 0.5f - 0.5f;
-            c_bin = c_rot + 4.0f * 0.5f - 0.5f;
-        };
-        auto in_cell = [&](float r_bin, float c_bin) {
-            return !(r_bin <= -1.0f || r_bin >= 4.0f || c_bin <= -1.0f || c_bin >= 4.0f) &&
-                   (int)floorf(r_bin) == r0 && (int)floorf(c_bin) == c0;
-        };
+            if (sin_t < 0.0f) {        // c_bin non-decreasing along the row
+                lo = max(lo, first_true([&](int q) { return c_lo(cbv(q)); }, ww, cross_c(tc)));
+                hi = min(hi, first_true([&](int q) { return cbv(q) >= (float)(c0 + 1); }, ww, cross_c((float)(c0 + 1))));
+            } else if (sin_t > 0.0f) { // non-increasing
+                hi = min(hi, first_true([&](int q) { return !c_lo(cbv(q)); }, ww, cross_c(tc)));
+                lo = max(lo, first_true([&](int q) { return cbv(q) < (float)(c0 + 1); }, ww, cross_c((float)(c0 + 1))));
+            } else {
+                const float c = cbv(0);
+                if (!(c_lo(c) && c < (float)(c0 + 1))) hi = 0;
+            }
+            s_rows[cell * kCellRows + qy] = (uint16_t)(hi > lo ? (lo << 8) | (hi - lo) : 0);
+        }
+    }
+    __syncthreads();
+    if (any) {
         // gradient sample of box pixel (qy, qx): weight, orientation and trilinear accumulation
         // into the lane's 4 corners x 8 orientations (pipeline.rs:1186-1189, :1394-1440)
         auto add_pixel = [&](float gxa, float gxb, float gya, float gyb, int qy, int qx) {
@@ -332,7 +413,7 @@ __device__ __forceinline__ void descriptor_body(const PyrView &pv, const Cand *k
             if (m <= 1e-7f) return;
             const float a = rem_euclid(grad_atan2(gdy, gdx), kTwoPi);
             float r_rot, c_rot, r_bin, c_bin;
-            bins(qy, qx, r_rot, c_rot, r_bin, c_bin);
+            bins_at(qy, qx, r_rot, c_rot, r_bin, c_bin);
             const float wm = m * __expf((r_rot * r_rot + c_rot * c_rot) * exp_scale);
             const float o_bin = rem_euclid(a - theta, kTwoPi) * bin_per_rad;
             const int o0 = (int)floorf(o_bin);
@@ -351,45 +432,47 @@ __device__ __forceinline__ void descriptor_body(const PyrView &pv, const Cand *k
                 }
             }
         };
-        if (ww <= 256 && hh <= 256) {
-            // Two phases per chunk of kListCap box pixels so the expensive part runs on dense
-            // lanes: (A) the lane tests the chunk's pixels and records those of its cell in a
-            // lane-private LDS list (entry i at list[i * kDescThreads], packed qy << 8 | qx);
-            // (B) it processes the list in batches whose gradient loads are issued together.
-            uint16_t *list = s_list + lane;
-            for (int chunk = sub; chunk < total; chunk += kListCap * kLanesPerCell) {
-                const int chunk_end = min(chunk + kListCap * kLanesPerCell, total);
-                int n = 0;
-                for (int p = chunk; p < chunk_end; p += kLanesPerCell) {
-                    int qy, qx;
-                    divmod_small(p, ww, inv_ww, qy, qx);
-                    float r_rot, c_rot, r_bin, c_bin;
-                    bins(qy, qx, r_rot, c_rot, r_bin, c_bin);
-                    if (in_cell(r_bin, c_bin)) list[(n++) * kDescThreads] = (uint16_t)((qy << 8) | qx);
-                }
-                for (int i0 = 0; i0 < n; i0 += kBatch) {
-                    float gxa[kBatch], gxb[kBatch], gya[kBatch], gyb[kBatch];
-                    int pq[kBatch];
+        if (by_rows) {
+            // pixels sub, sub + 5, ... of the cell's rows in order, kFeatBatch at a time with
+            // their gradient loads issued together
+            const uint16_t *rows = s_rows + cell * kCellRows;
+            int r = 0, base = 0, len = rows[0] & 255, kq = sub;
+            bool more = true;
+            while (more) {
+                int pq[kFeatBatch];
+                bool ok[kFeatBatch];
 #pragma unroll
-                    for (int u = 0; u < kBatch; u++) {
-                        const int i = min(i0 + u, n - 1);
-                        pq[u] = list[i * kDescThreads];
-                        const float *row = l.g + (size_t)(ya + (pq[u] >> 8)) * l.W + (xa + (pq[u] & 255));
-                        gxa[u] = row[1]; gxb[u] = row[-1]; gyb[u] = row[l.W]; gya[u] = row[-l.W];
+                for (int u = 0; u < kFeatBatch; u++) {
+                    while (r < hh && kq >= base + len) {
+                        base += len;
+                        r++;
+                        len = r < hh ? rows[r] & 255 : 0;
                     }
-#pragma unroll
-                    for (int u = 0; u < kBatch; u++)
-                        if (i0 + u < n) add_pixel(gxa[u], gxb[u], gya[u], gyb[u], pq[u] >> 8, pq[u] & 255);
+                    ok[u] = r < hh;
+                    pq[u] = ok[u] ? (r << 8) | ((rows[r] >> 8) + kq - base) : (u > 0 ? pq[u - 1] : -1);
+                    kq += kLanesPerCell;
                 }
+                more = ok[kFeatBatch - 1];
+                if (!ok[0]) break;
+                float gxa[kFeatBatch], gxb[kFeatBatch], gya[kFeatBatch], gyb[kFeatBatch];
+#pragma unroll
+                for (int u = 0; u < kFeatBatch; u++) {
+                    const float *row = l.g + (size_t)(ya + (pq[u] >> 8)) * l.W + (xa + (pq[u] & 255));
+                    gxa[u] = row[1]; gxb[u] = row[-1]; gyb[u] = row[l.W]; gya[u] = row[-l.W];
+                }
+#pragma unroll
+                for (int u = 0; u < kFeatBatch; u++)
+                    if (ok[u]) add_pixel(gxa[u], gxb[u], gya[u], gyb[u], pq[u] >> 8, pq[u] & 255);
             }
         } else {
-            // boxes of 256+ pixels a side (keypoint scales far beyond what detection produces):
-            // the same pixels, tested and accumulated one at a time
+            // boxes too large for the row intervals (keypoint scales far beyond what detection
+            // produces): every box pixel tested and accumulated one at a time
+            const int total = ww * hh;
             for (int p = sub; p < total; p += kLanesPerCell) {
                 const int qy = p / ww, qx = p - qy * ww;
                 float r_rot, c_rot, r_bin, c_bin;
-                bins(qy, qx, r_rot, c_rot, r_bin, c_bin);
-                if (!in_cell(r_bin, c_bin)) continue;
+                bins_at(qy, qx, r_rot, c_rot, r_bin, c_bin);
+                if (!(r_lo(r_bin) && r_bin < (float)(r0 + 1) && c_lo(c_bin) && c_bin < (float)(c0 + 1))) continue;
                 const float *row = l.g + (size_t)(ya + qy) * l.W + (xa + qx);
                 add_pixel(row[1], row[-1], row[-l.W], row[l.W], qy, qx);
             }
@@ -457,7 +540,7 @@ __device__ __forceinline__ void descriptor_body(const PyrView &pv, const Cand *k
     }
 }
 
-__global__ __launch_bounds__(kDescThreads) void descriptor_kernel(PyrView pv, const Cand *kps, const float *ori,
+__global__ __launch_bounds__(kFeatThreads) void descriptor_kernel(PyrView pv, const Cand *kps, const float *ori,
                                                                   const uint32_t *feat_kp, const uint32_t *feat_j,
                                                                   const uint32_t *nfeat, uint32_t cap, int kind,
                                                                   ps_feature *out_f, float *out_d) {
@@ -470,7 +553,7 @@ hipError_t launch_descriptors(const PyrView &pv, const Cand *kps, const float *o
                               const uint32_t *feat_j, const uint32_t *nfeat, uint32_t cap, int kind, ps_feature *out_f,
                               float *out_d, hipStream_t st) {
     if (cap == 0) return hipSuccess;
-    descriptor_kernel<<<cap, kDescThreads, 0, st>>>(pv, kps, ori, feat_kp, feat_j, nfeat, cap, kind, out_f, out_d);
+    descriptor_kernel<<<cap, kFeatThreads, 0, st>>>(pv, kps, ori, feat_kp, feat_j, nfeat, cap, kind, out_f, out_d);
     return hipGetLastError();
 }
 
@@ -529,7 +612,7 @@ hipError_t launch_group_offsets(const uint32_t *counts, uint32_t stride, uint32_
     return hipGetLastError();
 }
 
-__global__ __launch_bounds__(kDescThreads) void orient_group_kernel(const DescJob *__restrict__ jobs, uint32_t n_jobs,
+__global__ __launch_bounds__(kOriThreads) void orient_group_kernel(const DescJob *__restrict__ jobs, uint32_t n_jobs,
                                                                     const uint32_t *__restrict__ kp_off) {
     const uint32_t total = kp_off[n_jobs];
     uint32_t jb = blockIdx.x < total ? find_off(kp_off, n_jobs, blockIdx.x) : 0;
@@ -549,7 +632,7 @@ __global__ __launch_bounds__(kScanThreads) void feature_scan_group_kernel(const 
                       global_ptr(J.nfeat));
 }
 
-__global__ __launch_bounds__(kDescThreads) void descriptor_group_kernel(const DescJob *__restrict__ jobs, uint32_t n_jobs,
+__global__ __launch_bounds__(kFeatThreads) void descriptor_group_kernel(const DescJob *__restrict__ jobs, uint32_t n_jobs,
                                                                         const uint32_t *__restrict__ feat_off, int kind) {
     const uint32_t total = feat_off[n_jobs];
     uint32_t jb = blockIdx.x < total ? find_off(feat_off, n_jobs, blockIdx.x) : 0;
@@ -576,15 +659,15 @@ hipError_t launch_describe_group(const DescJob *jobs, uint32_t n_jobs, uint32_t 
                                  uint32_t *kp_off, uint32_t *feat_off, const uint32_t *kp_counts,
                                  uint32_t kp_stride, const uint32_t *feat_counts, hipStream_t st) {
     if (n_jobs == 0) return hipSuccess;
-    static const uint32_t g_ori = resident_grid((const void *)orient_group_kernel, kDescThreads);
-    static const uint32_t g_desc = resident_grid((const void *)descriptor_group_kernel, kDescThreads);
+    static const uint32_t g_ori = resident_grid((const void *)orient_group_kernel, kOriThreads);
+    static const uint32_t g_desc = resident_grid((const void *)descriptor_group_kernel, kFeatThreads);
     if (!g_ori || !g_desc) return hipErrorInvalidValue;
     hipError_t e;
     if ((e = launch_group_offsets(kp_counts, kp_stride, max_kp, 1, n_jobs, kp_off, st)) != hipSuccess) return e;
-    orient_group_kernel<<<g_ori, kDescThreads, 0, st>>>(jobs, n_jobs, kp_off);
+    orient_group_kernel<<<g_ori, kOriThreads, 0, st>>>(jobs, n_jobs, kp_off);
     feature_scan_group_kernel<<<n_jobs, kScanThreads, 0, st>>>(jobs, max_kp, cap);
     if ((e = launch_group_offsets(feat_counts, 1, cap, 1, n_jobs, feat_off, st)) != hipSuccess) return e;
-    descriptor_group_kernel<<<g_desc, kDescThreads, 0, st>>>(jobs, n_jobs, feat_off, kind);
+    descriptor_group_kernel<<<g_desc, kFeatThreads, 0, st>>>(jobs, n_jobs, feat_off, kind);
     return hipGetLastError();
 }
 
