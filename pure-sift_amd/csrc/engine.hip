@@ -458,8 +458,10 @@ ps_status enqueue_keypoints(ps_engine *e, const void *src, bool src_u8, size_t h
     prof_mark(e, 0);
     PS_TRY(enqueue_scale_space(e, src, src_u8, h, w, stride, cfg, true));
     prof_mark(e, 2);
+    const ps::PyrPlan &p = e->plan;
+    const bool packed = p.n_oct <= 8 && p.S < 16 && (p.n_oct == 0 || (p.h[0] <= 4096 && p.w[0] <= 4096));
     PS_HIP(ps::launch_rank(as<ps::Cand>(e->cand), cnt + 0, ps::kCandCap, limit, as<ps::Cand>(e->sorted), cnt + 2,
-                           e->rank.p, e->stream));
+                           e->rank.p, packed, e->stream));
     prof_mark(e, 3);
     return PS_OK;
 }

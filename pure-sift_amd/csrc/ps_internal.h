@@ -88,8 +88,10 @@ hipError_t launch_extrema(const float *gauss_octave, int h, int w, int S, float 
 // Orders candidates: scan order if n <= limit, else (|response| desc, scan order); writes the
 // first min(n, limit) into out and that count into out_count.
 size_t rank_scratch_bytes(uint32_t cap_in);
+// packed_keys: octave-0 rows and columns < 4096, fewer than 16 slices and 8 octaves, so a
+// candidate's scan key packs into 31 bits (enables the direct ordering of up to 32768 candidates).
 hipError_t launch_rank(const Cand *cand, const uint32_t *count, uint32_t cap_in, uint32_t limit,
-                       Cand *out, uint32_t *out_count, void *scratch, hipStream_t st);
+                       Cand *out, uint32_t *out_count, void *scratch, bool packed_keys, hipStream_t st);
 
 // describe.hip
 hipError_t launch_orientations(const PyrView &pv, const Cand *kps, const uint32_t *nkp, uint32_t max_kp,

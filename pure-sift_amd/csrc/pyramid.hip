@@ -914,8 +914,83 @@ __device__ __forceinline__ void rank_select_body(const Cand *c, const uint32_t *
     if (threadIdx.x == 0) { words[1] = s_prefix; words[2] = 1; }
 }
 
+// Direct ordering of up to kFastN candidates whose scan keys pack into 31 bits (octave < 8,
+// slice < 16, octave-local row and column < 4096): each candidate's output position is the
+// number of candidates that precede it, counted over all pairs on one 64-bit key per candidate
+// -- (~|response| bits, packed scan key) when n > limit, the packed scan key alone otherwise --
+// so no threshold select or subset gather is needed. Keys are distinct (one candidate per
+// octave, slice and pixel), so the positions are a permutation. A (256-candidate tile, j-chunk)
+// grid writes one partial count per candidate and chunk; the scatter sums them. Larger or
+// unpackable inputs take the select / gather / count path below, whose kernels then run and
+// these return at once (and the other way round).
+constexpr uint32_t kFastN = 32768;
+constexpr uint32_t kFastChunks = 32;
+constexpr int kFastPerThread = 1;   // candidates per thread (more per thread measured slower)
+
+// k: octave << 56 | slice << 48 | row << 24 | column
+__device__ __forceinline__ uint64_t fast_key(uint64_t k, float response, bool by_resp) {
+    const uint32_t k31 = (uint32_t)(k >> 56) << 28 | (uint32_t)((k >> 48) & 15u) << 24 |
+                         (uint32_t)((k >> 24) & 4095u) << 12 | (uint32_t)(k & 4095u);
+    return by_resp ? (uint64_t)~__float_as_uint(fabsf(response)) << 32 | k31 : (uint64_t)k31;
+}
+
+__device__ __forceinline__ bool fast_order(const uint32_t *count, uint32_t cap_in, bool fast) {
+    return fast && min(*count, cap_in) <= kFastN;
+}
+
+__global__ __launch_bounds__(256) void rank_fast_count_kernel(const Cand *c, const uint32_t *count, uint32_t cap_in,
+                                                              uint32_t limit, uint32_t *part) {
+    __shared__ uint64_t s_k[kFastN / kFastChunks];
+    const uint32_t n = min(*count, cap_in);
+    const uint32_t per = (n + kFastChunks - 1) / kFastChunks;
+    const uint32_t j0 = blockIdx.y * per, j1 = min(j0 + per, n);
+    constexpr uint32_t kTile = 256 * kFastPerThread;
+    if (n > kFastN || blockIdx.x * kTile >= n || j0 >= j1) return;
+    const bool by_resp = n > limit;
+    // every load of the thread (its chunk entries and its own candidates) in flight together
+    constexpr int kPer = kFastN / kFastChunks / 256;
+    uint64_t kj[kPer], kc[kFastPerThread];
+    float rj[kPer], rc[kFastPerThread];
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+        const uint32_t j = min(j0 + threadIdx.x + 256 * u, j1 - 1);
+        kj[u] = c[j].key;
+        rj[u] = c[j].response;
+    }
+#pragma unroll
+    for (int v = 0; v < kFastPerThread; v++) {
+        const uint32_t i = min(blockIdx.x * kTile + threadIdx.x + 256 * v, n - 1);
+        kc[v] = c[i].key;
+        rc[v] = c[i].response;
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; u++)
+        if (j0 + threadIdx.x + 256 * u < j1) s_k[threadIdx.x + 256 * u] = fast_key(kj[u], rj[u], by_resp);
+    uint64_t ki[kFastPerThread];
+    uint32_t acc[kFastPerThread];
+#pragma unroll
+    for (int v = 0; v < kFastPerThread; v++) {
+        ki[v] = fast_key(kc[v], rc[v], by_resp);
+        acc[v] = 0;
+    }
+    __syncthreads();
+    const uint32_t m = j1 - j0;
+#pragma unroll 4
+    for (uint32_t t = 0; t < m; t++) {
+        const uint64_t kt = s_k[t];
+#pragma unroll
+        for (int v = 0; v < kFastPerThread; v++) acc[v] += kt < ki[v] ? 1u : 0u;
+    }
+#pragma unroll
+    for (int v = 0; v < kFastPerThread; v++) {
+        const uint32_t i = blockIdx.x * kTile + threadIdx.x + 256 * v;
+        if (i < n) part[(size_t)blockIdx.y * kFastN + i] = acc[v];
+    }
+}
+
 __global__ __launch_bounds__(1024) void rank_select_kernel(const Cand *c, const uint32_t *count, uint32_t cap_in,
-                                                           uint32_t limit, uint32_t *words) {
+                                                           uint32_t limit, uint32_t *words, bool fast) {
+    if (fast_order(count, cap_in, fast)) return;
     rank_select_body(c, count, cap_in, limit, words);
 }
 
@@ -962,9 +1037,9 @@ __device__ __forceinline__ void rank_gather_block(const Cand *c, uint32_t n, uin
 
 __global__ __launch_bounds__(256) void rank_gather_kernel(const Cand *c, const uint32_t *count, uint32_t cap_in,
                                                           uint32_t limit, uint32_t *subset, RankRec *rec,
-                                                          uint32_t *rank, uint32_t *words) {
+                                                          uint32_t *rank, uint32_t *words, bool fast) {
     const uint32_t n = min(*count, cap_in);
-    if (blockIdx.x * blockDim.x >= n) return;
+    if (blockIdx.x * blockDim.x >= n || fast_order(count, cap_in, fast)) return;
     rank_gather_block(c, n, limit, subset, rec, rank, words, blockIdx.x * blockDim.x);
 }
 
@@ -1011,7 +1086,9 @@ __device__ __forceinline__ void rank_count_body(const RankRec *rec, const uint32
 }
 
 __global__ __launch_bounds__(kRankTile) void rank_count_kernel(const RankRec *rec, const uint32_t *words,
-                                                               uint32_t *rank) {
+                                                               uint32_t *rank, const uint32_t *count, uint32_t cap_in,
+                                                               bool fast) {
+    if (fast_order(count, cap_in, fast)) return;
     rank_count_body(rec, words, rank, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
 }
 
@@ -1022,8 +1099,24 @@ __device__ __forceinline__ void rank_scatter_one(const Cand *c, const uint32_t *
 }
 
 __global__ void rank_scatter_kernel(const Cand *c, const uint32_t *subset, const uint32_t *words, uint32_t limit,
-                                    const uint32_t *rank, Cand *out, uint32_t *out_count) {
-    rank_scatter_one(c, subset, words[0], limit, rank, out, out_count, blockIdx.x * blockDim.x + threadIdx.x);
+                                    const uint32_t *rank, Cand *out, uint32_t *out_count, const uint32_t *count,
+                                    uint32_t cap_in, bool fast, const uint32_t *part) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (fast_order(count, cap_in, fast)) {
+        const uint32_t n = min(*count, cap_in);
+        if (i == 0) *out_count = min(n, limit);
+        if (i >= n) return;
+        const uint32_t per = (n + kFastChunks - 1) / kFastChunks;
+        uint32_t p[kFastChunks];
+#pragma unroll
+        for (uint32_t ch = 0; ch < kFastChunks; ch++) p[ch] = ch * per < n ? part[(size_t)ch * kFastN + i] : 0u;
+        uint32_t r = 0;
+#pragma unroll
+        for (uint32_t ch = 0; ch < kFastChunks; ch++) r += p[ch];
+        if (r < limit) out[r] = c[i];
+        return;
+    }
+    rank_scatter_one(c, subset, words[0], limit, rank, out, out_count, i);
 }
 
 // scratch of one image's ordering: rank[cap_in], subset[cap_in], 64 words, RankRec[cap_in]
@@ -1036,16 +1129,23 @@ size_t rank_scratch_bytes(uint32_t cap_in) {
 }
 
 hipError_t launch_rank(const Cand *cand, const uint32_t *count, uint32_t cap_in, uint32_t limit, Cand *out,
-                       uint32_t *out_count, void *scratch, hipStream_t st) {
+                       uint32_t *out_count, void *scratch, bool packed_keys, hipStream_t st) {
     uint32_t *rank = static_cast<uint32_t *>(scratch);
     uint32_t *subset = rank + cap_in;
     uint32_t *words = subset + cap_in;
     RankRec *rec = reinterpret_cast<RankRec *>(words + 64);
+    // the direct path's partial counts reuse the record area (kFastChunks x kFastN words)
+    const bool fast = packed_keys && (size_t)cap_in * sizeof(RankRec) >= (size_t)kFastChunks * kFastN * 4;
+    uint32_t *part = reinterpret_cast<uint32_t *>(rec);
     const unsigned blocks = (cap_in + 255) / 256;
-    rank_select_kernel<<<1, 1024, 0, st>>>(cand, count, cap_in, limit, words);
-    rank_gather_kernel<<<blocks, 256, 0, st>>>(cand, count, cap_in, limit, subset, rec, rank, words);
-    rank_count_kernel<<<dim3(kRankGrid, 4 * kRankGrid), kRankTile, 0, st>>>(rec, words, rank);
-    rank_scatter_kernel<<<blocks, 256, 0, st>>>(cand, subset, words, limit, rank, out, out_count);
+    if (fast)
+        rank_fast_count_kernel<<<dim3(kFastN / (256 * kFastPerThread), kFastChunks), 256, 0, st>>>(cand, count, cap_in,
+                                                                                                 limit, part);
+    rank_select_kernel<<<1, 1024, 0, st>>>(cand, count, cap_in, limit, words, fast);
+    rank_gather_kernel<<<blocks, 256, 0, st>>>(cand, count, cap_in, limit, subset, rec, rank, words, fast);
+    rank_count_kernel<<<dim3(kRankGrid, 4 * kRankGrid), kRankTile, 0, st>>>(rec, words, rank, count, cap_in, fast);
+    rank_scatter_kernel<<<blocks, 256, 0, st>>>(cand, subset, words, limit, rank, out, out_count, count, cap_in, fast,
+                                                part);
     return hipGetLastError();
 }
 

@@ -4,8 +4,9 @@ same seeded inputs.
 Bars:
   * Gaussian/DoG pyramid, keypoint x/y/response/octave/layer, match indices,
     distances and ratios: bit-exact.
-  * keypoint scale: within 1 ulp (the device evaluates k^ds in f64 and rounds once;
-    glibc powf differs from that in ~0.06% of arguments), i.e. << 1e-4 octave.
+  * keypoint scale: within 2 ulp (the device evaluates k^ds in f64 and rounds once;
+    glibc powf differs from that by 1 ulp in ~0.06% of arguments, and the product
+    eff_scale * k^ds can round that to 2 ulp), i.e. << 1e-4 octave.
   * descriptors: L2 <= 1e-3 per feature on unit vectors, orientation within 1e-3 rad,
     after aligning features by keypoint (orientation-count flips are counted and
     bounded, since device expf/atan2f differ from glibc in the last ulp).
@@ -79,7 +80,7 @@ def _cmp_keypoints(dev, ref):
     for f in ("x", "y", "response", "octave", "layer"):
         assert np.array_equal(dev[f], ref[f]), f
     ulp = np.abs(dev["scale"].view(np.int32).astype(np.int64) - ref["scale"].view(np.int32).astype(np.int64))
-    assert ulp.max(initial=0) <= 1
+    assert ulp.max(initial=0) <= 2
     return int((ulp > 0).sum())
 
 
@@ -197,6 +198,40 @@ def test_detect_and_compute_capped_1080p(gpu):
     rh, rd = psr.extract_features(img, 4, 3, 0.015, True)
     assert len(dev) == 4096 == len(rh)
     # the keypoint order after the cap is exact, so headers agree except for orientation flips
+    flips, _ = _check_features(dev, rh, rd)
+    assert flips <= 8
+
+
+@pytest.mark.parametrize("thr", [0.015, 0.005])
+def test_order_with_unpacked_scan_keys(gpu, thr):
+    """An image whose upsampled first octave is 4200 px wide: its scan keys do not pack into
+    31 bits, so the ordering takes the select / gather / count path (the direct all-pairs
+    ordering needs rows and columns < 4096). Uncapped keypoints in scan order, and the capped
+    features (15.7k candidates at contrast 0.005), equal the oracle's."""
+    a = synth.blob_image(2100, 240, 300, 3, True)
+    img = psr.luma_to_f32(a)
+    cfg = gpu.SiftConfig().octaves(4).scales(3).contrast_threshold(thr)
+    _cmp_keypoints(gpu.Sift(cfg).detect_keypoints(img), psr.detect_keypoints(img, 4, 3, thr, True))
+    dev = gpu.Sift(cfg).detect_and_compute(a)
+    rh, rd = psr.extract_features(img, 4, 3, thr, True)
+    assert len(dev) == len(rh) and (thr > 0.01 or len(rh) == 4096)
+    flips, _ = _check_features(dev, rh, rd)
+    assert flips <= 8
+
+
+def test_order_beyond_the_direct_limit_1080p(gpu):
+    """1080p at contrast 0.006: 55k candidates, more than the direct ordering takes (32768), so
+    the select / gather / count path orders them -- by scan key uncapped, by |response| for the
+    4096-feature cap -- and both equal the oracle's."""
+    u8, _, _ = synth.blob_pair(1920, 1080, 2000, seed=1, band_noise=True)
+    img = psr.luma_to_f32(u8)
+    cfg = gpu.SiftConfig().octaves(4).scales(3).contrast_threshold(0.006)
+    ref = psr.detect_keypoints(img, 4, 3, 0.006, True)
+    assert len(ref) > 32768
+    _cmp_keypoints(gpu.Sift(cfg).detect_keypoints(img), ref)
+    dev = gpu.Sift(cfg).detect_and_compute(u8)
+    rh, rd = psr.extract_features(img, 4, 3, 0.006, True)
+    assert len(dev) == 4096 == len(rh)
     flips, _ = _check_features(dev, rh, rd)
     assert flips <= 8
 
