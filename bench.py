@@ -6,11 +6,13 @@ octave upsampled, contrast 0.015, capped at 4096 features) and ratio-0.8 match i
 against the pre-extracted warped partner. Inputs are resident in HBM before the
 timed region; the whole step runs on one engine stream through the C ABI.
 
-Prints one JSON line. `roofline` covers the scale-space stage (pyramid + extrema, HBM bound):
-algorithmic bytes per image W*H + 4*P*(6 + 5 + 4) = 663,033,600 B at 1080p -- the u8 input, the
-6 Gaussian planes written per octave, the 5 source planes the blur levels read and the 4
-Gaussian planes the extrema pass reads (P = pixels summed over octaves) -- timed with HIP events
-on the engine's stream. `cpu_baseline` is the CPU oracle
+Prints one JSON line. `roofline` covers the scale-space stage (pyramid + extrema): algorithmic
+bytes per image W*H + 4*P*(5 + 4 + 4) = 574,905,600 B at 1080p -- the u8 input, the 5 Gaussian
+planes written per octave, the 4 source planes the blur levels read and the 4 Gaussian planes
+the extrema pass reads (P = pixels summed over octaves) -- over the stage time measured with
+HIP events on the engine's stream; beside it the stage's algorithmic VALU floor (the same
+arithmetic at the 32-bit VALU issue rate, no FMA), which binds slightly above HBM. `cpu_baseline`
+is the CPU oracle
 (the C restatement of the reference's Rayon path, parallelised at the same sites)
 timed on a bounded sample on this host.
 """
@@ -46,6 +48,32 @@ def alg_bytes(h, w, octaves=4, upsample=True):
 
 
 ALG_BYTES_1080P = alg_bytes(1080, 1920)
+
+
+def alg_valu_ops(h, w, octaves=4, upsample=True):
+    """Algorithmic 32-bit VALU operations of the scale-space stage (S = 3) per image, counting the
+    reference's own arithmetic with no FMA (each tap one multiply and one add): the u8 -> f32
+    conversion (2 per input pixel); per octave-0 pixel the 2x bilinear upsampling (4) and the
+    first blur (radius 4: 2 passes x 9 taps x 2); per pixel of every octave the blur levels
+    1..S+1 (radii 4, 5, 6, 8: 4 per tap) and the extremum scan (3 DoG differences, then per
+    searchable slice the 8-neighbour comparison and the threshold, ~10). Level S+2, the later
+    octaves' 2x decimation and the refinement of the few candidates are left out."""
+    hh, ww = (2 * h, 2 * w) if upsample else (h, w)
+    ops, first = 2 * h * w, True
+    for _ in range(octaves):
+        if hh < 3 or ww < 3:
+            break
+        px = hh * ww
+        if first:
+            ops += px * ((4 if upsample else 0) + 4 * 9)
+            first = False
+        ops += px * (4 * (9 + 11 + 13 + 17) + 3 + 3 * 10)
+        hh //= 2
+        ww //= 2
+    return ops
+
+
+ALG_VALU_OPS_1080P = alg_valu_ops(1080, 1920)
 
 
 def traffic_bytes_1080p():
@@ -925,7 +953,13 @@ def main():
                                "side streams",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_bytes_1080p(),
-                     "alg_bytes_per_image": ALG_BYTES_1080P},
+                     "alg_bytes_per_image": ALG_BYTES_1080P,
+                     "stage_us": pyr_ms * 1e3,
+                     "hbm_floor_us": ALG_BYTES_1080P / (HBM_PEAK_GBS * 1e9) * 1e6,
+                     "valu_floor_us": ALG_VALU_OPS_1080P / (VALU_INT_PEAK_TOPS * 1e12) * 1e6,
+                     "alg_valu_ops_per_image": ALG_VALU_OPS_1080P,
+                     "frac_of_binding_floor": max(ALG_BYTES_1080P / (HBM_PEAK_GBS * 1e9),
+                                                  ALG_VALU_OPS_1080P / (VALU_INT_PEAK_TOPS * 1e12)) / (pyr_ms * 1e-3)},
     }
     eng.close()
     if not args.no_fp64:
