@@ -71,13 +71,14 @@ __device__ __forceinline__ uint32_t bf16_bits_rne(float x) {
 
 // Matcher prep, one launch for both sides: |x|^2 of each row (any summation order: it only
 // ranks and bounds) and the row's bf16 split (128 words: hi pairs, then lo pairs; elements 2j
-// and 2j+1 in word j). Workgroups [0, tblocks) take target rows, the rest query rows, 64 rows
-// each (each wave 16, one at a time); rows in [n, rows) get norm +inf, so they never rank,
-// and a zero split. The largest target norm goes to tmax[gen & 1] as (gen << 32 | ordered
+// and 2j+1 in word j). Workgroups [0, tblocks) take target rows, the rest query rows,
+// 4 x kPrepRows rows each (each wave kPrepRows, their loads issued together); rows in [n, rows)
+// get norm +inf, so they never rank, and a zero split. The largest target norm goes to tmax[gen & 1] as (gen << 32 | ordered
 // bits) by one 64-bit atomicMax per workgroup, and workgroup 0 zeroes tmax[(gen + 1) & 1],
 // the word the next call (next generation) will take: stream order puts that store after
 // this generation's readers of the other word. A word of another generation (fresh memory
 // before its first clearing, a wrapped counter) reads as +inf, which is safe.
+constexpr int kPrepRows = 4;   // rows per wave (16 per workgroup: 512 workgroups for 4096 + 4096 rows)
 struct PrepSide {
     const float *x;
     const uint32_t *n_p;
@@ -94,15 +95,15 @@ __global__ __launch_bounds__(256) void nn_prep_kernel(PrepSide t, PrepSide q, ui
     const uint32_t blk = tside ? blockIdx.x : blockIdx.x - tblocks;
     const uint32_t n = min(*sd.n_p, sd.n_max);
     float wmax = 0.0f;
-    float2 v[16];
+    float2 v[kPrepRows];
 #pragma unroll
-    for (int k = 0; k < 16; k++) {   // all loads first so their latencies overlap
-        const uint32_t r = blk * 64 + w * 16 + k;
+    for (int k = 0; k < kPrepRows; k++) {   // all loads first so their latencies overlap
+        const uint32_t r = (blk * 4 + w) * kPrepRows + k;
         v[k] = r < n ? reinterpret_cast<const float2 *>(sd.x + (size_t)r * 128)[lane] : make_float2(0.f, 0.f);
     }
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-        const uint32_t r = blk * 64 + w * 16 + k;
+    for (int k = 0; k < kPrepRows; k++) {
+        const uint32_t r = (blk * 4 + w) * kPrepRows + k;
         float s = v[k].x * v[k].x + v[k].y * v[k].y;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
@@ -121,7 +122,11 @@ __global__ __launch_bounds__(256) void nn_prep_kernel(PrepSide t, PrepSide q, ui
     __syncthreads();
     if (threadIdx.x == 0) {
         const float m = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
-        atomicMax(tmax + (gen & 1), ((unsigned long long)gen << 32) | __float_as_uint(m));
+        // the word only grows during a generation: a workgroup whose maximum does not exceed
+        // the value it reads skips the atomic (hundreds of workgroups, one address)
+        const unsigned long long v = ((unsigned long long)gen << 32) | __float_as_uint(m);
+        if (__hip_atomic_load(tmax + (gen & 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < v)
+            atomicMax(tmax + (gen & 1), v);
     }
 }
 
@@ -457,7 +462,9 @@ hipError_t launch_nn_mfma(const float *qd, const uint32_t *nq, size_t nq_max, co
     p += nq_max * 128 * sizeof(uint32_t);
     uint32_t *tsplit = reinterpret_cast<uint32_t *>(p);
     const uint32_t trows = (uint32_t)tn_rows(nt_max);
-    const uint32_t tblocks = (trows + 63) / 64, qblocks = (uint32_t)((nq_max + 63) / 64);
+    constexpr uint32_t kPrepBlock = 4 * kPrepRows;
+    const uint32_t tblocks = (trows + kPrepBlock - 1) / kPrepBlock,
+                   qblocks = (uint32_t)((nq_max + kPrepBlock - 1) / kPrepBlock);
     const PrepSide ts{td, nt, (uint32_t)nt_max, trows, tn, tsplit};
     const PrepSide qs{qd, nq, (uint32_t)nq_max, (uint32_t)nq_max, qn, qsplit};
     nn_prep_kernel<<<tblocks + qblocks, 256, 0, st>>>(ts, qs, tblocks, tmax, gen);
@@ -532,7 +539,11 @@ __global__ __launch_bounds__(256) void nn_prep_sets_kernel(const float *x, const
     __syncthreads();
     if (threadIdx.x == 0) {
         const float m = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
-        atomicMax(tmax + (gen & 1), ((unsigned long long)gen << 32) | __float_as_uint(m));
+        // the word only grows during a generation: a workgroup whose maximum does not exceed
+        // the value it reads skips the atomic (hundreds of workgroups, one address)
+        const unsigned long long v = ((unsigned long long)gen << 32) | __float_as_uint(m);
+        if (__hip_atomic_load(tmax + (gen & 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < v)
+            atomicMax(tmax + (gen & 1), v);
     }
 }
 
