@@ -273,19 +273,22 @@ __global__ __launch_bounds__(256) void nn_mfma_kernel(const uint32_t *qs, const 
 // its own 4th); if that exceeds the exact second-best the result equals the full scan's,
 // otherwise the query is flagged and the workgroup scans all targets for it exactly
 // (pipeline.rs:1700-1740 order rules).
-// nn_rescore_block is one workgroup's share: queries 32 qblk .. + 31 (nq valid) of lists
+// nn_rescore_block is one workgroup's share (kRsThreads threads, kRsQ = kRsThreads / 8 queries):
+// queries kRsQ qblk .. + kRsQ - 1 (nq valid) of lists
 // part[(split * pstride + q) * 2 + half], targets td (nt of them, nt_scan scanned exactly).
+constexpr int kRsThreads = 128;   // 16 queries per workgroup: 256 workgroups for 4096 queries
+constexpr int kRsQ = kRsThreads / 8;
 __device__ __forceinline__ void nn_rescore_block(const Cand4 *part, int nsplit, size_t pstride, uint32_t qblk,
                                                  const float *qd, const float *qn, uint32_t nq, const float *td,
                                                  uint32_t nt, uint32_t nt_scan, const unsigned long long *tmax,
                                                  uint32_t gen, uint32_t *best_idx, float *best, float *second) {
-    __shared__ uint32_t s_flag[32];
+    __shared__ uint32_t s_flag[kRsQ];
     __shared__ uint32_t s_nflag;
-    __shared__ float sb[256], ss[256];
-    __shared__ uint32_t si[256];
+    __shared__ float sb[kRsThreads], ss[kRsThreads];
+    __shared__ uint32_t si[kRsThreads];
     if (threadIdx.x == 0) s_nflag = 0;
     __syncthreads();
-    const uint32_t q = qblk * 32 + (threadIdx.x >> 3);
+    const uint32_t q = qblk * kRsQ + (threadIdx.x >> 3);
     const int j = threadIdx.x & 7;
     const bool active = q < nq;
     // lane j: entry j % 4 of the half-(j / 4) list of every split
@@ -383,14 +386,14 @@ __device__ __forceinline__ void nn_rescore_block(const Cand4 *part, int nsplit, 
         const uint32_t fq = s_flag[f];
         float b = INFINITY, s2 = INFINITY;
         uint32_t bix = 0;
-        for (uint32_t t = threadIdx.x; t < nt_scan; t += 256) {
+        for (uint32_t t = threadIdx.x; t < nt_scan; t += kRsThreads) {
             const float d2 = seq_dist128(qd + (size_t)fq * 128, td + (size_t)t * 128);
             if (d2 < b) { s2 = b; b = d2; bix = t; }
             else if (d2 < s2) s2 = d2;
         }
         sb[threadIdx.x] = b; ss[threadIdx.x] = s2; si[threadIdx.x] = bix;
         __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
+        for (int w = kRsThreads / 2; w > 0; w >>= 1) {
             if (threadIdx.x < (unsigned)w) {
                 float B = sb[threadIdx.x], S = ss[threadIdx.x];
                 uint32_t BI = si[threadIdx.x];
@@ -412,7 +415,7 @@ __device__ __forceinline__ void nn_rescore_block(const Cand4 *part, int nsplit, 
     }
 }
 
-__global__ __launch_bounds__(256) void nn_rescore_kernel(const Cand4 *part, int nsplit, const float *qd, const float *qn,
+__global__ __launch_bounds__(kRsThreads) void nn_rescore_kernel(const Cand4 *part, int nsplit, const float *qd, const float *qn,
                                                          const uint32_t *nq_p, uint32_t nq_max, const float *td,
                                                          const uint32_t *nt_p, uint32_t nt_max,
                                                          const unsigned long long *tmax, uint32_t gen,
@@ -470,7 +473,7 @@ hipError_t launch_nn_mfma(const float *qd, const uint32_t *nq, size_t nq_max, co
     nn_prep_kernel<<<tblocks + qblocks, 256, 0, st>>>(ts, qs, tblocks, tmax, gen);
     dim3 grid((unsigned)((nq_max + kMQ - 1) / kMQ), (unsigned)ns);
     nn_mfma_kernel<<<grid, 256, 0, st>>>(qsplit, nq, (uint32_t)nq_max, tsplit, nt, (uint32_t)nt_max, tn, ns, part);
-    nn_rescore_kernel<<<(unsigned)((nq_max + 31) / 32), 256, 0, st>>>(part, ns, qd, qn, nq, (uint32_t)nq_max, td, nt,
+    nn_rescore_kernel<<<(unsigned)((nq_max + kRsQ - 1) / kRsQ), kRsThreads, 0, st>>>(part, ns, qd, qn, nq, (uint32_t)nq_max, td, nt,
                                                                       (uint32_t)nt_max, tmax, gen, best_idx, best,
                                                                       second);
     return hipGetLastError();
@@ -574,9 +577,9 @@ __global__ __launch_bounds__(256) void nn_mfma_sets_kernel(const uint32_t *qspli
     }
 }
 
-// Pass 2 per problem: grid (32-query blocks, problems); results at [k * cap + q].
-__global__ __launch_bounds__(256) void nn_rescore_sets_kernel(const Cand4 *part, int nsplit, const float *qd,
-                                                              const float *qn, const float *td, SetsView V,
+// Pass 2 per problem: grid (kRsQ-query blocks, problems); results at [k * cap + q].
+__global__ __launch_bounds__(kRsThreads) void nn_rescore_sets_kernel(const Cand4 *part, int nsplit, const float *qd,
+                                                                     const float *qn, const float *td, SetsView V,
                                                               const ps_match_problem *probs,
                                                               const unsigned long long *tmax, uint32_t gen,
                                                               uint32_t *best_idx, float *best, float *second) {
@@ -584,7 +587,7 @@ __global__ __launch_bounds__(256) void nn_rescore_sets_kernel(const Cand4 *part,
     ps_feature_set Q, T;
     uint32_t nq, nt;
     resolve_sets(probs[k], V, Q, T, nq, nt);
-    if (blockIdx.x * 32 >= nq) return;
+    if (blockIdx.x * kRsQ >= nq) return;
     const size_t o = (size_t)k * cap;
     nn_rescore_block(part + (size_t)k * nsplit * cap * 2, nsplit, cap, blockIdx.x, qd + (size_t)Q.offset * 128,
                      qn + Q.offset, nq, td + (size_t)T.offset * 128, nt, nt, tmax, gen, best_idx + o, best + o,
@@ -716,7 +719,7 @@ hipError_t launch_match_sets(const MatchSetsArgs &a, void *scratch, uint32_t gen
                                                                      1, tmax, gen);
         nn_mfma_sets_kernel<<<dim3((a.cap + kMQ - 1) / kMQ, a.n_probs, ns), 256, 0, st>>>(qs, ts, tn, V, a.probs, ns,
                                                                                           part);
-        nn_rescore_sets_kernel<<<dim3((a.cap + 31) / 32, a.n_probs), 256, 0, st>>>(part, ns, a.qd, qn, a.td, V, a.probs,
+        nn_rescore_sets_kernel<<<dim3((a.cap + kRsQ - 1) / kRsQ, a.n_probs), kRsThreads, 0, st>>>(part, ns, a.qd, qn, a.td, V, a.probs,
                                                                                    tmax, gen, bidx, best, second);
     }
     match_sets_finish_kernel<<<a.n_probs, 256, 0, st>>>(a.qf, a.tf, V, a.probs, bidx, best, second, a.ratio, out,
