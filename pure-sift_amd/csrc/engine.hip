@@ -342,17 +342,12 @@ ps::PyrView pyr_view(ps_engine *e) {
 // Octave o+1 depends only on level S of octave o, so the levels 1..S of all octaves form one
 // dependent chain: it stays on the engine stream (a cross-stream event costs ~10 us of
 // latency), and each octave's leaf work -- the levels above S and the extrema pass -- is forked
-// to a side stream once level S is written (octave 0's one octave later, leaf_delay); every
-// side stream is joined back into the engine stream before this returns. Candidate order is fixed later by the ordering stage, so it does
-// not matter which stream appends first. No fork happens while the engine stream is being
+// to a side stream once level S is written (octave 0's one octave later when octave 1 is small,
+// leaf_delay); every side stream is joined back into the engine stream before this returns.
+// Candidate order is fixed later by the ordering stage, so it does not matter which stream
+// appends first. No fork happens while the engine stream is being
 // captured: a capturing caller (the batched path) already overlaps whole images on separate
 // streams.
-// Chains of later octaves that the leaf work of octave o waits for before it is forked. Octave
-// 0's leaf (its level S+1 and its extrema, the largest work of the scale space) would otherwise
-// fill the device while octave 1's chain runs, and every later octave waits for that chain: it
-// is forked after octave 1's levels 1..S instead (1080p frame 0.455 -> 0.448 ms; delaying every
-// leaf by one octave, or octave 0's by two, measured slower: 0.464 / 0.469 ms).
-static int leaf_delay(int o) { return o == 0 ? 1 : 0; }
 
 ps_status enqueue_scale_space(ps_engine *e, const void *src, bool src_u8, size_t h, size_t w, size_t stride,
                               const ps_sift_config &cfg, bool extrema) {
@@ -433,7 +428,7 @@ ps_status enqueue_scale_space(ps_engine *e, const void *src, bool src_u8, size_t
         for (int s = 1; s <= p.S; s++) PS_HIP(level(o, s, e->stream));
         if (o == 0) prof_mark(e, 1);
         for (int j = 0; j <= o; j++)
-            if (!leaf_done[j] && std::min(j + leaf_delay(j), p.n_oct - 1) <= o) {
+            if (!leaf_done[j] && std::min(j + leaf_delay(j, (long)p.h[1] * p.w[1]), p.n_oct - 1) <= o) {
                 PS_TRY(leaf(j));
                 leaf_done[j] = true;
             }

@@ -30,6 +30,17 @@ enum AuxSlot {
 
 struct BatchState;   // batch.hip
 
+// Chains of later octaves that the leaf work of octave o waits for before it is forked, for an
+// image whose octave 1 has oct1_px pixels. Octave 0's leaf (its level S+1 and its extrema, the
+// largest work of the scale space) would otherwise fill the device while octave 1's chain runs,
+// and every later octave waits for that chain: where that chain is small enough to be
+// latency-bound it goes first, octave 0's leaf forked after octave 1's levels 1..S (1080p frame
+// 0.455 -> 0.448 ms, 720p 0.376 -> 0.350 ms; delaying every leaf by one octave, or octave 0's by
+// two, measured slower at 1080p: 0.464 / 0.469 ms). Larger octaves 1 keep the fork at octave 0
+// (a 3840x2160 frame, octave 1 8.3M px, and the batched scale space, whose grouped launches are
+// throughput-bound, measured slower with the delay).
+inline int leaf_delay(int o, long oct1_px) { return o == 0 && oct1_px <= (4L << 20) ? 1 : 0; }
+
 struct ps_engine {
     int device = 0;
     hipStream_t stream = nullptr;
