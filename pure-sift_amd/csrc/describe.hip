@@ -221,21 +221,29 @@ hipError_t launch_orientations(const PyrView &pv, const Cand *kps, const uint32_
 constexpr int kScanThreads = 1024;
 __device__ __forceinline__ void feature_scan_body(const uint32_t *nkp, const uint32_t *nori, uint32_t max_kp,
                                                   uint32_t cap, uint32_t *feat_kp, uint32_t *feat_j, uint32_t *nfeat) {
-    __shared__ uint32_t s[kScanThreads];
+    __shared__ uint32_t s_w[kScanThreads / 64];
     const uint32_t n = min(*nkp, max_kp);
     const uint32_t per = (max_kp + kScanThreads - 1) / kScanThreads;
     const uint32_t b = threadIdx.x * per, e = min(b + per, n);
     uint32_t local = 0;
     for (uint32_t i = b; i < e; i++) local += nori[i];
-    s[threadIdx.x] = local;
-    __syncthreads();
-    for (int d = 1; d < kScanThreads; d <<= 1) {   // Hillis-Steele inclusive scan
-        uint32_t v = threadIdx.x >= (unsigned)d ? s[threadIdx.x - d] : 0u;
-        __syncthreads();
-        s[threadIdx.x] += v;
-        __syncthreads();
+    // exclusive offset of this thread's keypoints: a scan inside the wave, then the wave totals
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t incl = local;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(incl, d);
+        if (lane >= (uint32_t)d) incl += t;
     }
-    uint32_t off = s[threadIdx.x] - local;
+    if (lane == 63) s_w[wv] = incl;
+    __syncthreads();
+    uint32_t off = incl - local, total = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanThreads / 64; k++) {
+        const uint32_t v = s_w[k];
+        if (k < wv) off += v;
+        total += v;
+    }
     for (uint32_t i = b; i < e; i++) {
         uint32_t c = nori[i];
         for (uint32_t j = 0; j < c; j++) {
@@ -243,7 +251,7 @@ __device__ __forceinline__ void feature_scan_body(const uint32_t *nkp, const uin
         }
         off += c;
     }
-    if (threadIdx.x == kScanThreads - 1) *nfeat = min(s[kScanThreads - 1], cap);
+    if (threadIdx.x == kScanThreads - 1) *nfeat = min(total, cap);
 }
 
 __global__ __launch_bounds__(kScanThreads) void feature_scan_kernel(const uint32_t *nkp, const uint32_t *nori,
@@ -258,10 +266,10 @@ hipError_t launch_feature_scan(const uint32_t *nkp, const uint32_t *nori, uint32
     return hipGetLastError();
 }
 
-// pipeline.rs:1352-1480 (+ RootSIFT, :1012-1019). One 256-thread workgroup per feature and no
+// pipeline.rs:1352-1480 (+ RootSIFT, :1012-1019). One 128-thread workgroup per feature and no
 // shared accumulators: a pixel that passes the window test has r_bin, c_bin in (-1, 4), so it
 // belongs to exactly one of the 25 cells (floor(r_bin), floor(c_bin)) in {-1..3}^2 and only
-// feeds the 4 spatial bins around that cell. Ten lanes own each cell and accumulate 4 corners
+// feeds the 4 spatial bins around that cell. Five lanes own each cell and accumulate 4 corners
 // x 8 orientations into lane-private LDS regions; a fixed-order sum over the owners of the 4
 // neighbouring cells then forms each of the 128 bins.
 //
@@ -271,8 +279,8 @@ hipError_t launch_feature_scan(const uint32_t *nkp, const uint32_t *nori, uint32
 // the window form one interval, bounded by where four monotone tests change value. Each
 // transition is located from its closed-form estimate and confirmed by evaluating the test on
 // both sides (a binary search when the estimate misses, e.g. for a nearly axis-parallel
-// boundary), so the intervals hold exactly the pixels the per-pixel test would keep. The ten
-// lanes then take every tenth pixel of the cell's row-major sequence.
+// boundary), so the intervals hold exactly the pixels the per-pixel test would keep. The five
+// lanes then take every fifth pixel of the cell's row-major sequence.
 constexpr int kFeatThreads = 128;   // descriptor workgroup: 5 lanes per cell, lanes 125..127 idle
 constexpr int kLanesPerCell = 5;
 constexpr int kPrivStride = 33;     // 32 accumulators + 1 pad

@@ -219,6 +219,19 @@ def test_order_with_unpacked_scan_keys(gpu, thr):
     assert flips <= 8
 
 
+def test_keypoints_exact_3000x1500(gpu):
+    """A 3000x1500 image: its octave 1 (3000 x 1500 px, more than 4M) keeps every octave's leaf
+    work (level S+1 and extrema) forked right after its own chain -- smaller images fork octave
+    0's after octave 1's chain -- and its 6000 px wide first octave takes the select / gather /
+    count ordering. 21k keypoints in scan order equal the oracle's."""
+    a = synth.blob_image(3000, 1500, 1500, 7, True)
+    img = psr.luma_to_f32(a)
+    cfg = gpu.SiftConfig().octaves(4).scales(3)
+    ref = psr.detect_keypoints(img, 4, 3, 0.015, True)
+    assert len(ref) > 20000
+    _cmp_keypoints(gpu.Sift(cfg).detect_keypoints(img), ref)
+
+
 def test_order_beyond_the_direct_limit_1080p(gpu):
     """1080p at contrast 0.006: 55k candidates, more than the direct ordering takes (32768), so
     the select / gather / count path orders them -- by scan key uncapped, by |response| for the
