@@ -2,7 +2,7 @@
 (fp64_errors.py), on the 640x480 pair and the 1080p frame pair: keypoint position and scale
 (the f32 Taylor solve alone, and the whole f32 chain), descriptors (at the feature's own
 orientation; orientation error to the float64 peak), match distances and ratios, and the
-homography's reprojection against a float64 DLT of its inliers. The bounds are the north-star
+homography's reprojection against a float64 DLT of its inliers. The bounds are the stated
 tolerances (1e-3 px, 1e-4 octave, 1e-3 descriptor L2) at the 99th percentile, where the rare
 branch flips of a float64 decision (the Taylor step's |offset| <= 0.5 test, an orientation
 bin) are counted and reported instead."""

@@ -1,7 +1,7 @@
 """GPU parity: the HIP pipeline (through the C ABI) against the CPU oracle on the
 same seeded inputs.
 
-Bars (from the north-star tolerances):
+Bars:
   * Gaussian/DoG pyramid, keypoint x/y/response/octave/layer, match indices,
     distances and ratios: bit-exact.
   * keypoint scale: within 2 ulp (the device evaluates k^ds in f64 and rounds once;
