@@ -352,6 +352,25 @@ def test_match_exact_on_oracle_features(gpu, mutual):
     assert np.array_equal(m, ref)
 
 
+@pytest.mark.parametrize("mutual", [False, True])
+def test_match_exact_beyond_4096_queries(gpu, mutual):
+    """6000 queries against 5000 targets (planted unit descriptors): past 16 blocks of 256
+    queries the ratio test and the ordered compaction take the flag + compaction launches
+    instead of the single finishing launch; the records equal the oracle's."""
+    q, db, _ = synth.unit_descriptors(6000, 5000, seed=33)
+    rng = np.random.default_rng(33)
+    qh = np.zeros(6000, gpu.FEAT_DTYPE)
+    th = np.zeros(5000, gpu.FEAT_DTYPE)
+    for h in (qh, th):
+        h["x"] = rng.uniform(0, 1920, len(h))
+        h["y"] = rng.uniform(0, 1080, len(h))
+        h["scale"] = rng.uniform(1, 8, len(h))
+    ref = psr.match(qh, q, th, db, 0.8, mutual)
+    m = gpu.Matcher(gpu.MatchOptions(0.8, mutual)).match_(gpu.Features(qh, q), gpu.Features(th, db))
+    assert len(ref) > 500
+    assert np.array_equal(m, ref)
+
+
 def test_mutual_known_answer(gpu):
     """pipeline_tests.rs:1221-1280"""
     def desc(v0, v1):
