@@ -805,6 +805,7 @@ ps_status ps_catalog_coarse_votes(const ps_catalog *c, const float *q_desc, size
 ps_status ps_catalog_search(const ps_catalog *c, const ps_feature *q, const float *q_desc, size_t nq, size_t top_k,
                             size_t shortlist_k, float ratio, float ransac_threshold, ps_catalog_hit *out, size_t cap,
                             size_t *out_count) {
+    TraceRange tr("ps.catalog_search");
     PS_TRY(check_fail_injection());
     PS_TRY(check_search_args(c, q, q_desc, nq, out, cap, out_count));
     *out_count = 0;

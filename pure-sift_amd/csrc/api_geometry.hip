@@ -152,6 +152,7 @@ ps_status ps_engine_pair_geometry_batch(ps_engine *e, const ps_feature_match *d_
 ps_status ps_pair_geometry_estimate(const ps_feature *q, size_t nq, const ps_feature *t, size_t nt,
                                     const ps_feature_match *m, size_t n, const ps_ransac_options *opt,
                                     uint32_t *out_best_inliers, ps_pair_geometry *out) {
+    TraceRange tr("ps.pair_geometry");
     PS_TRY(check_fail_injection());
     if (!opt || !out || (n && (!m || !q || !t || !out_best_inliers))) return set_err(PS_E_INVALID_ARG, "null argument");
     PS_TRY(check_reproj(opt->reproj_threshold));

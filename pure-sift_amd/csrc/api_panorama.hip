@@ -15,6 +15,7 @@ namespace pshost {
 // grouped exact ratio match on the engine stream (scratch: the caller's workspace slot)
 ps_status enqueue_match_sets(ps_engine *e, const ps::MatchSetsArgs &a, DevBuf &scratch, ps_feature_match *out,
                              uint32_t *counts) {
+    TraceRange tr("ps.match_sets");
     bool fresh = false;
     PS_TRY(ensure(scratch, ps::match_sets_scratch_bytes(a), &fresh));
     if (++e->nn_gen == 0) e->nn_gen = 1;
@@ -200,6 +201,7 @@ ps_status ps_engine_panorama_pairs(ps_engine *e, const uint8_t *d_pixels, const 
 ps_status ps_panorama_pair_edges(const uint8_t *const *luma, const uint32_t *heights, const uint32_t *widths,
                                  const uint32_t *row_strides, uint32_t n_images, const ps_panorama_options *opt,
                                  ps_pair_edge *out, size_t cap, size_t *out_count) {
+    TraceRange tr("ps.panorama_pair_edges");
     PS_TRY(check_fail_injection());
     if (!opt || !out_count) return set_err(PS_E_INVALID_ARG, "null argument");
     *out_count = 0;

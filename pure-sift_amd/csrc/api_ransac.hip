@@ -85,6 +85,7 @@ ps_status ps_homography_estimate(const ps_feature_match *m, size_t n, const ps_r
 ps_status ps_ransac_batch(const ps_feature_match *matches, const float *weights, const ps_ransac_problem *problems,
                           uint32_t n_problems, size_t total_matches, const ps_ransac_params *params,
                           uint32_t *inliers, ps_ransac_result *results) {
+    TraceRange tr("ps.ransac_batch");
     PS_TRY(check_fail_injection());
     PS_TRY(check_params(params));
     if (n_problems == 0) return PS_OK;

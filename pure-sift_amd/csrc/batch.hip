@@ -428,6 +428,7 @@ ps_status enqueue_scale_space_group(ps_engine *e, BatchState &B) {
 // The whole batched extraction: counters, scale space, ordering (4096 cap), orientations,
 // feature flattening and descriptors.
 ps_status enqueue_group(ps_engine *e, BatchState &B) {
+    TraceRange tr("ps.extract_batch");
     const uint32_t kMax = PS_MAX_FEATURES;
     PS_HIP(launch_clear_counters(as<uint32_t>(B.cnt), B.n, kErrWord, e->stream));
     PS_TRY(enqueue_scale_space_group(e, B));

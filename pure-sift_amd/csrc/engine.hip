@@ -351,6 +351,7 @@ ps::PyrView pyr_view(ps_engine *e) {
 
 ps_status enqueue_scale_space(ps_engine *e, const void *src, bool src_u8, size_t h, size_t w, size_t stride,
                               const ps_sift_config &cfg, bool extrema) {
+    TraceRange tr("ps.scale_space");
     ps::PyrPlan &p = e->plan;
     if (!ps::make_plan((int)h, (int)w, cfg, p)) return set_err(PS_E_CONFIG, "unsupported blur radius");
     PS_TRY(ensure(e->pyr, p.total_floats * sizeof(float)));
@@ -464,6 +465,7 @@ ps_status enqueue_keypoints(ps_engine *e, const void *src, bool src_u8, size_t h
 ps_status enqueue_extract(ps_engine *e, const void *src, bool src_u8, size_t h, size_t w, size_t stride,
                           const ps_sift_config &cfg, ps_feature *d_feat, float *d_desc, uint32_t cap,
                           uint32_t *d_count) {
+    TraceRange tr("ps.extract");
     const uint32_t kMax = PS_MAX_FEATURES;
     PS_TRY(enqueue_keypoints(e, src, src_u8, h, w, stride, cfg, kMax));
     PS_TRY(ensure(e->ori, kMax * 4 * sizeof(float)));
@@ -525,6 +527,7 @@ ps_status enqueue_nn(ps_engine *e, const float *qd, const uint32_t *nq, size_t n
 ps_status enqueue_match(ps_engine *e, const ps_feature *qf, const float *qd, const uint32_t *nq, size_t nq_max,
                         const ps_feature *tf, const float *td, const uint32_t *nt, size_t nt_max,
                         const ps_match_options &opt, ps_feature_match *out, uint32_t *out_count) {
+    TraceRange tr("ps.match");
     size_t nqm = std::max<size_t>(nq_max, 1), ntm = std::max<size_t>(nt_max, 1);
     prof_mark(e, 5);
     PS_TRY(ensure(e->bidx, nqm * sizeof(uint32_t)));

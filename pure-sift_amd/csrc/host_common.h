@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "ps_internal.h"
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 struct DevBuf {
     void *p = nullptr;
@@ -29,6 +30,16 @@ enum AuxSlot {
 };
 
 struct BatchState;   // batch.hip
+
+// A named host-side trace range (roctx) around an enqueue sequence: rocprofv3 --marker-trace
+// shows it beside the kernels the sequence launched (host time of the call; with no tracer
+// attached the calls return at once).
+struct TraceRange {
+    explicit TraceRange(const char *name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange &) = delete;
+    TraceRange &operator=(const TraceRange &) = delete;
+};
 
 // Chains of later octaves that the leaf work of octave o waits for before it is forked, for an
 // image whose octave 1 has oct1_px pixels. Octave 0's leaf (its level S+1 and its extrema, the
