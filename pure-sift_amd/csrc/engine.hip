@@ -350,13 +350,14 @@ ps::PyrView pyr_view(ps_engine *e) {
 // streams.
 
 ps_status enqueue_scale_space(ps_engine *e, const void *src, bool src_u8, size_t h, size_t w, size_t stride,
-                              const ps_sift_config &cfg, bool extrema) {
+                              const ps_sift_config &cfg, bool extrema, uint32_t *clear = nullptr, int n_clear = 0) {
     TraceRange tr("ps.scale_space");
     ps::PyrPlan &p = e->plan;
     if (!ps::make_plan((int)h, (int)w, cfg, p)) return set_err(PS_E_CONFIG, "unsupported blur radius");
     PS_TRY(ensure(e->pyr, p.total_floats * sizeof(float)));
     if (extrema) PS_TRY(ensure(e->cand, (size_t)ps::kCandCap * sizeof(ps::Cand)));
     if (p.n_oct == 0) {
+        if (clear) PS_HIP(ps::launch_fill_u32(clear, 0u, (size_t)n_clear, e->stream));
         prof_mark(e, 1);
         return PS_OK;
     }
@@ -405,6 +406,8 @@ ps_status enqueue_scale_space(ps_engine *e, const void *src, bool src_u8, size_t
     a.dst_g = P + p.gauss(0, 0);
     a.dst_d = nullptr;
     a.dst_src = nullptr;
+    a.clear = clear;
+    a.n_clear = n_clear;
     int mode = src_u8 ? (p.upsample ? ps::SRC_U8_UP : ps::SRC_U8) : (p.upsample ? ps::SRC_F32_UP : ps::SRC_F32);
     PS_HIP(ps::launch_blur(p.init_r, mode, a, p.init_taps, blur_tile(p.init_r, mode, a.h, a.w), e->stream));
     // octave o's leaf work -- its remaining levels and its extrema -- on a side stream beside the
@@ -450,9 +453,10 @@ ps_status enqueue_keypoints(ps_engine *e, const void *src, bool src_u8, size_t h
     uint32_t *cnt = as<uint32_t>(e->cnt);
     PS_TRY(ensure(e->rank, ps::rank_scratch_bytes(ps::kCandCap)));
     PS_TRY(ensure(e->sorted, (size_t)limit * sizeof(ps::Cand)));
-    PS_HIP(ps::launch_fill_u32(cnt, 0u, kErrWord, e->stream));   // every counter but the error word
     prof_mark(e, 0);
-    PS_TRY(enqueue_scale_space(e, src, src_u8, h, w, stride, cfg, true));
+    // every counter but the error word is zeroed by the first blur launch (or here when the image
+    // is too small for an octave)
+    PS_TRY(enqueue_scale_space(e, src, src_u8, h, w, stride, cfg, true, cnt, (int)kErrWord));
     prof_mark(e, 2);
     const ps::PyrPlan &p = e->plan;
     const bool packed = p.n_oct <= 8 && p.S < 16 && (p.n_oct == 0 || (p.h[0] <= 4096 && p.w[0] <= 4096));

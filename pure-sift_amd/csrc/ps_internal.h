@@ -67,6 +67,10 @@ struct BlurArgs {
     float *dst_g;        // Gaussian out (nullable)
     float *dst_d;        // DoG out = G - src (nullable)
     float *dst_src;      // the source value itself (nullable; materialises G0 for SRC_DOWN2)
+    // words the launch's first workgroup zeroes before anything else (the extraction's counters,
+    // cleared by its first blur instead of a launch of their own); nullable
+    uint32_t *clear = nullptr;
+    int n_clear = 0;
 };
 
 // host helpers (engine.hip)

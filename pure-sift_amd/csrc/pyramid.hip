@@ -79,6 +79,7 @@ __device__ __forceinline__ void blur_tile(const BlurArgs &a, const Taps &taps, i
     __shared__ __attribute__((aligned(16))) float s_in[IH * IWP];
     __shared__ __attribute__((aligned(16))) float s_tmp[IH * kTW];
     const int h = a.h, w = a.w;
+    if (a.clear && x0 == 0 && y0 == 0 && tid < a.n_clear) a.clear[tid] = 0u;
 
     // stage the clamped window: all loads of a thread are issued before its LDS stores so
     // they overlap instead of paying one memory latency each
@@ -198,6 +199,7 @@ __device__ __forceinline__ void blur_strip(const BlurArgs &a, const Taps &taps, 
     // the wave index is uniform: readfirstlane lets row arithmetic stay scalar
     const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int h = a.h, w = a.w;
+    if (a.clear && x0 == 0 && y_start == 0 && tid < a.n_clear) a.clear[tid] = 0u;
     const int y_end = min(y_start + chunk, h);
     const int n_steps = (y_end - y_start + TH - 1) / TH + P;
     const int hs = y_start - R;   // source row of horizontal-pass row 0
