@@ -187,6 +187,26 @@ def test_detect_and_compute_640x480(gpu, kind):
     assert np.array_equal(dev_f.header, dev.header) and np.array_equal(dev_f.descriptors, dev.descriptors)
 
 
+def test_counts_start_at_zero_each_extraction(gpu):
+    """An extraction's counters are zeroed by its first blur launch, or by one fill when the
+    image is too small for any octave (base level under 3 px: 2 x 2 without upsampling, 1 x 1
+    with it). Extractions right after one with hundreds of features report only their own
+    (none, like the oracle), and the first image extracted again is byte-equal to its first run."""
+    u8, img = _blob_f32(640, 480, 200, seed=0)
+    cfg = gpu.SiftConfig().octaves(4).scales(3)
+    first = gpu.Sift(cfg).detect_and_compute(u8)
+    assert len(first) > 100
+    for shape, up in [((2, 2), False), ((1, 1), True), ((2, 5), False)]:
+        tiny = np.full(shape, 0.5, np.float32)
+        tiny[0, 0] = 0.9
+        rh, _ = psr.extract_features(tiny, 4, 3, 0.015, up)
+        dev = gpu.Sift(gpu.SiftConfig().octaves(4).scales(3).upsample_first_octave(up)).detect_and_compute(tiny)
+        assert len(dev) == len(rh) == 0, shape
+        assert len(gpu.Sift(cfg).detect_keypoints(tiny)) == 0, shape
+    again = gpu.Sift(cfg).detect_and_compute(u8)
+    assert np.array_equal(again.header, first.header) and np.array_equal(again.descriptors, first.descriptors)
+
+
 def test_detect_and_compute_capped_1080p(gpu):
     """1080p band-noise image: > 4096 keypoints, so the stable |response| cap is exercised."""
     u8, _, _ = synth.blob_pair(1920, 1080, 2000, seed=1, band_noise=True)
